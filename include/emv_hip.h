@@ -1,0 +1,124 @@
+/*
+ * emv_hip.h -- C ABI of the MI355X (gfx950) Erickson-Monma-Veinott Steiner
+ * tree dynamic program.  Plain C types only; no HIP or torch types cross it.
+ *
+ * Drop-in boundary.  The reference computes the DP inside
+ *     index_t erickson_monma_veinott(steinerq_t *root, index_t list_soln)
+ * (reader/reader-el.c:2141-2343), which allocates the tables (2212-2259),
+ * runs emv_kernel (1964-2139, with dijkstra 1762-1880) and the traceback
+ * build_tree/backtrack (1907-1944).  That one call is replaced by
+ *     emv_open + emv_load_graph(root->n, root->m, root->pos, root->adj)
+ *     + emv_solve(root->k, root->kk, EMV_LIST, &res)
+ * and the host prints the reference's lines from the result.
+ *
+ * Conventions.  Every function returns EMV_OK (0) or a negative EMV_ERR_*
+ * status, never aborts and never throws; emv_last_error() gives the text.
+ * Vertex ids are 0-based.  The caller owns every input array and may free it
+ * when the call returns.  The library owns device memory and the pairs array
+ * of an emv_result until emv_result_free().  A context is driven by one host
+ * thread at a time.  Weights must be integers in [1, 2^31-1]; the graph must
+ * be connected (EMV_ERR_DISCONNECTED otherwise).
+ */
+#ifndef EMV_HIP_H
+#define EMV_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMV_OK                 0
+#define EMV_ERR_ARG           -1   /* bad argument */
+#define EMV_ERR_HIP           -2   /* HIP runtime error */
+#define EMV_ERR_NOMEM         -3   /* device or host allocation failed */
+#define EMV_ERR_STATE         -4   /* call out of order (no graph, no table) */
+#define EMV_ERR_DISCONNECTED  -5   /* a terminal does not reach every vertex */
+#define EMV_ERR_RANGE         -6   /* values exceed the supported integer range */
+#define EMV_ERR_NODEV         -7   /* no HIP device */
+#define EMV_ERR_CAPACITY      -8   /* request exceeds a configured capacity */
+#define EMV_ERR_ADMISSION     -9   /* serving: request can never fit in device memory */
+
+/* emv_solve flags */
+#define EMV_LIST        0x1u  /* run the device traceback; fill res->pairs */
+#define EMV_FORCE_I64   0x2u  /* int64 table even when int32 bounds hold */
+#define EMV_KEEP_TABLE  0x4u  /* keep the table on the device for emv_read_rows */
+#define EMV_FORCE_P64   0x8u  /* 64-bit merge packing even when 32-bit fits */
+
+typedef struct emv_ctx emv_ctx;
+
+typedef struct {
+    int64_t cost;            /* f[C][q], C = 2^(k-1)-1, q = kk[k-1] (reader-el.c:2137) */
+    int32_t k;
+    int32_t table_bytes;     /* 4 (int32 table) or 8 (int64 table) */
+    int32_t merge_bits;      /* 32 or 64: width of the packed merge accumulator */
+    int32_t code_bits;       /* low bits of a relaxation cell holding the record */
+    int64_t n;
+    int64_t dp_cells;        /* 2^k * n */
+    /* timings, milliseconds, from HIP events on the solve stream */
+    double zero_ms;          /* table allocation + initial fill */
+    double singleton_ms;     /* k single-source shortest paths */
+    double merge_ms;         /* sum over layers of the subset-merge kernels */
+    double relax_ms;         /* sum over layers of the relaxation (all rounds + unpack) */
+    double kernel_ms;        /* singleton + merge + relax, as the reference's [kernel:] */
+    double traceback_ms;
+    double total_ms;
+    /* algorithmic traffic models (bytes) */
+    double merge_alg_bytes;  /* cb*n*(3^k-2^(k+1)+1) + (cb+ab)*n*(2^k-k-1) */
+    double relax_alg_bytes;  /* (2^k-1)*(cb*n + cb*2m + (cb+ab)*n) + k*(4(n+1)+16m) */
+    int64_t merge_launches;
+    int64_t relax_rounds;    /* relaxation rounds launched, all layers */
+    int64_t device_bytes;    /* device memory held during the solve */
+    int64_t npairs;          /* traceback: number of (v,u) pairs */
+    int64_t *pairs;          /* 2*npairs, 0-based, reference DFS order (reader-el.c:1907-1944) */
+} emv_result;
+
+/* Context on HIP device `device`. */
+int emv_open(emv_ctx **ctx, int device);
+void emv_close(emv_ctx *ctx);
+const char *emv_last_error(const emv_ctx *ctx);
+
+/* Graph in the reference's steinerq_t layout (reader-el.c:696-704, built by
+ * root_build 706-899): for u < n, adj[pos[u]] = degree, followed by degree
+ * (neighbour, weight) pairs.  Rows u >= n (virtual sources) are ignored.
+ * Replaces the pos/adj hand-over at reader-el.c:2269. */
+int emv_load_graph(emv_ctx *ctx, int64_t n, int64_t m, const int64_t *pos,
+                   const int64_t *adj);
+
+/* Graph as the reference's edge list graph_t.edges (reader-el.c:358-373,
+ * graph_add_edge 421-438): 3*m int64 (u, v, w), 0-based. */
+int emv_load_edges(emv_ctx *ctx, int64_t n, int64_t m, const int64_t *edges);
+
+/* Full EMV DP for terminals kk[0..k-1] on the loaded graph: replaces
+ * emv_kernel + build_tree (reader-el.c:2212-2310).  1 <= k <= 30. */
+int emv_solve(emv_ctx *ctx, int64_t k, const int64_t *kk, uint32_t flags,
+              emv_result *res);
+
+/* After a solve with EMV_KEEP_TABLE: rows X0 .. X0+nrows-1 of the table as
+ * int64 (INF = INT64_MAX, reader-el.c:69) and of the record codes (-1 unset,
+ * Y for a merge record, 2^k + u for an edge record).  Either output may be
+ * NULL.  Each output holds nrows*n values. */
+int emv_read_rows(emv_ctx *ctx, int64_t X0, int64_t nrows, int64_t *f_out,
+                  int64_t *code_out);
+
+/* Releases the table kept by EMV_KEEP_TABLE. */
+int emv_release_table(emv_ctx *ctx);
+
+/* Single-source shortest paths on the loaded graph (reader-el.c:1762-1880;
+ * the -dijkstra command 2452-2478 and the k == 2 branch 2159-2209).
+ * d_out[v] = distance (INT64_MAX if unreachable); pred_out[v] = smallest
+ * neighbour u with d[u] + w(u,v) == d[v], -1 for s.  Either may be NULL. */
+int emv_sssp(emv_ctx *ctx, int64_t s, int64_t *d_out, int64_t *pred_out,
+             double *ms);
+
+void emv_result_free(emv_result *res);
+
+/* Device memory the solve of a k-terminal query on the loaded graph needs,
+ * in bytes (tables + relaxation staging). */
+int64_t emv_solve_footprint(const emv_ctx *ctx, int64_t k, uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMV_HIP_H */

@@ -1,0 +1,166 @@
+"""CPU-side checks of the product: the C ABI library loads and exports what
+include/emv_hip.h declares, its argument checks return status codes without
+a device, and the C host (emv-reader) keeps the reference's input and
+root-build lines (verify.py / report.py regexes) and fails loudly when no
+GPU is present."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import emv
+from golden_data import load_instance
+from stputil import write_stp
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(REPO, "steiner-edge-linear_amd")
+HEADERS = [os.path.join(REPO, "include", h) for h in os.listdir(os.path.join(REPO, "include"))
+           if h.endswith(".h")]
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(emv\w*)\s*\(", text, flags=re.M):
+            names.append(m.group(1))
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["emv_open", "emv_close", "emv_load_graph", "emv_load_edges", "emv_solve",
+                 "emv_read_rows", "emv_sssp", "emv_result_free", "emv_last_error"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(emv.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_result_struct_layout_matches_header():
+    # the ctypes mirror and the C struct agree on size (offsets follow)
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "emv_hip.h"
+int main(void){ printf("%zu %zu %zu\n", sizeof(emv_result), offsetof(emv_result, pairs),
+                       offsetof(emv_result, merge_alg_bytes)); return 0; }
+"""
+    tmp = os.path.join("/tmp", f"emv_layout_{os.getpid()}")
+    with open(tmp + ".c", "w") as fh:
+        fh.write(src)
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", tmp, tmp + ".c"], check=True)
+    out = subprocess.run([tmp], capture_output=True, text=True, check=True).stdout.split()
+    os.remove(tmp)
+    os.remove(tmp + ".c")
+    assert int(out[0]) == ctypes.sizeof(emv.EmvResult)
+    assert int(out[1]) == emv.EmvResult.pairs.offset
+    assert int(out[2]) == emv.EmvResult.merge_alg_bytes.offset
+
+
+def test_status_codes_without_device():
+    lib = emv.lib()
+    h = ctypes.c_void_p()
+    st = lib.emv_open(ctypes.byref(h), 0)
+    if st == 0:
+        lib.emv_close(h)
+        pytest.skip("a HIP device is present")
+    assert st in (-7, -2)
+    assert lib.emv_load_edges(None, 4, 0, None) == -1
+    assert lib.emv_solve(None, 2, None, 0, None) == -1
+    assert lib.emv_solve_footprint(None, 3, 0) == -1
+
+
+INPUT_RE = re.compile(r"(.*)n = (.*), m = (.*), k = (.*), cost = (.*) \[(.*)ms\] {peak:(.*)GiB} "
+                      r"{curr:(.*)GiB}", re.M | re.I)
+ROOT_RE = re.compile(r"root build:(.*)\[zero:(.*)ms\] \[pos:(.*)ms\] \[adj:(.*)ms\] \[term:(.*)ms\] "
+                     r"done. \[(.*)ms\] {peak:(.*)GiB} {curr:(.*)GiB}")
+BUILD_RE = re.compile(r"build:(.*), (.*)")
+
+
+def run_reader(args, **kw):
+    return subprocess.run([os.path.join(PKG, "emv-reader")] + args, capture_output=True, text=True,
+                          timeout=60, **kw)
+
+
+def test_reader_lines_without_command(tmp_path):
+    inst = load_instance("small-instances/B/b01")
+    p = str(tmp_path / "b01.stp")
+    write_stp(p, inst)
+    out = run_reader(["-in", p])
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.splitlines()
+    inp = [x for x in lines if x.startswith("input")][0]
+    g = INPUT_RE.search(inp)
+    assert [g.group(i).strip() for i in (2, 3, 4)] == ["50", "63", "9"]
+    assert ROOT_RE.search([x for x in lines if x.startswith("root build")][0])
+    assert [x for x in lines if x.startswith("terminals")][0] == \
+        "terminals: 48 49 22 35 27 12 37 34 24"
+    assert BUILD_RE.search([x for x in lines if x.startswith("build")][0])
+    for prefix in ["invoked as:", "random seed =", "command:", "command done", "grand total",
+                   "host:", "list solution:", "num threads:", "compiler:"]:
+        assert any(x.startswith(prefix) for x in lines), prefix
+
+
+def test_reader_stdin_and_seed(tmp_path):
+    inst = load_instance("small-instances/SP/se03")
+    p = str(tmp_path / "s.stp")
+    write_stp(p, inst)
+    with open(p) as fh:
+        out = run_reader(["-seed", "5"], stdin=fh)
+    assert out.returncode == 0
+    assert "no input file specified, defaulting to stdin" in out.stdout
+    assert "random seed = 5" in out.stdout
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("section graph\nsection graph\n", "nested sections"),
+    ("section bogus\nend\n", "invalid section"),
+    ("end\n", "no section to end"),
+    ("section graph\nnodes 3\nedges 1\ne 1 9 2\nend\n", "outside"),
+    ("section graph\nnodes 3\nedges 2\ne 1 2 2\nend\nsection terminals\nterminals 1\nt 1\nend\n",
+     "edge count"),
+])
+def test_reader_malformed_input_errors(tmp_path, text, msg):
+    p = str(tmp_path / "bad.stp")
+    with open(p, "w") as fh:
+        fh.write(text)
+    out = run_reader(["-in", p, "-el"])
+    assert out.returncode != 0
+    assert "ERROR" in out.stderr and msg in out.stderr
+
+
+def test_reader_fails_loudly_without_gpu(tmp_path):
+    lib = emv.lib()
+    h = ctypes.c_void_p()
+    if lib.emv_open(ctypes.byref(h), 0) == 0:
+        lib.emv_close(h)
+        pytest.skip("a HIP device is present")
+    inst = load_instance("small-instances/B/b01")
+    p = str(tmp_path / "b01.stp")
+    write_stp(p, inst)
+    out = run_reader(["-in", p, "-el"])
+    assert out.returncode != 0 and "ERROR" in out.stderr and "emv_open" in out.stderr
+
+
+def test_missing_file_and_args():
+    out = run_reader(["-in"])
+    assert out.returncode != 0 and "file name missing" in out.stderr
+    out = run_reader(["-in", "/nonexistent.stp"])
+    assert out.returncode != 0 and "unable to open" in out.stderr
+    out = run_reader(["-h"])
+    assert out.returncode == 0 and "usage" in out.stdout
+
+
+def test_python_reduction_matches_host_reduction_rule():
+    e = np.array([[0, 1, 5], [1, 2, 7]])
+    n2, red, kk, M = emv.group_reduce(3, e, [[0], [2]])
+    assert M == 13 and n2 == 5 and list(kk) == [3, 4]
+    assert red.tolist()[-2:] == [[3, 0, 13], [4, 2, 13]]

@@ -1,0 +1,279 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and the pinned CPU oracle.  Integer work, so everything is bit-exact:
+every DP cell, the cost, and the record codes (the GPU and the oracle share
+the deterministic tie rule; the reference's own record kinds are matched via
+digests).  Listed trees are validated, not compared, since the reference's
+edge-record choice depends on its heap order."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import emv
+import oracle
+from golden_data import CFG1, META, NAMES, digest, golden_table, kinds, load_instance
+from stputil import INF, Instance, adjacency, check_listed_tree, dijkstra, write_stp
+
+pytestmark = pytest.mark.gpu
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+READER = os.path.join(REPO, "steiner-edge-linear_amd", "emv-reader")
+
+
+def solve_full(ctx, inst, flags=0):
+    ctx.load_edges(inst.n, inst.edges)
+    sol = ctx.solve(inst.terminals, emv.LIST | emv.KEEP_TABLE | flags)
+    f, code = ctx.read_rows(0, 1 << inst.k)
+    ctx.release_table()
+    return sol, f, code
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_testset_table_bit_exact(gpu_ctx, name):
+    inst = load_instance(name)
+    mt = META[name]
+    print(f"case {name} n={inst.n} k={inst.k}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    assert sol.cost == mt["ref_cost"] == inst.cost
+    assert digest(f) == mt["f_sha256"]
+    assert digest(kinds(code, inst.k)) == mt["kind_sha256"]
+    ocost, of, ocode = oracle.emv(inst.n, inst.edges, inst.terminals)
+    np.testing.assert_array_equal(code, ocode)
+    C = (1 << (inst.k - 1)) - 1
+    opairs = oracle.traceback(inst.n, inst.k, ocode, int(inst.terminals[-1]), C)
+    np.testing.assert_array_equal(sol.pairs, opairs)
+    ok, why = check_listed_tree(inst, sol.pairs, sol.cost)
+    assert ok, why
+
+
+def test_regular_1000_full_table(gpu_ctx):
+    inst = load_instance(CFG1)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    gf, gcode = golden_table(CFG1)
+    np.testing.assert_array_equal(f, gf)
+    np.testing.assert_array_equal(kinds(code, inst.k), kinds(gcode, inst.k))
+    assert sol.cost == META[CFG1]["ref_cost"]
+    assert sol.table_bytes == 4 and sol.merge_bits == 32
+
+
+@pytest.mark.parametrize("flags", [emv.FORCE_P64, emv.FORCE_I64, emv.FORCE_I64 | emv.FORCE_P64])
+@pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
+def test_wide_paths_bit_exact(gpu_ctx, name, flags):
+    inst = load_instance(name)
+    print(f"case {name} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst, flags)
+    assert digest(f) == META[name]["f_sha256"]
+    assert digest(kinds(code, inst.k)) == META[name]["kind_sha256"]
+    assert sol.merge_bits == 64
+    if flags & emv.FORCE_I64:
+        assert sol.table_bytes == 8
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_regular_vs_oracle(gpu_ctx, seed):
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(2000, 6000)) // 2 * 2
+    d = int(rng.integers(3, 9))
+    k = int(rng.integers(5, 10))
+    e = emv.regular_graph(n, d, 1000, 500 + seed)
+    kk = emv.terminals(n, k, 500 + seed)
+    inst = Instance(n, e, kk)
+    print(f"case regular n={n} d={d} k={k} seed={seed}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+    assert sol.cost == ocost
+
+
+def test_grid_vs_oracle(gpu_ctx):
+    e = emv.grid_graph(60, 70, 100, 9)
+    n = 60 * 70
+    kk = emv.terminals(n, 8, 9)
+    inst = Instance(n, e, kk)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
+def test_edge_cases_k1_k2_duplicates(gpu_ctx):
+    inst = load_instance("small-instances/B/b01")
+    gpu_ctx.load_edges(inst.n, inst.edges)
+    # k = 1: the answer row C = 0 is never written (reader-el.c:2137)
+    sol = gpu_ctx.solve(inst.terminals[:1], emv.LIST)
+    assert sol.cost == INF and len(sol.pairs) == 0
+    # k = 2 through the DP equals the shortest path
+    sol = gpu_ctx.solve(inst.terminals[:2], emv.LIST)
+    d = dijkstra(adjacency(inst.n, inst.edges), int(inst.terminals[0]))
+    assert sol.cost == d[int(inst.terminals[1])]
+    # a repeated terminal changes nothing but the table shape
+    kk = np.array([inst.terminals[0], inst.terminals[1], inst.terminals[0], inst.terminals[2]])
+    sol = gpu_ctx.solve(kk, emv.KEEP_TABLE)
+    f, _ = gpu_ctx.read_rows(0, 16)
+    gpu_ctx.release_table()
+    ocost, of, _ = oracle.emv(inst.n, inst.edges, kk)
+    np.testing.assert_array_equal(f, of)
+    assert sol.cost == ocost
+
+
+def test_errors_are_status_codes(gpu_ctx):
+    inst = load_instance("small-instances/B/b01")
+    gpu_ctx.load_edges(inst.n, inst.edges)
+    with pytest.raises(emv.EmvError) as ei:
+        gpu_ctx.solve(np.array([0, inst.n + 5]))
+    assert ei.value.status == -1
+    with pytest.raises(emv.EmvError) as ei:
+        gpu_ctx.read_rows(0, 1)
+    assert ei.value.status == -4
+    # disconnected graph: two components
+    e = np.array([[0, 1, 3], [2, 3, 4]])
+    gpu_ctx.load_edges(4, e)
+    with pytest.raises(emv.EmvError) as ei:
+        gpu_ctx.solve(np.array([0, 2, 1]))
+    assert ei.value.status == -5
+
+
+def test_sssp_vs_dijkstra(gpu_ctx):
+    inst = load_instance(CFG1)
+    gpu_ctx.load_edges(inst.n, inst.edges)
+    d, p, _ = gpu_ctx.sssp(17)
+    od, op = oracle.sssp(inst.n, inst.edges, 17)
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(p, op)
+
+
+def test_load_graph_reference_layout(gpu_ctx):
+    # the reference's steinerq pos/adj layout (reader-el.c:696-859) gives the
+    # same solve as the edge list
+    inst = load_instance("small-instances/C/c01")
+    n = inst.n
+    deg = np.zeros(n, dtype=np.int64)
+    for u, v, _ in inst.edges:
+        deg[u] += 1
+        deg[v] += 1
+    pos = np.zeros(n + 1, dtype=np.int64)
+    pos[1:] = np.cumsum(1 + 2 * deg)
+    adj = np.zeros(pos[-1], dtype=np.int64)
+    fill = pos[:-1].copy()
+    for u, v, w in inst.edges:
+        for a, b in ((u, v), (v, u)):
+            p = pos[a]
+            c = adj[p]
+            adj[p + 1 + 2 * c] = b
+            adj[p + 2 + 2 * c] = w
+            adj[p] += 1
+    gpu_ctx.load_graph(n, inst.m, pos, adj)
+    sol = gpu_ctx.solve(inst.terminals)
+    assert sol.cost == inst.cost
+
+
+VERIFY_INPUT = re.compile(r"(.*)n = (.*), m = (.*), k = (.*), cost = (.*) \[(.*)ms\] {peak:(.*)GiB} "
+                          r"{curr:(.*)GiB}")
+VERIFY_EL_LIST = re.compile(r"erickson: \[zero:(.*)ms\] \[kernel:(.*)ms (.*)GiB/s\] \[traceback: "
+                            r"(.*)ms\] done. \[(.*)ms\] \[cost:(.*)] {peak:(.*)GiB} {curr:(.*)GiB}")
+VERIFY_EL = re.compile(r"erickson: \[zero:(.*)ms\] \[kernel:(.*)ms (.*)GiB/s\] done. \[(.*)ms\] "
+                       r"\[cost:(.*)] {peak:(.*)GiB} {curr:(.*)GiB}")
+
+
+def run_reader(args):
+    out = subprocess.run([READER] + args, capture_output=True, text=True, timeout=120)
+    return out.returncode, out.stdout, out.stderr
+
+
+@pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/I080/i080-001",
+                                  "small-instances/SP/se03", CFG1])
+def test_cli_drop_in_lines(name):
+    """The reference harness's regexes (verify.py:120-169) parse our lines."""
+    inst = load_instance(name)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "x.stp")
+        write_stp(path, inst)
+        if inst.cost >= 0:
+            with open(path, "a") as fh:
+                fh.write(f"cost {inst.cost}\n")
+        rc, out, err = run_reader(["-in", path, "-el", "-list"])
+        assert rc == 0, err
+        lines = out.splitlines()
+        el = [x for x in lines if x.startswith("erickson")][0]
+        toks = VERIFY_EL_LIST.search(el)
+        assert toks is not None, el
+        cost = int(toks.group(6).strip())
+        assert cost == META[name]["ref_cost"]
+        sol = [x for x in lines if x.startswith("solution")][0]
+        pairs = [tuple(int(a) - 1 for a in p.split()) for p in re.findall(r'"([^"]*)"', sol)]
+        ok, why = check_listed_tree(inst, pairs, cost)
+        assert ok, why
+        assert VERIFY_INPUT.search([x for x in lines if x.startswith("input")][0])
+        assert any(x.startswith("gpu:") for x in lines)
+        rc, out, err = run_reader(["-in", path, "-el"])
+        assert rc == 0
+        el = [x for x in out.splitlines() if x.startswith("erickson")][0]
+        assert VERIFY_EL.search(el) is not None
+
+
+def test_cli_cost_mismatch_aborts():
+    inst = load_instance("small-instances/B/b01")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "x.stp")
+        write_stp(path, inst)
+        with open(path, "a") as fh:
+            fh.write("cost 81\n")
+        rc, out, err = run_reader(["-in", path, "-el"])
+        assert rc != 0 and "ERROR" in err and "min_cost != cost" in err
+
+
+def test_cli_k2_path_and_dijkstra():
+    inst = load_instance("path/p02")
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "x.stp")
+        write_stp(path, inst)
+        rc, out, err = run_reader(["-in", path, "-el", "-list"])
+        assert rc == 0, err
+        el = [x for x in out.splitlines() if x.startswith("erickson")][0]
+        assert "[zero:" not in el and "[cost: 20]" in el
+        sol = [x for x in out.splitlines() if x.startswith("solution")][0]
+        pairs = [tuple(int(a) - 1 for a in p.split()) for p in re.findall(r'"([^"]*)"', sol)]
+        adj = adjacency(inst.n, inst.edges)
+        # a real path from kk[1] to kk[0] of total weight 20
+        assert pairs[0][0] == inst.terminals[1] and pairs[-1][1] == inst.terminals[0]
+        tot = 0
+        for a, b in pairs:
+            tot += min(w for x, w in adj[a] if x == b)
+        assert tot == 20
+        rc, out, err = run_reader(["-in", path, "-dijkstra", "-seed", "7"])
+        assert rc == 0, err
+
+
+def test_cli_group_reduction(tmp_path):
+    """Group Steiner by reduction: cost = OPT(reduced) - g*M and the printed
+    group tree is a real tree of that cost touching every group."""
+    R, C = 9, 11
+    e = emv.grid_graph(R, C, 50, 21)
+    n = R * C
+    grp = emv.groups(n, 4, 5, 21)
+    inst = Instance(n, e, emv.terminals(n, 2, 21))
+    path = str(tmp_path / "g.stp")
+    write_stp(path, inst)
+    gpath = str(tmp_path / "g.grp")
+    with open(gpath, "w") as fh:
+        for row in grp:
+            fh.write("g " + " ".join(str(int(v) + 1) for v in row) + "\n")
+    rc, out, err = run_reader(["-in", path, "-el", "-list", "-groups", gpath])
+    assert rc == 0, err
+    line = [x for x in out.splitlines() if x.startswith("group: [groups")][0]
+    gcost = int(re.search(r"\[cost: (-?\d+)\]", line).group(1))
+    nn, red, kk, M = emv.group_reduce(n, e, grp)
+    ocost, _, _ = oracle.emv(nn, red, kk)
+    assert gcost == ocost - len(grp) * M
+    sol = [x for x in out.splitlines() if x.startswith("group solution")][0]
+    pairs = [tuple(int(a) - 1 for a in p.split()) for p in re.findall(r'"([^"]*)"', sol)]
+    adj = adjacency(n, e)
+    tot = sum(min(w for x, w in adj[a] if x == b) for a, b in pairs)
+    assert tot == gcost
+    verts = {v for p in pairs for v in p}
+    for row in grp:
+        assert verts & set(int(v) for v in row)
