@@ -1,14 +1,16 @@
 // emv_hip.hip -- host orchestration and C ABI of the gfx950 EMV Steiner DP.
 //
 // Solve sequence (reader-el.c:2141-2343 is the reference driver):
-//   1. k single-source shortest paths, batched in one staging of k rows
-//      (reader-el.c:1988-2031), push rounds from the k sources.
+//   1. k single-source shortest paths in one staging chunk (lane t = source
+//      kk[t], reader-el.c:1988-2031): one dense pull round, then flagged
+//      rounds until no label drops.
 //   2. Bound B = sum_t max_v dist(kk[t], v) picks the table type (int32 when
-//      B < 2^31-1) and the merge accumulator (32-bit when (2B << k) fits).
+//      B < 2^31-1), the merge accumulator (32-bit when (2B << k) fits) and
+//      the staging label width (32-bit when 2B + max w fits).
 //   3. For each layer m = 2..k (reader-el.c:2033-2134), in batches of rows:
-//        merge  -> staging (d << CB | Y)
-//        relax  -> one pull sweep, then push rounds until no label changes
-//        unpack -> f rows, record rows
+//        merge  -> vertex-major staging (label, Y)
+//        relax  -> dense pull round, then flagged rounds until no label drops
+//        unpack -> table rows f[X], b[X]
 //   4. Answer f[C][q]; optional device traceback.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
@@ -19,6 +21,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <new>
 #include <vector>
 
 #include "emv_hip.h"
@@ -37,27 +40,25 @@ struct emv_ctx {
     int32_t *d_rp = nullptr;
     Edge *d_adj = nullptr;
     int64_t maxw = 0;
-    // table kept after a solve (EMV_KEEP_TABLE)
+    // table; the allocation is reused across solves
     void *d_f = nullptr;
     int32_t *d_b = nullptr;
-    size_t f_cap = 0, b_cap = 0;  // bytes held by d_f / d_b (reused across solves)
+    size_t f_cap = 0, b_cap = 0;
     int64_t np = 0;
-    int tbytes = 0;
+    int tbytes = 0;  // nonzero while a kept table is readable
     int64_t tk = 0;
-    // relaxation workspace
-    uint64_t *d_S = nullptr;
-    uint32_t *d_q[2] = {nullptr, nullptr};
-    uint32_t *d_stamp = nullptr;
-    int64_t ws_cells = 0;
-    uint32_t *d_cnt = nullptr;              // 3 rotating queue counters
+    // relaxation workspace, sized in chunks of 64 rows
+    void *d_D = nullptr;          // labels, 8 bytes per cell reserved
+    uint32_t *d_code = nullptr;   // record codes
+    uint8_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
+    int64_t ws_chunks = 0;
+    uint32_t *d_any = nullptr;               // 4 rotating "round has work" words
     unsigned long long *d_rowmax = nullptr;  // per singleton row
-    int64_t *d_misc = nullptr;              // traceback result etc.
-    uint32_t *d_xs = nullptr;               // row subset ids of a batch
+    int64_t *d_misc = nullptr;               // traceback result
+    uint32_t *d_xs = nullptr;                // subset ids of a batch
     int64_t xs_cap = 0;
-    int32_t *d_kk = nullptr;
-    uint32_t *h_pin = nullptr;              // pinned counter readback
-    uint32_t stamp_next = 1;
-    int64_t rounds = 0;
+    int32_t *d_src = nullptr;                // shortest-path sources
+    uint32_t *h_pin = nullptr;               // pinned readback of a work word
 };
 
 namespace {
@@ -82,9 +83,9 @@ void set_err(emv_ctx *c, const char *fmt, ...) {
         }                                                                                   \
     } while (0)
 
-#define CHK(expr)                 \
-    do {                          \
-        int _r = (expr);          \
+#define CHK(expr)                    \
+    do {                             \
+        int _r = (expr);             \
         if (_r != EMV_OK) return _r; \
     } while (0)
 
@@ -109,13 +110,12 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
+// Grid of `items` workgroups rounded up to a multiple of the XCD count.
 int grid_for(int64_t items, int nxcd, int64_t *per) {
     *per = (items + nxcd - 1) / nxcd;
-    int64_t g = *per * nxcd;
-    return (int)g;
+    return (int)(*per * nxcd);
 }
 
-// Drops the table memory; invalidate_table() only forgets its contents.
 int free_table(emv_ctx *c) {
     dfree(c->d_f);
     dfree(c->d_b);
@@ -146,60 +146,88 @@ int ensure_table(emv_ctx *c, size_t fbytes, size_t bbytes) {
     return EMV_OK;
 }
 
-int ensure_ws(emv_ctx *c, int64_t cells) {
-    if (cells <= c->ws_cells) return EMV_OK;
-    dfree(c->d_S);
-    dfree(c->d_q[0]);
-    dfree(c->d_q[1]);
-    dfree(c->d_stamp);
-    c->ws_cells = 0;
-    HIPCHK(c, hipMalloc((void **)&c->d_S, sizeof(uint64_t) * cells));
-    HIPCHK(c, hipMalloc((void **)&c->d_q[0], sizeof(uint32_t) * cells));
-    HIPCHK(c, hipMalloc((void **)&c->d_q[1], sizeof(uint32_t) * cells));
-    HIPCHK(c, hipMalloc((void **)&c->d_stamp, sizeof(uint32_t) * cells));
-    int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_stamp,
-                       cells, 0u);
+int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
+
+void free_ws(emv_ctx *c) {
+    dfree(c->d_D);
+    dfree(c->d_code);
+    dfree(c->d_flag[0]);
+    dfree(c->d_flag[1]);
+    c->ws_chunks = 0;
+}
+
+// Clears both activity-flag arrays (a completed relaxation leaves them clear;
+// this also covers one that stopped on an error).
+int clear_flags(emv_ctx *c) {
+    const int64_t items = c->ws_chunks * c->np;
+    int blocks = (int)std::min<int64_t>((items + kBlock - 1) / kBlock, 65536);
+    for (int i = 0; i < 2; i++)
+        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
+                           c->d_flag[i], items, (uint8_t)0);
     HIPCHK(c, hipGetLastError());
-    c->stamp_next = 1;
-    c->ws_cells = cells;
     return EMV_OK;
+}
+
+int ensure_ws(emv_ctx *c, int64_t chunks) {
+    if (chunks <= c->ws_chunks) return EMV_OK;
+    free_ws(c);
+    const int64_t cells = chunks * chunk_cells(c);
+    const int64_t items = chunks * c->np;
+    HIPCHK(c, hipMalloc(&c->d_D, sizeof(uint64_t) * cells));
+    HIPCHK(c, hipMalloc((void **)&c->d_code, sizeof(uint32_t) * cells));
+    HIPCHK(c, hipMalloc((void **)&c->d_flag[0], items));
+    HIPCHK(c, hipMalloc((void **)&c->d_flag[1], items));
+    c->ws_chunks = chunks;
+    return clear_flags(c);
 }
 
 int ensure_xs(emv_ctx *c, int64_t count) {
     if (count <= c->xs_cap) return EMV_OK;
     dfree(c->d_xs);
+    c->xs_cap = 0;
     HIPCHK(c, hipMalloc((void **)&c->d_xs, sizeof(uint32_t) * count));
     c->xs_cap = count;
     return EMV_OK;
 }
 
-// Push rounds from the queue q[j&1] (count cnt[j%3]) until a round adds
-// nothing.  Rounds are launched in groups between counter polls.
-int run_push_rounds(emv_ctx *c, int j, int CB, uint64_t pred_base, int VS, uint32_t stamp0,
-                    int64_t *rounds_out) {
+// Relaxation of `nchunks` staged chunks: one dense pull round over every
+// (chunk, vertex), then flagged rounds, launched in groups between polls of
+// the "round has work" word, until a round flags nothing.  Rounds launched
+// after convergence return at once.
+template <typename DT>
+int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *rounds_out) {
     const int grid = c->num_cu * 8;
-    const int group = 4;
-    int64_t rounds = 0;
+    DT *D = (DT *)c->d_D;
+    const int64_t nitems = nchunks * c->n;
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_any, (int64_t)4,
+                       0u);
+    int r = 0;
+    hipLaunchKernelGGL((k_relax_v<DT>), dim3(grid), dim3(kBlock), 0, c->stream, D, c->d_code,
+                       c->np, (int)c->n, c->d_rp, c->d_adj, pred_base, nitems, 1,
+                       c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r, c->nxcd);
+    HIPCHK(c, hipGetLastError());
+    r = 1;
+    int64_t rounds = 1;
+    int group = 2;
     for (;;) {
-        for (int g = 0; g < group; g++, j++) {
-            hipLaunchKernelGGL(k_push, dim3(grid), dim3(kBlock), 0, c->stream, c->d_S, c->np,
-                               c->d_rp, c->d_adj, CB, pred_base, c->d_q[j & 1],
-                               c->d_q[(j + 1) & 1], c->d_cnt, j, c->d_stamp,
-                               stamp0 + (uint32_t)j + 1u, VS);
+        for (int g = 0; g < group; g++, r++) {
+            hipLaunchKernelGGL((k_relax_v<DT>), dim3(grid), dim3(kBlock), 0, c->stream, D,
+                               c->d_code, c->np, (int)c->n, c->d_rp, c->d_adj, pred_base, nitems,
+                               0, c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r,
+                               c->nxcd);
             rounds++;
         }
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(c->h_pin, c->d_cnt + (j % 3), sizeof(uint32_t),
+        HIPCHK(c, hipMemcpyAsync(c->h_pin, c->d_any + (r & 3), sizeof(uint32_t),
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (*c->h_pin == 0) break;
-        if (rounds > (int64_t)4 * (c->n + 8)) {
-            set_err(c, "relaxation did not converge (negative or zero-weight cycle?)");
+        group = std::min(group * 2, 8);
+        if (rounds > (int64_t)2 * (c->n + 8)) {
+            set_err(c, "relaxation did not converge");
             return EMV_ERR_RANGE;
         }
     }
-    c->stamp_next = stamp0 + (uint32_t)j + 2u;
     *rounds_out += rounds;
     return EMV_OK;
 }
@@ -207,28 +235,23 @@ int run_push_rounds(emv_ctx *c, int j, int CB, uint64_t pred_base, int VS, uint3
 struct SolveState {
     int k;
     int64_t S2;  // 2^k
-    int CB;
-    uint64_t pred_base;
-    int VS;
+    uint32_t pred_base;
     int tbytes;
     int pbits;
+    int dbits;
 };
 
-template <typename T>
+template <typename T, typename P, typename DT>
 int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *res,
                std::vector<hipEvent_t> &ev) {
     const int k = st.k;
     const int64_t np = c->np;
     const int64_t n = c->n;
-    const int64_t max_rows = std::max<int64_t>(1, c->ws_cells / np);
-    const int64_t row_cap_vs = (int64_t)1 << (32 - st.VS);
-    const int64_t rows_per_batch =
-        std::min<int64_t>(std::min<int64_t>(max_rows, row_cap_vs), c->xs_cap);
+    const int64_t rows_per_batch = std::min<int64_t>(c->ws_chunks * kWave, c->xs_cap);
     std::vector<uint32_t> xs;
     for (int m = 2; m <= k; m++) {
         xs.clear();
-        // Gosper order: the m-subsets in increasing numeric order
-        // (reader-el.c:2040-2046).
+        // the m-subsets in increasing numeric order (reader-el.c:2040-2046)
         for (uint64_t X = (1ull << m) - 1; X < (uint64_t)st.S2;) {
             xs.push_back((uint32_t)X);
             uint64_t lo = X & (~X + 1);
@@ -236,46 +259,37 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             X = (((r ^ X) >> 2) / lo) | r;
         }
         for (size_t off = 0; off < xs.size(); off += (size_t)rows_per_batch) {
-            int nx = (int)std::min<int64_t>(rows_per_batch, (int64_t)(xs.size() - off));
+            const int nx = (int)std::min<int64_t>(rows_per_batch, (int64_t)(xs.size() - off));
+            const int64_t nchunks = (nx + kWave - 1) / kWave;
             HIPCHK(c, hipMemcpyAsync(c->d_xs, xs.data() + off, sizeof(uint32_t) * nx,
                                      hipMemcpyHostToDevice, c->stream));
             hipEvent_t e0, e1, e2;
             HIPCHK(c, hipEventCreate(&e0));
-            HIPCHK(c, hipEventCreate(&e1));
-            HIPCHK(c, hipEventCreate(&e2));
             ev.push_back(e0);
+            HIPCHK(c, hipEventCreate(&e1));
             ev.push_back(e1);
+            HIPCHK(c, hipEventCreate(&e2));
             ev.push_back(e2);
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            // merge
-            constexpr int VEC = VecOf<T>::N;
-            int64_t ntiles = np / (kBlock * VEC);
+            // merge -> staging
+            const int nsub = (int)(nchunks * (kWave / kXG));
+            const int64_t ntiles = np / kWave;
             int64_t per;
-            int grid = grid_for(ntiles * nx, c->nxcd, &per);
-            if (st.pbits == 32 && sizeof(T) == 4)
-                hipLaunchKernelGGL((k_merge<T, uint32_t>), dim3(grid), dim3(kBlock), 0, c->stream,
-                                   f, np, c->d_xs, nx, ntiles, k, st.CB, c->d_S, c->nxcd, per);
-            else
-                hipLaunchKernelGGL((k_merge<T, uint64_t>), dim3(grid), dim3(kBlock), 0, c->stream,
-                                   f, np, c->d_xs, nx, ntiles, k, st.CB, c->d_S, c->nxcd, per);
+            int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
+            hipLaunchKernelGGL((k_merge_v<T, P, DT>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+                               np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, c->d_code, c->nxcd,
+                               per);
             HIPCHK(c, hipGetLastError());
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
-            // relax: pull sweep, then push rounds
-            int64_t tpr = (n + kBlock - 1) / kBlock;
-            grid = grid_for(tpr * nx, c->nxcd, &per);
-            uint32_t stamp0 = c->stamp_next;
-            hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_cnt,
-                               (int64_t)3, 0u);
-            hipLaunchKernelGGL(k_pull, dim3(grid), dim3(kBlock), 0, c->stream, c->d_S, np, nx,
-                               (int)n, c->d_rp, c->d_adj, st.CB, st.pred_base, c->d_q[1],
-                               c->d_cnt, 1, c->d_stamp, stamp0 + 1u, st.VS, c->nxcd, per, tpr);
-            HIPCHK(c, hipGetLastError());
-            res->relax_rounds++;
-            CHK(run_push_rounds(c, 1, st.CB, st.pred_base, st.VS, stamp0, &res->relax_rounds));
+            // relax
+            CHK(relax_chunks<DT>(c, nchunks, st.pred_base, &res->relax_rounds));
             // unpack
-            hipLaunchKernelGGL((k_unpack<T>), dim3(grid), dim3(kBlock), 0, c->stream, c->d_S, np,
-                               c->d_xs, nx, (int)n, st.CB, f, b, c->nxcd, per, tpr);
+            const int64_t vtiles = (n + kWave - 1) / kWave;
+            grid = grid_for(nchunks * vtiles, c->nxcd, &per);
+            hipLaunchKernelGGL((k_unpack_v<T, DT>), dim3(grid), dim3(kBlock), 0, c->stream,
+                               (const DT *)c->d_D, c->d_code, np, c->d_xs, nx, (int)n, f, b,
+                               c->nxcd, per, vtiles);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e2, c->stream));
         }
@@ -283,14 +297,31 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
     return EMV_OK;
 }
 
+template <typename T>
+int run_layers_t(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *res,
+                 std::vector<hipEvent_t> &ev) {
+    if (st.pbits == 32) {
+        if (st.dbits == 32) return run_layers<T, uint32_t, uint32_t>(c, st, f, b, res, ev);
+        return run_layers<T, uint32_t, uint64_t>(c, st, f, b, res, ev);
+    }
+    if (st.dbits == 32) return run_layers<T, uint64_t, uint32_t>(c, st, f, b, res, ev);
+    return run_layers<T, uint64_t, uint64_t>(c, st, f, b, res, ev);
+}
+
 int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
              const std::vector<Edge> &adj, int64_t maxw) {
-    free_table(c);
+    HIPCHK(c, hipSetDevice(c->device));
+    invalidate_table(c);
+    // the table and the workspace are sized by np: drop them when it changes
+    const int64_t np = round_up(n, 1024);
+    if (np != c->np) {
+        free_table(c);
+        free_ws(c);
+    }
     dfree(c->d_rp);
     dfree(c->d_adj);
     c->n = 0;
     c->m = 0;
-    HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMalloc((void **)&c->d_rp, sizeof(int32_t) * (n + 1)));
     HIPCHK(c, hipMalloc((void **)&c->d_adj, sizeof(Edge) * std::max<size_t>(1, adj.size())));
     HIPCHK(c, hipMemcpy(c->d_rp, rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
@@ -300,11 +331,11 @@ int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
     c->n = n;
     c->m = m;
     c->maxw = maxw;
-    c->np = round_up(n, 1024);
+    c->np = np;
     return EMV_OK;
 }
 
-int check_weights_range(emv_ctx *c, int64_t w) {
+int check_weight(emv_ctx *c, int64_t w) {
     if (w < 1 || w > INT32_MAX) {
         set_err(c, "edge weight %lld outside [1, 2^31-1]", (long long)w);
         return EMV_ERR_RANGE;
@@ -312,25 +343,29 @@ int check_weights_range(emv_ctx *c, int64_t w) {
     return EMV_OK;
 }
 
-// Single-source pass shared by emv_sssp and the singleton phase: row t of
-// the staging starts at src[t] with code (1<<t) or 1.
-int sssp_rows(emv_ctx *c, const std::vector<int32_t> &src, bool bit_codes, int CB,
-              uint64_t pred_base, int VS, int64_t *rounds) {
-    int ns = (int)src.size();
-    int64_t cells = (int64_t)ns * c->np;
-    CHK(ensure_ws(c, cells));
-    int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_S, cells,
-                       ~0ull);
-    dfree(c->d_kk);
-    HIPCHK(c, hipMalloc((void **)&c->d_kk, sizeof(int32_t) * std::max(ns, 1)));
-    HIPCHK(c, hipMemcpyAsync(c->d_kk, src.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice,
+// Shortest paths from up to 64 sources in staging chunk 0 (64-bit labels).
+int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, bool bit_codes, uint32_t pred_base,
+               int64_t *rounds) {
+    const int ns = (int)src.size();
+    if (ns > kWave) {
+        set_err(c, "%d sources exceed %d", ns, kWave);
+        return EMV_ERR_RANGE;
+    }
+    CHK(ensure_ws(c, 1));
+    const int64_t cells = chunk_cells(c);
+    const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
+                       (uint64_t *)c->d_D, cells, ~0ull);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_code,
+                       cells, 0u);
+    dfree(c->d_src);
+    HIPCHK(c, hipMalloc((void **)&c->d_src, sizeof(int32_t) * std::max(ns, 1)));
+    HIPCHK(c, hipMemcpyAsync(c->d_src, src.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice,
                              c->stream));
-    uint32_t stamp0 = c->stamp_next;
-    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, c->stream, c->d_S, c->np, c->d_kk, ns,
-                       c->d_q[0], c->d_cnt, c->d_stamp, stamp0, VS, bit_codes ? 1u : 0u);
+    hipLaunchKernelGGL((k_seed_v<uint64_t>), dim3(1), dim3(kWave), 0, c->stream,
+                       (uint64_t *)c->d_D, c->d_code, c->d_src, ns, bit_codes ? 1 : 0);
     HIPCHK(c, hipGetLastError());
-    return run_push_rounds(c, 0, CB, pred_base, VS, stamp0, rounds);
+    return relax_chunks<uint64_t>(c, 1, pred_base, rounds);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -338,6 +373,13 @@ double ev_ms(hipEvent_t a, hipEvent_t b) {
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
     return (double)ms;
 }
+
+struct EventList {
+    std::vector<hipEvent_t> v;
+    ~EventList() {
+        for (auto e : v) (void)hipEventDestroy(e);
+    }
+};
 
 }  // namespace
 
@@ -364,7 +406,7 @@ extern "C" int emv_open(emv_ctx **out, int device) {
             rc = EMV_ERR_HIP;
             break;
         }
-        if (hipMalloc((void **)&c->d_cnt, sizeof(uint32_t) * 4) != hipSuccess ||
+        if (hipMalloc((void **)&c->d_any, sizeof(uint32_t) * 4) != hipSuccess ||
             hipMalloc((void **)&c->d_rowmax, sizeof(unsigned long long) * 64) != hipSuccess ||
             hipMalloc((void **)&c->d_misc, sizeof(int64_t) * 8) != hipSuccess ||
             hipHostMalloc((void **)&c->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault) !=
@@ -386,17 +428,14 @@ extern "C" void emv_close(emv_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_table(c);
+    free_ws(c);
     dfree(c->d_rp);
     dfree(c->d_adj);
-    dfree(c->d_S);
-    dfree(c->d_q[0]);
-    dfree(c->d_q[1]);
-    dfree(c->d_stamp);
-    dfree(c->d_cnt);
+    dfree(c->d_any);
     dfree(c->d_rowmax);
     dfree(c->d_misc);
     dfree(c->d_xs);
-    dfree(c->d_kk);
+    dfree(c->d_src);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -418,7 +457,7 @@ extern "C" int emv_load_edges(emv_ctx *c, int64_t n, int64_t m, const int64_t *e
             set_err(c, "edge %lld endpoint out of range", (long long)j);
             return EMV_ERR_ARG;
         }
-        CHK(check_weights_range(c, w));
+        CHK(check_weight(c, w));
         maxw = std::max(maxw, w);
         rp[u + 1]++;
         rp[v + 1]++;
@@ -426,7 +465,7 @@ extern "C" int emv_load_edges(emv_ctx *c, int64_t n, int64_t m, const int64_t *e
     for (int64_t u = 0; u < n; u++) rp[u + 1] += rp[u];
     std::vector<Edge> adj((size_t)(2 * m));
     std::vector<int32_t> fill(rp.begin(), rp.end() - 1);
-    // Both directions in edge-list order (reader-el.c:798-826).
+    // both directions in edge-list order (reader-el.c:798-826)
     for (int64_t j = 0; j < m; j++) {
         int32_t u = (int32_t)e[3 * j], v = (int32_t)e[3 * j + 1], w = (int32_t)e[3 * j + 2];
         adj[fill[u]++] = Edge{v, w};
@@ -467,7 +506,7 @@ extern "C" int emv_load_graph(emv_ctx *c, int64_t n, int64_t m, const int64_t *p
                 set_err(c, "neighbour %lld of %lld out of range", (long long)v, (long long)u);
                 return EMV_ERR_ARG;
             }
-            CHK(check_weights_range(c, w));
+            CHK(check_weight(c, w));
             maxw = std::max(maxw, w);
             out[rp[u] + i] = Edge{(int32_t)v, (int32_t)w};
         }
@@ -477,11 +516,11 @@ extern "C" int emv_load_graph(emv_ctx *c, int64_t n, int64_t m, const int64_t *p
 
 extern "C" int64_t emv_solve_footprint(const emv_ctx *c, int64_t k, uint32_t flags) {
     if (!c || c->n <= 0 || k < 1 || k > 30) return -1;
-    int64_t np = round_up(c->n, 1024);
-    int64_t S2 = (int64_t)1 << k;
-    int tb = (flags & EMV_FORCE_I64) ? 8 : 4;
-    int64_t rows = std::max<int64_t>(k, choose(k, k / 2));
-    int64_t ws = rows * np * (8 + 4 + 4 + 4);
+    const int64_t np = round_up(c->n, 1024);
+    const int64_t S2 = (int64_t)1 << k;
+    const int tb = (flags & EMV_FORCE_I64) ? 8 : 4;
+    const int64_t chunks = std::max<int64_t>(1, (choose(k, k / 2) + kWave - 1) / kWave);
+    const int64_t ws = chunks * np * (kWave * (8 + 4) + 12);
     return S2 * np * (tb + 4) + ws;
 }
 
@@ -516,98 +555,69 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     SolveState st;
     st.k = (int)k;
     st.S2 = (int64_t)1 << k;
-    st.pred_base = (uint64_t)st.S2;
-    st.CB = bits_for((uint64_t)st.S2 + (uint64_t)n);
-    int64_t np = c->np;
-    st.VS = bits_for((uint64_t)np - 1);
-    if (st.S2 + n >= INT32_MAX || st.VS > 31) {
+    st.pred_base = (uint32_t)st.S2;
+    const int64_t np = c->np;
+    if (st.S2 + n >= INT32_MAX) {
         set_err(c, "2^k + n = %lld does not fit the int32 record", (long long)(st.S2 + n));
         return EMV_ERR_RANGE;
     }
-    std::vector<hipEvent_t> ev;
-    auto cleanup_events = [&]() {
-        for (auto e : ev) (void)hipEventDestroy(e);
-        ev.clear();
-    };
+    // workspace for the largest layer, chunks of 64 rows
+    const int64_t want_chunks = std::max<int64_t>(1, (choose(k, k / 2) + kWave - 1) / kWave);
+    CHK(ensure_ws(c, want_chunks));
+    CHK(clear_flags(c));
+    CHK(ensure_xs(c, c->ws_chunks * kWave));
 
-    // ---- workspace for the largest layer (or k singleton rows)
-    int64_t rows_max = std::max<int64_t>(k, choose(k, k / 2));
-    int64_t row_cap_vs = (int64_t)1 << (32 - st.VS);
-    rows_max = std::min(rows_max, row_cap_vs);
-    {
-        int rc = ensure_ws(c, std::max<int64_t>(rows_max, k) * np);
-        if (rc) return rc;
+    EventList evl;
+    std::vector<hipEvent_t> &ev = evl.v;
+    hipEvent_t e_begin, e_single1, e_end, e_tb0, e_tb1, e_zero0, e_zero1;
+    for (hipEvent_t *p : {&e_begin, &e_single1, &e_end, &e_tb0, &e_tb1, &e_zero0, &e_zero1}) {
+        HIPCHK(c, hipEventCreate(p));
+        ev.push_back(*p);
     }
-    if (k > row_cap_vs) {
-        set_err(c, "k=%lld singleton rows exceed the queue index range", (long long)k);
-        return EMV_ERR_RANGE;
-    }
-    CHK(ensure_xs(c, rows_max));
-
-    hipEvent_t e_begin, e_single0, e_single1, e_end, e_tb0, e_tb1, e_zero0, e_zero1;
-    HIPCHK(c, hipEventCreate(&e_zero0));
-    HIPCHK(c, hipEventCreate(&e_zero1));
-    ev.push_back(e_zero0);
-    ev.push_back(e_zero1);
-    HIPCHK(c, hipEventCreate(&e_begin));
-    HIPCHK(c, hipEventCreate(&e_single0));
-    HIPCHK(c, hipEventCreate(&e_single1));
-    HIPCHK(c, hipEventCreate(&e_end));
-    HIPCHK(c, hipEventCreate(&e_tb0));
-    HIPCHK(c, hipEventCreate(&e_tb1));
-    ev.push_back(e_begin);
-    ev.push_back(e_single0);
-    ev.push_back(e_single1);
-    ev.push_back(e_end);
-    ev.push_back(e_tb0);
-    ev.push_back(e_tb1);
 
     HIPCHK(c, hipEventRecord(e_begin, c->stream));
-    // ---- 1. singleton SSSPs into staging rows 0..k-1
+    // ---- 1. singleton shortest paths in chunk 0, lane t = kk[t]
     std::vector<int32_t> src((size_t)k);
     for (int64_t t = 0; t < k; t++) src[t] = (int32_t)kk[t];
-    HIPCHK(c, hipEventRecord(e_single0, c->stream));
-    int64_t srounds = 0;
-    int rc = sssp_rows(c, src, true, st.CB, st.pred_base, st.VS, &srounds);
-    if (rc) { cleanup_events(); return rc; }
-    res->relax_rounds += srounds;
-    // ---- 2. bound: per-row maxima
+    CHK(sssp_chunk(c, src, true, st.pred_base, &res->relax_rounds));
+    // ---- 2. bound from the per-row maxima
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, c->stream, c->d_rowmax,
                        (int64_t)64, 0ull);
-    hipLaunchKernelGGL((k_unpack_singletons<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
-                       dim3(kBlock), 0, c->stream, c->d_S, np, c->d_kk, (int)k, (int)n, st.CB,
-                       (int32_t *)nullptr, (int32_t *)nullptr, c->d_rowmax);
+    hipLaunchKernelGGL((k_unpack_singletons_v<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
+                       dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src, (int)k,
+                       (int)n, (int32_t *)nullptr, (int32_t *)nullptr, c->d_rowmax);
     HIPCHK(c, hipGetLastError());
     std::vector<unsigned long long> rowmax((size_t)k);
     HIPCHK(c, hipMemcpyAsync(rowmax.data(), c->d_rowmax, sizeof(unsigned long long) * k,
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const unsigned long long dinf = ~0ull >> st.CB;
     unsigned long long B = 0;
     for (int64_t t = 0; t < k; t++) {
-        if (rowmax[t] >= dinf) {
-            cleanup_events();
+        if (rowmax[t] == ~0ull) {
             set_err(c, "terminal %lld does not reach every vertex (graph disconnected)",
-                    (long long)kk[t]);
+                    (long long)kk[t] + 1);
             return EMV_ERR_DISCONNECTED;
+        }
+        if (rowmax[t] >= (1ull << 56)) {
+            set_err(c, "distance bound overflows");
+            return EMV_ERR_RANGE;
         }
         B += rowmax[t];
     }
-    // merge sums reach 2B; staging needs 2B < 2^(64-CB)
-    if (bits_for(2 * B) > 64 - st.CB || bits_for(2 * B) + k > 64) {
-        cleanup_events();
+    // merge sums reach 2B; labels reach 2B + max w before a relaxation
+    const unsigned long long top = 2 * B + (unsigned long long)c->maxw;
+    if (bits_for(2 * B) + k > 64 || top >= (1ull << 62)) {
         set_err(c, "distance bound %llu exceeds the 64-bit packing", B);
         return EMV_ERR_RANGE;
     }
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
     st.pbits = (st.tbytes == 4 && bits_for(2 * B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
+    st.dbits = (top < 0xFFFFFFFFull && !(flags & EMV_FORCE_P64)) ? 32 : 64;
 
     // ---- table allocation + fill
-    int64_t cells = st.S2 * np;
+    const int64_t cells = st.S2 * np;
     HIPCHK(c, hipEventRecord(e_zero0, c->stream));
     CHK(ensure_table(c, (size_t)cells * st.tbytes, (size_t)cells * 4));
-    c->tbytes = st.tbytes;
-    c->tk = k;
     {
         int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
         if (st.tbytes == 4) {
@@ -627,28 +637,29 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     }
     HIPCHK(c, hipEventRecord(e_zero1, c->stream));
     if (st.tbytes == 4)
-        hipLaunchKernelGGL((k_unpack_singletons<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
-                           dim3(kBlock), 0, c->stream, c->d_S, np, c->d_kk, (int)k, (int)n, st.CB,
-                           (int32_t *)c->d_f, c->d_b, c->d_rowmax);
+        hipLaunchKernelGGL((k_unpack_singletons_v<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
+                           dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src,
+                           (int)k, (int)n, (int32_t *)c->d_f, c->d_b, c->d_rowmax);
     else
-        hipLaunchKernelGGL((k_unpack_singletons<int64_t>), dim3((n + kBlock - 1) / kBlock, k),
-                           dim3(kBlock), 0, c->stream, c->d_S, np, c->d_kk, (int)k, (int)n, st.CB,
-                           (int64_t *)c->d_f, c->d_b, c->d_rowmax);
+        hipLaunchKernelGGL((k_unpack_singletons_v<int64_t>), dim3((n + kBlock - 1) / kBlock, k),
+                           dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src,
+                           (int)k, (int)n, (int64_t *)c->d_f, c->d_b, c->d_rowmax);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(e_single1, c->stream));
+    c->tbytes = st.tbytes;
+    c->tk = k;
 
     // ---- 3. layers
-    size_t ev_layers0 = ev.size();
+    const size_t ev_layers0 = ev.size();
     if (st.tbytes == 4)
-        rc = run_layers<int32_t>(c, st, (int32_t *)c->d_f, c->d_b, res, ev);
+        CHK(run_layers_t<int32_t>(c, st, (int32_t *)c->d_f, c->d_b, res, ev));
     else
-        rc = run_layers<int64_t>(c, st, (int64_t *)c->d_f, c->d_b, res, ev);
-    if (rc) { cleanup_events(); return rc; }
+        CHK(run_layers_t<int64_t>(c, st, (int64_t *)c->d_f, c->d_b, res, ev));
     HIPCHK(c, hipEventRecord(e_end, c->stream));
 
     // ---- 4. answer
-    int64_t C = (k == 1) ? 0 : (((int64_t)1 << (k - 1)) - 1);
-    int64_t q = kk[k - 1];
+    const int64_t C = (k == 1) ? 0 : (((int64_t)1 << (k - 1)) - 1);
+    const int64_t q = kk[k - 1];
     int64_t cost = 0;
     if (C == 0) {
         cost = INT64_MAX;  // row 0 is never written (reader-el.c:2137 with k == 1)
@@ -667,33 +678,37 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // ---- traceback
     HIPCHK(c, hipEventRecord(e_tb0, c->stream));
     if ((flags & EMV_LIST) && C != 0) {
-        // An optimal tree emits at most n-1 edges plus k shortcut pairs; the
-        // pending-frame stack holds at most one frame per terminal split.
-        int64_t cap = 2 * (n + k) + 64;
-        int64_t stcap = 4 * k + 64;
+        // an optimal tree emits at most n-1 edges plus k shortcut pairs; the
+        // pending-frame stack holds at most one frame per terminal split
+        const int64_t cap = 2 * (n + k) + 64;
+        const int64_t stcap = 4 * k + 64;
         int64_t *d_pairs = nullptr, *d_stack = nullptr;
         HIPCHK(c, hipMalloc((void **)&d_pairs, sizeof(int64_t) * 2 * cap));
-        HIPCHK(c, hipMalloc((void **)&d_stack, sizeof(int64_t) * 2 * stcap));
+        if (hipMalloc((void **)&d_stack, sizeof(int64_t) * 2 * stcap) != hipSuccess) {
+            (void)hipFree(d_pairs);
+            set_err(c, "traceback stack allocation failed");
+            return EMV_ERR_NOMEM;
+        }
         hipLaunchKernelGGL(k_traceback, dim3(1), dim3(64), 0, c->stream, c->d_b, np, (int)k, q, C,
                            d_pairs, cap, d_stack, stcap, c->d_misc);
-        int64_t np_out = 0;
-        hipError_t e1 = hipMemcpyAsync(&np_out, c->d_misc, sizeof(int64_t),
-                                       hipMemcpyDeviceToHost, c->stream);
+        int64_t cnt = 0;
+        hipError_t e1 = hipMemcpyAsync(&cnt, c->d_misc, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                       c->stream);
         hipError_t e2 = hipStreamSynchronize(c->stream);
-        if (e1 != hipSuccess || e2 != hipSuccess || np_out < 0) {
+        if (e1 != hipSuccess || e2 != hipSuccess || cnt < 0) {
             (void)hipFree(d_pairs);
             (void)hipFree(d_stack);
-            cleanup_events();
-            set_err(c, "traceback failed (%lld)", (long long)np_out);
-            return np_out < 0 ? EMV_ERR_CAPACITY : EMV_ERR_HIP;
+            set_err(c, "traceback failed (%lld)", (long long)cnt);
+            return cnt < 0 ? EMV_ERR_CAPACITY : EMV_ERR_HIP;
         }
-        res->npairs = np_out;
-        res->pairs = (int64_t *)malloc(sizeof(int64_t) * 2 * std::max<int64_t>(np_out, 1));
-        if (np_out > 0)
-            (void)hipMemcpy(res->pairs, d_pairs, sizeof(int64_t) * 2 * np_out,
-                            hipMemcpyDeviceToHost);
+        res->npairs = cnt;
+        res->pairs = (int64_t *)malloc(sizeof(int64_t) * 2 * std::max<int64_t>(cnt, 1));
+        hipError_t e3 = hipSuccess;
+        if (cnt > 0)
+            e3 = hipMemcpy(res->pairs, d_pairs, sizeof(int64_t) * 2 * cnt, hipMemcpyDeviceToHost);
         (void)hipFree(d_pairs);
         (void)hipFree(d_stack);
+        HIPCHK(c, e3);
     } else if (flags & EMV_LIST) {
         res->npairs = 0;
         res->pairs = (int64_t *)malloc(sizeof(int64_t) * 2);
@@ -701,33 +716,33 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     HIPCHK(c, hipEventRecord(e_tb1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
-    // ---- timings and models
+    // ---- timings and traffic models
     res->cost = cost;
     res->k = (int32_t)k;
     res->n = n;
     res->table_bytes = st.tbytes;
     res->merge_bits = st.pbits;
-    res->code_bits = st.CB;
+    res->code_bits = st.dbits;
     res->dp_cells = st.S2 * n;
     res->zero_ms = ev_ms(e_zero0, e_zero1);
-    res->singleton_ms = ev_ms(e_single0, e_single1) - res->zero_ms;
+    res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;
     for (size_t i = ev_layers0; i + 2 < ev.size(); i += 3) {
         res->merge_ms += ev_ms(ev[i], ev[i + 1]);
         res->relax_ms += ev_ms(ev[i + 1], ev[i + 2]);
     }
     res->kernel_ms = ev_ms(e_begin, e_end) - res->zero_ms;
     res->traceback_ms = ev_ms(e_tb0, e_tb1);
-    double cb = st.tbytes, ab = 4.0, dn = (double)n, dm = (double)c->m;
+    const double cb = st.tbytes, ab = 4.0, dn = (double)n, dm = (double)c->m;
     double p3 = 1.0;
     for (int64_t i = 0; i < k; i++) p3 *= 3.0;
     res->merge_alg_bytes = cb * dn * (p3 - 2.0 * (double)st.S2 + 1.0) +
                            (cb + ab) * dn * ((double)st.S2 - (double)k - 1.0);
     res->relax_alg_bytes = ((double)st.S2 - 1.0) * (cb * dn + cb * 2 * dm + (cb + ab) * dn) +
                            (double)k * (4.0 * (dn + 1) + 16.0 * dm);
-    res->device_bytes = cells * (st.tbytes + 4) + c->ws_cells * 20;
+    res->device_bytes = (int64_t)(c->f_cap + c->b_cap) +
+                        c->ws_chunks * np * (int64_t)(kWave * 12 + 12);
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
-    cleanup_events();
     if (!(flags & EMV_KEEP_TABLE)) invalidate_table(c);
     return EMV_OK;
 }
@@ -739,7 +754,7 @@ extern "C" int emv_read_rows(emv_ctx *c, int64_t X0, int64_t nrows, int64_t *f_o
         set_err(c, "no table kept (solve with EMV_KEEP_TABLE)");
         return EMV_ERR_STATE;
     }
-    int64_t S2 = (int64_t)1 << c->tk;
+    const int64_t S2 = (int64_t)1 << c->tk;
     if (X0 < 0 || nrows < 0 || X0 + nrows > S2) {
         set_err(c, "rows [%lld, %lld) outside [0, %lld)", (long long)X0, (long long)(X0 + nrows),
                 (long long)S2);
@@ -749,7 +764,7 @@ extern "C" int emv_read_rows(emv_ctx *c, int64_t X0, int64_t nrows, int64_t *f_o
     const int64_t n = c->n, np = c->np;
     std::vector<char> buf((size_t)(np * std::max(c->tbytes, 4)));
     for (int64_t r = 0; r < nrows; r++) {
-        int64_t X = X0 + r;
+        const int64_t X = X0 + r;
         if (f_out) {
             HIPCHK(c, hipMemcpy(buf.data(), (char *)c->d_f + (size_t)(X * np) * c->tbytes,
                                 (size_t)n * c->tbytes, hipMemcpyDeviceToHost));
@@ -784,36 +799,38 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
         return EMV_ERR_ARG;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    const uint64_t pred_base = 2;
-    int CB = bits_for(pred_base + (uint64_t)c->n);
-    int VS = bits_for((uint64_t)c->np - 1);
+    const uint32_t pred_base = 2;
+    EventList evl;
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
+    evl.v.push_back(a);
     HIPCHK(c, hipEventCreate(&b));
+    evl.v.push_back(b);
     HIPCHK(c, hipEventRecord(a, c->stream));
     int64_t rounds = 0;
     std::vector<int32_t> src(1, (int32_t)s);
-    int rc = sssp_rows(c, src, false, CB, pred_base, VS, &rounds);
-    if (rc) {
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        return rc;
-    }
+    CHK(sssp_chunk(c, src, false, pred_base, &rounds));
     int64_t *dd = nullptr, *dp = nullptr;
     HIPCHK(c, hipMalloc((void **)&dd, sizeof(int64_t) * c->n));
-    HIPCHK(c, hipMalloc((void **)&dp, sizeof(int64_t) * c->n));
-    hipLaunchKernelGGL(k_unpack_sssp, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->d_S,
-                       (int)c->n, CB, pred_base, dd, dp);
-    HIPCHK(c, hipEventRecord(b, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (hipMalloc((void **)&dp, sizeof(int64_t) * c->n) != hipSuccess) {
+        (void)hipFree(dd);
+        set_err(c, "allocation failed");
+        return EMV_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_unpack_sssp_v, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
+                       (const uint64_t *)c->d_D, c->d_code, (int)c->n, pred_base, dd, dp);
+    hipError_t e0 = hipEventRecord(b, c->stream);
+    hipError_t e1 = hipStreamSynchronize(c->stream);
     if (ms) *ms = ev_ms(a, b);
-    if (d_out) HIPCHK(c, hipMemcpy(d_out, dd, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost));
-    if (pred_out)
-        HIPCHK(c, hipMemcpy(pred_out, dp, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost));
+    hipError_t e2 = hipSuccess, e3 = hipSuccess;
+    if (d_out) e2 = hipMemcpy(d_out, dd, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost);
+    if (pred_out) e3 = hipMemcpy(pred_out, dp, sizeof(int64_t) * c->n, hipMemcpyDeviceToHost);
     (void)hipFree(dd);
     (void)hipFree(dp);
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    HIPCHK(c, e0);
+    HIPCHK(c, e1);
+    HIPCHK(c, e2);
+    HIPCHK(c, e3);
     return EMV_OK;
 }
 

@@ -1,19 +1,25 @@
 // emv_kernels.hpp -- gfx950 device code of the EMV Steiner DP.
 //
-// Table layout in HBM (subset-major, as reader-el.c:690-692, rows padded):
-//   f[X * np + v]   cost, int32 or int64         X in [0, 2^k), v in [0, np)
-//   b[X * np + v]   int32 record code: Y (merge split, Y == X terminates)
-//                   or 2^k + u (edge from predecessor u)
-// np is n rounded up to a multiple of 1024; padding cells stay 0.
+// Table in HBM (subset-major as reader-el.c:690-692, rows padded to np, a
+// multiple of 1024; padding cells stay 0):
+//   f[X * np + v]   cost, int32 or int64
+//   b[X * np + v]   int32 record code: Y for a merge split (Y == X stops the
+//                   traceback), 2^k + u for an edge from predecessor u
 //
-// Relaxation staging: one uint64 per cell of the layer batch,
-//   S = (d << CB) | code,  code < 2^CB
-// so that a single 64-bit atomicMin keeps the smallest distance and, among
-// equal distances, the smallest code: the merge record (code = Y < 2^k) beats
-// every edge record (code = 2^k + u), and edge records prefer the smallest u.
-// That reproduces the reference's rule that the merge record survives unless
-// the shortest path strictly improves on the merged value (reader-el.c:2122)
-// and makes the records independent of the schedule.
+// Layer staging (vertex-major): the rows of one layer batch are grouped in
+// chunks of 64; chunk c holds, for every vertex v, the 64 rows' labels
+// side by side:
+//   D[(c * np + v) * 64 + lane]     distance label of row c*64+lane at v
+//   CODE[(c * np + v) * 64 + lane]  its record code
+// One wave owns one (chunk, vertex) item: lane = row.  A neighbour's labels
+// for the whole chunk are one coalesced 256 B (u32) or 512 B (u64) load, and
+// the adjacency of v is read once for 64 rows.
+//
+// Records are decided by the lexicographic minimum of (distance, code).  A
+// merge code (Y < 2^k) is below every edge code (2^k + u), so the merge
+// record survives unless a path strictly improves on the merged value -- the
+// reference's rule (reader-el.c:2117-2126) -- and among equal improvements
+// the smallest neighbour id wins, independent of the schedule.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -26,13 +32,8 @@ struct Edge {
 };
 
 constexpr int kBlock = 256;
-
-typedef int32_t v4i32 __attribute__((ext_vector_type(4)));
-typedef int64_t v2i64 __attribute__((ext_vector_type(2)));
-
-template <typename T> struct VecOf;
-template <> struct VecOf<int32_t> { static constexpr int N = 4; typedef v4i32 type; };
-template <> struct VecOf<int64_t> { static constexpr int N = 2; typedef v2i64 type; };
+constexpr int kWave = 64;
+constexpr int kXG = 8;  // subset rows per merge wave item
 
 // Work-item mapping that keeps neighbouring items on one XCD.  Workgroups
 // are dealt to the XCDs round-robin (workgroup b runs on XCD b % nxcd), so
@@ -50,236 +51,253 @@ __global__ void k_fill(T *__restrict__ p, int64_t count, T val) {
     for (; i < count; i += stride) p[i] = val;
 }
 
-// ------------------------------------------------------ singleton seeding
-// Row t of the staging holds the SSSP from kk[t]: all INF except the source,
-// whose code 1<<t is the reference's (kk[t], {t}) record (reader-el.c:2025).
-__global__ void k_seed(uint64_t *__restrict__ S, int64_t np, const int32_t *__restrict__ src,
-                       int nsrc, uint32_t *__restrict__ q, uint32_t *__restrict__ cnt,
-                       uint32_t *__restrict__ stamp, uint32_t stamp_val, int VS,
-                       uint32_t code_base_is_bit) {
+// ------------------------------------------------------------- seeding
+// Chunk 0, lane t: the shortest-path labels from src[t].  The source's code
+// is 1<<t for the singleton rows (the reference's (kk[t], {t}) record,
+// reader-el.c:2025) or 1 for a plain shortest-path query.
+template <typename DT>
+__global__ void k_seed_v(DT *__restrict__ D, uint32_t *__restrict__ CODE,
+                         const int32_t *__restrict__ src, int nsrc, int bit_codes) {
     int t = threadIdx.x;
     if (t < nsrc) {
-        int64_t cell = (int64_t)t * np + src[t];
-        uint64_t code = code_base_is_bit ? (1ull << t) : 1ull;
-        S[cell] = code;  // d = 0
-        stamp[cell] = stamp_val;
-        q[t] = ((uint32_t)t << VS) | (uint32_t)src[t];
-    }
-    if (t == 0) {
-        cnt[0] = (uint32_t)nsrc;
-        cnt[1] = 0;
-        cnt[2] = 0;
-    }
-}
-
-// Wave-aggregated append of `cell` to a queue (one atomic per wave).
-__device__ __forceinline__ void wave_append(bool want, uint32_t entry, uint32_t *__restrict__ q,
-                                            uint32_t *__restrict__ cnt) {
-    uint64_t mask = __ballot(want);
-    if (mask == 0) return;
-    int lane = __lane_id();
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (want) {
-        uint64_t below = mask & ((1ull << lane) - 1ull);
-        q[base + (uint32_t)__popcll(below)] = entry;
+        int64_t idx = (int64_t)src[t] * kWave + t;
+        D[idx] = 0;
+        CODE[idx] = bit_codes ? (1u << t) : 1u;
     }
 }
 
 // ------------------------------------------------------------ relaxation
-// Pull sweep over every cell of a batch (first round of a layer).  Each
-// thread owns one cell, so its write needs no atomic; concurrent readers see
-// the old or the new label, both valid upper bounds.
-__global__ __launch_bounds__(kBlock) void k_pull(uint64_t *__restrict__ S, int64_t np, int nrows,
-                                                 int n, const int32_t *__restrict__ rp,
-                                                 const Edge *__restrict__ adj, int CB,
-                                                 uint64_t pred_base, uint32_t *__restrict__ qout,
-                                                 uint32_t *__restrict__ cnt, int jout,
-                                                 uint32_t *__restrict__ stamp, uint32_t stamp_val,
-                                                 int VS, int nxcd, int64_t per_xcd,
-                                                 int64_t tiles_per_row) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(jout + 1) % 3] = 0;
-    int64_t item = xcd_item(nxcd, per_xcd);
-    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
-    int64_t row = item / tiles_per_row;
-    int64_t tile = item - row * tiles_per_row;
-    bool want = false;
-    uint32_t entry = 0;
-    int v = (int)(tile * kBlock + threadIdx.x);
-    if (row < nrows && v < n) {
-        uint64_t *Sr = S + row * np;
-        uint64_t cur = Sr[v];
-        uint64_t best = cur;
-        const uint64_t dinf = ~0ull >> CB;
-        for (int e = rp[v]; e < rp[v + 1]; e++) {
-            Edge ed = adj[e];
-            uint64_t du = Sr[ed.v] >> CB;
-            if (du == dinf) continue;
-            uint64_t cand = ((du + (uint64_t)ed.w) << CB) | (pred_base + (uint64_t)ed.v);
-            best = cand < best ? cand : best;
-        }
-        if (best < cur) {
-            Sr[v] = best;
-            if ((best >> CB) < (cur >> CB)) {
-                want = true;
-                entry = ((uint32_t)row << VS) | (uint32_t)v;
-                stamp[row * np + v] = stamp_val;
-            }
-        }
-    }
-    wave_append(want, entry, qout, &cnt[jout]);
-}
-
-// Push round: every queued cell offers d + w to its neighbours with a 64-bit
-// atomicMin; a neighbour whose distance drops is queued once for the next
-// round (stamp dedup).  Input/output counters rotate over cnt[0..2]; this
-// round clears the counter the next round will write.
-__global__ __launch_bounds__(kBlock) void k_push(uint64_t *__restrict__ S, int64_t np,
-                                                 const int32_t *__restrict__ rp,
-                                                 const Edge *__restrict__ adj, int CB,
-                                                 uint64_t pred_base,
-                                                 const uint32_t *__restrict__ qin,
-                                                 uint32_t *__restrict__ qout,
-                                                 uint32_t *__restrict__ cnt, int j,
-                                                 uint32_t *__restrict__ stamp, uint32_t stamp_val,
-                                                 int VS) {
-    const uint32_t nin = cnt[j % 3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[(j + 2) % 3] = 0;
-    uint32_t *cout = &cnt[(j + 1) % 3];
-    const uint32_t vmask = (1u << VS) - 1u;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // Round the trip count up to a whole number of waves so that every lane
-    // of a wave reaches the wave-wide append together.
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < nin; base += stride) {
-        uint64_t i = base + threadIdx.x;
-        bool active = i < nin;
-        uint32_t ent = active ? qin[i] : 0u;
-        int64_t row = ent >> VS;
-        int u = (int)(ent & vmask);
-        uint64_t *Sr = S + row * np;
-        uint64_t du = active ? (Sr[u] >> CB) : 0;
-        int e0 = active ? rp[u] : 0, e1 = active ? rp[u + 1] : 0;
-        uint64_t code = pred_base + (uint64_t)u;
-        // Lanes walk their own adjacency; the append is done once per edge
-        // slot with all lanes of the wave present.
-        int deg = e1 - e0;
-        int maxdeg = deg;
-        for (int off = 32; off > 0; off >>= 1) {
-            int o = __shfl_xor(maxdeg, off);
-            maxdeg = o > maxdeg ? o : maxdeg;
-        }
-        for (int t = 0; t < maxdeg; t++) {
-            bool want = false;
-            uint32_t entry = 0;
-            if (t < deg) {
-                Edge ed = adj[e0 + t];
-                uint64_t cand = ((du + (uint64_t)ed.w) << CB) | code;
-                uint64_t *dst = Sr + ed.v;
-                if (cand < *dst) {
-                    uint64_t old = atomicMin((unsigned long long *)dst, (unsigned long long)cand);
-                    if ((cand >> CB) < (old >> CB)) {
-                        uint32_t prev = atomicMax(&stamp[row * np + ed.v], stamp_val);
-                        if (prev < stamp_val) {
-                            want = true;
-                            entry = ((uint32_t)row << VS) | (uint32_t)ed.v;
+// One pull round over (chunk, vertex) items, item i = (chunk i / n, vertex
+// i % n).  Round r processes the items flagged in F_cur (every item when
+// `dense`); a wave loads 64 consecutive flags at once and walks the set ones.
+// Per item the wave recomputes, for its 64 rows, the lexicographic minimum of
+// its own (label, code) and (neighbour label + w, 2^k + neighbour) over the
+// whole adjacency and writes what changed.  If any label dropped it flags
+// every neighbour in F_next and raises any[(r+1)&3].  Flags are bytes written
+// with plain stores (idempotent, no atomics); a round clears the flags it
+// consumed, so F_next is clean when the round starts, and clears
+// any[(r+2)&3] for the round after next.  Pulling the full adjacency after
+// every neighbour change makes the final (label, code) of each cell the exact
+// lexicographic minimum, independent of the order waves run in.
+template <typename DT>
+__global__ __launch_bounds__(kBlock) void k_relax_v(
+    DT *__restrict__ D, uint32_t *__restrict__ CODE, int64_t np, int n,
+    const int32_t *__restrict__ rp, const Edge *__restrict__ adj, uint32_t pred_base,
+    int64_t nitems, int dense, uint8_t *__restrict__ Fcur, uint8_t *__restrict__ Fnext,
+    uint32_t *__restrict__ any, int r, int nxcd) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) any[(r + 2) & 3] = 0;
+    if (!dense && any[r & 3] == 0) return;  // nothing flagged for this round
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int64_t waves_per_block = kBlock / kWave;
+    const int64_t per = gridDim.x / nxcd;
+    const int64_t b = blockIdx.x;
+    const int64_t wave0 = ((b % nxcd) * per + b / nxcd) * waves_per_block + wid;
+    const int64_t wstride = (int64_t)gridDim.x * waves_per_block;
+    const int64_t nblk = (nitems + kWave - 1) / kWave;
+    const DT INF = ~(DT)0;
+    bool raised = false;
+    for (int64_t blk = wave0; blk < nblk; blk += wstride) {
+        const int64_t base = blk * kWave;
+        // flags of the 64 items base .. base+63
+        int64_t mi = base + lane;
+        int64_t mc = mi / n;
+        int64_t mf = mc * np + (mi - mc * n);
+        bool fl = false;
+        if (mi < nitems) fl = dense ? true : (Fcur[mf] != 0);
+        uint64_t act = __ballot(fl);
+        if (!dense && fl) Fcur[mf] = 0;
+        while (act) {
+            const int t = __ffsll((unsigned long long)act) - 1;
+            act &= act - 1;
+            const int64_t it = base + t;
+            const int64_t c = it / n;
+            const int v = (int)(it - c * n);
+            DT *Dc = D + c * np * kWave;
+            uint32_t *Cc = CODE + c * np * kWave;
+            const int64_t idx = (int64_t)v * kWave + lane;
+            const DT cur = Dc[idx];
+            const uint32_t cc = Cc[idx];
+            DT best = cur;
+            uint32_t bc = cc;
+            const int e0 = rp[v], e1 = rp[v + 1];
+            for (int eb = e0; eb < e1; eb += kWave) {
+                const int cnt = min(kWave, e1 - eb);
+                Edge my = {0, 0};
+                if (lane < cnt) my = adj[eb + lane];
+                for (int t0 = 0; t0 < cnt; t0 += 4) {
+                    // four neighbours per step: their loads issue together
+                    DT du[4];
+                    int uu[4], ww[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int tt = min(t0 + q, cnt - 1);
+                        uu[q] = __builtin_amdgcn_readlane(my.v, tt);
+                        ww[q] = __builtin_amdgcn_readlane(my.w, tt);
+                        du[q] = Dc[(int64_t)uu[q] * kWave + lane];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (t0 + q < cnt && du[q] != INF) {
+                            const DT nd = du[q] + (DT)ww[q];
+                            const uint32_t nc = pred_base + (uint32_t)uu[q];
+                            if (nd < best || (nd == best && nc < bc)) {
+                                best = nd;
+                                bc = nc;
+                            }
                         }
                     }
                 }
             }
-            wave_append(want, entry, qout, cout);
+            const bool dchg = best < cur;
+            if (dchg) Dc[idx] = best;
+            if (bc != cc) Cc[idx] = bc;
+            if (__ballot(dchg) != 0) {
+                for (int e = e0 + lane; e < e1; e += kWave) Fnext[c * np + adj[e].v] = 1;
+                raised = true;
+            }
         }
     }
+    if (raised && lane == 0) any[(r + 1) & 3] = 1;
 }
 
 // ------------------------------------------------------------------ merge
 // f[X][v] = min over nonempty Y strictly inside X of f[Y][v] + f[X\Y][v]
 // (reader-el.c:2068-2088).  Only Y without the top bit of X is enumerated:
-// the pair {Y, X\Y} is visited once and Y is the smaller of the two.  The
+// each pair {Y, X\Y} is visited once and Y is the smaller of the two.  The
 // accumulator packs (sum << k) | Y, so an unsigned min keeps the smallest
 // sum and, among equal sums, the smallest Y -- the reference's first strict
-// minimum over increasing Y.  Each workgroup handles one X on a tile of
-// 256*VEC vertices; the tiles of one XCD are consecutive, so the rows a tile
-// needs are shared in that XCD's L2.
-template <typename T, typename P>
-__global__ __launch_bounds__(kBlock) void k_merge(const T *__restrict__ f, int64_t np,
-                                                  const uint32_t *__restrict__ xs, int nx,
-                                                  int64_t ntiles, int k, int CB,
-                                                  uint64_t *__restrict__ S, int nxcd,
-                                                  int64_t per_xcd) {
-    constexpr int VEC = VecOf<T>::N;
-    typedef typename VecOf<T>::type VT;
+// minimum over increasing Y (strict '<' at 2078).
+//
+// Work item = one wave: 64 consecutive vertices (one per lane, 256 B
+// coalesced row loads) x kXG consecutive rows of the layer batch.  Items are
+// tile-major and dealt to XCDs in contiguous ranges, so the waves of an XCD
+// sweep the rows of one vertex tile together and the tile's lower-layer rows
+// stay in that XCD's L2.  Results go straight to the vertex-major staging:
+// each lane writes its kXG labels (and codes) as one contiguous run.
+template <typename T, typename P, typename DT>
+__global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int64_t np,
+                                                    const uint32_t *__restrict__ xs, int nx,
+                                                    int nsub, int64_t ntiles, int k,
+                                                    DT *__restrict__ D,
+                                                    uint32_t *__restrict__ CODE, int nxcd,
+                                                    int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
-    int64_t item = xcd_item(nxcd, per_xcd);
-    int64_t tile = item / nx;
-    int xi = (int)(item - tile * nx);
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t tile = item / nsub;
+    const int sub = (int)(item - tile * nsub);
     if (tile >= ntiles) return;
-    const uint32_t X = xs[xi];
-    const uint32_t top = 1u << (31 - __clz(X));
-    const uint32_t Xp = X ^ top;
-    const int64_t v0 = tile * (kBlock * VEC) + (int64_t)threadIdx.x * VEC;
-    P acc[VEC];
+    const int64_t v = tile * kWave + lane;
+    const T *fv = f + v;
+    const DT INF = ~(DT)0;
+    const P ymask = ((P)1 << k) - 1;
+    DT dv[kXG];
+    uint32_t yv[kXG];
 #pragma unroll
-    for (int j = 0; j < VEC; j++) acc[j] = ~(P)0;
-    for (uint32_t Y = Xp; Y != 0; Y = (Y - 1) & Xp) {
-        VT a = *(const VT *)(f + (int64_t)Y * np + v0);
-        VT c = *(const VT *)(f + (int64_t)(X ^ Y) * np + v0);
-#pragma unroll
-        for (int j = 0; j < VEC; j++) {
-            P s = (((P)a[j] + (P)c[j]) << k) | (P)Y;
-            acc[j] = s < acc[j] ? s : acc[j];
+    for (int jx = 0; jx < kXG; jx++) {
+        const int xi = sub * kXG + jx;
+        dv[jx] = INF;
+        yv[jx] = 0;
+        if (xi < nx) {
+            const uint32_t X = xs[xi];
+            const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+            P acc = ~(P)0;
+            uint32_t Y = Xp;
+            while (Y != 0) {
+                // four pairs per trip keep eight loads in flight
+                const uint32_t Y1 = (Y - 1) & Xp;
+                const uint32_t Y2 = Y1 ? ((Y1 - 1) & Xp) : 0u;
+                const uint32_t Y3 = Y2 ? ((Y2 - 1) & Xp) : 0u;
+                const T a0 = fv[(int64_t)Y * np], c0 = fv[(int64_t)(X ^ Y) * np];
+                T a1 = 0, c1 = 0, a2 = 0, c2 = 0, a3 = 0, c3 = 0;
+                if (Y1) { a1 = fv[(int64_t)Y1 * np]; c1 = fv[(int64_t)(X ^ Y1) * np]; }
+                if (Y2) { a2 = fv[(int64_t)Y2 * np]; c2 = fv[(int64_t)(X ^ Y2) * np]; }
+                if (Y3) { a3 = fv[(int64_t)Y3 * np]; c3 = fv[(int64_t)(X ^ Y3) * np]; }
+                P s = (((P)a0 + (P)c0) << k) | (P)Y;
+                acc = s < acc ? s : acc;
+                if (Y1) { s = (((P)a1 + (P)c1) << k) | (P)Y1; acc = s < acc ? s : acc; }
+                if (Y2) { s = (((P)a2 + (P)c2) << k) | (P)Y2; acc = s < acc ? s : acc; }
+                if (Y3) { s = (((P)a3 + (P)c3) << k) | (P)Y3; acc = s < acc ? s : acc; }
+                Y = Y3 ? ((Y3 - 1) & Xp) : 0u;
+            }
+            dv[jx] = (DT)(acc >> k);
+            yv[jx] = (uint32_t)(acc & ymask);
         }
     }
-    uint64_t *Sr = S + (int64_t)xi * np + v0;
-    const P ymask = ((P)1 << k) - 1;
+    const int chunk = sub / (kWave / kXG);
+    const int loff = (sub % (kWave / kXG)) * kXG;
+    DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
+    uint32_t *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
 #pragma unroll
-    for (int j = 0; j < VEC; j++) {
-        uint64_t d = (uint64_t)(acc[j] >> k);
-        uint64_t y = (uint64_t)(acc[j] & ymask);
-        Sr[j] = (d << CB) | y;
+    for (int jx = 0; jx < kXG; jx++) {
+        dd[jx] = dv[jx];
+        cd[jx] = yv[jx];
     }
 }
 
 // ----------------------------------------------------------------- unpack
-// Staging rows -> table rows f[X], b[X] for a batch of layer rows.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_unpack(const uint64_t *__restrict__ S, int64_t np,
-                                                   const uint32_t *__restrict__ xs, int nrows,
-                                                   int n, int CB, T *__restrict__ f,
-                                                   int32_t *__restrict__ b, int nxcd,
-                                                   int64_t per_xcd, int64_t tiles_per_row) {
+// Vertex-major staging -> table rows f[X], b[X] through an LDS transpose of
+// 64 vertices x 64 rows per workgroup.
+template <typename T, typename DT>
+__global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
+                                                     const uint32_t *__restrict__ CODE,
+                                                     int64_t np, const uint32_t *__restrict__ xs,
+                                                     int nx, int n, T *__restrict__ f,
+                                                     int32_t *__restrict__ b, int nxcd,
+                                                     int64_t per_xcd, int64_t vtiles) {
+    __shared__ DT sd[kWave][kWave + 1];
+    __shared__ uint32_t sc[kWave][kWave + 1];
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
-    int64_t item = xcd_item(nxcd, per_xcd);
-    int64_t row = item / tiles_per_row;
-    int64_t tile = item - row * tiles_per_row;
-    int v = (int)(tile * kBlock + threadIdx.x);
-    if (row >= nrows || v >= n) return;
-    uint64_t s = S[row * np + v];
-    int64_t X = xs[row];
-    f[X * np + v] = (T)(s >> CB);
-    b[X * np + v] = (int32_t)(s & ((1ull << CB) - 1ull));
+    const int64_t item = xcd_item(nxcd, per_xcd);
+    const int64_t chunk = item / vtiles;
+    const int64_t vt = item - chunk * vtiles;
+    if (chunk * kWave >= nx) return;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int grp = t / kWave;
+    const int64_t v0 = vt * kWave;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int vv = grp + r * (kBlock / kWave);
+        const int64_t idx = ((chunk * np) + v0 + vv) * kWave + lane;
+        sd[vv][lane] = D[idx];
+        sc[vv][lane] = CODE[idx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int xl = grp + r * (kBlock / kWave);
+        const int64_t xi = chunk * kWave + xl;
+        const int64_t v = v0 + lane;
+        if (xi < nx && v < n) {
+            const int64_t X = xs[xi];
+            f[X * np + v] = (T)sd[lane][xl];
+            b[X * np + v] = (int32_t)sc[lane][xl];
+        }
+    }
 }
 
-// Singleton rows: f[1<<t][v] = dist(kk[t], v); record (kk[t], {t}) for every
-// v (reader-el.c:2015-2027).  Also the per-row maximum distance (bounds the
-// table: f[X][v] <= sum over t in X of dist(kk[t], v)).
+// Singleton rows: f[1<<t][v] = dist(kk[t], v) from chunk 0 lane t; record
+// (kk[t], {t}) for every v (reader-el.c:2015-2027).  Also the per-row
+// maximum distance, which bounds the table: f[X][v] <= sum over t in X of
+// dist(kk[t], v).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_unpack_singletons(
-    const uint64_t *__restrict__ S, int64_t np, const int32_t *__restrict__ kk, int k, int n,
-    int CB, T *__restrict__ f, int32_t *__restrict__ b, unsigned long long *__restrict__ rowmax) {
-    int t = blockIdx.y;
-    int v = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
+    const uint64_t *__restrict__ D, int64_t np, const int32_t *__restrict__ kk, int k, int n,
+    T *__restrict__ f, int32_t *__restrict__ b, unsigned long long *__restrict__ rowmax) {
+    const int t = blockIdx.y;
+    const int v = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long d = 0;
     if (v < n) {
-        d = S[(int64_t)t * np + v] >> CB;
-        int64_t X = 1ll << t;
+        d = D[(int64_t)v * kWave + t];
         if (f) {
+            const int64_t X = 1ll << t;
             f[X * np + v] = (T)d;
             b[X * np + v] = (v == kk[t]) ? (int32_t)X : (int32_t)(((int64_t)1 << k) + kk[t]);
         }
     }
-    // wave max then one atomic per wave
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(d, off);
         d = o > d ? o : d;
@@ -287,15 +305,15 @@ __global__ __launch_bounds__(kBlock) void k_unpack_singletons(
     if (__lane_id() == 0) atomicMax(&rowmax[t], d);
 }
 
-// Distances and predecessors of one SSSP row (for -dijkstra / k == 2).
-__global__ void k_unpack_sssp(const uint64_t *__restrict__ S, int n, int CB, uint64_t pred_base,
-                              int64_t *__restrict__ d, int64_t *__restrict__ pred) {
+// Distances and predecessors of one shortest-path query (chunk 0, lane 0).
+__global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, const uint32_t *__restrict__ CODE,
+                                int n, uint32_t pred_base, int64_t *__restrict__ d,
+                                int64_t *__restrict__ pred) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    uint64_t s = S[v];
-    uint64_t dd = s >> CB;
-    uint64_t code = s & ((1ull << CB) - 1ull);
-    d[v] = (dd == (~0ull >> CB)) ? INT64_MAX : (int64_t)dd;
+    uint64_t dd = D[(int64_t)v * kWave];
+    uint32_t code = CODE[(int64_t)v * kWave];
+    d[v] = (dd == ~0ull) ? INT64_MAX : (int64_t)dd;
     pred[v] = (code >= pred_base) ? (int64_t)(code - pred_base) : -1;
 }
 
@@ -303,14 +321,14 @@ __global__ void k_unpack_sssp(const uint64_t *__restrict__ S, int n, int CB, uin
 // Depth-first walk of the records from (q, C) in the reference's order
 // (reader-el.c:1907-1932): an edge record emits (v,u) and continues at
 // (u, X); a merge record visits (v, Y) then (v, X\Y); Y == X stops.
-// One lane walks; the records are a few hundred steps for real trees.
+// One lane walks; real trees are a few hundred steps.
 __global__ void k_traceback(const int32_t *__restrict__ b, int64_t np, int k, int64_t q,
                             int64_t C, int64_t *__restrict__ pairs, int64_t cap,
                             int64_t *__restrict__ stack, int64_t stcap,
                             int64_t *__restrict__ result) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int64_t S = 1ll << k;
-    int64_t sp = 0, np_ = 0;
+    int64_t sp = 0, npairs = 0;
     stack[0] = q;
     stack[1] = C;
     sp = 1;
@@ -322,10 +340,10 @@ __global__ void k_traceback(const int32_t *__restrict__ b, int64_t np, int k, in
         if (r < 0) continue;
         if (r >= S) {
             int64_t u = r - S;
-            if (np_ >= cap) { result[0] = -1; return; }
-            pairs[2 * np_] = v;
-            pairs[2 * np_ + 1] = u;
-            np_++;
+            if (npairs >= cap) { result[0] = -1; return; }
+            pairs[2 * npairs] = v;
+            pairs[2 * npairs + 1] = u;
+            npairs++;
             stack[2 * sp] = u;
             stack[2 * sp + 1] = X;
             sp++;
@@ -340,7 +358,7 @@ __global__ void k_traceback(const int32_t *__restrict__ b, int64_t np, int k, in
             sp++;
         }
     }
-    result[0] = np_;
+    result[0] = npairs;
 }
 
 }  // namespace emv
