@@ -43,7 +43,8 @@ extern "C" {
 #define EMV_LIST        0x1u  /* run the device traceback; fill res->pairs */
 #define EMV_FORCE_I64   0x2u  /* int64 table even when int32 bounds hold */
 #define EMV_KEEP_TABLE  0x4u  /* keep the table on the device for emv_read_rows */
-#define EMV_FORCE_P64   0x8u  /* 64-bit merge packing even when 32-bit fits */
+#define EMV_FORCE_P64   0x8u  /* 64-bit merge packing and relaxation labels */
+#define EMV_WIDE_LABELS 0x10u /* no 16-bit relaxation labels even when they fit */
 
 typedef struct emv_ctx emv_ctx;
 
@@ -52,7 +53,7 @@ typedef struct {
     int32_t k;
     int32_t table_bytes;     /* 4 (int32 table) or 8 (int64 table) */
     int32_t merge_bits;      /* 32 or 64: width of the packed merge accumulator */
-    int32_t code_bits;       /* low bits of a relaxation cell holding the record */
+    int32_t label_bits;      /* width of the relaxation labels: 16, 32 or 64 */
     int64_t n;
     int64_t dp_cells;        /* 2^k * n */
     /* timings, milliseconds, from HIP events on the solve stream */

@@ -35,10 +35,14 @@ struct emv_ctx {
     char err[512] = {0};
     int nxcd = 8;
     int num_cu = 256;
+    int relax_nxcd = 1;  // relaxation sweep: 1 = grid-wide order, nxcd = per-XCD ranges
+    int merge_persistent = 0;  // 1: per-XCD persistent merge for the large layers
     // graph (CSR on the device)
     int64_t n = 0, m = 0;
     int32_t *d_rp = nullptr;
     Edge *d_adj = nullptr;
+    Edge *d_ell = nullptr;  // padded fixed-degree copy (ell_deg > 0)
+    int ell_deg = 0;
     int64_t maxw = 0;
     // table; the allocation is reused across solves
     void *d_f = nullptr;
@@ -51,6 +55,7 @@ struct emv_ctx {
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     uint32_t *d_code = nullptr;   // record codes
     uint8_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
+    void *d_M = nullptr;          // packed merge accumulators, row-major, 8 bytes per cell reserved
     int64_t ws_chunks = 0;
     uint32_t *d_any = nullptr;               // 4 rotating "round has work" words
     unsigned long long *d_rowmax = nullptr;  // per singleton row
@@ -59,6 +64,7 @@ struct emv_ctx {
     int64_t xs_cap = 0;
     int32_t *d_src = nullptr;                // shortest-path sources
     uint32_t *h_pin = nullptr;               // pinned readback of a work word
+    uint32_t h_xs0 = 0;                      // first subset of the current batch
 };
 
 namespace {
@@ -90,6 +96,16 @@ void set_err(emv_ctx *c, const char *fmt, ...) {
     } while (0)
 
 int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// Row stride of the table: n rounded up to 64-vertex tiles, plus one tile so
+// that the stride is an odd multiple of 256 B.  The rows of one tile then
+// rotate over the L2 and HBM channels instead of all landing on the channel
+// that a power-of-two stride would pick.
+int64_t row_stride(int64_t n) {
+    int64_t np = round_up(n, kWave);
+    if ((np / kWave) % 2 == 0) np += kWave;
+    return np;
+}
 
 int bits_for(uint64_t x) {  // number of bits to hold values 0..x
     int b = 0;
@@ -149,6 +165,7 @@ int ensure_table(emv_ctx *c, size_t fbytes, size_t bbytes) {
 int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
 
 void free_ws(emv_ctx *c) {
+    dfree(c->d_M);
     dfree(c->d_D);
     dfree(c->d_code);
     dfree(c->d_flag[0]);
@@ -177,6 +194,7 @@ int ensure_ws(emv_ctx *c, int64_t chunks) {
     HIPCHK(c, hipMalloc((void **)&c->d_code, sizeof(uint32_t) * cells));
     HIPCHK(c, hipMalloc((void **)&c->d_flag[0], items));
     HIPCHK(c, hipMalloc((void **)&c->d_flag[1], items));
+    HIPCHK(c, hipMalloc(&c->d_M, sizeof(uint64_t) * cells));
     c->ws_chunks = chunks;
     return clear_flags(c);
 }
@@ -194,27 +212,39 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // (chunk, vertex), then flagged rounds, launched in groups between polls of
 // the "round has work" word, until a round flags nothing.  Rounds launched
 // after convergence return at once.
+template <typename DT, int DMAX>
+void launch_relax_round(emv_ctx *c, int64_t nitems, uint32_t pred_base, int dense, int r) {
+    const int grid = c->num_cu * 8;
+    hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
+                       (DT *)c->d_D, c->d_code, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
+                       pred_base, nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
+                       c->d_any, r, c->relax_nxcd);
+}
+
+template <typename DT>
+void relax_round(emv_ctx *c, int64_t nitems, uint32_t pred_base, int dense, int r) {
+    switch (c->ell_deg) {
+    case 4: launch_relax_round<DT, 4>(c, nitems, pred_base, dense, r); break;
+    case 8: launch_relax_round<DT, 8>(c, nitems, pred_base, dense, r); break;
+    case 16: launch_relax_round<DT, 16>(c, nitems, pred_base, dense, r); break;
+    default: launch_relax_round<DT, 0>(c, nitems, pred_base, dense, r); break;
+    }
+}
+
 template <typename DT>
 int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *rounds_out) {
-    const int grid = c->num_cu * 8;
-    DT *D = (DT *)c->d_D;
     const int64_t nitems = nchunks * c->n;
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_any, (int64_t)4,
                        0u);
     int r = 0;
-    hipLaunchKernelGGL((k_relax_v<DT>), dim3(grid), dim3(kBlock), 0, c->stream, D, c->d_code,
-                       c->np, (int)c->n, c->d_rp, c->d_adj, pred_base, nitems, 1,
-                       c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r, c->nxcd);
+    relax_round<DT>(c, nitems, pred_base, 1, r);
     HIPCHK(c, hipGetLastError());
     r = 1;
     int64_t rounds = 1;
     int group = 2;
     for (;;) {
         for (int g = 0; g < group; g++, r++) {
-            hipLaunchKernelGGL((k_relax_v<DT>), dim3(grid), dim3(kBlock), 0, c->stream, D,
-                               c->d_code, c->np, (int)c->n, c->d_rp, c->d_adj, pred_base, nitems,
-                               0, c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r,
-                               c->nxcd);
+            relax_round<DT>(c, nitems, pred_base, 0, r);
             rounds++;
         }
         HIPCHK(c, hipGetLastError());
@@ -241,6 +271,56 @@ struct SolveState {
     int dbits;
 };
 
+// Rows per merge wave: few rows per wave when the batch has few rows (the
+// top layers), so that every tile still spreads over many waves.
+int merge_rows_per_wave(int nx) {
+    if (nx >= 1024) return 8;
+    if (nx >= 256) return 4;
+    if (nx >= 64) return 2;
+    return 1;
+}
+
+template <typename T, typename P, typename DT, int XG>
+int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
+    const int nsub = (int)(nchunks * (kWave / XG));
+    const int64_t ntiles = c->np / kWave;
+    int64_t per;
+    const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
+    hipLaunchKernelGGL((k_merge_v<T, P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+                       c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, c->d_code, c->nxcd,
+                       per);
+    HIPCHK(c, hipGetLastError());
+    return EMV_OK;
+}
+
+template <typename T, typename P, typename DT>
+int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
+    const int grid_p = c->num_cu * 8;
+    const int64_t waves_x = (int64_t)(grid_p / c->nxcd) * (kBlock / kWave);
+    const int m = __builtin_popcount(c->h_xs0);
+    const int64_t W = (int64_t)nx * ((1ll << (m - 1)) - 1);
+    if (c->merge_persistent && W >= 8 * waves_x) {
+        // persistent per-XCD merge into the packed accumulators, then pack
+        const int64_t cells = (int64_t)nx * c->np;
+        const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+        hipLaunchKernelGGL(k_fill<P>, dim3(blocks), dim3(kBlock), 0, c->stream, (P *)c->d_M,
+                           cells, ~(P)0);
+        hipLaunchKernelGGL((k_merge_p<T, P>), dim3(grid_p), dim3(kBlock), 0, c->stream, f, c->np,
+                           c->d_xs, nx, k, c->np / kWave, (P *)c->d_M, c->nxcd);
+        const int64_t vtiles = (c->n + kWave - 1) / kWave;
+        hipLaunchKernelGGL((k_pack_v<P, DT>), dim3(nchunks * vtiles), dim3(kBlock), 0, c->stream,
+                           (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, c->d_code, vtiles);
+        HIPCHK(c, hipGetLastError());
+        return EMV_OK;
+    }
+    switch (merge_rows_per_wave(nx)) {
+    case 8: return launch_merge_xg<T, P, DT, 8>(c, f, nx, nchunks, k);
+    case 4: return launch_merge_xg<T, P, DT, 4>(c, f, nx, nchunks, k);
+    case 2: return launch_merge_xg<T, P, DT, 2>(c, f, nx, nchunks, k);
+    default: return launch_merge_xg<T, P, DT, 1>(c, f, nx, nchunks, k);
+    }
+}
+
 template <typename T, typename P, typename DT>
 int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *res,
                std::vector<hipEvent_t> &ev) {
@@ -263,6 +343,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             const int64_t nchunks = (nx + kWave - 1) / kWave;
             HIPCHK(c, hipMemcpyAsync(c->d_xs, xs.data() + off, sizeof(uint32_t) * nx,
                                      hipMemcpyHostToDevice, c->stream));
+            c->h_xs0 = xs[off];
             hipEvent_t e0, e1, e2;
             HIPCHK(c, hipEventCreate(&e0));
             ev.push_back(e0);
@@ -272,21 +353,15 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             ev.push_back(e2);
             HIPCHK(c, hipEventRecord(e0, c->stream));
             // merge -> staging
-            const int nsub = (int)(nchunks * (kWave / kXG));
-            const int64_t ntiles = np / kWave;
-            int64_t per;
-            int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
-            hipLaunchKernelGGL((k_merge_v<T, P, DT>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                               np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, c->d_code, c->nxcd,
-                               per);
-            HIPCHK(c, hipGetLastError());
+            CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k)));
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             // relax
             CHK(relax_chunks<DT>(c, nchunks, st.pred_base, &res->relax_rounds));
             // unpack
             const int64_t vtiles = (n + kWave - 1) / kWave;
-            grid = grid_for(nchunks * vtiles, c->nxcd, &per);
+            int64_t per;
+            const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
             hipLaunchKernelGGL((k_unpack_v<T, DT>), dim3(grid), dim3(kBlock), 0, c->stream,
                                (const DT *)c->d_D, c->d_code, np, c->d_xs, nx, (int)n, f, b,
                                c->nxcd, per, vtiles);
@@ -301,9 +376,11 @@ template <typename T>
 int run_layers_t(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *res,
                  std::vector<hipEvent_t> &ev) {
     if (st.pbits == 32) {
+        if (st.dbits == 16) return run_layers<T, uint32_t, uint16_t>(c, st, f, b, res, ev);
         if (st.dbits == 32) return run_layers<T, uint32_t, uint32_t>(c, st, f, b, res, ev);
         return run_layers<T, uint32_t, uint64_t>(c, st, f, b, res, ev);
     }
+    if (st.dbits == 16) return run_layers<T, uint64_t, uint16_t>(c, st, f, b, res, ev);
     if (st.dbits == 32) return run_layers<T, uint64_t, uint32_t>(c, st, f, b, res, ev);
     return run_layers<T, uint64_t, uint64_t>(c, st, f, b, res, ev);
 }
@@ -313,15 +390,37 @@ int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
     HIPCHK(c, hipSetDevice(c->device));
     invalidate_table(c);
     // the table and the workspace are sized by np: drop them when it changes
-    const int64_t np = round_up(n, 1024);
+    const int64_t np = row_stride(n);
     if (np != c->np) {
         free_table(c);
         free_ws(c);
     }
     dfree(c->d_rp);
     dfree(c->d_adj);
+    dfree(c->d_ell);
+    c->ell_deg = 0;
     c->n = 0;
     c->m = 0;
+    // padded fixed-degree copy when the maximum degree is small and the
+    // padding at most doubles the adjacency
+    int64_t maxdeg = 0;
+    for (int64_t u = 0; u < n; u++) maxdeg = std::max<int64_t>(maxdeg, rp[u + 1] - rp[u]);
+    int ell = 0;
+    for (int d : {4, 8, 16})
+        if (maxdeg <= d) {
+            ell = d;
+            break;
+        }
+    if (ell && (int64_t)ell * n > 2 * std::max<int64_t>(1, 2 * m)) ell = 0;
+    if (getenv("EMV_NO_ELL")) ell = 0;
+    if (ell) {
+        std::vector<Edge> e((size_t)(n * ell), Edge{-1, 0});
+        for (int64_t u = 0; u < n; u++)
+            for (int32_t i = rp[u]; i < rp[u + 1]; i++) e[(size_t)(u * ell + (i - rp[u]))] = adj[i];
+        HIPCHK(c, hipMalloc((void **)&c->d_ell, sizeof(Edge) * e.size()));
+        HIPCHK(c, hipMemcpy(c->d_ell, e.data(), sizeof(Edge) * e.size(), hipMemcpyHostToDevice));
+        c->ell_deg = ell;
+    }
     HIPCHK(c, hipMalloc((void **)&c->d_rp, sizeof(int32_t) * (n + 1)));
     HIPCHK(c, hipMalloc((void **)&c->d_adj, sizeof(Edge) * std::max<size_t>(1, adj.size())));
     HIPCHK(c, hipMemcpy(c->d_rp, rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
@@ -402,6 +501,10 @@ extern "C" int emv_open(emv_ctx **out, int device) {
         c->num_cu = prop.multiProcessorCount;
         // MI355X: 8 XCDs of 32 CUs; workgroups are dealt round-robin over XCDs.
         c->nxcd = (c->num_cu % 32 == 0 && c->num_cu >= 64) ? c->num_cu / 32 : 1;
+        const char *rx = getenv("EMV_RELAX_XCD");
+        c->relax_nxcd = (rx && atoi(rx) > 0) ? c->nxcd : 1;
+        const char *mp = getenv("EMV_MERGE_PERSISTENT");
+        c->merge_persistent = (mp && atoi(mp) > 0) ? 1 : 0;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = EMV_ERR_HIP;
             break;
@@ -431,6 +534,7 @@ extern "C" void emv_close(emv_ctx *c) {
     free_ws(c);
     dfree(c->d_rp);
     dfree(c->d_adj);
+    dfree(c->d_ell);
     dfree(c->d_any);
     dfree(c->d_rowmax);
     dfree(c->d_misc);
@@ -516,11 +620,11 @@ extern "C" int emv_load_graph(emv_ctx *c, int64_t n, int64_t m, const int64_t *p
 
 extern "C" int64_t emv_solve_footprint(const emv_ctx *c, int64_t k, uint32_t flags) {
     if (!c || c->n <= 0 || k < 1 || k > 30) return -1;
-    const int64_t np = round_up(c->n, 1024);
+    const int64_t np = row_stride(c->n);
     const int64_t S2 = (int64_t)1 << k;
     const int tb = (flags & EMV_FORCE_I64) ? 8 : 4;
     const int64_t chunks = std::max<int64_t>(1, (choose(k, k / 2) + kWave - 1) / kWave);
-    const int64_t ws = chunks * np * (kWave * (8 + 4) + 12);
+    const int64_t ws = chunks * np * (kWave * (8 + 4 + 8) + 2);
     return S2 * np * (tb + 4) + ws;
 }
 
@@ -604,15 +708,22 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         }
         B += rowmax[t];
     }
-    // merge sums reach 2B; labels reach 2B + max w before a relaxation
-    const unsigned long long top = 2 * B + (unsigned long long)c->maxw;
-    if (bits_for(2 * B) + k > 64 || top >= (1ull << 62)) {
+    // Every cell obeys f[X][v] <= sum over t in X of dist(kk[t], v) <= B, so
+    // a merge sum f[Y][v] + f[X\Y][v] and every stored label are at most B,
+    // and a relaxation candidate label + w is at most B + max w.
+    const unsigned long long top = B + (unsigned long long)c->maxw;
+    if (bits_for(B) + k > 64 || top >= (1ull << 62)) {
         set_err(c, "distance bound %llu exceeds the 64-bit packing", B);
         return EMV_ERR_RANGE;
     }
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
-    st.pbits = (st.tbytes == 4 && bits_for(2 * B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
-    st.dbits = (top < 0xFFFFFFFFull && !(flags & EMV_FORCE_P64)) ? 32 : 64;
+    st.pbits = (st.tbytes == 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
+    if (flags & EMV_FORCE_P64)
+        st.dbits = 64;
+    else if (top < 0xFFFFull && !(flags & EMV_WIDE_LABELS))
+        st.dbits = 16;
+    else
+        st.dbits = (top < 0xFFFFFFFFull) ? 32 : 64;
 
     // ---- table allocation + fill
     const int64_t cells = st.S2 * np;
@@ -722,7 +833,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->n = n;
     res->table_bytes = st.tbytes;
     res->merge_bits = st.pbits;
-    res->code_bits = st.dbits;
+    res->label_bits = st.dbits;
     res->dp_cells = st.S2 * n;
     res->zero_ms = ev_ms(e_zero0, e_zero1);
     res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;
@@ -740,7 +851,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->relax_alg_bytes = ((double)st.S2 - 1.0) * (cb * dn + cb * 2 * dm + (cb + ab) * dn) +
                            (double)k * (4.0 * (dn + 1) + 16.0 * dm);
     res->device_bytes = (int64_t)(c->f_cap + c->b_cap) +
-                        c->ws_chunks * np * (int64_t)(kWave * 12 + 12);
+                        c->ws_chunks * np * (int64_t)(kWave * 20 + 2);
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     if (!(flags & EMV_KEEP_TABLE)) invalidate_table(c);

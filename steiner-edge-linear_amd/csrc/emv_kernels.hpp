@@ -1,7 +1,7 @@
 // emv_kernels.hpp -- gfx950 device code of the EMV Steiner DP.
 //
-// Table in HBM (subset-major as reader-el.c:690-692, rows padded to np, a
-// multiple of 1024; padding cells stay 0):
+// Table in HBM (subset-major as reader-el.c:690-692, rows padded to np, an
+// odd number of 64-vertex tiles; padding cells stay 0):
 //   f[X * np + v]   cost, int32 or int64
 //   b[X * np + v]   int32 record code: Y for a merge split (Y == X stops the
 //                   traceback), 2^k + u for an edge from predecessor u
@@ -33,7 +33,6 @@ struct Edge {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
-constexpr int kXG = 8;  // subset rows per merge wave item
 
 // Work-item mapping that keeps neighbouring items on one XCD.  Workgroups
 // are dealt to the XCDs round-robin (workgroup b runs on XCD b % nxcd), so
@@ -79,82 +78,168 @@ __global__ void k_seed_v(DT *__restrict__ D, uint32_t *__restrict__ CODE,
 // any[(r+2)&3] for the round after next.  Pulling the full adjacency after
 // every neighbour change makes the final (label, code) of each cell the exact
 // lexicographic minimum, independent of the order waves run in.
+//
+// Adjacency: DMAX > 0 uses the padded fixed-degree copy ell[v * DMAX + j]
+// (missing neighbours have v = -1).  The 64 items of a flag block load their
+// rows into registers with one coalesced load, so an item's neighbour labels
+// are gathered with no dependent load in front, and two items are processed
+// together to keep 2 * DMAX gathers in flight.  DMAX == 0 walks the CSR.
 template <typename DT>
+__device__ __forceinline__ void relax_fold(DT du, int u, int w, uint32_t pred_base, DT INF,
+                                           DT &best, uint32_t &bc) {
+    if (u >= 0 && du != INF) {
+        const DT nd = du + (DT)w;
+        const uint32_t nc = pred_base + (uint32_t)u;
+        if (nd < best || (nd == best && nc < bc)) {
+            best = nd;
+            bc = nc;
+        }
+    }
+}
+
+template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_relax_v(
     DT *__restrict__ D, uint32_t *__restrict__ CODE, int64_t np, int n,
-    const int32_t *__restrict__ rp, const Edge *__restrict__ adj, uint32_t pred_base,
-    int64_t nitems, int dense, uint8_t *__restrict__ Fcur, uint8_t *__restrict__ Fnext,
-    uint32_t *__restrict__ any, int r, int nxcd) {
+    const int32_t *__restrict__ rp, const Edge *__restrict__ adj, const Edge *__restrict__ ell,
+    uint32_t pred_base, int64_t nitems, int dense, uint8_t *__restrict__ Fcur,
+    uint8_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r, int nxcd) {
     if (blockIdx.x == 0 && threadIdx.x == 0) any[(r + 2) & 3] = 0;
     if (!dense && any[r & 3] == 0) return;  // nothing flagged for this round
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
+    // nxcd > 1: each XCD sweeps its own contiguous range of items; nxcd == 1:
+    // the whole grid sweeps the items in order.
     const int64_t waves_per_block = kBlock / kWave;
     const int64_t per = gridDim.x / nxcd;
     const int64_t b = blockIdx.x;
     const int64_t wave0 = ((b % nxcd) * per + b / nxcd) * waves_per_block + wid;
     const int64_t wstride = (int64_t)gridDim.x * waves_per_block;
     const int64_t nblk = (nitems + kWave - 1) / kWave;
-    const DT INF = ~(DT)0;
+    const DT INF = (DT)~(DT)0;
     bool raised = false;
     for (int64_t blk = wave0; blk < nblk; blk += wstride) {
         const int64_t base = blk * kWave;
-        // flags of the 64 items base .. base+63
-        int64_t mi = base + lane;
-        int64_t mc = mi / n;
-        int64_t mf = mc * np + (mi - mc * n);
+        const int64_t mi = base + lane;
+        const int64_t mc = mi / n;
+        const int mv = (int)(mi - mc * n);
+        const int64_t mf = mc * np + mv;
         bool fl = false;
         if (mi < nitems) fl = dense ? true : (Fcur[mf] != 0);
         uint64_t act = __ballot(fl);
         if (!dense && fl) Fcur[mf] = 0;
-        while (act) {
-            const int t = __ffsll((unsigned long long)act) - 1;
-            act &= act - 1;
-            const int64_t it = base + t;
-            const int64_t c = it / n;
-            const int v = (int)(it - c * n);
-            DT *Dc = D + c * np * kWave;
-            uint32_t *Cc = CODE + c * np * kWave;
-            const int64_t idx = (int64_t)v * kWave + lane;
-            const DT cur = Dc[idx];
-            const uint32_t cc = Cc[idx];
-            DT best = cur;
-            uint32_t bc = cc;
-            const int e0 = rp[v], e1 = rp[v + 1];
-            for (int eb = e0; eb < e1; eb += kWave) {
-                const int cnt = min(kWave, e1 - eb);
-                Edge my = {0, 0};
-                if (lane < cnt) my = adj[eb + lane];
-                for (int t0 = 0; t0 < cnt; t0 += 4) {
-                    // four neighbours per step: their loads issue together
-                    DT du[4];
-                    int uu[4], ww[4];
+        if (act == 0) continue;
+        if constexpr (DMAX > 0) {
+            Edge e[DMAX];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int tt = min(t0 + q, cnt - 1);
-                        uu[q] = __builtin_amdgcn_readlane(my.v, tt);
-                        ww[q] = __builtin_amdgcn_readlane(my.w, tt);
-                        du[q] = Dc[(int64_t)uu[q] * kWave + lane];
-                    }
+            for (int j = 0; j < DMAX; j++) e[j] = Edge{-1, 0};
+            if (mi < nitems) {
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        if (t0 + q < cnt && du[q] != INF) {
-                            const DT nd = du[q] + (DT)ww[q];
-                            const uint32_t nc = pred_base + (uint32_t)uu[q];
-                            if (nd < best || (nd == best && nc < bc)) {
-                                best = nd;
-                                bc = nc;
-                            }
+                for (int j = 0; j < DMAX; j++) e[j] = ell[(int64_t)mv * DMAX + j];
+            }
+            while (act) {
+                const int t1 = __ffsll((unsigned long long)act) - 1;
+                act &= act - 1;
+                const bool two = act != 0;
+                const int t2 = two ? __ffsll((unsigned long long)act) - 1 : t1;
+                if (two) act &= act - 1;
+                const int64_t ca = (base + t1) / n, cb = (base + t2) / n;
+                const int va = (int)(base + t1 - ca * n), vb = (int)(base + t2 - cb * n);
+                DT *Da = D + ca * np * kWave;
+                DT *Db = D + cb * np * kWave;
+                const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
+                const DT cura = Da[ia];
+                const DT curb = two ? Db[ib] : INF;
+                const uint32_t cca = CODE[ca * np * kWave + ia];
+                const uint32_t ccb = two ? CODE[cb * np * kWave + ib] : 0u;
+                int ua[DMAX], wa[DMAX], ub[DMAX], wb[DMAX];
+                DT da[DMAX], db[DMAX];
+#pragma unroll
+                for (int j = 0; j < DMAX; j++) {
+                    ua[j] = __builtin_amdgcn_readlane(e[j].v, t1);
+                    wa[j] = __builtin_amdgcn_readlane(e[j].w, t1);
+                    ub[j] = __builtin_amdgcn_readlane(e[j].v, t2);
+                    wb[j] = __builtin_amdgcn_readlane(e[j].w, t2);
+                }
+#pragma unroll
+                for (int j = 0; j < DMAX; j++) {
+                    da[j] = ua[j] >= 0 ? Da[(int64_t)ua[j] * kWave + lane] : INF;
+                    db[j] = (two && ub[j] >= 0) ? Db[(int64_t)ub[j] * kWave + lane] : INF;
+                }
+                DT besta = cura, bestb = curb;
+                uint32_t bca = cca, bcb = ccb;
+#pragma unroll
+                for (int j = 0; j < DMAX; j++) {
+                    relax_fold(da[j], ua[j], wa[j], pred_base, INF, besta, bca);
+                    relax_fold(db[j], ub[j], wb[j], pred_base, INF, bestb, bcb);
+                }
+                const bool chga = besta < cura, chgb = two && bestb < curb;
+                if (chga) Da[ia] = besta;
+                if (bca != cca) CODE[ca * np * kWave + ia] = bca;
+                if (chgb) Db[ib] = bestb;
+                if (two && bcb != ccb) CODE[cb * np * kWave + ib] = bcb;
+                const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
+                if (anya || anyb) {
+                    // lane j < DMAX flags neighbour j
+                    int na = -1, nb = -1;
+#pragma unroll
+                    for (int j = 0; j < DMAX; j++) {
+                        if (lane == j) {
+                            na = ua[j];
+                            nb = ub[j];
                         }
                     }
+                    if (anya && na >= 0) Fnext[ca * np + na] = 1;
+                    if (anyb && nb >= 0) Fnext[cb * np + nb] = 1;
+                    raised = true;
                 }
             }
-            const bool dchg = best < cur;
-            if (dchg) Dc[idx] = best;
-            if (bc != cc) Cc[idx] = bc;
-            if (__ballot(dchg) != 0) {
-                for (int e = e0 + lane; e < e1; e += kWave) Fnext[c * np + adj[e].v] = 1;
-                raised = true;
+        } else {
+            int me0 = 0, me1 = 0;
+            if (mi < nitems) {
+                me0 = rp[mv];
+                me1 = rp[mv + 1];
+            }
+            while (act) {
+                const int t = __ffsll((unsigned long long)act) - 1;
+                act &= act - 1;
+                const int64_t c = (base + t) / n;
+                const int v = (int)(base + t - c * n);
+                DT *Dc = D + c * np * kWave;
+                uint32_t *Cc = CODE + c * np * kWave;
+                const int64_t idx = (int64_t)v * kWave + lane;
+                const DT cur = Dc[idx];
+                const uint32_t cc = Cc[idx];
+                DT best = cur;
+                uint32_t bc = cc;
+                const int e0 = __builtin_amdgcn_readlane(me0, t);
+                const int e1 = __builtin_amdgcn_readlane(me1, t);
+                for (int eb = e0; eb < e1; eb += kWave) {
+                    const int cnt = min(kWave, e1 - eb);
+                    Edge my = {0, 0};
+                    if (lane < cnt) my = adj[eb + lane];
+                    for (int t0 = 0; t0 < cnt; t0 += 4) {
+                        // four neighbours per step: their loads issue together
+                        DT du[4];
+                        int uu[4], ww[4];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const int tt = min(t0 + q, cnt - 1);
+                            uu[q] = __builtin_amdgcn_readlane(my.v, tt);
+                            ww[q] = __builtin_amdgcn_readlane(my.w, tt);
+                            du[q] = Dc[(int64_t)uu[q] * kWave + lane];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            if (t0 + q < cnt) relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
+                    }
+                }
+                const bool dchg = best < cur;
+                if (dchg) Dc[idx] = best;
+                if (bc != cc) Cc[idx] = bc;
+                if (__ballot(dchg) != 0) {
+                    for (int e = e0 + lane; e < e1; e += kWave) Fnext[c * np + adj[e].v] = 1;
+                    raised = true;
+                }
             }
         }
     }
@@ -168,14 +253,48 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
 // accumulator packs (sum << k) | Y, so an unsigned min keeps the smallest
 // sum and, among equal sums, the smallest Y -- the reference's first strict
 // minimum over increasing Y (strict '<' at 2078).
-//
+template <typename T, typename P>
+__device__ __forceinline__ P merge_one(const T *__restrict__ fv, int64_t np, uint32_t X, int k) {
+    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+    const uint32_t npairs = (1u << __popc(Xp)) - 1u;
+    P acc = ~(P)0;
+    uint32_t Y = Xp;  // subsets of Xp in decreasing order: Y <- (Y - 1) & Xp
+    uint32_t i = 0;
+    // eight pairs per trip, sixteen row loads in flight, no guards
+    for (; i + 8 <= npairs; i += 8) {
+        uint32_t Ys[8];
+        Ys[0] = Y;
+#pragma unroll
+        for (int q = 1; q < 8; q++) Ys[q] = (Ys[q - 1] - 1) & Xp;
+        T a[8], c[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            a[q] = fv[(int64_t)Ys[q] * np];
+            c[q] = fv[(int64_t)(X ^ Ys[q]) * np];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const P s = (((P)a[q] + (P)c[q]) << k) | (P)Ys[q];
+            acc = s < acc ? s : acc;
+        }
+        Y = (Ys[7] - 1) & Xp;
+    }
+    for (; i < npairs; i++) {
+        const P s = (((P)fv[(int64_t)Y * np] + (P)fv[(int64_t)(X ^ Y) * np]) << k) | (P)Y;
+        acc = s < acc ? s : acc;
+        Y = (Y - 1) & Xp;
+    }
+    return acc;
+}
+
 // Work item = one wave: 64 consecutive vertices (one per lane, 256 B
-// coalesced row loads) x kXG consecutive rows of the layer batch.  Items are
+// coalesced row loads) x XG consecutive rows of the layer batch.  Items are
 // tile-major and dealt to XCDs in contiguous ranges, so the waves of an XCD
 // sweep the rows of one vertex tile together and the tile's lower-layer rows
 // stay in that XCD's L2.  Results go straight to the vertex-major staging:
-// each lane writes its kXG labels (and codes) as one contiguous run.
-template <typename T, typename P, typename DT>
+// each lane writes its XG labels (and codes) as one contiguous run.  Rows
+// past nx (the tail of the last chunk) are written as INF.
+template <typename T, typename P, typename DT, int XG>
 __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int64_t np,
                                                     const uint32_t *__restrict__ xs, int nx,
                                                     int nsub, int64_t ntiles, int k,
@@ -191,49 +310,138 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
     if (tile >= ntiles) return;
     const int64_t v = tile * kWave + lane;
     const T *fv = f + v;
-    const DT INF = ~(DT)0;
+    const DT INF = (DT)~(DT)0;
     const P ymask = ((P)1 << k) - 1;
-    DT dv[kXG];
-    uint32_t yv[kXG];
+    DT dv[XG];
+    uint32_t yv[XG];
 #pragma unroll
-    for (int jx = 0; jx < kXG; jx++) {
-        const int xi = sub * kXG + jx;
+    for (int jx = 0; jx < XG; jx++) {
+        const int xi = sub * XG + jx;
         dv[jx] = INF;
         yv[jx] = 0;
         if (xi < nx) {
-            const uint32_t X = xs[xi];
-            const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
-            P acc = ~(P)0;
-            uint32_t Y = Xp;
-            while (Y != 0) {
-                // four pairs per trip keep eight loads in flight
-                const uint32_t Y1 = (Y - 1) & Xp;
-                const uint32_t Y2 = Y1 ? ((Y1 - 1) & Xp) : 0u;
-                const uint32_t Y3 = Y2 ? ((Y2 - 1) & Xp) : 0u;
-                const T a0 = fv[(int64_t)Y * np], c0 = fv[(int64_t)(X ^ Y) * np];
-                T a1 = 0, c1 = 0, a2 = 0, c2 = 0, a3 = 0, c3 = 0;
-                if (Y1) { a1 = fv[(int64_t)Y1 * np]; c1 = fv[(int64_t)(X ^ Y1) * np]; }
-                if (Y2) { a2 = fv[(int64_t)Y2 * np]; c2 = fv[(int64_t)(X ^ Y2) * np]; }
-                if (Y3) { a3 = fv[(int64_t)Y3 * np]; c3 = fv[(int64_t)(X ^ Y3) * np]; }
-                P s = (((P)a0 + (P)c0) << k) | (P)Y;
-                acc = s < acc ? s : acc;
-                if (Y1) { s = (((P)a1 + (P)c1) << k) | (P)Y1; acc = s < acc ? s : acc; }
-                if (Y2) { s = (((P)a2 + (P)c2) << k) | (P)Y2; acc = s < acc ? s : acc; }
-                if (Y3) { s = (((P)a3 + (P)c3) << k) | (P)Y3; acc = s < acc ? s : acc; }
-                Y = Y3 ? ((Y3 - 1) & Xp) : 0u;
-            }
+            const P acc = merge_one<T, P>(fv, np, xs[xi], k);
             dv[jx] = (DT)(acc >> k);
             yv[jx] = (uint32_t)(acc & ymask);
         }
     }
-    const int chunk = sub / (kWave / kXG);
-    const int loff = (sub % (kWave / kXG)) * kXG;
+    const int chunk = sub / (kWave / XG);
+    const int loff = (sub % (kWave / XG)) * XG;
     DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
     uint32_t *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
 #pragma unroll
-    for (int jx = 0; jx < kXG; jx++) {
+    for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
         cd[jx] = yv[jx];
+    }
+}
+
+// Persistent merge: each XCD owns the vertex tiles tile = xcd, xcd + nxcd, ...
+// and all waves resident on that XCD split one tile's iteration space
+// (row xi of the batch, pair index p) evenly, so the XCD reads the lower
+// layers of a single 64-vertex tile at a time -- at most 2^k rows of 256 B,
+// which its 4 MB L2 holds.  A wave's slice covers whole or partial rows; each
+// finished row segment is folded into the packed accumulator M[xi][v] with an
+// unsigned atomicMin (the min is order-independent, so the split changes
+// nothing).  M must start at all ones.  Pairs of a row are enumerated in
+// increasing Y: pair p is Y = deposit(p + 1, Xp) and the next subset of Xp is
+// ((Y | ~Xp) + 1) & Xp.
+__device__ __forceinline__ uint32_t deposit_bits(uint32_t x, uint32_t mask) {
+    uint32_t r = 0;
+    for (uint32_t bb = 1; mask != 0; bb <<= 1) {
+        const uint32_t low = mask & (~mask + 1u);
+        if (x & bb) r |= low;
+        mask ^= low;
+    }
+    return r;
+}
+
+template <typename T, typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_p(const T *__restrict__ f, int64_t np,
+                                                    const uint32_t *__restrict__ xs, int nx,
+                                                    int k, int64_t ntiles, P *__restrict__ M,
+                                                    int nxcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int xcd = blockIdx.x % nxcd;
+    const int64_t waves_x = (int64_t)(gridDim.x / nxcd) * (kBlock / kWave);
+    const int64_t w = (int64_t)(blockIdx.x / nxcd) * (kBlock / kWave) + wid;
+    const uint32_t npairs = (1u << (__popc(xs[0]) - 1)) - 1u;
+    const int64_t W = (int64_t)nx * npairs;
+    const int64_t lo = W * w / waves_x, hi = W * (w + 1) / waves_x;
+    if (lo >= hi) return;
+    for (int64_t tile = xcd; tile < ntiles; tile += nxcd) {
+        const int64_t v = tile * kWave + lane;
+        const T *fv = f + v;
+        int64_t i = lo;
+        while (i < hi) {
+            const int xi = (int)(i / npairs);
+            const uint32_t p = (uint32_t)(i - (int64_t)xi * npairs);
+            const uint32_t cnt = (uint32_t)min((int64_t)(npairs - p), hi - i);
+            const uint32_t X = xs[xi];
+            const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+            const uint32_t nXp = ~Xp;
+            uint32_t Y = deposit_bits(p + 1u, Xp);
+            P acc = ~(P)0;
+            uint32_t j = 0;
+            for (; j + 8 <= cnt; j += 8) {
+                uint32_t Ys[8];
+                Ys[0] = Y;
+#pragma unroll
+                for (int q = 1; q < 8; q++) Ys[q] = ((Ys[q - 1] | nXp) + 1u) & Xp;
+                T a[8], c[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    a[q] = fv[(int64_t)Ys[q] * np];
+                    c[q] = fv[(int64_t)(X ^ Ys[q]) * np];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const P s = (((P)a[q] + (P)c[q]) << k) | (P)Ys[q];
+                    acc = s < acc ? s : acc;
+                }
+                Y = ((Ys[7] | nXp) + 1u) & Xp;
+            }
+            for (; j < cnt; j++) {
+                const P s = (((P)fv[(int64_t)Y * np] + (P)fv[(int64_t)(X ^ Y) * np]) << k) | (P)Y;
+                acc = s < acc ? s : acc;
+                Y = ((Y | nXp) + 1u) & Xp;
+            }
+            atomicMin(&M[(int64_t)xi * np + v], acc);
+            i += cnt;
+        }
+    }
+}
+
+// Packed accumulators M[xi][v] -> vertex-major staging (label, Y) through an
+// LDS transpose of 64 rows x 64 vertices per workgroup; rows past nx are INF.
+template <typename P, typename DT>
+__global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int64_t np, int nx,
+                                                   int k, DT *__restrict__ D,
+                                                   uint32_t *__restrict__ CODE, int64_t vtiles) {
+    __shared__ P sm[kWave][kWave + 1];
+    const int64_t chunk = blockIdx.x / vtiles;
+    const int64_t vt = blockIdx.x - chunk * vtiles;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int grp = t / kWave;
+    const int64_t v0 = vt * kWave;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int xl = grp + r * (kBlock / kWave);
+        const int64_t xi = chunk * kWave + xl;
+        sm[xl][lane] = (xi < nx) ? M[xi * np + v0 + lane] : ~(P)0;
+    }
+    __syncthreads();
+    const P ymask = ((P)1 << k) - 1;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int vv = grp + r * (kBlock / kWave);
+        const P a = sm[lane][vv];
+        const int64_t idx = (chunk * np + v0 + vv) * kWave + lane;
+        const bool real = (chunk * kWave + lane) < nx;
+        D[idx] = real ? (DT)(a >> k) : ~(DT)0;
+        CODE[idx] = real ? (uint32_t)(a & ymask) : 0u;
     }
 }
 

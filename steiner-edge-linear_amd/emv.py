@@ -20,6 +20,7 @@ LIST = 0x1
 FORCE_I64 = 0x2
 KEEP_TABLE = 0x4
 FORCE_P64 = 0x8
+WIDE_LABELS = 0x10
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",
           -5: "ERR_DISCONNECTED", -6: "ERR_RANGE", -7: "ERR_NODEV", -8: "ERR_CAPACITY",
@@ -31,7 +32,7 @@ _i64p = ctypes.POINTER(ctypes.c_int64)
 class EmvResult(ctypes.Structure):
     _fields_ = [
         ("cost", ctypes.c_int64), ("k", ctypes.c_int32), ("table_bytes", ctypes.c_int32),
-        ("merge_bits", ctypes.c_int32), ("code_bits", ctypes.c_int32), ("n", ctypes.c_int64),
+        ("merge_bits", ctypes.c_int32), ("label_bits", ctypes.c_int32), ("n", ctypes.c_int64),
         ("dp_cells", ctypes.c_int64), ("zero_ms", ctypes.c_double),
         ("singleton_ms", ctypes.c_double), ("merge_ms", ctypes.c_double),
         ("relax_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double),
@@ -89,6 +90,7 @@ class Solution:
         self.k = r.k
         self.table_bytes = r.table_bytes
         self.merge_bits = r.merge_bits
+        self.label_bits = r.label_bits
         self.dp_cells = r.dp_cells
         self.singleton_ms = r.singleton_ms
         self.merge_ms = r.merge_ms

@@ -4,7 +4,7 @@ Test/measurement infrastructure: reads the SQLite output of
 `rocprofv3 --kernel-trace --stats -d <dir> -o <name> -- <cmd>` and prints a
 markdown table (calls, total ms, average/min/max us, share of GPU time).
 
-    python oracle/prof_summary.py <results.db> [--title T] [--cmd C]
+    python steiner-edge-linear_amd/profsum.py <results.db> [--title T] [--cmd C]
 """
 import argparse
 import sqlite3

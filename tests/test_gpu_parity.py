@@ -56,20 +56,26 @@ def test_regular_1000_full_table(gpu_ctx):
     np.testing.assert_array_equal(f, gf)
     np.testing.assert_array_equal(kinds(code, inst.k), kinds(gcode, inst.k))
     assert sol.cost == META[CFG1]["ref_cost"]
-    assert sol.table_bytes == 4 and sol.merge_bits == 32
+    assert sol.table_bytes == 4 and sol.merge_bits == 32 and sol.label_bits == 16
 
 
-@pytest.mark.parametrize("flags", [emv.FORCE_P64, emv.FORCE_I64, emv.FORCE_I64 | emv.FORCE_P64])
+@pytest.mark.parametrize("flags", [emv.WIDE_LABELS, emv.FORCE_P64, emv.FORCE_I64,
+                                   emv.FORCE_I64 | emv.FORCE_P64])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
 def test_wide_paths_bit_exact(gpu_ctx, name, flags):
+    """Every width combination (16/32/64-bit labels, 32/64-bit merge
+    accumulators, int32/int64 table) gives the reference's table."""
     inst = load_instance(name)
     print(f"case {name} flags={flags}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst, flags)
     assert digest(f) == META[name]["f_sha256"]
     assert digest(kinds(code, inst.k)) == META[name]["kind_sha256"]
-    assert sol.merge_bits == 64
+    if flags & emv.FORCE_P64:
+        assert sol.merge_bits == 64 and sol.label_bits == 64
+    if flags == emv.WIDE_LABELS:
+        assert sol.label_bits == 32
     if flags & emv.FORCE_I64:
-        assert sol.table_bytes == 8
+        assert sol.table_bytes == 8 and sol.merge_bits == 64
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -277,3 +283,17 @@ def test_cli_group_reduction(tmp_path):
     verts = {v for p in pairs for v in p}
     for row in grp:
         assert verts & set(int(v) for v in row)
+
+
+def test_csr_and_ell_paths_agree(gpu_ctx, monkeypatch):
+    """The padded fixed-degree adjacency and the CSR walk give the same table."""
+    n, d, k = 3000, 6, 7
+    e = emv.regular_graph(n, d, 1000, 77)
+    kk = emv.terminals(n, k, 77)
+    inst = Instance(n, e, kk)
+    _, f1, c1 = solve_full(gpu_ctx, inst)
+    monkeypatch.setenv("EMV_NO_ELL", "1")
+    _, f2, c2 = solve_full(gpu_ctx, inst)
+    monkeypatch.delenv("EMV_NO_ELL")
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(c1, c2)
