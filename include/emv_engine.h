@@ -1,0 +1,109 @@
+/*
+ * emv_engine.h -- C ABI of the multi-instance EMV engine: batched solves of
+ * many independent instances and a continuous-batching query server on a
+ * resident graph.  Plain C types only.
+ *
+ * The reference solves one instance per process (reader-el.c:2355-2536,
+ * experiments/report.py:693 spawns one reader-el per instance) and has no
+ * batching or serving interface.  These entry points have no counterpart
+ * there: they run many reference-equivalent solves (every DP cell and cost
+ * identical to erickson_monma_veinott, reader-el.c:2141) in shared launches.
+ *
+ * Conventions as in emv_hip.h: EMV_OK or a negative EMV_ERR_* status,
+ * emv_engine_last_error() for the text, 0-based vertex ids, inputs owned by
+ * the caller.  Engine instances use int32 tables and 32-bit labels; an
+ * instance whose worst-case cost k*(n-1)*max w reaches 2^31 is refused with
+ * EMV_ERR_RANGE (emv_solve handles it).  k must be in [2, 30].
+ */
+#ifndef EMV_ENGINE_H
+#define EMV_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct emv_engine emv_engine;
+
+/* One batched instance: its own graph (edge list as graph_t.edges,
+ * reader-el.c:358-373) and terminals. */
+typedef struct {
+    int64_t n;
+    int64_t m;
+    const int64_t *edges; /* 3*m: u, v, w (0-based) */
+    int64_t k;
+    const int64_t *kk;
+} emv_instance;
+
+/* Tunables (emv_engine_default_params fills the defaults). */
+typedef struct {
+    int32_t relax_blocks_per_cu;  /* relaxation grid = CUs * this (1..32), default 8 */
+    int32_t merge_rows_per_wave;  /* 1, 2, 4, 8, or 0 = by layer size */
+    int32_t use_graph;            /* batch: replay steps 2..kmax from a hipGraph */
+    int32_t graph_round_margin;   /* rounds added to each replayed step's recorded count
+                                     (negative values force short replays, for tests) */
+} emv_engine_params;
+
+typedef struct {
+    double total_ms;
+    int64_t steps;
+    int64_t relax_rounds;
+    int64_t dp_cells;       /* sum over instances of 2^k * n */
+    int32_t graph_used;     /* the solve was a graph replay */
+    int32_t graph_fallback; /* a replay ran out of rounds and the solve ran eagerly */
+} emv_batch_stats;
+
+typedef struct {
+    int32_t state;      /* 0 queued, 1 running, 2 done */
+    int32_t status;     /* EMV_OK or the error that ended the request */
+    int64_t cost;       /* f[C][q] once done */
+    int32_t k;
+    int32_t stage;      /* next DP stage: 1 singletons, m >= 2 layer m */
+    double queue_ms;    /* submit -> admission */
+    double latency_ms;  /* submit -> done */
+    int64_t npairs;     /* traceback pairs held (EMV_LIST) */
+} emv_request_info;
+
+int emv_engine_default_params(emv_engine_params *p);
+int emv_engine_set_params(emv_engine *e, const emv_engine_params *p);
+const char *emv_engine_last_error(const emv_engine *e);
+void emv_engine_destroy(emv_engine *e);
+
+/* ---- batch: instances admitted together, stepped in lockstep ---------- */
+int emv_batch_create(int device, int64_t ninst, const emv_instance *inst, emv_engine **out);
+/* Solves every instance; costs[i] for instance i.  flags: EMV_LIST,
+ * EMV_KEEP_TABLE (emv_hip.h).  The first solve runs eagerly and records the
+ * relaxation rounds each step needed; later solves replay a hipGraph when
+ * use_graph is set. */
+int emv_batch_solve(emv_engine *e, uint32_t flags, int64_t *costs, emv_batch_stats *st);
+
+/* ---- server: one resident graph, requests admitted as memory frees ---- */
+/* arena_bytes: device memory for request tables (0 = 60% of free memory);
+ * max_inflight: requests that may run at once. */
+int emv_server_create(int device, int64_t n, int64_t m, const int64_t *edges,
+                      int64_t arena_bytes, int max_inflight, emv_engine **out);
+/* Queues a terminal-set query.  EMV_ERR_ADMISSION if its table footprint
+ * 2 * 2^k * np * 4 bytes can never fit the arena. */
+int emv_server_submit(emv_engine *e, int64_t k, const int64_t *kk, uint32_t flags,
+                      int64_t *ticket);
+/* Admits queued requests that fit (FIFO), then advances every running
+ * request by one DP stage in shared launches. */
+int emv_server_step(emv_engine *e, int64_t *running, int64_t *queued);
+int emv_server_poll(emv_engine *e, int64_t ticket, emv_request_info *info);
+int64_t emv_server_free_bytes(const emv_engine *e);
+int64_t emv_request_footprint(const emv_engine *e, int64_t k);
+
+/* ---- results of batch instances (ticket = index) and server requests --- */
+int emv_engine_read_rows(emv_engine *e, int64_t ticket, int64_t X0, int64_t nrows,
+                         int64_t *f_out, int64_t *code_out);
+int emv_engine_pairs(emv_engine *e, int64_t ticket, int64_t *pairs, int64_t cap,
+                     int64_t *npairs);
+/* Frees a finished request's table kept with EMV_KEEP_TABLE. */
+int emv_engine_release(emv_engine *e, int64_t ticket);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMV_ENGINE_H */

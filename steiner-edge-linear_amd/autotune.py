@@ -1,0 +1,92 @@
+"""Autotuner of the multi-instance engine.
+
+For a batch it times graph replays of the whole solve over a small grid of
+engine shapes -- relaxation workgroups per CU (the relaxation grid) and merge
+rows per wave (how many subset rows one wave merges for its 64-vertex tile)
+-- keeps the fastest, and caches the choice in JSON keyed by the device's CU
+count and a signature of the batch (its instances' n, m and k).
+
+    params, table = tune_batch(batch, instances)   # sets the params on batch
+"""
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+CANDIDATES = [(rb, rows) for rb in (4, 8, 16) for rows in (0, 1, 2)]
+
+
+def cache_path():
+    p = os.environ.get("EMV_TUNE_CACHE")
+    if p:
+        return p
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "emv-mi355x", "tune.json")
+
+
+def signature(instances, num_cu):
+    h = hashlib.sha1()
+    h.update(str(num_cu).encode())
+    for n, e, kk in instances:
+        h.update(f"{n},{len(e)},{len(kk)};".encode())
+    return h.hexdigest()[:16]
+
+
+def _load(path):
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, data):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as fh:
+            json.dump(data, fh, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def tune_batch(batch, instances, num_cu=256, reps=2, use_cache=True, log=None):
+    path = cache_path()
+    key = signature(instances, num_cu)
+    cache = _load(path) if use_cache else {}
+    if key in cache:
+        p = cache[key]["params"]
+        batch.set_params(relax_blocks_per_cu=p[0], merge_rows_per_wave=p[1], use_graph=1)
+        return tuple(p), cache[key]["table"]
+    table = []
+    ref = None
+    for rb, rows in CANDIDATES:
+        batch.set_params(relax_blocks_per_cu=rb, merge_rows_per_wave=rows, use_graph=0)
+        costs, _ = batch.solve()  # eager: records the rounds of this shape
+        if ref is None:
+            ref = costs
+        elif not np.array_equal(ref, costs):
+            raise RuntimeError("autotune: results differ between engine shapes")
+        batch.set_params(relax_blocks_per_cu=rb, merge_rows_per_wave=rows, use_graph=1)
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            costs, st = batch.solve()
+            ms = (time.perf_counter() - t0) * 1e3
+            if not np.array_equal(ref, costs):
+                raise RuntimeError("autotune: replay results differ")
+            best = min(best, ms)
+        table.append([rb, rows, best])
+        if log:
+            log(f"autotune: blocks/CU={rb} rows/wave={rows or 'auto'}: {best:.1f} ms")
+    rb, rows, _ = min(table, key=lambda r: r[2])
+    batch.set_params(relax_blocks_per_cu=rb, merge_rows_per_wave=rows, use_graph=0)
+    batch.solve()  # re-record the rounds for the chosen shape
+    batch.set_params(relax_blocks_per_cu=rb, merge_rows_per_wave=rows, use_graph=1)
+    if use_cache:
+        cache[key] = {"params": [rb, rows], "table": table}
+        _store(path, cache)
+    return (rb, rows), table

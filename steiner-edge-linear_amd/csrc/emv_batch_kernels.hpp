@@ -1,0 +1,358 @@
+// emv_batch_kernels.hpp -- gfx950 kernels of the multi-instance EMV engine.
+//
+// One engine step advances every active instance by one DP stage in the same
+// launches: instances at their singleton stage contribute one chunk of k
+// seeded shortest-path rows, instances at layer m contribute the chunks of
+// their m-subsets.  Instances may sit on different graphs (batched solves)
+// or share one resident graph (serving), and sit at different layers.
+//
+// The per-instance table keeps the single-instance layout in one arena
+// region: costs f[tbase + X * np + v] (int32), then the record codes
+// b[tbase + 2^k * np + X * np + v] (int32).
+// Staging chunks keep the vertex-major layout D[dbase + v * 64 + lane] with
+// 32-bit labels.  Work items are located through prefix arrays over the
+// descriptors (binary search), so instances of any size share one grid.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "emv_kernels.hpp"
+
+namespace emv {
+
+// One staging chunk: up to 64 rows of one instance.
+struct GChunk {
+    int64_t dbase;  // element offset of the chunk in D / CODE (np * 64 cells)
+    int64_t fbase;  // offset of the chunk's activity flags (np words)
+    int64_t tbase;  // element offset of the instance's table
+    int32_t n, np;  // instance vertices, table row stride
+    int32_t voff;   // first vertex of the instance's graph in the shared adjacency
+    int32_t k;      // instance terminals; codes of edge records start at 2^k
+    int32_t x0;     // first row of the chunk in the step's row list
+    int32_t nx;     // rows in the chunk (lane j < nx is row x0 + j)
+    int32_t kind;   // 0: layer rows, 1: singleton rows (row j = terminal j)
+    int32_t inst;   // engine slot of the instance
+};
+
+// One instance at a layer: its chunks are consecutive, its merge items are
+// tile-major over all of them (so one tile's rows are read together).
+struct GMerge {
+    int32_t chunk0, nchunks;
+    int32_t ntiles;  // np / 64
+    int32_t nsub;    // nchunks * 64 / XG
+};
+
+__device__ __forceinline__ int find_seg(const int64_t *__restrict__ pre, int nseg, int64_t i) {
+    // largest s with pre[s] <= i, pre[0] = 0, pre[nseg] = total
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------ singleton seeding
+// Every cell of every singleton chunk: INF label, code 0; then the sources.
+__global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict__ cell_pre,
+                        int nch, const int32_t *__restrict__ kk_all,
+                        const int32_t *__restrict__ kk_off, uint32_t *__restrict__ D,
+                        uint32_t *__restrict__ CODE) {
+    const int64_t total = cell_pre[nch];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int g = find_seg(cell_pre, nch, i);
+        const GChunk c = ch[g];
+        const int64_t off = i - cell_pre[g];  // v * 64 + lane
+        const int lane = (int)(off & (kWave - 1));
+        const int v = (int)(off / kWave);
+        uint32_t d = ~0u, code = 0u;
+        if (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) {
+            d = 0u;
+            code = 1u << lane;
+        }
+        D[c.dbase + off] = d;
+        CODE[c.dbase + off] = code;
+    }
+}
+
+// ------------------------------------------------------------------ merge
+// Same recurrence and packing as k_merge_v (reader-el.c:2068-2088), per
+// instance: f rows of the instance's table, shift by the instance's k.
+template <typename P, int XG>
+__global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f,
+                                                   const GChunk *__restrict__ ch,
+                                                   const GMerge *__restrict__ mg,
+                                                   const int64_t *__restrict__ item_pre, int nmg,
+                                                   const uint32_t *__restrict__ xs,
+                                                   uint32_t *__restrict__ D,
+                                                   uint32_t *__restrict__ CODE) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t item = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (item >= item_pre[nmg]) return;
+    const int s = find_seg(item_pre, nmg, item);
+    const GMerge M = mg[s];
+    const GChunk c0 = ch[M.chunk0];
+    const int64_t local = item - item_pre[s];
+    const int64_t tile = local / M.nsub;
+    const int sub = (int)(local - tile * M.nsub);
+    const int64_t v = tile * kWave + lane;
+    const int32_t *fv = f + c0.tbase + v;
+    const int k = c0.k;
+    const P ymask = ((P)1 << k) - 1;
+    int nrows = 0;
+    for (int q = 0; q < M.nchunks; q++) nrows += ch[M.chunk0 + q].nx;
+    uint32_t dv[XG], yv[XG];
+#pragma unroll
+    for (int jx = 0; jx < XG; jx++) {
+        const int xi = sub * XG + jx;
+        dv[jx] = ~0u;
+        yv[jx] = 0u;
+        if (xi < nrows) {
+            const P acc = merge_one<int32_t, P>(fv, (int64_t)c0.np, xs[c0.x0 + xi], k);
+            dv[jx] = (uint32_t)(acc >> k);
+            yv[jx] = (uint32_t)(acc & ymask);
+        }
+    }
+    const int chunk = sub / (kWave / XG);
+    const int loff = (sub % (kWave / XG)) * XG;
+    const GChunk cc = ch[M.chunk0 + chunk];
+    uint32_t *dd = D + cc.dbase + v * kWave + loff;
+    uint32_t *cd = CODE + cc.dbase + v * kWave + loff;
+#pragma unroll
+    for (int jx = 0; jx < XG; jx++) {
+        dd[jx] = dv[jx];
+        cd[jx] = yv[jx];
+    }
+}
+
+// Flags neighbour nb (< 0: none) of a changed item of chunk c and appends
+// its item id (item_pre of the chunk + nb) while the list has room.
+__device__ __forceinline__ void gflag(const GChunk &c, int64_t base_item, int nb,
+                                      uint32_t *__restrict__ Fnext, uint32_t *__restrict__ Qnext,
+                                      uint32_t *__restrict__ cnext, uint32_t qcap, bool list_open) {
+    bool want = false;
+    uint32_t ent = 0;
+    if (nb >= 0) {
+        uint32_t *fp = Fnext + c.fbase + nb;
+        if (*fp == 0) {
+            *fp = 1;
+            want = list_open;
+            ent = (uint32_t)(base_item + nb);
+        }
+    }
+    wave_append(want, ent, Qnext, cnext, qcap);
+}
+
+// ------------------------------------------------------------ relaxation
+// The pull round of k_relax_v (worklist or flag scan, same rules) over the
+// items of many chunks: item i of chunk g is vertex i - item_pre[g] of that
+// chunk's instance.  Adjacency is the shared padded fixed-degree copy
+// (DMAX > 0, neighbour ids local to the instance's graph) or the shared CSR.
+template <int DMAX>
+__global__ __launch_bounds__(kBlock) void k_grelax(
+    uint32_t *__restrict__ D, uint32_t *__restrict__ CODE, const GChunk *__restrict__ ch,
+    const int64_t *__restrict__ item_pre, int nch, const int32_t *__restrict__ rp,
+    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int dense,
+    uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
+    uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
+    uint32_t *__restrict__ cnt, int r) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        any[(r + 2) & 3] = 0;
+        cnt[(r + 2) & 3] = 0;
+    }
+    if (!dense && any[r & 3] == 0) return;
+    const uint32_t qn = dense ? 0u : cnt[r & 3];
+    const bool use_list = !dense && qn < qcap;
+    uint32_t *cnext = &cnt[(r + 1) & 3];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave0 = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    const int64_t nitems = item_pre[nch];
+    const int64_t span = use_list ? (int64_t)qn : nitems;
+    const int64_t nblk = (span + kWave - 1) / kWave;
+    const uint32_t INF = ~0u;
+    bool raised = false;
+    for (int64_t blk = wave0; blk < nblk; blk += wstride) {
+        const int64_t j = blk * kWave + lane;
+        int64_t mi = -1;
+        if (j < span) mi = use_list ? (int64_t)Qcur[j] : j;
+        int mg = 0, mv = 0;
+        bool fl = false;
+        if (mi >= 0) {
+            mg = find_seg(item_pre, nch, mi);
+            mv = (int)(mi - item_pre[mg]);
+            const int64_t mf = ch[mg].fbase + mv;
+            if (dense) {
+                fl = true;
+            } else if (use_list) {
+                fl = atomicExch(&Fcur[mf], 0u) != 0;  // one of the listed copies
+            } else if (Fcur[mf] != 0) {
+                fl = true;
+                Fcur[mf] = 0;
+            }
+        }
+        uint64_t act = __ballot(fl);
+        if (act == 0) continue;
+        const bool list_open = *(volatile uint32_t *)cnext < qcap;
+        int me0 = 0, me1 = 0;
+        Edge e[DMAX > 0 ? DMAX : 1];
+        if constexpr (DMAX > 0) {
+#pragma unroll
+            for (int q = 0; q < DMAX; q++) e[q] = Edge{-1, 0};
+            if (fl) {
+                const int64_t gv = (int64_t)ch[mg].voff + mv;
+#pragma unroll
+                for (int q = 0; q < DMAX; q++) e[q] = ell[gv * DMAX + q];
+            }
+        } else {
+            if (fl) {
+                const int64_t gv = (int64_t)ch[mg].voff + mv;
+                me0 = rp[gv];
+                me1 = rp[gv + 1];
+            }
+        }
+        while (act) {
+            const int t = __ffsll((unsigned long long)act) - 1;
+            act &= act - 1;
+            const int g = __builtin_amdgcn_readlane(mg, t);
+            const int v = __builtin_amdgcn_readlane(mv, t);
+            const GChunk c = ch[g];
+            const uint32_t pred_base = 1u << c.k;
+            uint32_t *Dc = D + c.dbase;
+            uint32_t *Cc = CODE + c.dbase;
+            const int64_t idx = (int64_t)v * kWave + lane;
+            const uint32_t cur = Dc[idx];
+            const uint32_t cc = Cc[idx];
+            uint32_t best = cur, bc = cc;
+            int nb = -1;  // lane q < degree holds neighbour q (for the flags)
+            if constexpr (DMAX > 0) {
+                int uu[DMAX], ww[DMAX];
+                uint32_t du[DMAX];
+#pragma unroll
+                for (int q = 0; q < DMAX; q++) {
+                    uu[q] = __builtin_amdgcn_readlane(e[q].v, t);
+                    ww[q] = __builtin_amdgcn_readlane(e[q].w, t);
+                }
+#pragma unroll
+                for (int q = 0; q < DMAX; q++)
+                    du[q] = uu[q] >= 0 ? Dc[(int64_t)uu[q] * kWave + lane] : INF;
+#pragma unroll
+                for (int q = 0; q < DMAX; q++) {
+                    relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
+                    if (lane == q) nb = uu[q];
+                }
+            } else {
+                const int e0 = __builtin_amdgcn_readlane(me0, t);
+                const int e1 = __builtin_amdgcn_readlane(me1, t);
+                for (int eb = e0; eb < e1; eb += kWave) {
+                    const int cnt_e = min(kWave, e1 - eb);
+                    Edge my = {0, 0};
+                    if (lane < cnt_e) my = adj[eb + lane];
+                    for (int tt = 0; tt < cnt_e; tt++) {
+                        const int u = __builtin_amdgcn_readlane(my.v, tt);
+                        const int w = __builtin_amdgcn_readlane(my.w, tt);
+                        relax_fold(Dc[(int64_t)u * kWave + lane], u, w, pred_base, INF, best, bc);
+                    }
+                }
+            }
+            const bool dchg = best < cur;
+            if (dchg) Dc[idx] = best;
+            if (bc != cc) Cc[idx] = bc;
+            if (__ballot(dchg) != 0) {
+                const int64_t base_item = item_pre[g];
+                if constexpr (DMAX > 0) {
+                    gflag(c, base_item, nb, Fnext, Qnext, cnext, qcap, list_open);
+                } else {
+                    const int e0 = __builtin_amdgcn_readlane(me0, t);
+                    const int e1 = __builtin_amdgcn_readlane(me1, t);
+                    for (int eb = e0; eb < e1; eb += kWave) {
+                        const int e2 = eb + lane;
+                        gflag(c, base_item, e2 < e1 ? adj[e2].v : -1, Fnext, Qnext, cnext, qcap,
+                              list_open);
+                    }
+                }
+                raised = true;
+            }
+        }
+    }
+    if (raised && lane == 0) any[(r + 1) & 3] = 1;
+}
+
+// ----------------------------------------------------------------- unpack
+// Staging chunks -> table rows of their instances (LDS transpose of 64
+// vertices x 64 rows per workgroup).  Singleton chunks write f[1<<t][v] and
+// the record (kk[t], {t}) (reader-el.c:2015-2027), and fold the per-row
+// maximum into rowmax[inst * 64 + t].
+__global__ __launch_bounds__(kBlock) void k_gunpack(
+    const uint32_t *__restrict__ D, const uint32_t *__restrict__ CODE,
+    const GChunk *__restrict__ ch, const int64_t *__restrict__ blk_pre, int nch,
+    const uint32_t *__restrict__ xs, const int32_t *__restrict__ kk_all,
+    const int32_t *__restrict__ kk_off, int32_t *__restrict__ f, int32_t *__restrict__ b,
+    unsigned long long *__restrict__ rowmax) {
+    __shared__ uint32_t sd[kWave][kWave + 1];
+    __shared__ uint32_t sc[kWave][kWave + 1];
+    const int64_t item = blockIdx.x;
+    if (item >= blk_pre[nch]) return;
+    const int g = find_seg(blk_pre, nch, item);
+    const GChunk c = ch[g];
+    const int64_t v0 = (item - blk_pre[g]) * kWave;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int grp = t / kWave;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int vv = grp + r * (kBlock / kWave);
+        const int64_t idx = c.dbase + (v0 + vv) * kWave + lane;
+        sd[vv][lane] = D[idx];
+        sc[vv][lane] = CODE[idx];
+    }
+    __syncthreads();
+    const int64_t S2 = 1ll << c.k;
+    const int64_t bbase = c.tbase + S2 * c.np;  // records follow the costs
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int xl = grp + r * (kBlock / kWave);
+        const int64_t v = v0 + lane;
+        unsigned long long dmax = 0;
+        if (xl < c.nx && v < c.n) {
+            const uint32_t d = sd[lane][xl];
+            if (c.kind == 0) {
+                const int64_t X = xs[c.x0 + xl];
+                f[c.tbase + X * c.np + v] = (int32_t)d;
+                b[bbase + X * c.np + v] = (int32_t)sc[lane][xl];
+            } else {
+                const int64_t X = 1ll << xl;
+                const int32_t src = kk_all[kk_off[c.inst] + xl];
+                f[c.tbase + X * c.np + v] = (d == ~0u) ? INT32_MAX : (int32_t)d;
+                b[bbase + X * c.np + v] = (v == src) ? (int32_t)X : (int32_t)(S2 + src);
+                dmax = d;
+            }
+        }
+        if (c.kind == 1) {
+            for (int off = 32; off > 0; off >>= 1) {
+                const unsigned long long o = __shfl_xor(dmax, off);
+                dmax = o > dmax ? o : dmax;
+            }
+            if (lane == 0 && xl < c.nx) atomicMax(&rowmax[(int64_t)c.inst * kWave + xl], dmax);
+        }
+    }
+}
+
+// Per-step convergence word for graph replays: out[slot] = any[r & 3].
+__global__ void k_copy_word(const uint32_t *__restrict__ any, int r, uint32_t *__restrict__ out,
+                            int slot) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[slot] = any[r & 3];
+}
+
+// Answer cells f[C][q] of finished instances.
+__global__ void k_gather_costs(const int32_t *__restrict__ f, const int64_t *__restrict__ cell,
+                               int n, int64_t *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int32_t x = f[cell[i]];
+        out[i] = (x == INT32_MAX) ? INT64_MAX : (int64_t)x;
+    }
+}
+
+}  // namespace emv

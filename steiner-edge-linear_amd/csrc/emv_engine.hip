@@ -1,0 +1,1236 @@
+// emv_engine.hip -- multi-instance EMV engine: batched solves and serving.
+//
+// The engine keeps graphs resident (one shared adjacency), owns a device
+// arena for the per-instance tables and advances every active instance by
+// one DP stage per step (reader-el.c:1988-2134 per instance): the singleton
+// shortest paths for newly admitted instances, the merge + relaxation of
+// layer m for the others.  All instances of a step share one merge launch,
+// one set of relaxation rounds and one unpack launch.
+//
+//   batch  : many instances, possibly on different graphs, admitted at once
+//            and stepped in lockstep; steps 2..kmax can be captured in a
+//            hipGraph with a fixed number of relaxation rounds per step,
+//            checked after the replay (a step that had not converged makes
+//            the solve run again eagerly).
+//   server : one resident graph, requests admitted by their table footprint
+//            2^k * np * 8 bytes against the arena as they arrive, requests at
+//            different layers stepped together (continuous batching).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "emv_batch_kernels.hpp"
+#include "emv_engine.h"
+#include "emv_hip.h"
+
+using namespace emv;
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+int bits_for(uint64_t x) {
+    int b = 0;
+    while (b < 64 && (x >> b) != 0) b++;
+    return b;
+}
+
+int64_t row_stride(int64_t n) {
+    int64_t np = (n + kWave - 1) / kWave * kWave;
+    if ((np / kWave) % 2 == 0) np += kWave;
+    return np;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+// First-fit allocator over [0, cap) cells with coalescing frees.
+struct Arena {
+    int64_t cap = 0;
+    std::map<int64_t, int64_t> free_;  // offset -> length
+    void reset(int64_t c) {
+        cap = c;
+        free_.clear();
+        if (c > 0) free_[0] = c;
+    }
+    int64_t alloc(int64_t len) {
+        len = (len + 255) / 256 * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= len) {
+                const int64_t off = it->first, rest = it->second - len;
+                free_.erase(it);
+                if (rest > 0) free_[off + len] = rest;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t len) {
+        len = (len + 255) / 256 * 256;
+        auto it = free_.emplace(off, len).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    int64_t largest() const {
+        int64_t m = 0;
+        for (auto &kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+};
+
+struct GraphRec {
+    int64_t voff, n, np, maxw;
+    int maxdeg;
+};
+
+enum { ST_QUEUED = 0, ST_RUNNING = 1, ST_DONE = 2 };
+
+struct Inst {
+    int64_t ticket = 0;
+    int graph = 0;
+    int k = 0;
+    std::vector<int32_t> kk;
+    uint32_t flags = 0;
+    int state = ST_QUEUED;
+    int stage = 1;  // next stage: 1 = singletons, m >= 2 = layer m
+    int pbits = 32;
+    int status = EMV_OK;
+    int64_t abase = -1, acells = 0;  // arena region: f then b, S * np cells each
+    int64_t cost = 0;
+    std::vector<int64_t> pairs;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int slot = -1;  // row of the kk table while running
+};
+
+}  // namespace
+
+struct emv_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    char err[512] = {0};
+    // resident graphs
+    std::vector<GraphRec> graphs;
+    int32_t *d_rp = nullptr;
+    Edge *d_adj = nullptr;
+    Edge *d_ell = nullptr;
+    int ell_deg = 0;
+    int64_t gvert = 0;
+    // table arena (int32 cells)
+    int32_t *d_arena = nullptr;
+    Arena arena;
+    // instances
+    std::vector<Inst> inst;
+    std::deque<int64_t> queue;  // tickets waiting for admission
+    int64_t next_ticket = 0;
+    int max_slots = 0;
+    std::vector<int> free_slots;
+    int32_t *d_kk = nullptr;     // max_slots * 64 terminals
+    int32_t *d_kkoff = nullptr;  // slot -> offset (slot * 64)
+    unsigned long long *d_rowmax = nullptr;  // max_slots * 64
+    // staging
+    uint32_t *d_D = nullptr, *d_code = nullptr;
+    int64_t stage_cells = 0;
+    uint32_t *d_flag[2] = {nullptr, nullptr};
+    int64_t flag_bytes = 0;  // flag words held per array
+    uint32_t *d_any = nullptr;
+    uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
+    uint32_t *d_q[2] = {nullptr, nullptr};
+    uint32_t qcap = 0;  // worklist rounds off unless EMV_QCAP sets a capacity
+    uint32_t *h_pin = nullptr;
+    // per-step descriptor buffer (eager steps reuse one; graph steps own one each)
+    std::vector<void *> desc_bufs;
+    std::vector<int64_t> desc_caps;
+    // tunables
+    emv_engine_params prm;
+    // counters
+    int64_t steps = 0, relax_rounds = 0;
+    double relax_ms_acc = 0, merge_ms_acc = 0;
+    std::vector<int> rounds_per_step;  // eager: rounds used by step s (batch)
+};
+
+namespace {
+
+void eset(emv_engine *e, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(e->err, sizeof(e->err), fmt, ap);
+    va_end(ap);
+}
+
+#define EHIP(e, expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _x = (expr);                                                            \
+        if (_x != hipSuccess) {                                                            \
+            eset((e), "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_x), __FILE__,     \
+                 __LINE__);                                                                \
+            return (_x == hipErrorOutOfMemory || _x == hipErrorMemoryAllocation)           \
+                       ? EMV_ERR_NOMEM                                                     \
+                       : EMV_ERR_HIP;                                                      \
+        }                                                                                  \
+    } while (0)
+
+#define ECHK(expr)                   \
+    do {                             \
+        int _r = (expr);             \
+        if (_r != EMV_OK) return _r; \
+    } while (0)
+
+// The m-subsets of k bits in increasing numeric order (reader-el.c:2040-2046).
+void subsets_of_size(int k, int m, std::vector<uint32_t> &out) {
+    for (uint64_t X = (1ull << m) - 1; X < (1ull << k);) {
+        out.push_back((uint32_t)X);
+        const uint64_t lo = X & (~X + 1);
+        const uint64_t r = X + lo;
+        X = (((r ^ X) >> 2) / lo) | r;
+    }
+}
+
+int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
+                  const std::vector<int64_t> &ns) {
+    const size_t G = ns.size();
+    std::vector<int32_t> rp(1, 0);
+    std::vector<Edge> adj;
+    int64_t voff = 0;
+    int maxdeg_all = 0;
+    for (size_t g = 0; g < G; g++) {
+        const int64_t n = ns[g];
+        const std::vector<int64_t> &E = edges[g];
+        const int64_t m = (int64_t)E.size() / 3;
+        std::vector<int32_t> deg(n, 0);
+        int64_t maxw = 0;
+        for (int64_t j = 0; j < m; j++) {
+            const int64_t u = E[3 * j], v = E[3 * j + 1], w = E[3 * j + 2];
+            if (u < 0 || v < 0 || u >= n || v >= n) {
+                eset(e, "graph %zu: edge %lld endpoint out of range", g, (long long)j);
+                return EMV_ERR_ARG;
+            }
+            if (w < 1 || w > INT32_MAX) {
+                eset(e, "graph %zu: weight %lld outside [1, 2^31-1]", g, (long long)w);
+                return EMV_ERR_RANGE;
+            }
+            maxw = std::max(maxw, w);
+            deg[u]++;
+            deg[v]++;
+        }
+        const int64_t e0 = (int64_t)adj.size();
+        std::vector<int64_t> fill(n);
+        int64_t run = e0;
+        int maxdeg = 0;
+        for (int64_t u = 0; u < n; u++) {
+            fill[u] = run;
+            run += deg[u];
+            maxdeg = std::max(maxdeg, deg[u]);
+            rp.push_back((int32_t)run);
+        }
+        if (run >= INT32_MAX) {
+            eset(e, "adjacency exceeds 2^31 entries");
+            return EMV_ERR_RANGE;
+        }
+        adj.resize(run);
+        for (int64_t j = 0; j < m; j++) {
+            const int32_t u = (int32_t)E[3 * j], v = (int32_t)E[3 * j + 1];
+            const int32_t w = (int32_t)E[3 * j + 2];
+            adj[fill[u]++] = Edge{v, w};
+            adj[fill[v]++] = Edge{u, w};
+        }
+        GraphRec gr;
+        gr.voff = voff;
+        gr.n = n;
+        gr.np = row_stride(n);
+        gr.maxw = maxw;
+        gr.maxdeg = maxdeg;
+        e->graphs.push_back(gr);
+        voff += n;
+        maxdeg_all = std::max(maxdeg_all, maxdeg);
+    }
+    e->gvert = voff;
+    EHIP(e, hipMalloc((void **)&e->d_rp, sizeof(int32_t) * rp.size()));
+    EHIP(e, hipMemcpy(e->d_rp, rp.data(), sizeof(int32_t) * rp.size(), hipMemcpyHostToDevice));
+    EHIP(e, hipMalloc((void **)&e->d_adj, sizeof(Edge) * std::max<size_t>(adj.size(), 1)));
+    if (!adj.empty())
+        EHIP(e, hipMemcpy(e->d_adj, adj.data(), sizeof(Edge) * adj.size(), hipMemcpyHostToDevice));
+    int ell = 0;
+    for (int d : {4, 8, 16})
+        if (maxdeg_all <= d) {
+            ell = d;
+            break;
+        }
+    if (ell && (int64_t)ell * voff > 2 * std::max<int64_t>((int64_t)adj.size(), 1)) ell = 0;
+    if (getenv("EMV_NO_ELL")) ell = 0;
+    if (ell) {
+        std::vector<Edge> el((size_t)(voff * ell), Edge{-1, 0});
+        for (int64_t gv = 0; gv < voff; gv++)
+            for (int32_t i = rp[gv]; i < rp[gv + 1]; i++)
+                el[(size_t)(gv * ell + (i - rp[gv]))] = adj[i];
+        EHIP(e, hipMalloc((void **)&e->d_ell, sizeof(Edge) * el.size()));
+        EHIP(e, hipMemcpy(e->d_ell, el.data(), sizeof(Edge) * el.size(), hipMemcpyHostToDevice));
+    }
+    e->ell_deg = ell;
+    return EMV_OK;
+}
+
+int ensure_slots(emv_engine *e, int slots) {
+    if (slots <= e->max_slots) return EMV_OK;
+    // grows only while nothing is running (create time)
+    dfree(e->d_kk);
+    dfree(e->d_kkoff);
+    dfree(e->d_rowmax);
+    EHIP(e, hipMalloc((void **)&e->d_kk, sizeof(int32_t) * slots * kWave));
+    EHIP(e, hipMalloc((void **)&e->d_kkoff, sizeof(int32_t) * slots));
+    EHIP(e, hipMalloc((void **)&e->d_rowmax, sizeof(unsigned long long) * slots * kWave));
+    std::vector<int32_t> off(slots);
+    for (int s = 0; s < slots; s++) off[s] = s * kWave;
+    EHIP(e, hipMemcpy(e->d_kkoff, off.data(), sizeof(int32_t) * slots, hipMemcpyHostToDevice));
+    for (int s = e->max_slots; s < slots; s++) e->free_slots.push_back(s);
+    e->max_slots = slots;
+    return EMV_OK;
+}
+
+int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
+    if (cells > e->stage_cells) {
+        dfree(e->d_D);
+        dfree(e->d_code);
+        e->stage_cells = 0;
+        const int64_t c = cells + cells / 4;
+        EHIP(e, hipMalloc((void **)&e->d_D, sizeof(uint32_t) * c));
+        EHIP(e, hipMalloc((void **)&e->d_code, sizeof(uint32_t) * c));
+        e->stage_cells = c;
+    }
+    if (flags > e->flag_bytes) {
+        dfree(e->d_flag[0]);
+        dfree(e->d_flag[1]);
+        e->flag_bytes = 0;
+        const int64_t c = flags + flags / 4;
+        EHIP(e, hipMalloc((void **)&e->d_flag[0], sizeof(uint32_t) * c));
+        EHIP(e, hipMalloc((void **)&e->d_flag[1], sizeof(uint32_t) * c));
+        const int blocks = (int)std::min<int64_t>((c + kBlock - 1) / kBlock, 65536);
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           e->d_flag[0], c, 0u);
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           e->d_flag[1], c, 0u);
+        EHIP(e, hipGetLastError());
+        e->flag_bytes = c;
+    }
+    return EMV_OK;
+}
+
+// Host-side plan of one step, packed into one descriptor upload.
+struct StepPlan {
+    std::vector<GChunk> ch;
+    std::vector<GMerge> mg;
+    std::vector<int64_t> relax_pre, merge_pre, unpack_pre, seed_pre;
+    std::vector<uint32_t> xs;
+    std::vector<int> seed_chunks;  // indices into ch
+    std::vector<int64_t> insts;    // tickets stepped
+    int xg = 1;
+    int pbits = 32;
+    int64_t cells = 0, flagbytes = 0;
+    // device layout (byte offsets into the descriptor buffer)
+    size_t off_ch = 0, off_mg = 0, off_rp = 0, off_mp = 0, off_up = 0, off_sp = 0, off_xs = 0,
+           off_seedch = 0, bytes = 0;
+    std::vector<char> blob;
+    std::vector<GChunk> seed_ch;
+};
+
+void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P) {
+    P = StepPlan();
+    int64_t dcur = 0, fcur = 0;
+    int maxrows = 0;
+    std::vector<uint32_t> rows;
+    for (int64_t t : tickets) {
+        Inst &I = e->inst[t];
+        const GraphRec &G = e->graphs[I.graph];
+        const int64_t S2 = 1ll << I.k;
+        P.insts.push_back(t);
+        if (I.stage == 1) {
+            GChunk c;
+            c.dbase = dcur;
+            c.fbase = fcur;
+            c.tbase = I.abase;
+            c.n = (int32_t)G.n;
+            c.np = (int32_t)G.np;
+            c.voff = (int32_t)G.voff;
+            c.k = I.k;
+            c.x0 = 0;
+            c.nx = I.k;
+            c.kind = 1;
+            c.inst = I.slot;
+            P.seed_chunks.push_back((int)P.ch.size());
+            P.ch.push_back(c);
+            dcur += G.np * kWave;
+            fcur += G.np;
+            continue;
+        }
+        rows.clear();
+        subsets_of_size(I.k, I.stage, rows);
+        const int nrows = (int)rows.size();
+        maxrows = std::max(maxrows, nrows);
+        if (I.pbits == 64) P.pbits = 64;
+        GMerge M;
+        M.chunk0 = (int)P.ch.size();
+        M.nchunks = (nrows + kWave - 1) / kWave;
+        M.ntiles = (int32_t)(G.np / kWave);
+        M.nsub = 0;  // set once the step's rows per wave is known
+        const int x0 = (int)P.xs.size();
+        P.xs.insert(P.xs.end(), rows.begin(), rows.end());
+        for (int q = 0; q < M.nchunks; q++) {
+            GChunk c;
+            c.dbase = dcur;
+            c.fbase = fcur;
+            c.tbase = I.abase;
+            c.n = (int32_t)G.n;
+            c.np = (int32_t)G.np;
+            c.voff = (int32_t)G.voff;
+            c.k = I.k;
+            c.x0 = x0 + q * kWave;
+            c.nx = std::min(kWave, nrows - q * kWave);
+            c.kind = 0;
+            c.inst = I.slot;
+            P.ch.push_back(c);
+            dcur += G.np * kWave;
+            fcur += G.np;
+        }
+        P.mg.push_back(M);
+        (void)S2;
+    }
+    if (e->prm.merge_rows_per_wave > 0)
+        P.xg = e->prm.merge_rows_per_wave;
+    else
+        P.xg = maxrows >= 1024 ? 8 : maxrows >= 256 ? 4 : maxrows >= 64 ? 2 : 1;
+    P.relax_pre.push_back(0);
+    P.unpack_pre.push_back(0);
+    for (const GChunk &c : P.ch) {
+        P.relax_pre.push_back(P.relax_pre.back() + c.n);
+        P.unpack_pre.push_back(P.unpack_pre.back() + (c.n + kWave - 1) / kWave);
+    }
+    P.merge_pre.push_back(0);
+    for (GMerge &M : P.mg) {
+        M.nsub = M.nchunks * (kWave / P.xg);
+        P.merge_pre.push_back(P.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
+    }
+    P.seed_pre.push_back(0);
+    for (int g : P.seed_chunks) {
+        P.seed_ch.push_back(P.ch[g]);
+        P.seed_pre.push_back(P.seed_pre.back() + (int64_t)P.ch[g].np * kWave);
+    }
+    P.cells = dcur;
+    P.flagbytes = fcur;
+    // pack
+    auto place = [&](size_t bytes) {
+        size_t o = (P.bytes + 255) / 256 * 256;
+        P.bytes = o + bytes;
+        return o;
+    };
+    P.off_ch = place(sizeof(GChunk) * P.ch.size());
+    P.off_mg = place(sizeof(GMerge) * P.mg.size());
+    P.off_rp = place(sizeof(int64_t) * P.relax_pre.size());
+    P.off_mp = place(sizeof(int64_t) * P.merge_pre.size());
+    P.off_up = place(sizeof(int64_t) * P.unpack_pre.size());
+    P.off_sp = place(sizeof(int64_t) * P.seed_pre.size());
+    P.off_xs = place(sizeof(uint32_t) * std::max<size_t>(P.xs.size(), 1));
+    P.off_seedch = place(sizeof(GChunk) * P.seed_ch.size());
+    P.blob.assign(P.bytes, 0);
+    auto put = [&](size_t off, const void *src, size_t n) {
+        if (n) memcpy(P.blob.data() + off, src, n);
+    };
+    put(P.off_ch, P.ch.data(), sizeof(GChunk) * P.ch.size());
+    put(P.off_mg, P.mg.data(), sizeof(GMerge) * P.mg.size());
+    put(P.off_rp, P.relax_pre.data(), sizeof(int64_t) * P.relax_pre.size());
+    put(P.off_mp, P.merge_pre.data(), sizeof(int64_t) * P.merge_pre.size());
+    put(P.off_up, P.unpack_pre.data(), sizeof(int64_t) * P.unpack_pre.size());
+    put(P.off_sp, P.seed_pre.data(), sizeof(int64_t) * P.seed_pre.size());
+    put(P.off_xs, P.xs.data(), sizeof(uint32_t) * P.xs.size());
+    put(P.off_seedch, P.seed_ch.data(), sizeof(GChunk) * P.seed_ch.size());
+}
+
+int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
+    while (e->desc_bufs.size() <= idx) {
+        e->desc_bufs.push_back(nullptr);
+        e->desc_caps.push_back(0);
+    }
+    if ((size_t)e->desc_caps[idx] < bytes) {
+        if (e->desc_bufs[idx]) (void)hipFree(e->desc_bufs[idx]);
+        e->desc_bufs[idx] = nullptr;
+        e->desc_caps[idx] = 0;
+        EHIP(e, hipMalloc(&e->desc_bufs[idx], bytes * 2));
+        e->desc_caps[idx] = (int64_t)bytes * 2;
+    }
+    *out = (char *)e->desc_bufs[idx];
+    return EMV_OK;
+}
+
+template <int DMAX>
+void launch_grelax(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
+    const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
+    hipLaunchKernelGGL((k_grelax<DMAX>), dim3(grid), dim3(kBlock), 0, e->stream, e->d_D,
+                       e->d_code, (const GChunk *)(db + P.off_ch),
+                       (const int64_t *)(db + P.off_rp), (int)P.ch.size(), e->d_rp, e->d_adj,
+                       e->d_ell, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
+                       (const uint32_t *)e->d_q[r & 1], e->d_q[(r + 1) & 1], e->qcap, e->d_any,
+                       e->d_cnt, r);
+}
+
+void grelax_round(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
+    switch (e->ell_deg) {
+    case 4: launch_grelax<4>(e, db, P, dense, r); break;
+    case 8: launch_grelax<8>(e, db, P, dense, r); break;
+    case 16: launch_grelax<16>(e, db, P, dense, r); break;
+    default: launch_grelax<0>(e, db, P, dense, r); break;
+    }
+}
+
+template <typename P_, int XG>
+void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P) {
+    const int64_t items = P.merge_pre.back();
+    const int64_t blocks = (items + (kBlock / kWave) - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((k_gmerge<P_, XG>), dim3((unsigned)blocks), dim3(kBlock), 0, e->stream,
+                       e->d_arena, (const GChunk *)(db + P.off_ch), (const GMerge *)(db + P.off_mg),
+                       (const int64_t *)(db + P.off_mp), (int)P.mg.size(),
+                       (const uint32_t *)(db + P.off_xs), e->d_D, e->d_code);
+}
+
+template <typename P_>
+void launch_gmerge_p(emv_engine *e, char *db, const StepPlan &P) {
+    switch (P.xg) {
+    case 8: launch_gmerge_xg<P_, 8>(e, db, P); break;
+    case 4: launch_gmerge_xg<P_, 4>(e, db, P); break;
+    case 2: launch_gmerge_xg<P_, 2>(e, db, P); break;
+    default: launch_gmerge_xg<P_, 1>(e, db, P); break;
+    }
+}
+
+// Launches of one step up to (not including) the relaxation rounds.
+int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
+    if (!P.seed_ch.empty()) {
+        const int64_t cells = P.seed_pre.back();
+        const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
+        hipLaunchKernelGGL(k_gseed, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
+                           (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D, e->d_code);
+    }
+    if (!P.mg.empty() && P.merge_pre.back() > 0) {
+        if (P.pbits == 64)
+            launch_gmerge_p<uint64_t>(e, db, P);
+        else
+            launch_gmerge_p<uint32_t>(e, db, P);
+    }
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_any, (int64_t)4,
+                       0u);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_cnt, (int64_t)4,
+                       0u);
+    EHIP(e, hipGetLastError());
+    return EMV_OK;
+}
+
+int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
+    const int64_t blocks = P.unpack_pre.back();
+    if (blocks > 0)
+        hipLaunchKernelGGL(k_gunpack, dim3((unsigned)blocks), dim3(kBlock), 0, e->stream, e->d_D,
+                           e->d_code, (const GChunk *)(db + P.off_ch),
+                           (const int64_t *)(db + P.off_up), (int)P.ch.size(),
+                           (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_arena,
+                           e->d_arena, e->d_rowmax);
+    EHIP(e, hipGetLastError());
+    return EMV_OK;
+}
+
+int run_relax_eager(emv_engine *e, char *db, const StepPlan &P, int *rounds_used) {
+    int r = 0;
+    grelax_round(e, db, P, 1, r);
+    EHIP(e, hipGetLastError());
+    r = 1;
+    int rounds = 1, group = 2;
+    for (;;) {
+        for (int g = 0; g < group; g++, r++, rounds++) grelax_round(e, db, P, 0, r);
+        EHIP(e, hipGetLastError());
+        EHIP(e, hipMemcpyAsync(e->h_pin, e->d_any + (r & 3), sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, e->stream));
+        EHIP(e, hipStreamSynchronize(e->stream));
+        if (*e->h_pin == 0) break;
+        group = std::min(group * 2, 32);
+        if (rounds > 4000000) {
+            eset(e, "relaxation did not converge");
+            return EMV_ERR_RANGE;
+        }
+    }
+    // the rounds that did work: the last group may have run empty rounds
+    *rounds_used = rounds;
+    e->relax_rounds += rounds;
+    return EMV_OK;
+}
+
+int read_costs_and_finish(emv_engine *e, const std::vector<int64_t> &done);
+
+// After a step: singleton instances get their bound (and P width), layer
+// instances advance; instances past their last layer finish.
+int advance(emv_engine *e, const StepPlan &P) {
+    std::vector<int> single_slots;
+    for (int64_t t : P.insts)
+        if (e->inst[t].stage == 1) single_slots.push_back(e->inst[t].slot);
+    std::vector<unsigned long long> rm;
+    if (!single_slots.empty()) {
+        rm.resize((size_t)e->max_slots * kWave);
+        EHIP(e, hipMemcpyAsync(rm.data(), e->d_rowmax,
+                               sizeof(unsigned long long) * e->max_slots * kWave,
+                               hipMemcpyDeviceToHost, e->stream));
+        EHIP(e, hipStreamSynchronize(e->stream));
+    }
+    std::vector<int64_t> done;
+    for (int64_t t : P.insts) {
+        Inst &I = e->inst[t];
+        if (I.stage == 1) {
+            unsigned long long B = 0;
+            bool disc = false;
+            for (int j = 0; j < I.k; j++) {
+                const unsigned long long d = rm[(size_t)I.slot * kWave + j];
+                if (d >= 0xFFFFFFFFull) disc = true;
+                B += d;
+            }
+            if (disc) {
+                I.status = EMV_ERR_DISCONNECTED;
+                done.push_back(t);
+                continue;
+            }
+            I.pbits = (bits_for(B) + I.k <= 32) ? 32 : 64;
+            if (B + (unsigned long long)e->graphs[I.graph].maxw >= 0xFFFFFFFFull ||
+                B >= (unsigned long long)INT32_MAX) {
+                I.status = EMV_ERR_RANGE;
+                done.push_back(t);
+                continue;
+            }
+            I.stage = 2;
+        } else {
+            I.stage++;
+        }
+        if (I.stage > I.k) done.push_back(t);
+    }
+    return read_costs_and_finish(e, done);
+}
+
+int traceback_inst(emv_engine *e, Inst &I);
+
+int read_costs_and_finish(emv_engine *e, const std::vector<int64_t> &done) {
+    for (int64_t t : done) {
+        Inst &I = e->inst[t];
+        const GraphRec &G = e->graphs[I.graph];
+        if (I.status == EMV_OK) {
+            const int64_t C = (I.k == 1) ? 0 : (((int64_t)1 << (I.k - 1)) - 1);
+            if (C == 0) {
+                I.cost = INT64_MAX;
+            } else {
+                int32_t v = 0;
+                EHIP(e, hipMemcpyAsync(&v, e->d_arena + I.abase + C * G.np + I.kk[I.k - 1],
+                                       sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+                EHIP(e, hipStreamSynchronize(e->stream));
+                I.cost = (v == INT32_MAX) ? INT64_MAX : (int64_t)v;
+                if (I.flags & EMV_LIST) ECHK(traceback_inst(e, I));
+            }
+        }
+        I.state = ST_DONE;
+        I.t_done = now_ms();
+        if (!(I.flags & EMV_KEEP_TABLE) && I.abase >= 0) {
+            e->arena.release(I.abase, I.acells);
+            I.abase = -1;
+        }
+        if (I.slot >= 0) {
+            e->free_slots.push_back(I.slot);
+            I.slot = -1;
+        }
+    }
+    return EMV_OK;
+}
+
+int traceback_inst(emv_engine *e, Inst &I) {
+    const GraphRec &G = e->graphs[I.graph];
+    const int64_t S2 = 1ll << I.k;
+    const int64_t C = (((int64_t)1 << (I.k - 1)) - 1);
+    const int64_t cap = 2 * (G.n + I.k) + 64, stcap = 4 * I.k + 64;
+    int64_t *d_pairs = nullptr, *d_stack = nullptr, *d_res = nullptr;
+    EHIP(e, hipMalloc((void **)&d_pairs, sizeof(int64_t) * 2 * cap));
+    if (hipMalloc((void **)&d_stack, sizeof(int64_t) * 2 * stcap) != hipSuccess ||
+        hipMalloc((void **)&d_res, sizeof(int64_t)) != hipSuccess) {
+        dfree(d_pairs);
+        dfree(d_stack);
+        eset(e, "traceback allocation failed");
+        return EMV_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(k_traceback, dim3(1), dim3(64), 0, e->stream,
+                       e->d_arena + I.abase + S2 * G.np, (int64_t)G.np, I.k,
+                       (int64_t)I.kk[I.k - 1], C, d_pairs, cap, d_stack, stcap, d_res);
+    int64_t cnt = 0;
+    hipError_t x1 = hipMemcpyAsync(&cnt, d_res, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream);
+    hipError_t x2 = hipStreamSynchronize(e->stream);
+    if (x1 == hipSuccess && x2 == hipSuccess && cnt >= 0) {
+        I.pairs.resize((size_t)(2 * cnt));
+        if (cnt > 0)
+            x1 = hipMemcpy(I.pairs.data(), d_pairs, sizeof(int64_t) * 2 * cnt,
+                           hipMemcpyDeviceToHost);
+    }
+    dfree(d_pairs);
+    dfree(d_stack);
+    dfree(d_res);
+    if (x1 != hipSuccess || x2 != hipSuccess || cnt < 0) {
+        eset(e, "traceback failed");
+        return cnt < 0 ? EMV_ERR_CAPACITY : EMV_ERR_HIP;
+    }
+    return EMV_OK;
+}
+
+// Admission: FIFO; the head waits until its table fits the arena.
+int admit(emv_engine *e) {
+    while (!e->queue.empty() && !e->free_slots.empty()) {
+        const int64_t t = e->queue.front();
+        Inst &I = e->inst[t];
+        const GraphRec &G = e->graphs[I.graph];
+        const int64_t cells = 2 * ((int64_t)1 << I.k) * G.np;
+        const int64_t off = e->arena.alloc(cells);
+        if (off < 0) break;
+        e->queue.pop_front();
+        I.abase = off;
+        I.acells = cells;
+        I.slot = e->free_slots.back();
+        e->free_slots.pop_back();
+        I.state = ST_RUNNING;
+        I.stage = 1;
+        I.t_admit = now_ms();
+        std::vector<int32_t> k64(kWave, -1);
+        for (int j = 0; j < I.k; j++) k64[j] = I.kk[j];
+        EHIP(e, hipMemcpyAsync(e->d_kk + (int64_t)I.slot * kWave, k64.data(),
+                               sizeof(int32_t) * kWave, hipMemcpyHostToDevice, e->stream));
+        // table: f = 0 (padding stays 0), row 0 INF, b = -1; rowmax = 0
+        const int64_t S2 = 1ll << I.k;
+        const int blocks = (int)std::min<int64_t>((S2 * G.np + kBlock - 1) / kBlock, 1 << 18);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           e->d_arena + off, S2 * G.np, 0);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream, e->d_arena + off,
+                           G.np, INT32_MAX);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           e->d_arena + off + S2 * G.np, S2 * G.np, -1);
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, e->stream,
+                           e->d_rowmax + (int64_t)I.slot * kWave, (int64_t)kWave, 0ull);
+        EHIP(e, hipGetLastError());
+    }
+    return EMV_OK;
+}
+
+// One eager step over every running instance.
+int step_eager(emv_engine *e, int *rounds_used) {
+    ECHK(admit(e));
+    std::vector<int64_t> run;
+    for (size_t t = 0; t < e->inst.size(); t++)
+        if (e->inst[t].state == ST_RUNNING) run.push_back((int64_t)t);
+    *rounds_used = 0;
+    if (run.empty()) return EMV_OK;
+    StepPlan P;
+    build_plan(e, run, P);
+    ECHK(ensure_stage(e, P.cells, P.flagbytes));
+    char *db = nullptr;
+    ECHK(desc_buffer(e, 0, P.bytes, &db));
+    EHIP(e, hipMemcpyAsync(db, P.blob.data(), P.bytes, hipMemcpyHostToDevice, e->stream));
+    ECHK(enqueue_front(e, db, P));
+    ECHK(run_relax_eager(e, db, P, rounds_used));
+    ECHK(enqueue_back(e, db, P));
+    e->steps++;
+    return advance(e, P);
+}
+
+int engine_open(int device, emv_engine **out) {
+    emv_engine *e = new (std::nothrow) emv_engine();
+    if (!e) return EMV_ERR_NOMEM;
+    e->device = device;
+    e->prm.relax_blocks_per_cu = 8;
+    e->prm.merge_rows_per_wave = 0;
+    e->prm.use_graph = 1;
+    e->prm.graph_round_margin = 2;
+    *out = e;
+    if (hipSetDevice(device) != hipSuccess) return EMV_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EMV_ERR_HIP;
+    e->num_cu = prop.multiProcessorCount;
+    EHIP(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    EHIP(e, hipMalloc((void **)&e->d_any, sizeof(uint32_t) * 4));
+    EHIP(e, hipMalloc((void **)&e->d_cnt, sizeof(uint32_t) * 4));
+    if (const char *qc = getenv("EMV_QCAP")) e->qcap = (uint32_t)std::max(0, atoi(qc));
+    EHIP(e, hipMalloc((void **)&e->d_q[0], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
+    EHIP(e, hipMalloc((void **)&e->d_q[1], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
+    EHIP(e, hipHostMalloc((void **)&e->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault));
+    return EMV_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" int emv_engine_default_params(emv_engine_params *p) {
+    if (!p) return EMV_ERR_ARG;
+    p->relax_blocks_per_cu = 8;
+    p->merge_rows_per_wave = 0;
+    p->use_graph = 1;
+    p->graph_round_margin = 2;
+    return EMV_OK;
+}
+
+extern "C" int emv_engine_set_params(emv_engine *e, const emv_engine_params *p) {
+    if (!e || !p) return EMV_ERR_ARG;
+    if (p->relax_blocks_per_cu < 1 || p->relax_blocks_per_cu > 32 ||
+        (p->merge_rows_per_wave != 0 && p->merge_rows_per_wave != 1 &&
+         p->merge_rows_per_wave != 2 && p->merge_rows_per_wave != 4 &&
+         p->merge_rows_per_wave != 8) ||
+        p->graph_round_margin < -1000 || p->graph_round_margin > 1000) {
+        eset(e, "engine parameters out of range");
+        return EMV_ERR_ARG;
+    }
+    e->prm = *p;
+    return EMV_OK;
+}
+
+extern "C" const char *emv_engine_last_error(const emv_engine *e) {
+    return e ? e->err : "no engine";
+}
+
+extern "C" void emv_engine_destroy(emv_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    dfree(e->d_rp);
+    dfree(e->d_adj);
+    dfree(e->d_ell);
+    dfree(e->d_arena);
+    dfree(e->d_kk);
+    dfree(e->d_kkoff);
+    dfree(e->d_rowmax);
+    dfree(e->d_D);
+    dfree(e->d_code);
+    dfree(e->d_flag[0]);
+    dfree(e->d_flag[1]);
+    dfree(e->d_any);
+    dfree(e->d_cnt);
+    dfree(e->d_q[0]);
+    dfree(e->d_q[1]);
+    for (void *p : e->desc_bufs)
+        if (p) (void)hipFree(p);
+    if (e->h_pin) (void)hipHostFree(e->h_pin);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static int check_instance(emv_engine *e, const GraphRec &G, int64_t k, const int64_t *kk) {
+    if (k < 2 || k > 30 || !kk) {
+        eset(e, "k=%lld outside [2, 30]", (long long)k);
+        return EMV_ERR_ARG;
+    }
+    for (int64_t j = 0; j < k; j++)
+        if (kk[j] < 0 || kk[j] >= G.n) {
+            eset(e, "terminal %lld out of range", (long long)kk[j]);
+            return EMV_ERR_ARG;
+        }
+    // int32 table and 32-bit labels: every cell is at most k*(n-1)*max w
+    if ((double)k * (double)(G.n - 1) * (double)G.maxw >= (double)INT32_MAX ||
+        (1ll << k) + G.n >= INT32_MAX) {
+        eset(e, "instance exceeds the engine's 32-bit range (use emv_solve)");
+        return EMV_ERR_RANGE;
+    }
+    return EMV_OK;
+}
+
+static int submit(emv_engine *e, int graph, int64_t k, const int64_t *kk, uint32_t flags,
+                  int64_t *ticket) {
+    const GraphRec &G = e->graphs[graph];
+    ECHK(check_instance(e, G, k, kk));
+    const int64_t cells = 2 * ((int64_t)1 << k) * G.np;
+    if ((cells + 255) / 256 * 256 > e->arena.cap) {
+        eset(e, "request needs %lld table bytes, more than the arena's %lld", (long long)(cells * 4),
+             (long long)(e->arena.cap * 4));
+        return EMV_ERR_ADMISSION;
+    }
+    Inst I;
+    I.ticket = e->next_ticket++;
+    I.graph = graph;
+    I.k = (int)k;
+    I.kk.assign(kk, kk + k);
+    I.flags = flags;
+    I.t_submit = now_ms();
+    e->inst.push_back(I);
+    e->queue.push_back(I.ticket);
+    *ticket = I.ticket;
+    return EMV_OK;
+}
+
+// ---------------------------------------------------------------- batch
+
+extern "C" int emv_batch_create(int device, int64_t ninst, const emv_instance *in,
+                                emv_engine **out) {
+    if (!out || ninst < 1 || !in) return EMV_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EMV_ERR_NODEV;
+    if (device < 0 || device >= ndev) return EMV_ERR_ARG;
+    emv_engine *e = nullptr;
+    int rc = engine_open(device, &e);
+    if (rc != EMV_OK) {
+        emv_engine_destroy(e);
+        return rc;
+    }
+    std::vector<std::vector<int64_t>> edges((size_t)ninst);
+    std::vector<int64_t> ns((size_t)ninst);
+    for (int64_t i = 0; i < ninst; i++) {
+        if (in[i].n < 1 || in[i].m < 0 || (in[i].m > 0 && !in[i].edges)) {
+            eset(e, "instance %lld: bad graph", (long long)i);
+            *out = e;
+            return EMV_ERR_ARG;
+        }
+        ns[i] = in[i].n;
+        edges[i].assign(in[i].edges, in[i].edges + 3 * in[i].m);
+    }
+    *out = e;
+    ECHK(upload_graphs(e, edges, ns));
+    int64_t cells = 0;
+    for (int64_t i = 0; i < ninst; i++) {
+        ECHK(check_instance(e, e->graphs[i], in[i].k, in[i].kk));
+        cells += (2 * ((int64_t)1 << in[i].k) * e->graphs[i].np + 255) / 256 * 256;
+    }
+    EHIP(e, hipMalloc((void **)&e->d_arena, sizeof(int32_t) * cells));
+    e->arena.reset(cells);
+    ECHK(ensure_slots(e, (int)ninst));
+    for (int64_t i = 0; i < ninst; i++) {
+        int64_t t;
+        ECHK(submit(e, (int)i, in[i].k, in[i].kk, 0, &t));
+    }
+    return EMV_OK;
+}
+
+// Re-queues every instance of a batch (tables are rewritten by the solve).
+static int batch_reset(emv_engine *e, uint32_t flags) {
+    for (Inst &I : e->inst) {
+        if (I.abase >= 0) e->arena.release(I.abase, I.acells);
+        if (I.slot >= 0) e->free_slots.push_back(I.slot);
+        I.abase = -1;
+        I.slot = -1;
+        I.state = ST_QUEUED;
+        I.stage = 1;
+        I.status = EMV_OK;
+        I.flags = flags;
+        I.pairs.clear();
+        I.t_submit = now_ms();
+    }
+    e->queue.clear();
+    for (Inst &I : e->inst) e->queue.push_back(I.ticket);
+    return EMV_OK;
+}
+
+static int batch_solve_eager(emv_engine *e, uint32_t flags, emv_batch_stats *st) {
+    ECHK(batch_reset(e, flags));
+    e->rounds_per_step.clear();
+    for (;;) {
+        bool any_running = false;
+        for (const Inst &I : e->inst)
+            if (I.state != ST_DONE) any_running = true;
+        if (!any_running) break;
+        int used = 0;
+        ECHK(step_eager(e, &used));
+        e->rounds_per_step.push_back(used);
+    }
+    (void)st;
+    return EMV_OK;
+}
+
+// Lockstep replay of steps 2..kmax captured in one graph.
+static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
+    *ok = false;
+    ECHK(batch_reset(e, flags));
+    int used = 0;
+    ECHK(step_eager(e, &used));  // singletons + bounds, eager
+    for (const Inst &I : e->inst)
+        if (I.state == ST_DONE && I.status != EMV_OK) return EMV_OK;  // let eager report it
+    // plans of the remaining lockstep steps
+    std::vector<StepPlan> plans;
+    std::vector<std::vector<int64_t>> runs;
+    int kmax = 0;
+    for (const Inst &I : e->inst) kmax = std::max(kmax, I.k);
+    for (int m = 2; m <= kmax; m++) {
+        std::vector<int64_t> run;
+        for (const Inst &I : e->inst)
+            if (I.state == ST_RUNNING && I.k >= m) run.push_back(I.ticket);
+        runs.push_back(run);
+    }
+    // rounds per step from the last eager solve (+ margin)
+    if ((int)e->rounds_per_step.size() < kmax) return EMV_OK;
+    plans.resize(runs.size());
+    int64_t max_cells = 0, max_flags = 0;
+    // build with the stage each instance will have at that step
+    std::vector<int> saved_stage(e->inst.size());
+    for (size_t t = 0; t < e->inst.size(); t++) saved_stage[t] = e->inst[t].stage;
+    for (size_t s = 0; s < runs.size(); s++) {
+        for (int64_t t : runs[s]) e->inst[t].stage = (int)s + 2;
+        build_plan(e, runs[s], plans[s]);
+        max_cells = std::max(max_cells, plans[s].cells);
+        max_flags = std::max(max_flags, plans[s].flagbytes);
+    }
+    for (size_t t = 0; t < e->inst.size(); t++) e->inst[t].stage = saved_stage[t];
+    ECHK(ensure_stage(e, max_cells, max_flags));
+    std::vector<char *> dbs(plans.size());
+    for (size_t s = 0; s < plans.size(); s++) {
+        ECHK(desc_buffer(e, 1 + s, plans[s].bytes, &dbs[s]));
+        EHIP(e, hipMemcpyAsync(dbs[s], plans[s].blob.data(), plans[s].bytes,
+                               hipMemcpyHostToDevice, e->stream));
+    }
+    uint32_t *d_conv = nullptr;
+    EHIP(e, hipMalloc((void **)&d_conv, sizeof(uint32_t) * std::max<size_t>(plans.size(), 1)));
+    EHIP(e, hipStreamSynchronize(e->stream));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int rc = EMV_OK;
+    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipFree(d_conv);
+        eset(e, "hipStreamBeginCapture failed");
+        return EMV_ERR_HIP;
+    }
+    for (size_t s = 0; s < plans.size() && rc == EMV_OK; s++) {
+        rc = enqueue_front(e, dbs[s], plans[s]);
+        const int R = std::max(1, e->rounds_per_step[s + 1] + e->prm.graph_round_margin);
+        grelax_round(e, dbs[s], plans[s], 1, 0);
+        for (int r = 1; r < R; r++) grelax_round(e, dbs[s], plans[s], 0, r);
+        hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, e->stream, e->d_any, R, d_conv,
+                           (int)s);
+        if (rc == EMV_OK) rc = enqueue_back(e, dbs[s], plans[s]);
+    }
+    hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    if (rc != EMV_OK || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipFree(d_conv);
+        if (rc == EMV_OK) {
+            eset(e, "graph capture failed: %s", hipGetErrorString(ce));
+            rc = EMV_ERR_HIP;
+        }
+        return rc;
+    }
+    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ie == hipSuccess) ie = hipGraphLaunch(exec, e->stream);
+    if (ie == hipSuccess) ie = hipStreamSynchronize(e->stream);
+    std::vector<uint32_t> conv(plans.size(), 1);
+    if (ie == hipSuccess)
+        ie = hipMemcpy(conv.data(), d_conv, sizeof(uint32_t) * plans.size(), hipMemcpyDeviceToHost);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(d_conv);
+    if (ie != hipSuccess) {
+        eset(e, "graph replay failed: %s", hipGetErrorString(ie));
+        return EMV_ERR_HIP;
+    }
+    for (uint32_t w : conv)
+        if (w != 0) return EMV_OK;  // a step ran out of rounds: caller re-solves eagerly
+    // bookkeeping the eager steps would have done
+    for (size_t s = 0; s < plans.size(); s++) {
+        e->steps++;
+        e->relax_rounds += std::max(1, e->rounds_per_step[s + 1] + e->prm.graph_round_margin);
+        std::vector<int64_t> done;
+        for (int64_t t : runs[s]) {
+            Inst &I = e->inst[t];
+            I.stage++;
+            if (I.stage > I.k) done.push_back(t);
+        }
+        ECHK(read_costs_and_finish(e, done));
+    }
+    *ok = true;
+    return EMV_OK;
+}
+
+extern "C" int emv_batch_solve(emv_engine *e, uint32_t flags, int64_t *costs,
+                               emv_batch_stats *st) {
+    if (!e) return EMV_ERR_ARG;
+    EHIP(e, hipSetDevice(e->device));
+    const double t0 = now_ms();
+    const int64_t steps0 = e->steps, rounds0 = e->relax_rounds;
+    bool done = false;
+    int graph_used = 0, graph_fallback = 0;
+    if (e->prm.use_graph && !e->rounds_per_step.empty()) {
+        ECHK(batch_solve_graph(e, flags, &done));
+        if (done) graph_used = 1;
+        else graph_fallback = 1;
+    }
+    if (!done) {
+        // flags may be dirty after a replay that ran out of rounds
+        if (graph_fallback && e->flag_bytes > 0) {
+            const int blocks = (int)std::min<int64_t>((e->flag_bytes + kBlock - 1) / kBlock, 65536);
+            for (int i = 0; i < 2; i++)
+                hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                                   e->d_flag[i], e->flag_bytes, 0u);
+        }
+        ECHK(batch_solve_eager(e, flags, st));
+    }
+    int rc = EMV_OK;
+    for (size_t i = 0; i < e->inst.size(); i++) {
+        if (costs) costs[i] = e->inst[i].cost;
+        if (e->inst[i].status != EMV_OK && rc == EMV_OK) {
+            rc = e->inst[i].status;
+            eset(e, "instance %zu failed (%d)", i, rc);
+        }
+    }
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->total_ms = now_ms() - t0;
+        st->steps = e->steps - steps0;
+        st->relax_rounds = e->relax_rounds - rounds0;
+        st->graph_used = graph_used;
+        st->graph_fallback = graph_fallback;
+        int64_t cells = 0;
+        for (const Inst &I : e->inst) cells += ((int64_t)1 << I.k) * e->graphs[I.graph].n;
+        st->dp_cells = cells;
+    }
+    return rc;
+}
+
+extern "C" int emv_engine_read_rows(emv_engine *e, int64_t ticket, int64_t X0, int64_t nrows,
+                                    int64_t *f_out, int64_t *code_out) {
+    if (!e || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    const Inst &I = e->inst[ticket];
+    if (I.abase < 0 || I.state != ST_DONE) {
+        eset(e, "no table kept for ticket %lld", (long long)ticket);
+        return EMV_ERR_STATE;
+    }
+    const GraphRec &G = e->graphs[I.graph];
+    const int64_t S2 = 1ll << I.k;
+    if (X0 < 0 || nrows < 0 || X0 + nrows > S2) return EMV_ERR_ARG;
+    EHIP(e, hipSetDevice(e->device));
+    std::vector<int32_t> buf((size_t)G.np);
+    for (int64_t r = 0; r < nrows; r++) {
+        const int64_t X = X0 + r;
+        if (f_out) {
+            EHIP(e, hipMemcpy(buf.data(), e->d_arena + I.abase + X * G.np, sizeof(int32_t) * G.n,
+                              hipMemcpyDeviceToHost));
+            for (int64_t v = 0; v < G.n; v++)
+                f_out[r * G.n + v] = buf[v] == INT32_MAX ? INT64_MAX : (int64_t)buf[v];
+        }
+        if (code_out) {
+            EHIP(e, hipMemcpy(buf.data(), e->d_arena + I.abase + S2 * G.np + X * G.np,
+                              sizeof(int32_t) * G.n, hipMemcpyDeviceToHost));
+            for (int64_t v = 0; v < G.n; v++) code_out[r * G.n + v] = buf[v];
+        }
+    }
+    return EMV_OK;
+}
+
+extern "C" int emv_engine_pairs(emv_engine *e, int64_t ticket, int64_t *pairs, int64_t cap,
+                                int64_t *npairs) {
+    if (!e || !npairs || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    const Inst &I = e->inst[ticket];
+    *npairs = (int64_t)I.pairs.size() / 2;
+    if (pairs) {
+        if (cap < *npairs) return EMV_ERR_CAPACITY;
+        memcpy(pairs, I.pairs.data(), sizeof(int64_t) * I.pairs.size());
+    }
+    return EMV_OK;
+}
+
+extern "C" int emv_engine_release(emv_engine *e, int64_t ticket) {
+    if (!e || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    Inst &I = e->inst[ticket];
+    if (I.state == ST_DONE && I.abase >= 0) {
+        e->arena.release(I.abase, I.acells);
+        I.abase = -1;
+    }
+    return EMV_OK;
+}
+
+// --------------------------------------------------------------- server
+
+extern "C" int emv_server_create(int device, int64_t n, int64_t m, const int64_t *edges,
+                                 int64_t arena_bytes, int max_inflight, emv_engine **out) {
+    if (!out || n < 1 || m < 0 || (m > 0 && !edges) || max_inflight < 1 || max_inflight > 4096)
+        return EMV_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return EMV_ERR_NODEV;
+    if (device < 0 || device >= ndev) return EMV_ERR_ARG;
+    emv_engine *e = nullptr;
+    int rc = engine_open(device, &e);
+    *out = e;
+    if (rc != EMV_OK) return rc;
+    std::vector<std::vector<int64_t>> E(1);
+    E[0].assign(edges, edges + 3 * m);
+    std::vector<int64_t> ns(1, n);
+    ECHK(upload_graphs(e, E, ns));
+    if (arena_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        EHIP(e, hipMemGetInfo(&fr, &tot));
+        // keep room for staging and the runtime
+        arena_bytes = (int64_t)(fr * 0.6);
+    }
+    const int64_t cells = arena_bytes / 4 / 256 * 256;
+    EHIP(e, hipMalloc((void **)&e->d_arena, sizeof(int32_t) * std::max<int64_t>(cells, 256)));
+    e->arena.reset(cells);
+    ECHK(ensure_slots(e, max_inflight));
+    return EMV_OK;
+}
+
+extern "C" int emv_server_submit(emv_engine *e, int64_t k, const int64_t *kk, uint32_t flags,
+                                 int64_t *ticket) {
+    if (!e || !ticket) return EMV_ERR_ARG;
+    return submit(e, 0, k, kk, flags, ticket);
+}
+
+extern "C" int emv_server_step(emv_engine *e, int64_t *running, int64_t *queued) {
+    if (!e) return EMV_ERR_ARG;
+    EHIP(e, hipSetDevice(e->device));
+    int used = 0;
+    ECHK(step_eager(e, &used));
+    int64_t r = 0, q = 0;
+    for (const Inst &I : e->inst) {
+        if (I.state == ST_RUNNING) r++;
+        if (I.state == ST_QUEUED) q++;
+    }
+    if (running) *running = r;
+    if (queued) *queued = q;
+    return EMV_OK;
+}
+
+extern "C" int emv_server_poll(emv_engine *e, int64_t ticket, emv_request_info *info) {
+    if (!e || !info || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    const Inst &I = e->inst[ticket];
+    info->state = I.state;
+    info->status = I.status;
+    info->cost = I.cost;
+    info->k = I.k;
+    info->stage = I.stage;
+    info->queue_ms = (I.state == ST_QUEUED) ? 0.0 : I.t_admit - I.t_submit;
+    info->latency_ms = (I.state == ST_DONE) ? I.t_done - I.t_submit : 0.0;
+    info->npairs = (int64_t)I.pairs.size() / 2;
+    return EMV_OK;
+}
+
+extern "C" int64_t emv_server_free_bytes(const emv_engine *e) {
+    return e ? e->arena.largest() * 4 : -1;
+}
+
+extern "C" int64_t emv_request_footprint(const emv_engine *e, int64_t k) {
+    if (!e || e->graphs.empty() || k < 1 || k > 30) return -1;
+    return 2 * ((int64_t)1 << k) * e->graphs[0].np * 4;
+}

@@ -54,7 +54,10 @@ struct emv_ctx {
     // relaxation workspace, sized in chunks of 64 rows
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     uint32_t *d_code = nullptr;   // record codes
-    uint8_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
+    uint32_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
+    uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
+    uint32_t qcap = 0;
+    uint32_t *d_cnt = nullptr;                // 4 rotating worklist counters
     void *d_M = nullptr;          // packed merge accumulators, row-major, 8 bytes per cell reserved
     int64_t ws_chunks = 0;
     uint32_t *d_any = nullptr;               // 4 rotating "round has work" words
@@ -165,6 +168,9 @@ int ensure_table(emv_ctx *c, size_t fbytes, size_t bbytes) {
 int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
 
 void free_ws(emv_ctx *c) {
+    dfree(c->d_q[0]);
+    dfree(c->d_q[1]);
+    c->qcap = 0;
     dfree(c->d_M);
     dfree(c->d_D);
     dfree(c->d_code);
@@ -179,8 +185,8 @@ int clear_flags(emv_ctx *c) {
     const int64_t items = c->ws_chunks * c->np;
     int blocks = (int)std::min<int64_t>((items + kBlock - 1) / kBlock, 65536);
     for (int i = 0; i < 2; i++)
-        hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
-                           c->d_flag[i], items, (uint8_t)0);
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
+                           c->d_flag[i], items, 0u);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
 }
@@ -192,9 +198,17 @@ int ensure_ws(emv_ctx *c, int64_t chunks) {
     const int64_t items = chunks * c->np;
     HIPCHK(c, hipMalloc(&c->d_D, sizeof(uint64_t) * cells));
     HIPCHK(c, hipMalloc((void **)&c->d_code, sizeof(uint32_t) * cells));
-    HIPCHK(c, hipMalloc((void **)&c->d_flag[0], items));
-    HIPCHK(c, hipMalloc((void **)&c->d_flag[1], items));
+    HIPCHK(c, hipMalloc((void **)&c->d_flag[0], sizeof(uint32_t) * items));
+    HIPCHK(c, hipMalloc((void **)&c->d_flag[1], sizeof(uint32_t) * items));
     HIPCHK(c, hipMalloc(&c->d_M, sizeof(uint64_t) * cells));
+    // Worklist rounds are off unless EMV_QCAP sets a capacity: every append
+    // is an atomic on one counter, which costs more per round than scanning
+    // the flags in order (measured on the 8-regular and grid workloads).
+    int64_t qcap = 0;
+    if (const char *qc = getenv("EMV_QCAP")) qcap = std::min<int64_t>(items, std::max(0ll, atoll(qc)));
+    HIPCHK(c, hipMalloc((void **)&c->d_q[0], sizeof(uint32_t) * std::max<int64_t>(qcap, 1)));
+    HIPCHK(c, hipMalloc((void **)&c->d_q[1], sizeof(uint32_t) * std::max<int64_t>(qcap, 1)));
+    c->qcap = (uint32_t)qcap;
     c->ws_chunks = chunks;
     return clear_flags(c);
 }
@@ -218,7 +232,8 @@ void launch_relax_round(emv_ctx *c, int64_t nitems, uint32_t pred_base, int dens
     hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
                        (DT *)c->d_D, c->d_code, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        pred_base, nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
-                       c->d_any, r, c->relax_nxcd);
+                       (const uint32_t *)c->d_q[r & 1], c->d_q[(r + 1) & 1], c->qcap, c->d_any,
+                       c->d_cnt, r, c->relax_nxcd);
 }
 
 template <typename DT>
@@ -236,6 +251,8 @@ int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *round
     const int64_t nitems = nchunks * c->n;
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_any, (int64_t)4,
                        0u);
+    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_cnt, (int64_t)4,
+                       0u);
     int r = 0;
     relax_round<DT>(c, nitems, pred_base, 1, r);
     HIPCHK(c, hipGetLastError());
@@ -252,7 +269,7 @@ int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *round
                                  hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (*c->h_pin == 0) break;
-        group = std::min(group * 2, 8);
+        group = std::min(group * 2, 32);
         if (rounds > (int64_t)2 * (c->n + 8)) {
             set_err(c, "relaxation did not converge");
             return EMV_ERR_RANGE;
@@ -510,6 +527,7 @@ extern "C" int emv_open(emv_ctx **out, int device) {
             break;
         }
         if (hipMalloc((void **)&c->d_any, sizeof(uint32_t) * 4) != hipSuccess ||
+            hipMalloc((void **)&c->d_cnt, sizeof(uint32_t) * 4) != hipSuccess ||
             hipMalloc((void **)&c->d_rowmax, sizeof(unsigned long long) * 64) != hipSuccess ||
             hipMalloc((void **)&c->d_misc, sizeof(int64_t) * 8) != hipSuccess ||
             hipHostMalloc((void **)&c->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault) !=
@@ -536,6 +554,7 @@ extern "C" void emv_close(emv_ctx *c) {
     dfree(c->d_adj);
     dfree(c->d_ell);
     dfree(c->d_any);
+    dfree(c->d_cnt);
     dfree(c->d_rowmax);
     dfree(c->d_misc);
     dfree(c->d_xs);

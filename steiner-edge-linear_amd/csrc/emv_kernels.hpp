@@ -67,20 +67,32 @@ __global__ void k_seed_v(DT *__restrict__ D, uint32_t *__restrict__ CODE,
 
 // ------------------------------------------------------------ relaxation
 // One pull round over (chunk, vertex) items, item i = (chunk i / n, vertex
-// i % n).  Round r processes the items flagged in F_cur (every item when
-// `dense`); a wave loads 64 consecutive flags at once and walks the set ones.
+// i % n).  Every item whose neighbour's label dropped in the previous round
+// is flagged (a byte in F_cur); round 0 (`dense`) takes every item.  A round
+// takes its items in one of two ways, decided on the device from the
+// previous round's counters:
+//   worklist : the flagged items were also appended to Q_cur (cnt[r&3] of
+//              them, at most qcap) -- a thin frontier costs only its size;
+//   flag scan: too many were flagged for the list, so the wave reads 64
+//              consecutive flags at once and walks the set ones.
 // Per item the wave recomputes, for its 64 rows, the lexicographic minimum of
 // its own (label, code) and (neighbour label + w, 2^k + neighbour) over the
 // whole adjacency and writes what changed.  If any label dropped it flags
-// every neighbour in F_next and raises any[(r+1)&3].  Flags are bytes written
-// with plain stores (idempotent, no atomics); a round clears the flags it
-// consumed, so F_next is clean when the round starts, and clears
-// any[(r+2)&3] for the round after next.  Pulling the full adjacency after
-// every neighbour change makes the final (label, code) of each cell the exact
-// lexicographic minimum, independent of the order waves run in.
+// every neighbour in F_next (a writer that finds the flag clear also appends
+// the neighbour to Q_next while the list has room) and raises any[(r+1)&3].
+// Each flagged item is processed by exactly one wave per round: the flag
+// scan reads each flag once, and a listed item is taken only by the wave
+// whose atomic exchange clears its flag, so duplicate list entries are
+// skipped.  One writer per cell per round keeps the masked (label, code)
+// updates free of races between waves.  A
+// round clears the flags it consumed, so F_next is clean when the round
+// starts, and clears any/cnt[(r+2)&3] for the round after next.  Pulling the
+// full adjacency after every neighbour change makes the final (label, code)
+// of each cell the exact lexicographic minimum, independent of the order
+// waves run in.
 //
 // Adjacency: DMAX > 0 uses the padded fixed-degree copy ell[v * DMAX + j]
-// (missing neighbours have v = -1).  The 64 items of a flag block load their
+// (missing neighbours have v = -1).  The 64 items a wave holds load their
 // rows into registers with one coalesced load, so an item's neighbour labels
 // are gathered with no dependent load in front, and two items are processed
 // together to keep 2 * DMAX gathers in flight.  DMAX == 0 walks the CSR.
@@ -97,44 +109,103 @@ __device__ __forceinline__ void relax_fold(DT du, int u, int w, uint32_t pred_ba
     }
 }
 
+// Wave-aggregated append of `entry` to a list of capacity cap (one atomic per
+// wave).  Every lane of the wave must call it.  Entries past cap are dropped;
+// the counter still counts them, which tells the next round to scan flags.
+__device__ __forceinline__ void wave_append(bool want, uint32_t entry, uint32_t *__restrict__ q,
+                                            uint32_t *__restrict__ cnt, uint32_t cap) {
+    const uint64_t mask = __ballot(want);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(cnt, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (want) {
+        const uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (at < cap) q[at] = entry;
+    }
+}
+
+// Flags one neighbour per lane (nb < 0: none) of a changed item of chunk c;
+// a neighbour whose flag was clear is also appended to the list while it has
+// room.  Writers racing on one flag may both append it; the consumer's
+// atomic exchange lets only one of them through.
+__device__ __forceinline__ void flag_neighbours(int64_t c, int64_t np, int n, int nb,
+                                                uint32_t *__restrict__ Fnext,
+                                                uint32_t *__restrict__ Qnext,
+                                                uint32_t *__restrict__ cnext, uint32_t qcap,
+                                                bool list_open) {
+    bool want = false;
+    uint32_t ent = 0;
+    if (nb >= 0) {
+        uint32_t *fp = Fnext + c * np + nb;
+        if (*fp == 0) {
+            *fp = 1;
+            want = list_open;
+            ent = (uint32_t)(c * n + nb);
+        }
+    }
+    wave_append(want, ent, Qnext, cnext, qcap);
+}
+
 template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_relax_v(
     DT *__restrict__ D, uint32_t *__restrict__ CODE, int64_t np, int n,
     const int32_t *__restrict__ rp, const Edge *__restrict__ adj, const Edge *__restrict__ ell,
-    uint32_t pred_base, int64_t nitems, int dense, uint8_t *__restrict__ Fcur,
-    uint8_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r, int nxcd) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) any[(r + 2) & 3] = 0;
+    uint32_t pred_base, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
+    uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur, uint32_t *__restrict__ Qnext,
+    uint32_t qcap, uint32_t *__restrict__ any, uint32_t *__restrict__ cnt, int r, int nxcd) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        any[(r + 2) & 3] = 0;
+        cnt[(r + 2) & 3] = 0;
+    }
     if (!dense && any[r & 3] == 0) return;  // nothing flagged for this round
+    const uint32_t qn = dense ? 0u : cnt[r & 3];
+    const bool use_list = !dense && qn < qcap;
+    uint32_t *cnext = &cnt[(r + 1) & 3];
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
-    // nxcd > 1: each XCD sweeps its own contiguous range of items; nxcd == 1:
-    // the whole grid sweeps the items in order.
+    // nxcd > 1: each XCD sweeps its own contiguous range of item blocks;
+    // nxcd == 1: the whole grid sweeps them in order.
     const int64_t waves_per_block = kBlock / kWave;
     const int64_t per = gridDim.x / nxcd;
     const int64_t b = blockIdx.x;
     const int64_t wave0 = ((b % nxcd) * per + b / nxcd) * waves_per_block + wid;
     const int64_t wstride = (int64_t)gridDim.x * waves_per_block;
-    const int64_t nblk = (nitems + kWave - 1) / kWave;
+    const int64_t span = use_list ? (int64_t)qn : nitems;
+    const int64_t nblk = (span + kWave - 1) / kWave;
     const DT INF = (DT)~(DT)0;
     bool raised = false;
     for (int64_t blk = wave0; blk < nblk; blk += wstride) {
-        const int64_t base = blk * kWave;
-        const int64_t mi = base + lane;
-        const int64_t mc = mi / n;
-        const int mv = (int)(mi - mc * n);
-        const int64_t mf = mc * np + mv;
+        const int64_t j = blk * kWave + lane;
+        int64_t mi = -1;
+        if (j < span) mi = use_list ? (int64_t)Qcur[j] : j;
+        int mc = 0, mv = 0;
         bool fl = false;
-        if (mi < nitems) fl = dense ? true : (Fcur[mf] != 0);
+        if (mi >= 0) {
+            mc = (int)(mi / n);
+            mv = (int)(mi - (int64_t)mc * n);
+            const int64_t mf = (int64_t)mc * np + mv;
+            if (dense) {
+                fl = true;
+            } else if (use_list) {
+                fl = atomicExch(&Fcur[mf], 0u) != 0;  // one of the listed copies
+            } else if (Fcur[mf] != 0) {
+                fl = true;
+                Fcur[mf] = 0;
+            }
+        }
         uint64_t act = __ballot(fl);
-        if (!dense && fl) Fcur[mf] = 0;
         if (act == 0) continue;
+        const bool list_open = *(volatile uint32_t *)cnext < qcap;
         if constexpr (DMAX > 0) {
             Edge e[DMAX];
 #pragma unroll
-            for (int j = 0; j < DMAX; j++) e[j] = Edge{-1, 0};
-            if (mi < nitems) {
+            for (int q = 0; q < DMAX; q++) e[q] = Edge{-1, 0};
+            if (fl) {
 #pragma unroll
-                for (int j = 0; j < DMAX; j++) e[j] = ell[(int64_t)mv * DMAX + j];
+                for (int q = 0; q < DMAX; q++) e[q] = ell[(int64_t)mv * DMAX + q];
             }
             while (act) {
                 const int t1 = __ffsll((unsigned long long)act) - 1;
@@ -142,8 +213,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 const bool two = act != 0;
                 const int t2 = two ? __ffsll((unsigned long long)act) - 1 : t1;
                 if (two) act &= act - 1;
-                const int64_t ca = (base + t1) / n, cb = (base + t2) / n;
-                const int va = (int)(base + t1 - ca * n), vb = (int)(base + t2 - cb * n);
+                const int64_t ca = __builtin_amdgcn_readlane(mc, t1);
+                const int64_t cb = __builtin_amdgcn_readlane(mc, t2);
+                const int va = __builtin_amdgcn_readlane(mv, t1);
+                const int vb = __builtin_amdgcn_readlane(mv, t2);
                 DT *Da = D + ca * np * kWave;
                 DT *Db = D + cb * np * kWave;
                 const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
@@ -154,23 +227,23 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 int ua[DMAX], wa[DMAX], ub[DMAX], wb[DMAX];
                 DT da[DMAX], db[DMAX];
 #pragma unroll
-                for (int j = 0; j < DMAX; j++) {
-                    ua[j] = __builtin_amdgcn_readlane(e[j].v, t1);
-                    wa[j] = __builtin_amdgcn_readlane(e[j].w, t1);
-                    ub[j] = __builtin_amdgcn_readlane(e[j].v, t2);
-                    wb[j] = __builtin_amdgcn_readlane(e[j].w, t2);
+                for (int q = 0; q < DMAX; q++) {
+                    ua[q] = __builtin_amdgcn_readlane(e[q].v, t1);
+                    wa[q] = __builtin_amdgcn_readlane(e[q].w, t1);
+                    ub[q] = __builtin_amdgcn_readlane(e[q].v, t2);
+                    wb[q] = __builtin_amdgcn_readlane(e[q].w, t2);
                 }
 #pragma unroll
-                for (int j = 0; j < DMAX; j++) {
-                    da[j] = ua[j] >= 0 ? Da[(int64_t)ua[j] * kWave + lane] : INF;
-                    db[j] = (two && ub[j] >= 0) ? Db[(int64_t)ub[j] * kWave + lane] : INF;
+                for (int q = 0; q < DMAX; q++) {
+                    da[q] = ua[q] >= 0 ? Da[(int64_t)ua[q] * kWave + lane] : INF;
+                    db[q] = (two && ub[q] >= 0) ? Db[(int64_t)ub[q] * kWave + lane] : INF;
                 }
                 DT besta = cura, bestb = curb;
                 uint32_t bca = cca, bcb = ccb;
 #pragma unroll
-                for (int j = 0; j < DMAX; j++) {
-                    relax_fold(da[j], ua[j], wa[j], pred_base, INF, besta, bca);
-                    relax_fold(db[j], ub[j], wb[j], pred_base, INF, bestb, bcb);
+                for (int q = 0; q < DMAX; q++) {
+                    relax_fold(da[q], ua[q], wa[q], pred_base, INF, besta, bca);
+                    relax_fold(db[q], ub[q], wb[q], pred_base, INF, bestb, bcb);
                 }
                 const bool chga = besta < cura, chgb = two && bestb < curb;
                 if (chga) Da[ia] = besta;
@@ -179,31 +252,31 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 if (two && bcb != ccb) CODE[cb * np * kWave + ib] = bcb;
                 const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
                 if (anya || anyb) {
-                    // lane j < DMAX flags neighbour j
+                    // lane q < DMAX takes neighbour q
                     int na = -1, nb = -1;
 #pragma unroll
-                    for (int j = 0; j < DMAX; j++) {
-                        if (lane == j) {
-                            na = ua[j];
-                            nb = ub[j];
+                    for (int q = 0; q < DMAX; q++) {
+                        if (lane == q) {
+                            na = ua[q];
+                            nb = ub[q];
                         }
                     }
-                    if (anya && na >= 0) Fnext[ca * np + na] = 1;
-                    if (anyb && nb >= 0) Fnext[cb * np + nb] = 1;
+                    if (anya) flag_neighbours(ca, np, n, na, Fnext, Qnext, cnext, qcap, list_open);
+                    if (anyb) flag_neighbours(cb, np, n, nb, Fnext, Qnext, cnext, qcap, list_open);
                     raised = true;
                 }
             }
         } else {
             int me0 = 0, me1 = 0;
-            if (mi < nitems) {
+            if (fl) {
                 me0 = rp[mv];
                 me1 = rp[mv + 1];
             }
             while (act) {
                 const int t = __ffsll((unsigned long long)act) - 1;
                 act &= act - 1;
-                const int64_t c = (base + t) / n;
-                const int v = (int)(base + t - c * n);
+                const int64_t c = __builtin_amdgcn_readlane(mc, t);
+                const int v = __builtin_amdgcn_readlane(mv, t);
                 DT *Dc = D + c * np * kWave;
                 uint32_t *Cc = CODE + c * np * kWave;
                 const int64_t idx = (int64_t)v * kWave + lane;
@@ -214,30 +287,34 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 const int e0 = __builtin_amdgcn_readlane(me0, t);
                 const int e1 = __builtin_amdgcn_readlane(me1, t);
                 for (int eb = e0; eb < e1; eb += kWave) {
-                    const int cnt = min(kWave, e1 - eb);
+                    const int cnt_e = min(kWave, e1 - eb);
                     Edge my = {0, 0};
-                    if (lane < cnt) my = adj[eb + lane];
-                    for (int t0 = 0; t0 < cnt; t0 += 4) {
+                    if (lane < cnt_e) my = adj[eb + lane];
+                    for (int t0 = 0; t0 < cnt_e; t0 += 4) {
                         // four neighbours per step: their loads issue together
                         DT du[4];
                         int uu[4], ww[4];
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
-                            const int tt = min(t0 + q, cnt - 1);
+                            const int tt = min(t0 + q, cnt_e - 1);
                             uu[q] = __builtin_amdgcn_readlane(my.v, tt);
                             ww[q] = __builtin_amdgcn_readlane(my.w, tt);
                             du[q] = Dc[(int64_t)uu[q] * kWave + lane];
                         }
 #pragma unroll
                         for (int q = 0; q < 4; q++)
-                            if (t0 + q < cnt) relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
+                            if (t0 + q < cnt_e) relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
                     }
                 }
                 const bool dchg = best < cur;
                 if (dchg) Dc[idx] = best;
                 if (bc != cc) Cc[idx] = bc;
                 if (__ballot(dchg) != 0) {
-                    for (int e = e0 + lane; e < e1; e += kWave) Fnext[c * np + adj[e].v] = 1;
+                    for (int eb = e0; eb < e1; eb += kWave) {
+                        const int e = eb + lane;
+                        flag_neighbours(c, np, n, e < e1 ? adj[e].v : -1, Fnext, Qnext, cnext, qcap,
+                                        list_open);
+                    }
                     raised = true;
                 }
             }
@@ -514,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
 }
 
 // Distances and predecessors of one shortest-path query (chunk 0, lane 0).
-__global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, const uint32_t *__restrict__ CODE,
+[[maybe_unused]] static __global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, const uint32_t *__restrict__ CODE,
                                 int n, uint32_t pred_base, int64_t *__restrict__ d,
                                 int64_t *__restrict__ pred) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -530,7 +607,7 @@ __global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, const uint32_t *
 // (reader-el.c:1907-1932): an edge record emits (v,u) and continues at
 // (u, X); a merge record visits (v, Y) then (v, X\Y); Y == X stops.
 // One lane walks; real trees are a few hundred steps.
-__global__ void k_traceback(const int32_t *__restrict__ b, int64_t np, int k, int64_t q,
+[[maybe_unused]] static __global__ void k_traceback(const int32_t *__restrict__ b, int64_t np, int k, int64_t q,
                             int64_t C, int64_t *__restrict__ pairs, int64_t cap,
                             int64_t *__restrict__ stack, int64_t stcap,
                             int64_t *__restrict__ result) {

@@ -252,3 +252,177 @@ def group_reduce(n, edges, grp):
 
 def uniform(seed, stream, index):
     return gen().emvgen_u64(seed, stream, index)
+
+
+# ----------------------------------------------------- multi-instance engine
+
+class EmvInstance(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("m", ctypes.c_int64), ("edges", _i64p),
+                ("k", ctypes.c_int64), ("kk", _i64p)]
+
+
+class EngineParams(ctypes.Structure):
+    _fields_ = [("relax_blocks_per_cu", ctypes.c_int32), ("merge_rows_per_wave", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("graph_round_margin", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("steps", ctypes.c_int64),
+                ("relax_rounds", ctypes.c_int64), ("dp_cells", ctypes.c_int64),
+                ("graph_used", ctypes.c_int32), ("graph_fallback", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("cost", ctypes.c_int64),
+                ("k", ctypes.c_int32), ("stage", ctypes.c_int32), ("queue_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("npairs", ctypes.c_int64)]
+
+
+_ENG_BOUND = False
+
+
+def _bind_engine():
+    global _ENG_BOUND
+    if _ENG_BOUND:
+        return lib()
+    L = lib()
+    vp = ctypes.c_void_p
+    L.emv_engine_default_params.argtypes = [ctypes.POINTER(EngineParams)]
+    L.emv_engine_set_params.argtypes = [vp, ctypes.POINTER(EngineParams)]
+    L.emv_engine_last_error.argtypes = [vp]
+    L.emv_engine_last_error.restype = ctypes.c_char_p
+    L.emv_engine_destroy.argtypes = [vp]
+    L.emv_engine_destroy.restype = None
+    L.emv_batch_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(EmvInstance),
+                                   ctypes.POINTER(vp)]
+    L.emv_batch_solve.argtypes = [vp, ctypes.c_uint32, _i64p, ctypes.POINTER(BatchStats)]
+    L.emv_server_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _i64p,
+                                    ctypes.c_int64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.emv_server_submit.argtypes = [vp, ctypes.c_int64, _i64p, ctypes.c_uint32,
+                                    ctypes.POINTER(ctypes.c_int64)]
+    L.emv_server_step.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.POINTER(ctypes.c_int64)]
+    L.emv_server_poll.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(RequestInfo)]
+    L.emv_server_free_bytes.argtypes = [vp]
+    L.emv_server_free_bytes.restype = ctypes.c_int64
+    L.emv_request_footprint.argtypes = [vp, ctypes.c_int64]
+    L.emv_request_footprint.restype = ctypes.c_int64
+    L.emv_engine_read_rows.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64p,
+                                       _i64p]
+    L.emv_engine_pairs.argtypes = [vp, ctypes.c_int64, _i64p, ctypes.c_int64,
+                                   ctypes.POINTER(ctypes.c_int64)]
+    L.emv_engine_release.argtypes = [vp, ctypes.c_int64]
+    _ENG_BOUND = True
+    return L
+
+
+class _Engine:
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        self._keep = []
+
+    def _check(self, st):
+        if st != 0:
+            msg = _bind_engine().emv_engine_last_error(self._h) if self._h else b"engine"
+            raise EmvError(st, msg.decode())
+
+    def close(self):
+        if self._h:
+            _bind_engine().emv_engine_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_params(self, relax_blocks_per_cu=8, merge_rows_per_wave=0, use_graph=1,
+                   graph_round_margin=2):
+        p = EngineParams(relax_blocks_per_cu, merge_rows_per_wave, use_graph, graph_round_margin)
+        self._check(_bind_engine().emv_engine_set_params(self._h, ctypes.byref(p)))
+
+    def read_rows(self, ticket, X0, nrows, n, codes=True):
+        f = np.empty(nrows * n, dtype=np.int64)
+        c = np.empty(nrows * n, dtype=np.int64) if codes else None
+        self._check(_bind_engine().emv_engine_read_rows(self._h, ticket, X0, nrows, _p(f),
+                                                        _p(c) if codes else None))
+        return f.reshape(nrows, n), (c.reshape(nrows, n) if codes else None)
+
+    def pairs(self, ticket):
+        L = _bind_engine()
+        cnt = ctypes.c_int64(0)
+        self._check(L.emv_engine_pairs(self._h, ticket, None, 0, ctypes.byref(cnt)))
+        out = np.empty(2 * max(cnt.value, 1), dtype=np.int64)
+        self._check(L.emv_engine_pairs(self._h, ticket, _p(out), cnt.value, ctypes.byref(cnt)))
+        return out[: 2 * cnt.value].reshape(-1, 2)
+
+    def release(self, ticket):
+        self._check(_bind_engine().emv_engine_release(self._h, ticket))
+
+
+class Batch(_Engine):
+    """Many independent instances solved together (one merge launch, shared
+    relaxation rounds and one unpack launch per DP layer)."""
+
+    def __init__(self, instances, device=0):
+        super().__init__()
+        L = _bind_engine()
+        arr = (EmvInstance * len(instances))()
+        for i, (n, edges, kk) in enumerate(instances):
+            e = np.ascontiguousarray(edges, dtype=np.int64).reshape(-1)
+            t = np.ascontiguousarray(kk, dtype=np.int64)
+            self._keep += [e, t]
+            arr[i] = EmvInstance(n, len(e) // 3, _p(e), len(t), _p(t))
+        st = L.emv_batch_create(device, len(instances), arr, ctypes.byref(self._h))
+        if st != 0:
+            msg = L.emv_engine_last_error(self._h).decode() if self._h else "emv_batch_create"
+            self.close()
+            raise EmvError(st, msg)
+        self.count = len(instances)
+        self._keep = []
+
+    def solve(self, flags=0):
+        costs = np.empty(self.count, dtype=np.int64)
+        st = BatchStats()
+        self._check(_bind_engine().emv_batch_solve(self._h, flags, _p(costs), ctypes.byref(st)))
+        return costs, st
+
+
+class Server(_Engine):
+    """Continuous-batching query server on one resident graph."""
+
+    def __init__(self, n, edges, arena_bytes=0, max_inflight=64, device=0):
+        super().__init__()
+        L = _bind_engine()
+        e = np.ascontiguousarray(edges, dtype=np.int64).reshape(-1)
+        st = L.emv_server_create(device, n, len(e) // 3, _p(e), arena_bytes, max_inflight,
+                                 ctypes.byref(self._h))
+        if st != 0:
+            msg = L.emv_engine_last_error(self._h).decode() if self._h else "emv_server_create"
+            self.close()
+            raise EmvError(st, msg)
+        self.n = n
+
+    def submit(self, kk, flags=0):
+        t = np.ascontiguousarray(kk, dtype=np.int64)
+        ticket = ctypes.c_int64(-1)
+        self._check(_bind_engine().emv_server_submit(self._h, len(t), _p(t), flags,
+                                                     ctypes.byref(ticket)))
+        return ticket.value
+
+    def step(self):
+        r, q = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(_bind_engine().emv_server_step(self._h, ctypes.byref(r), ctypes.byref(q)))
+        return r.value, q.value
+
+    def poll(self, ticket):
+        info = RequestInfo()
+        self._check(_bind_engine().emv_server_poll(self._h, ticket, ctypes.byref(info)))
+        return info
+
+    def free_bytes(self):
+        return _bind_engine().emv_server_free_bytes(self._h)
+
+    def footprint(self, k):
+        return _bind_engine().emv_request_footprint(self._h, k)

@@ -1,0 +1,140 @@
+"""Batched solves and the query server (multi-instance engine) on the GPU:
+every instance's table, records and cost equal the oracle's, alone and in
+the mix, eagerly and from a hipGraph replay.  Integer work: tolerance 0."""
+import numpy as np
+import pytest
+
+import emv
+import oracle
+from golden_data import META, load_instance
+from stputil import check_listed_tree, Instance
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_instances(count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(2 ** rng.uniform(8, 11.5))
+        d = int(rng.integers(3, 9))
+        if (n * d) % 2:
+            n += 1
+        k = int(rng.integers(3, 9))
+        e = emv.regular_graph(n, d, 1000, 900 + 17 * seed + i)
+        kk = emv.terminals(n, k, 900 + 17 * seed + i)
+        out.append((n, e, kk))
+    return out
+
+
+def test_batch_matches_oracle_eager_and_graph():
+    insts = mixed_instances(12, 1)
+    with emv.Batch(insts) as b:
+        c1, s1 = b.solve(emv.KEEP_TABLE | emv.LIST)
+        assert s1.graph_used == 0
+        for i, (n, e, kk) in enumerate(insts):
+            ocost, of, ocode = oracle.emv(n, e, kk)
+            assert c1[i] == ocost
+            f, code = b.read_rows(i, 0, 1 << len(kk), n)
+            np.testing.assert_array_equal(f, of)
+            np.testing.assert_array_equal(code, ocode)
+            ok, why = check_listed_tree(Instance(n, e, kk), b.pairs(i), c1[i])
+            assert ok, why
+        # second solve: replayed from a graph, identical tables
+        c2, s2 = b.solve(emv.KEEP_TABLE)
+        assert s2.graph_used == 1 or s2.graph_fallback == 1
+        np.testing.assert_array_equal(c1, c2)
+        for i, (n, e, kk) in enumerate(insts):
+            _, of, ocode = oracle.emv(n, e, kk)
+            f, code = b.read_rows(i, 0, 1 << len(kk), n)
+            np.testing.assert_array_equal(f, of)
+            np.testing.assert_array_equal(code, ocode)
+
+
+def test_batch_of_testset_instances():
+    names = ["small-instances/B/b01", "small-instances/C/c01", "small-instances/P6E/p619",
+             "small-instances/I080/i080-011", "small-instances/D/d06", "small-instances/SP/se03"]
+    insts = [load_instance(nm) for nm in names]
+    with emv.Batch([(x.n, x.edges, x.terminals) for x in insts]) as b:
+        costs, _ = b.solve()
+        for x, c in zip(insts, costs):
+            assert c == x.cost
+        b.set_params(use_graph=1, graph_round_margin=0)
+        costs2, st = b.solve()
+        np.testing.assert_array_equal(costs, costs2)
+
+
+def test_batch_graph_replay_short_of_rounds_falls_back():
+    """A replay given too few relaxation rounds (forced from the host with a
+    negative margin) is caught by the per-step convergence words and the
+    batch is solved again eagerly, with the same results."""
+    insts = mixed_instances(4, 2)
+    with emv.Batch(insts) as b:
+        c1, _ = b.solve()
+        b.set_params(use_graph=1, graph_round_margin=-6)
+        c2, st = b.solve()
+        assert st.graph_fallback == 1 and st.graph_used == 0
+        np.testing.assert_array_equal(c1, c2)
+        b.set_params(use_graph=1, graph_round_margin=2)
+        c3, st = b.solve()
+        assert st.graph_used == 1
+        np.testing.assert_array_equal(c1, c3)
+
+
+def test_server_results_alone_equal_in_the_mix():
+    R, C = 40, 50
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 41)
+    rng = np.random.default_rng(5)
+    queries = [emv.distinct(n, int(rng.integers(3, 8)), 4000 + i, 2) for i in range(10)]
+    # alone: one request at a time
+    alone = []
+    with emv.Server(n, e, arena_bytes=256 << 20, max_inflight=16) as s:
+        for q in queries:
+            t = s.submit(q)
+            while s.poll(t).state != 2:
+                s.step()
+            info = s.poll(t)
+            assert info.status == 0
+            alone.append(info.cost)
+    # in the mix: all submitted at once, stepped together at different layers
+    with emv.Server(n, e, arena_bytes=256 << 20, max_inflight=16) as s:
+        tickets = []
+        for i, q in enumerate(queries):
+            tickets.append(s.submit(q, emv.KEEP_TABLE))
+            if i % 3 == 0:
+                s.step()  # later requests join while earlier ones are mid-way
+        while any(s.poll(t).state != 2 for t in tickets):
+            s.step()
+        for t, q, c in zip(tickets, queries, alone):
+            info = s.poll(t)
+            assert info.status == 0 and info.cost == c
+            ocost, of, ocode = oracle.emv(n, e, q)
+            assert c == ocost
+            f, code = s.read_rows(t, 0, 1 << len(q), n)
+            np.testing.assert_array_equal(f, of)
+            np.testing.assert_array_equal(code, ocode)
+            s.release(t)
+
+
+def test_server_admission_by_footprint():
+    R, C = 30, 30
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 43)
+    with emv.Server(n, e, arena_bytes=0, max_inflight=4) as probe:
+        fp8 = probe.footprint(8)
+    # arena for one k=8 request at a time
+    with emv.Server(n, e, arena_bytes=fp8 + 4096, max_inflight=4) as s:
+        with pytest.raises(emv.EmvError) as ei:
+            s.submit(emv.distinct(n, 10, 1, 2))  # 4x the arena: never fits
+        assert ei.value.status == -9
+        t1 = s.submit(emv.distinct(n, 8, 2, 2))
+        t2 = s.submit(emv.distinct(n, 8, 3, 2))
+        s.step()
+        assert s.poll(t1).state == 1 and s.poll(t2).state == 0  # t2 waits for memory
+        while s.poll(t2).state != 2:
+            s.step()
+        for t, seed in ((t1, 2), (t2, 3)):
+            q = emv.distinct(n, 8, seed, 2)
+            assert s.poll(t).cost == oracle.emv(n, e, q)[0]
+        assert s.poll(t2).queue_ms > 0.0

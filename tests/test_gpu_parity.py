@@ -297,3 +297,19 @@ def test_csr_and_ell_paths_agree(gpu_ctx, monkeypatch):
     monkeypatch.delenv("EMV_NO_ELL")
     np.testing.assert_array_equal(f1, f2)
     np.testing.assert_array_equal(c1, c2)
+
+
+@pytest.mark.parametrize("qcap", ["64", "100000"])
+def test_worklist_rounds_agree(gpu_ctx, monkeypatch, qcap):
+    """Rounds taken from the worklist (EMV_QCAP > 0) give the flag-scan
+    result cell for cell; 64 forces list overflow back to the flag scan."""
+    inst = Instance(3600, emv.grid_graph(60, 60, 100, 12), emv.terminals(3600, 7, 12))
+    _, f1, c1 = solve_full(gpu_ctx, inst)
+    monkeypatch.setenv("EMV_QCAP", qcap)
+    ctx = emv.Context(0)
+    try:
+        _, f2, c2 = solve_full(ctx, inst)
+    finally:
+        ctx.close()
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(c1, c2)
