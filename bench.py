@@ -6,14 +6,29 @@ random connected 8-regular graph, n = 262,144, m = 1,048,576, weights
 uniform in [1, 1000], k = 14 uniform terminals (generator seed 2), int32
 table of 2^14 x 2^18 cells.  The graph is resident in HBM before timing.
 
-Prints one JSON line (rank 0).  `value` = DP cells (2^k * n) per second of
-wall time over the timed steps.  `roofline` covers the dominant kernel
-(largest share of device time), with achieved = algorithmic bytes / kernel
-time from HIP events on the library's stream.  `cpu_baseline` times the
-reference reader-el (oracle/_ref, OpenMP) on a bounded sample of the same
-family on this host's cores.
+Prints one JSON line.  `value` = DP cells (2^k * n) per second of wall time
+over the timed steps.  `roofline` covers the dominant kernel (largest share
+of device time), with achieved = algorithmic bytes / kernel time from HIP
+events on the library's stream.  `cpu_baseline` times the reference
+reader-el (oracle/_ref, OpenMP) on a bounded sample of the same family on
+this host's cores.
+
+Two more legs go into the same line, each compared at identical costs with
+the plain one-instance-at-a-time GPU path (Context.solve):
+  "batched": 256 uneven random regular instances (n = round(2^U[10,16]),
+             d ~ U{3..8}, k ~ U{6..12}, seeds 3000+i) solved by one Batch
+             (shared launches, autotuned shape, hipGraph replay) vs one
+             plain solve per instance with the graph already loaded;
+  "served":  512 x 512 grid (w ~ U[1,100], seed 4), terminal-set queries
+             (k ~ U{4..12}, seed 40) on a continuous-batching Server: a
+             burst (all queries at t=0) and Poisson streams at rates swept
+             as multiples of the plain path's serial capacity, with p50/p99
+             latency from arrival to completion; the plain serial figures
+             replay the same arrival times through a FIFO single server
+             whose service times are the measured plain solve times.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+                    [--legs plain,batch,serve] [--queries 24]
 """
 import argparse
 import json
@@ -34,6 +49,10 @@ METRIC = "DP cells/s k=14 n=2^18 (% HBM roofline); served solves/s + p99 vs plai
 WORKLOAD = dict(workload="EMV plain single instance: random connected 8-regular graph "
                 "n=262144 m=1048576, w~U[1,1000], k=14 uniform terminals, seed 2",
                 n=262144, d=8, wmax=1000, k=14, seed=2)
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def cpu_baseline():
@@ -59,8 +78,6 @@ def cpu_baseline():
             kind = "reference"
         else:
             sys.path.insert(0, os.path.join(REPO, "oracle"))
-            import numpy as np
-
             import emv
             import oracle
 
@@ -70,10 +87,231 @@ def cpu_baseline():
             oracle.emv(n, e, kk, threads)
             ms = (time.perf_counter() - t0) * 1e3
             kind = "port"
-            del np
     cells = (1 << k) * n
     return dict(value=cells / (ms / 1e3), unit="DP cells/s", cores=threads, kind=kind,
                 sample=sample, kernel_ms=ms)
+
+
+def plain_leg(args):
+    import emv
+
+    wl = dict(WORKLOAD)
+    wl["n"], wl["k"] = args.n, args.k
+    t0 = time.perf_counter()
+    e = emv.regular_graph(wl["n"], wl["d"], wl["wmax"], wl["seed"])
+    kk = emv.terminals(wl["n"], wl["k"], wl["seed"])
+    log(f"generated n={wl['n']} m={len(e)} k={wl['k']} in {time.perf_counter() - t0:.1f}s")
+    ctx = emv.Context(0)
+    ctx.load_edges(wl["n"], e)
+    cost = None
+    for i in range(args.warmup):
+        sol = ctx.solve(kk)
+        cost = sol.cost
+        log(f"warmup {i} kernel={sol.kernel_ms:.1f} ms merge={sol.merge_ms:.1f} "
+            f"relax={sol.relax_ms:.1f} single={sol.singleton_ms:.1f} cost={sol.cost}")
+    sols = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        sol = ctx.solve(kk)  # returns after the library's stream has drained
+        sols.append(sol)
+        log(f"step {i} kernel={sol.kernel_ms:.1f} ms merge={sol.merge_ms:.1f} "
+            f"relax={sol.relax_ms:.1f} single={sol.singleton_ms:.1f} rounds={sol.relax_rounds}")
+    wall = time.perf_counter() - t0
+    ctx.close()
+    for s in sols:
+        if cost is not None and s.cost != cost:
+            raise RuntimeError(f"cost changed between steps: {s.cost} != {cost}")
+    cost = sols[-1].cost
+    cells = sols[-1].dp_cells
+    merge_ms = sum(s.merge_ms for s in sols) / len(sols)
+    relax_ms = sum(s.relax_ms for s in sols) / len(sols)
+    kern = {
+        "merge": dict(bytes=sols[-1].merge_alg_bytes, ms=merge_ms,
+                      launches=sols[-1].merge_launches),
+        "relax": dict(bytes=sols[-1].relax_alg_bytes, ms=relax_ms,
+                      launches=sols[-1].relax_rounds),
+    }
+    for v in kern.values():
+        v["achieved_GBs"] = v["bytes"] / (v["ms"] * 1e6) if v["ms"] > 0 else 0.0
+        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+        v["cells_per_s"] = cells / (v["ms"] / 1e3) if v["ms"] > 0 else 0.0
+        v["avg_launch_us"] = v["ms"] * 1e3 / max(v["launches"], 1)
+    dom = "merge" if merge_ms >= relax_ms else "relax"
+    roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["achieved_GBs"], 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(kern[dom]["frac"], 4), traffic=None,
+                merge_relax_frac=round((kern["merge"]["bytes"] + kern["relax"]["bytes"]) /
+                                       ((merge_ms + relax_ms) * 1e6) / HBM_PEAK_GBS, 4))
+    return dict(
+        value=cells * args.steps / wall,
+        ms_per_step=wall * 1e3 / args.steps,
+        dtype="int32" if sols[-1].table_bytes == 4 else "int64",
+        config={"workload": wl["workload"] if (args.n, args.k) == (262144, 14)
+                else f"8-regular n={args.n} k={args.k} seed 2",
+                "n": wl["n"], "m": int(len(e)), "k": wl["k"], "dp_cells": cells,
+                "table": f"int{8 * sols[-1].table_bytes}",
+                "merge_acc_bits": sols[-1].merge_bits, "label_bits": sols[-1].label_bits},
+        cost=cost,
+        kernel_ms=sum(s.kernel_ms for s in sols) / len(sols),
+        singleton_ms=sum(s.singleton_ms for s in sols) / len(sols),
+        merge_ms=merge_ms, relax_ms=relax_ms, kernels=kern, roofline=roof)
+
+
+def batch_leg(args):
+    """256 uneven instances: plain per-instance solves vs one Batch."""
+    import numpy as np
+
+    import autotune
+    import emv
+    import workloads
+
+    insts = workloads.batch_instances(args.instances)
+    cells = sum((1 << len(kk)) * n for n, e, kk in insts)
+    ctx = emv.Context(0)
+    ctx.load_edges(insts[0][0], insts[0][1])
+    ctx.solve(insts[0][2])  # warm: code objects, workspace
+    plain_costs, t_solve, t_load = [], 0.0, 0.0
+    for n, e, kk in insts:
+        t0 = time.perf_counter()
+        ctx.load_edges(n, e)
+        t1 = time.perf_counter()
+        plain_costs.append(ctx.solve(kk).cost)
+        t_solve += time.perf_counter() - t1
+        t_load += t1 - t0
+    ctx.close()
+    plain_costs = np.array(plain_costs)
+    log(f"batch leg: plain {len(insts)} solves {t_solve:.3f}s (+{t_load:.3f}s graph loads)")
+    b = emv.Batch(insts)
+    t0 = time.perf_counter()
+    params, table = autotune.tune_batch(b, insts, reps=1, log=log)
+    tune_s = time.perf_counter() - t0
+    times, used = [], []
+    for i in range(args.batch_steps):
+        t0 = time.perf_counter()
+        costs, st = b.solve()
+        times.append(time.perf_counter() - t0)
+        used.append(int(st.graph_used))
+        if not np.array_equal(costs, plain_costs):
+            raise RuntimeError("batched costs differ from the plain per-instance costs")
+        log(f"batch leg: solve {i} {times[-1]:.3f}s graph={st.graph_used} "
+            f"fallback={st.graph_fallback} rounds={st.relax_rounds}")
+    b.close()
+    bt = sum(times) / len(times)
+    return dict(
+        workload=f"{len(insts)} random connected d-regular instances, n=round(2^U[10,16]), "
+                 "d~U{3..8}, k~U{6..12}, w~U[1,1000], seeds 3000+i",
+        instances=len(insts), dp_cells=cells, costs_equal=True,
+        plain_s=t_solve, plain_graph_load_s=t_load,
+        plain_instances_per_s=len(insts) / t_solve, plain_cells_per_s=cells / t_solve,
+        batched_s=bt, batched_instances_per_s=len(insts) / bt, batched_cells_per_s=cells / bt,
+        speedup=t_solve / bt, graph_replay=used,
+        autotune=dict(relax_blocks_per_cu=params[0], merge_rows_per_wave=params[1],
+                      seconds=round(tune_s, 2), table=table))
+
+
+def _percentile(xs, p):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs), p)) if len(xs) else None
+
+
+def _serve_stream(srv, qs, arr):
+    """Submits query i at arrival time arr[i] (s, from now), steps the server
+    while anything is in flight; returns latencies (arrival -> completion
+    seen after a step), costs and the completion time of the last query."""
+    Q = len(qs)
+    lat, costs = [None] * Q, [None] * Q
+    pending = {}
+    i = 0
+    t0 = time.perf_counter()
+    last = 0.0
+    while i < Q or pending:
+        now = time.perf_counter() - t0
+        while i < Q and arr[i] <= now:
+            pending[srv.submit(qs[i])] = i
+            i += 1
+        if not pending:
+            time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
+            continue
+        srv.step()
+        tnow = time.perf_counter() - t0
+        for tk in list(pending):
+            info = srv.poll(tk)
+            if info.state == 2:
+                j = pending.pop(tk)
+                if info.status != 0:
+                    raise RuntimeError(f"served query {j} failed with status {info.status}")
+                lat[j] = tnow - arr[j]
+                costs[j] = info.cost
+                last = tnow
+    return lat, costs, last
+
+
+def _fifo_latencies(arr, service):
+    """Latencies of a FIFO single server with the given service times."""
+    end, lat = 0.0, []
+    for a, s in zip(arr, service):
+        end = max(a, end) + s
+        lat.append(end - a)
+    return lat, end
+
+
+def serve_leg(args):
+    import numpy as np
+
+    import emv
+    import workloads
+
+    R = C = 512
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 4)
+    qs = workloads.grid_queries(n, args.queries, 40)
+    ctx = emv.Context(0)
+    ctx.load_edges(n, e)
+    ctx.solve(qs[0])  # warm
+    service, plain_costs = [], []
+    for q in qs:
+        t0 = time.perf_counter()
+        plain_costs.append(ctx.solve(q).cost)
+        service.append(time.perf_counter() - t0)
+    ctx.close()
+    total = sum(service)
+    log(f"serve leg: plain serial {len(qs)} queries {total:.2f}s "
+        f"(p50 {_percentile(service, 50) * 1e3:.0f} ms)")
+    srv = emv.Server(n, e, max_inflight=64)
+    _serve_stream(srv, qs[:1], [0.0])  # warm
+    lat, costs, last = _serve_stream(srv, qs, [0.0] * len(qs))
+    if costs != plain_costs:
+        raise RuntimeError("served costs differ from the plain costs (burst)")
+    burst = dict(solves_per_s=len(qs) / last, plain_solves_per_s=len(qs) / total,
+                 speedup=total / last, p50_ms=_percentile(lat, 50) * 1e3,
+                 p99_ms=_percentile(lat, 99) * 1e3)
+    log(f"serve leg: burst {last:.2f}s ({burst['speedup']:.2f}x plain)")
+    sweep = []
+    for idx, f in enumerate(args.rates):
+        lam = f * len(qs) / total
+        arr = list(workloads.poisson_arrivals(len(qs), lam, 40 + idx))
+        lat, costs, last = _serve_stream(srv, qs, arr)
+        if costs != plain_costs:
+            raise RuntimeError(f"served costs differ from the plain costs (rate {lam:.3f}/s)")
+        plat, pend = _fifo_latencies(arr, service)
+        sweep.append(dict(rate_per_s=lam, rate_vs_plain_capacity=f,
+                          solves_per_s=len(qs) / last, p50_ms=_percentile(lat, 50) * 1e3,
+                          p99_ms=_percentile(lat, 99) * 1e3,
+                          plain_solves_per_s=len(qs) / pend,
+                          plain_p50_ms=_percentile(plat, 50) * 1e3,
+                          plain_p99_ms=_percentile(plat, 99) * 1e3))
+        log(f"serve leg: rate {lam:.2f}/s served p50 {sweep[-1]['p50_ms']:.0f} ms "
+            f"p99 {sweep[-1]['p99_ms']:.0f} ms; plain p50 {sweep[-1]['plain_p50_ms']:.0f} ms "
+            f"p99 {sweep[-1]['plain_p99_ms']:.0f} ms")
+    srv.close()
+    return dict(
+        workload=f"512x512 grid, 4-neighbour, w~U[1,100], seed 4; {len(qs)} terminal-set "
+                 "queries k~U{4..12}, seed 40",
+        queries=len(qs), costs_equal=True, plain_serial_s=total,
+        plain_service_p50_ms=_percentile(service, 50) * 1e3,
+        plain_service_p99_ms=_percentile(service, 99) * 1e3,
+        burst=burst, poisson=sweep,
+        latency="arrival to completion, p99 over the stream's queries")
 
 
 def main():
@@ -84,91 +322,36 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=WORKLOAD["n"])
     ap.add_argument("--k", type=int, default=WORKLOAD["k"])
+    ap.add_argument("--legs", default="plain,batch,serve")
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--queries", type=int, default=24)
+    ap.add_argument("--rates", default="0.5,1,2")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU sharding is described, not built", file=sys.stderr)
         sys.exit(2)
+    args.rates = [float(x) for x in args.rates.split(",") if x]
+    legs = set(args.legs.split(","))
 
-    import emv
-
-    wl = dict(WORKLOAD)
-    wl["n"], wl["k"] = args.n, args.k
-    t0 = time.perf_counter()
-    e = emv.regular_graph(wl["n"], wl["d"], wl["wmax"], wl["seed"])
-    kk = emv.terminals(wl["n"], wl["k"], wl["seed"])
-    gen_s = time.perf_counter() - t0
-    print(f"bench: generated n={wl['n']} m={len(e)} k={wl['k']} in {gen_s:.1f}s", file=sys.stderr,
-          flush=True)
-    ctx = emv.Context(0)
-    ctx.load_edges(wl["n"], e)
-    cost = None
-    for i in range(args.warmup):
-        sol = ctx.solve(kk)
-        cost = sol.cost
-        print(f"bench: warmup {i} kernel={sol.kernel_ms:.1f} ms merge={sol.merge_ms:.1f} "
-              f"relax={sol.relax_ms:.1f} single={sol.singleton_ms:.1f} cost={sol.cost}",
-              file=sys.stderr, flush=True)
-    sols = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        sol = ctx.solve(kk)  # returns after the library's stream has drained
-        sols.append(sol)
-        print(f"bench: step {i} kernel={sol.kernel_ms:.1f} ms merge={sol.merge_ms:.1f} "
-              f"relax={sol.relax_ms:.1f} single={sol.singleton_ms:.1f} rounds={sol.relax_rounds}",
-              file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    ctx.close()
-    for s in sols:
-        if cost is not None and s.cost != cost:
-            raise RuntimeError(f"cost changed between steps: {s.cost} != {cost}")
-    cost = sols[-1].cost
-    cells = sols[-1].dp_cells
-    ms_per_step = wall * 1e3 / args.steps
-    merge_ms = sum(s.merge_ms for s in sols) / len(sols)
-    relax_ms = sum(s.relax_ms for s in sols) / len(sols)
-    kern = {
-        "merge": dict(bytes=sols[-1].merge_alg_bytes, ms=merge_ms,
-                      launches=sols[-1].merge_launches),
-        "relax": dict(bytes=sols[-1].relax_alg_bytes, ms=relax_ms,
-                      rounds=sols[-1].relax_rounds),
-    }
-    for v in kern.values():
-        v["achieved_GBs"] = v["bytes"] / (v["ms"] * 1e6) if v["ms"] > 0 else 0.0
-        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
-    dom = "merge" if merge_ms >= relax_ms else "relax"
-    roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["achieved_GBs"], 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(kern[dom]["frac"], 4), traffic=None,
-                merge_relax_frac=round((kern["merge"]["bytes"] + kern["relax"]["bytes"]) /
-                                       ((merge_ms + relax_ms) * 1e6) / HBM_PEAK_GBS, 4))
-    out = {
-        "metric": METRIC,
-        "value": cells * args.steps / wall,
-        "unit": "DP cells/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32" if sols[-1].table_bytes == 4 else "int64",
-        "data": "synthetic",
-        "config": {"workload": wl["workload"] if (args.n, args.k) == (262144, 14)
-                   else f"8-regular n={args.n} k={args.k} seed 2",
-                   "n": wl["n"], "m": int(len(e)), "k": wl["k"], "dp_cells": cells,
-                   "table": f"int{8 * sols[-1].table_bytes}",
-                   "merge_acc_bits": sols[-1].merge_bits},
-        "cost": cost,
-        "kernel_ms": sum(s.kernel_ms for s in sols) / len(sols),
-        "singleton_ms": sum(s.singleton_ms for s in sols) / len(sols),
-        "merge_ms": merge_ms,
-        "relax_ms": relax_ms,
-        "kernels": kern,
-        "roofline": roof,
-    }
-    if not args.no_cpu:
-        print("bench: cpu baseline (reference reader-el)...", file=sys.stderr, flush=True)
+    out = {"metric": METRIC, "value": None, "unit": "DP cells/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic"}
+    if "plain" in legs:
+        p = plain_leg(args)
+        out["value"] = p.pop("value")
+        out["ms_per_step"] = p.pop("ms_per_step")
+        out["dtype"] = p.pop("dtype")
+        out["config"] = p.pop("config")
+        out.update(p)
+    if "batch" in legs:
+        out["batched"] = batch_leg(args)
+    if "serve" in legs:
+        out["served"] = serve_leg(args)
+    if not args.no_cpu and "plain" in legs:
+        log("cpu baseline (reference reader-el)...")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 

@@ -10,8 +10,10 @@
 // region: costs f[tbase + X * np + v] (int32), then the record codes
 // b[tbase + 2^k * np + X * np + v] (int32).
 // Staging chunks keep the vertex-major layout D[dbase + v * 64 + lane] with
-// 32-bit labels.  Work items are located through prefix arrays over the
-// descriptors (binary search), so instances of any size share one grid.
+// 32-bit labels, and the merge's packed word (sum << k) | Y (64-bit) in
+// CODE[dbase + v * 64 + lane].  Work items are located through prefix arrays
+// over the descriptors (binary search), so instances of any size share one
+// grid.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -53,11 +55,10 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ pre, int nse
 }
 
 // ------------------------------------------------------ singleton seeding
-// Every cell of every singleton chunk: INF label, code 0; then the sources.
+// Every cell of every singleton chunk: INF label; then label 0 at the sources.
 __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict__ cell_pre,
                         int nch, const int32_t *__restrict__ kk_all,
-                        const int32_t *__restrict__ kk_off, uint32_t *__restrict__ D,
-                        uint32_t *__restrict__ CODE) {
+                        const int32_t *__restrict__ kk_off, uint32_t *__restrict__ D) {
     const int64_t total = cell_pre[nch];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
@@ -66,13 +67,7 @@ __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict
         const int64_t off = i - cell_pre[g];  // v * 64 + lane
         const int lane = (int)(off & (kWave - 1));
         const int v = (int)(off / kWave);
-        uint32_t d = ~0u, code = 0u;
-        if (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) {
-            d = 0u;
-            code = 1u << lane;
-        }
-        D[c.dbase + off] = d;
-        CODE[c.dbase + off] = code;
+        D[c.dbase + off] = (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) ? 0u : ~0u;
     }
 }
 
@@ -86,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
                                                    const int64_t *__restrict__ item_pre, int nmg,
                                                    const uint32_t *__restrict__ xs,
                                                    uint32_t *__restrict__ D,
-                                                   uint32_t *__restrict__ CODE) {
+                                                   uint64_t *__restrict__ CODE) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t item = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (item >= item_pre[nmg]) return;
@@ -99,30 +94,30 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     const int64_t v = tile * kWave + lane;
     const int32_t *fv = f + c0.tbase + v;
     const int k = c0.k;
-    const P ymask = ((P)1 << k) - 1;
     int nrows = 0;
     for (int q = 0; q < M.nchunks; q++) nrows += ch[M.chunk0 + q].nx;
-    uint32_t dv[XG], yv[XG];
+    uint32_t dv[XG];
+    uint64_t pv[XG];
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         const int xi = sub * XG + jx;
         dv[jx] = ~0u;
-        yv[jx] = 0u;
+        pv[jx] = ~0ull;
         if (xi < nrows) {
             const P acc = merge_one<int32_t, P>(fv, (int64_t)c0.np, xs[c0.x0 + xi], k);
             dv[jx] = (uint32_t)(acc >> k);
-            yv[jx] = (uint32_t)(acc & ymask);
+            pv[jx] = (uint64_t)acc;
         }
     }
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
     const GChunk cc = ch[M.chunk0 + chunk];
     uint32_t *dd = D + cc.dbase + v * kWave + loff;
-    uint32_t *cd = CODE + cc.dbase + v * kWave + loff;
+    uint64_t *cd = CODE + cc.dbase + v * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
-        cd[jx] = yv[jx];
+        cd[jx] = pv[jx];
     }
 }
 
@@ -151,7 +146,7 @@ __device__ __forceinline__ void gflag(const GChunk &c, int64_t base_item, int nb
 // (DMAX > 0, neighbour ids local to the instance's graph) or the shared CSR.
 template <int DMAX>
 __global__ __launch_bounds__(kBlock) void k_grelax(
-    uint32_t *__restrict__ D, uint32_t *__restrict__ CODE, const GChunk *__restrict__ ch,
+    uint32_t *__restrict__ D, const GChunk *__restrict__ ch,
     const int64_t *__restrict__ item_pre, int nch, const int32_t *__restrict__ rp,
     const Edge *__restrict__ adj, const Edge *__restrict__ ell, int dense,
     uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
@@ -218,13 +213,10 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
             const int g = __builtin_amdgcn_readlane(mg, t);
             const int v = __builtin_amdgcn_readlane(mv, t);
             const GChunk c = ch[g];
-            const uint32_t pred_base = 1u << c.k;
             uint32_t *Dc = D + c.dbase;
-            uint32_t *Cc = CODE + c.dbase;
             const int64_t idx = (int64_t)v * kWave + lane;
             const uint32_t cur = Dc[idx];
-            const uint32_t cc = Cc[idx];
-            uint32_t best = cur, bc = cc;
+            uint32_t best = cur;
             int nb = -1;  // lane q < degree holds neighbour q (for the flags)
             if constexpr (DMAX > 0) {
                 int uu[DMAX], ww[DMAX];
@@ -239,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     du[q] = uu[q] >= 0 ? Dc[(int64_t)uu[q] * kWave + lane] : INF;
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
+                    best = relax_min(du[q], uu[q], ww[q], INF, best);
                     if (lane == q) nb = uu[q];
                 }
             } else {
@@ -252,13 +244,12 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     for (int tt = 0; tt < cnt_e; tt++) {
                         const int u = __builtin_amdgcn_readlane(my.v, tt);
                         const int w = __builtin_amdgcn_readlane(my.w, tt);
-                        relax_fold(Dc[(int64_t)u * kWave + lane], u, w, pred_base, INF, best, bc);
+                        best = relax_min(Dc[(int64_t)u * kWave + lane], u, w, INF, best);
                     }
                 }
             }
             const bool dchg = best < cur;
             if (dchg) Dc[idx] = best;
-            if (bc != cc) Cc[idx] = bc;
             if (__ballot(dchg) != 0) {
                 const int64_t base_item = item_pre[g];
                 if constexpr (DMAX > 0) {
@@ -281,17 +272,20 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
 
 // ----------------------------------------------------------------- unpack
 // Staging chunks -> table rows of their instances (LDS transpose of 64
-// vertices x 64 rows per workgroup).  Singleton chunks write f[1<<t][v] and
-// the record (kk[t], {t}) (reader-el.c:2015-2027), and fold the per-row
-// maximum into rowmax[inst * 64 + t].
+// vertices x 64 rows per workgroup).  Layer chunks decide each record from
+// the converged labels (cell_record over the instance's adjacency).
+// Singleton chunks write f[1<<t][v] and the record (kk[t], {t})
+// (reader-el.c:2015-2027), and fold the per-row maximum into
+// rowmax[inst * 64 + t].
 __global__ __launch_bounds__(kBlock) void k_gunpack(
-    const uint32_t *__restrict__ D, const uint32_t *__restrict__ CODE,
+    const uint32_t *__restrict__ D, const uint64_t *__restrict__ CODE,
     const GChunk *__restrict__ ch, const int64_t *__restrict__ blk_pre, int nch,
     const uint32_t *__restrict__ xs, const int32_t *__restrict__ kk_all,
-    const int32_t *__restrict__ kk_off, int32_t *__restrict__ f, int32_t *__restrict__ b,
+    const int32_t *__restrict__ kk_off, const int32_t *__restrict__ rp,
+    const Edge *__restrict__ adj, int32_t *__restrict__ f, int32_t *__restrict__ b,
     unsigned long long *__restrict__ rowmax) {
     __shared__ uint32_t sd[kWave][kWave + 1];
-    __shared__ uint32_t sc[kWave][kWave + 1];
+    __shared__ int32_t sc[kWave][kWave + 1];
     const int64_t item = blockIdx.x;
     if (item >= blk_pre[nch]) return;
     const int g = find_seg(blk_pre, nch, item);
@@ -299,13 +293,21 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
     const int64_t v0 = (item - blk_pre[g]) * kWave;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    const int grp = t / kWave;
-#pragma unroll
+    const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
+    const uint32_t *Dc = D + c.dbase;
+    const bool real = lane < c.nx;
+#pragma unroll 1
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int vv = grp + r * (kBlock / kWave);
-        const int64_t idx = c.dbase + (v0 + vv) * kWave + lane;
-        sd[vv][lane] = D[idx];
-        sc[vv][lane] = CODE[idx];
+        const int64_t v = v0 + vv;
+        const int64_t idx = v * kWave + lane;
+        const uint32_t L = Dc[idx];
+        int32_t code = 0;
+        if (c.kind == 0 && v < c.n)
+            code = cell_record<uint32_t, uint64_t>(Dc, L, CODE[c.dbase + idx], c.k, real,
+                                                   c.voff + (int)v, rp, adj);
+        sd[vv][lane] = L;
+        sc[vv][lane] = code;
     }
     __syncthreads();
     const int64_t S2 = 1ll << c.k;
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
             if (c.kind == 0) {
                 const int64_t X = xs[c.x0 + xl];
                 f[c.tbase + X * c.np + v] = (int32_t)d;
-                b[bbase + X * c.np + v] = (int32_t)sc[lane][xl];
+                b[bbase + X * c.np + v] = sc[lane][xl];
             } else {
                 const int64_t X = 1ll << xl;
                 const int32_t src = kk_all[kk_off[c.inst] + xl];

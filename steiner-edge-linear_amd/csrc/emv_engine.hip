@@ -156,7 +156,8 @@ struct emv_engine {
     int32_t *d_kkoff = nullptr;  // slot -> offset (slot * 64)
     unsigned long long *d_rowmax = nullptr;  // max_slots * 64
     // staging
-    uint32_t *d_D = nullptr, *d_code = nullptr;
+    uint32_t *d_D = nullptr;
+    uint64_t *d_code = nullptr;  // merge's packed words
     int64_t stage_cells = 0;
     uint32_t *d_flag[2] = {nullptr, nullptr};
     int64_t flag_bytes = 0;  // flag words held per array
@@ -321,7 +322,7 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
         e->stage_cells = 0;
         const int64_t c = cells + cells / 4;
         EHIP(e, hipMalloc((void **)&e->d_D, sizeof(uint32_t) * c));
-        EHIP(e, hipMalloc((void **)&e->d_code, sizeof(uint32_t) * c));
+        EHIP(e, hipMalloc((void **)&e->d_code, sizeof(uint64_t) * c));
         e->stage_cells = c;
     }
     if (flags > e->flag_bytes) {
@@ -491,7 +492,7 @@ template <int DMAX>
 void launch_grelax(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
     const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
     hipLaunchKernelGGL((k_grelax<DMAX>), dim3(grid), dim3(kBlock), 0, e->stream, e->d_D,
-                       e->d_code, (const GChunk *)(db + P.off_ch),
+                       (const GChunk *)(db + P.off_ch),
                        (const int64_t *)(db + P.off_rp), (int)P.ch.size(), e->d_rp, e->d_adj,
                        e->d_ell, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
                        (const uint32_t *)e->d_q[r & 1], e->d_q[(r + 1) & 1], e->qcap, e->d_any,
@@ -534,7 +535,7 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
         const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
         hipLaunchKernelGGL(k_gseed, dim3(blocks), dim3(kBlock), 0, e->stream,
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
-                           (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D, e->d_code);
+                           (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
     }
     if (!P.mg.empty() && P.merge_pre.back() > 0) {
         if (P.pbits == 64)
@@ -556,8 +557,8 @@ int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
         hipLaunchKernelGGL(k_gunpack, dim3((unsigned)blocks), dim3(kBlock), 0, e->stream, e->d_D,
                            e->d_code, (const GChunk *)(db + P.off_ch),
                            (const int64_t *)(db + P.off_up), (int)P.ch.size(),
-                           (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_arena,
-                           e->d_arena, e->d_rowmax);
+                           (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_rp,
+                           e->d_adj, e->d_arena, e->d_arena, e->d_rowmax);
     EHIP(e, hipGetLastError());
     return EMV_OK;
 }

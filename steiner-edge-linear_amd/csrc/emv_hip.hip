@@ -8,9 +8,9 @@
 //      B < 2^31-1), the merge accumulator (32-bit when (2B << k) fits) and
 //      the staging label width (32-bit when 2B + max w fits).
 //   3. For each layer m = 2..k (reader-el.c:2033-2134), in batches of rows:
-//        merge  -> vertex-major staging (label, Y)
+//        merge  -> vertex-major staging (label, packed (sum << k) | Y)
 //        relax  -> dense pull round, then flagged rounds until no label drops
-//        unpack -> table rows f[X], b[X]
+//        unpack -> records from the converged labels; table rows f[X], b[X]
 //   4. Answer f[C][q]; optional device traceback.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
@@ -36,7 +36,6 @@ struct emv_ctx {
     int nxcd = 8;
     int num_cu = 256;
     int relax_nxcd = 1;  // relaxation sweep: 1 = grid-wide order, nxcd = per-XCD ranges
-    int merge_persistent = 0;  // 1: per-XCD persistent merge for the large layers
     // graph (CSR on the device)
     int64_t n = 0, m = 0;
     int32_t *d_rp = nullptr;
@@ -53,12 +52,11 @@ struct emv_ctx {
     int64_t tk = 0;
     // relaxation workspace, sized in chunks of 64 rows
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
-    uint32_t *d_code = nullptr;   // record codes
+    void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
     uint32_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
     uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
     uint32_t qcap = 0;
     uint32_t *d_cnt = nullptr;                // 4 rotating worklist counters
-    void *d_M = nullptr;          // packed merge accumulators, row-major, 8 bytes per cell reserved
     int64_t ws_chunks = 0;
     uint32_t *d_any = nullptr;               // 4 rotating "round has work" words
     unsigned long long *d_rowmax = nullptr;  // per singleton row
@@ -67,7 +65,6 @@ struct emv_ctx {
     int64_t xs_cap = 0;
     int32_t *d_src = nullptr;                // shortest-path sources
     uint32_t *h_pin = nullptr;               // pinned readback of a work word
-    uint32_t h_xs0 = 0;                      // first subset of the current batch
 };
 
 namespace {
@@ -171,7 +168,6 @@ void free_ws(emv_ctx *c) {
     dfree(c->d_q[0]);
     dfree(c->d_q[1]);
     c->qcap = 0;
-    dfree(c->d_M);
     dfree(c->d_D);
     dfree(c->d_code);
     dfree(c->d_flag[0]);
@@ -197,10 +193,9 @@ int ensure_ws(emv_ctx *c, int64_t chunks) {
     const int64_t cells = chunks * chunk_cells(c);
     const int64_t items = chunks * c->np;
     HIPCHK(c, hipMalloc(&c->d_D, sizeof(uint64_t) * cells));
-    HIPCHK(c, hipMalloc((void **)&c->d_code, sizeof(uint32_t) * cells));
+    HIPCHK(c, hipMalloc(&c->d_code, sizeof(uint64_t) * cells));
     HIPCHK(c, hipMalloc((void **)&c->d_flag[0], sizeof(uint32_t) * items));
     HIPCHK(c, hipMalloc((void **)&c->d_flag[1], sizeof(uint32_t) * items));
-    HIPCHK(c, hipMalloc(&c->d_M, sizeof(uint64_t) * cells));
     // Worklist rounds are off unless EMV_QCAP sets a capacity: every append
     // is an atomic on one counter, which costs more per round than scanning
     // the flags in order (measured on the 8-regular and grid workloads).
@@ -227,41 +222,41 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // the "round has work" word, until a round flags nothing.  Rounds launched
 // after convergence return at once.
 template <typename DT, int DMAX>
-void launch_relax_round(emv_ctx *c, int64_t nitems, uint32_t pred_base, int dense, int r) {
+void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
     const int grid = c->num_cu * 8;
     hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
-                       (DT *)c->d_D, c->d_code, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
-                       pred_base, nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
+                       (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
+                       nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
                        (const uint32_t *)c->d_q[r & 1], c->d_q[(r + 1) & 1], c->qcap, c->d_any,
                        c->d_cnt, r, c->relax_nxcd);
 }
 
 template <typename DT>
-void relax_round(emv_ctx *c, int64_t nitems, uint32_t pred_base, int dense, int r) {
+void relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
     switch (c->ell_deg) {
-    case 4: launch_relax_round<DT, 4>(c, nitems, pred_base, dense, r); break;
-    case 8: launch_relax_round<DT, 8>(c, nitems, pred_base, dense, r); break;
-    case 16: launch_relax_round<DT, 16>(c, nitems, pred_base, dense, r); break;
-    default: launch_relax_round<DT, 0>(c, nitems, pred_base, dense, r); break;
+    case 4: launch_relax_round<DT, 4>(c, nitems, dense, r); break;
+    case 8: launch_relax_round<DT, 8>(c, nitems, dense, r); break;
+    case 16: launch_relax_round<DT, 16>(c, nitems, dense, r); break;
+    default: launch_relax_round<DT, 0>(c, nitems, dense, r); break;
     }
 }
 
 template <typename DT>
-int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *rounds_out) {
+int relax_chunks(emv_ctx *c, int64_t nchunks, int64_t *rounds_out) {
     const int64_t nitems = nchunks * c->n;
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_any, (int64_t)4,
                        0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_cnt, (int64_t)4,
                        0u);
     int r = 0;
-    relax_round<DT>(c, nitems, pred_base, 1, r);
+    relax_round<DT>(c, nitems, 1, r);
     HIPCHK(c, hipGetLastError());
     r = 1;
     int64_t rounds = 1;
     int group = 2;
     for (;;) {
         for (int g = 0; g < group; g++, r++) {
-            relax_round<DT>(c, nitems, pred_base, 0, r);
+            relax_round<DT>(c, nitems, 0, r);
             rounds++;
         }
         HIPCHK(c, hipGetLastError());
@@ -282,7 +277,6 @@ int relax_chunks(emv_ctx *c, int64_t nchunks, uint32_t pred_base, int64_t *round
 struct SolveState {
     int k;
     int64_t S2;  // 2^k
-    uint32_t pred_base;
     int tbytes;
     int pbits;
     int dbits;
@@ -304,7 +298,7 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     int64_t per;
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
     hipLaunchKernelGGL((k_merge_v<T, P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                       c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, c->d_code, c->nxcd,
+                       c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
                        per);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
@@ -312,24 +306,6 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
 
 template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
-    const int grid_p = c->num_cu * 8;
-    const int64_t waves_x = (int64_t)(grid_p / c->nxcd) * (kBlock / kWave);
-    const int m = __builtin_popcount(c->h_xs0);
-    const int64_t W = (int64_t)nx * ((1ll << (m - 1)) - 1);
-    if (c->merge_persistent && W >= 8 * waves_x) {
-        // persistent per-XCD merge into the packed accumulators, then pack
-        const int64_t cells = (int64_t)nx * c->np;
-        const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
-        hipLaunchKernelGGL(k_fill<P>, dim3(blocks), dim3(kBlock), 0, c->stream, (P *)c->d_M,
-                           cells, ~(P)0);
-        hipLaunchKernelGGL((k_merge_p<T, P>), dim3(grid_p), dim3(kBlock), 0, c->stream, f, c->np,
-                           c->d_xs, nx, k, c->np / kWave, (P *)c->d_M, c->nxcd);
-        const int64_t vtiles = (c->n + kWave - 1) / kWave;
-        hipLaunchKernelGGL((k_pack_v<P, DT>), dim3(nchunks * vtiles), dim3(kBlock), 0, c->stream,
-                           (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, c->d_code, vtiles);
-        HIPCHK(c, hipGetLastError());
-        return EMV_OK;
-    }
     switch (merge_rows_per_wave(nx)) {
     case 8: return launch_merge_xg<T, P, DT, 8>(c, f, nx, nchunks, k);
     case 4: return launch_merge_xg<T, P, DT, 4>(c, f, nx, nchunks, k);
@@ -360,7 +336,6 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             const int64_t nchunks = (nx + kWave - 1) / kWave;
             HIPCHK(c, hipMemcpyAsync(c->d_xs, xs.data() + off, sizeof(uint32_t) * nx,
                                      hipMemcpyHostToDevice, c->stream));
-            c->h_xs0 = xs[off];
             hipEvent_t e0, e1, e2;
             HIPCHK(c, hipEventCreate(&e0));
             ev.push_back(e0);
@@ -374,14 +349,14 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             // relax
-            CHK(relax_chunks<DT>(c, nchunks, st.pred_base, &res->relax_rounds));
+            CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
             // unpack
             const int64_t vtiles = (n + kWave - 1) / kWave;
             int64_t per;
             const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
-            hipLaunchKernelGGL((k_unpack_v<T, DT>), dim3(grid), dim3(kBlock), 0, c->stream,
-                               (const DT *)c->d_D, c->d_code, np, c->d_xs, nx, (int)n, f, b,
-                               c->nxcd, per, vtiles);
+            hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0, c->stream,
+                               (const DT *)c->d_D, (const P *)c->d_code, np, c->d_xs, nx, (int)n,
+                               k, c->d_rp, c->d_adj, f, b, c->nxcd, per, vtiles);
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e2, c->stream));
         }
@@ -460,8 +435,7 @@ int check_weight(emv_ctx *c, int64_t w) {
 }
 
 // Shortest paths from up to 64 sources in staging chunk 0 (64-bit labels).
-int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, bool bit_codes, uint32_t pred_base,
-               int64_t *rounds) {
+int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, int64_t *rounds) {
     const int ns = (int)src.size();
     if (ns > kWave) {
         set_err(c, "%d sources exceed %d", ns, kWave);
@@ -472,16 +446,14 @@ int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, bool bit_codes, uint
     const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
     hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
                        (uint64_t *)c->d_D, cells, ~0ull);
-    hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_code,
-                       cells, 0u);
     dfree(c->d_src);
     HIPCHK(c, hipMalloc((void **)&c->d_src, sizeof(int32_t) * std::max(ns, 1)));
     HIPCHK(c, hipMemcpyAsync(c->d_src, src.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice,
                              c->stream));
     hipLaunchKernelGGL((k_seed_v<uint64_t>), dim3(1), dim3(kWave), 0, c->stream,
-                       (uint64_t *)c->d_D, c->d_code, c->d_src, ns, bit_codes ? 1 : 0);
+                       (uint64_t *)c->d_D, c->d_src, ns);
     HIPCHK(c, hipGetLastError());
-    return relax_chunks<uint64_t>(c, 1, pred_base, rounds);
+    return relax_chunks<uint64_t>(c, 1, rounds);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -520,8 +492,6 @@ extern "C" int emv_open(emv_ctx **out, int device) {
         c->nxcd = (c->num_cu % 32 == 0 && c->num_cu >= 64) ? c->num_cu / 32 : 1;
         const char *rx = getenv("EMV_RELAX_XCD");
         c->relax_nxcd = (rx && atoi(rx) > 0) ? c->nxcd : 1;
-        const char *mp = getenv("EMV_MERGE_PERSISTENT");
-        c->merge_persistent = (mp && atoi(mp) > 0) ? 1 : 0;
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = EMV_ERR_HIP;
             break;
@@ -643,7 +613,7 @@ extern "C" int64_t emv_solve_footprint(const emv_ctx *c, int64_t k, uint32_t fla
     const int64_t S2 = (int64_t)1 << k;
     const int tb = (flags & EMV_FORCE_I64) ? 8 : 4;
     const int64_t chunks = std::max<int64_t>(1, (choose(k, k / 2) + kWave - 1) / kWave);
-    const int64_t ws = chunks * np * (kWave * (8 + 4 + 8) + 2);
+    const int64_t ws = chunks * np * (kWave * (8 + 8) + 8);
     return S2 * np * (tb + 4) + ws;
 }
 
@@ -678,7 +648,6 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     SolveState st;
     st.k = (int)k;
     st.S2 = (int64_t)1 << k;
-    st.pred_base = (uint32_t)st.S2;
     const int64_t np = c->np;
     if (st.S2 + n >= INT32_MAX) {
         set_err(c, "2^k + n = %lld does not fit the int32 record", (long long)(st.S2 + n));
@@ -702,7 +671,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // ---- 1. singleton shortest paths in chunk 0, lane t = kk[t]
     std::vector<int32_t> src((size_t)k);
     for (int64_t t = 0; t < k; t++) src[t] = (int32_t)kk[t];
-    CHK(sssp_chunk(c, src, true, st.pred_base, &res->relax_rounds));
+    CHK(sssp_chunk(c, src, &res->relax_rounds));
     // ---- 2. bound from the per-row maxima
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, c->stream, c->d_rowmax,
                        (int64_t)64, 0ull);
@@ -870,7 +839,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->relax_alg_bytes = ((double)st.S2 - 1.0) * (cb * dn + cb * 2 * dm + (cb + ab) * dn) +
                            (double)k * (4.0 * (dn + 1) + 16.0 * dm);
     res->device_bytes = (int64_t)(c->f_cap + c->b_cap) +
-                        c->ws_chunks * np * (int64_t)(kWave * 20 + 2);
+                        c->ws_chunks * np * (int64_t)(kWave * 16 + 8);
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     if (!(flags & EMV_KEEP_TABLE)) invalidate_table(c);
@@ -929,7 +898,6 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
         return EMV_ERR_ARG;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    const uint32_t pred_base = 2;
     EventList evl;
     hipEvent_t a, b;
     HIPCHK(c, hipEventCreate(&a));
@@ -939,7 +907,7 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
     HIPCHK(c, hipEventRecord(a, c->stream));
     int64_t rounds = 0;
     std::vector<int32_t> src(1, (int32_t)s);
-    CHK(sssp_chunk(c, src, false, pred_base, &rounds));
+    CHK(sssp_chunk(c, src, &rounds));
     int64_t *dd = nullptr, *dp = nullptr;
     HIPCHK(c, hipMalloc((void **)&dd, sizeof(int64_t) * c->n));
     if (hipMalloc((void **)&dp, sizeof(int64_t) * c->n) != hipSuccess) {
@@ -948,7 +916,7 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
         return EMV_ERR_NOMEM;
     }
     hipLaunchKernelGGL(k_unpack_sssp_v, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
-                       (const uint64_t *)c->d_D, c->d_code, (int)c->n, pred_base, dd, dp);
+                       (const uint64_t *)c->d_D, (int)c->n, c->d_rp, c->d_adj, dd, dp);
     hipError_t e0 = hipEventRecord(b, c->stream);
     hipError_t e1 = hipStreamSynchronize(c->stream);
     if (ms) *ms = ev_ms(a, b);

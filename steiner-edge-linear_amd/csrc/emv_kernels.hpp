@@ -10,16 +10,21 @@
 // chunks of 64; chunk c holds, for every vertex v, the 64 rows' labels
 // side by side:
 //   D[(c * np + v) * 64 + lane]     distance label of row c*64+lane at v
-//   CODE[(c * np + v) * 64 + lane]  its record code
+//   CODE[(c * np + v) * 64 + lane]  the merge's packed word (sum << k) | Y
 // One wave owns one (chunk, vertex) item: lane = row.  A neighbour's labels
-// for the whole chunk are one coalesced 256 B (u32) or 512 B (u64) load, and
-// the adjacency of v is read once for 64 rows.
+// for the whole chunk are one coalesced 128 B (u16), 256 B (u32) or 512 B
+// (u64) load, and the adjacency of v is read once for 64 rows.
 //
-// Records are decided by the lexicographic minimum of (distance, code).  A
-// merge code (Y < 2^k) is below every edge code (2^k + u), so the merge
-// record survives unless a path strictly improves on the merged value -- the
-// reference's rule (reader-el.c:2117-2126) -- and among equal improvements
-// the smallest neighbour id wins, independent of the schedule.
+// Relaxation rounds move labels only.  Records are decided once, from the
+// converged labels, when the staging is unpacked: the merge record (split
+// Y) stays when the label still equals the merged value -- the reference
+// keeps it exactly when no path strictly improves on the merge
+// (reader-el.c:2117-2126) -- and otherwise the record is the edge from the
+// smallest neighbour u with label(u) + w(u, v) == label(v).  That is the
+// lexicographic minimum of (distance, code) over the merge candidate
+// (merged value, Y) and the edge candidates (label(u) + w, 2^k + u), with
+// merge codes below every edge code, so it does not depend on the order in
+// which the rounds ran.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -51,45 +56,36 @@ __global__ void k_fill(T *__restrict__ p, int64_t count, T val) {
 }
 
 // ------------------------------------------------------------- seeding
-// Chunk 0, lane t: the shortest-path labels from src[t].  The source's code
-// is 1<<t for the singleton rows (the reference's (kk[t], {t}) record,
-// reader-el.c:2025) or 1 for a plain shortest-path query.
+// Chunk 0, lane t: the shortest-path labels from src[t] (label 0 at the
+// source; every other label starts at INF).
 template <typename DT>
-__global__ void k_seed_v(DT *__restrict__ D, uint32_t *__restrict__ CODE,
-                         const int32_t *__restrict__ src, int nsrc, int bit_codes) {
+__global__ void k_seed_v(DT *__restrict__ D, const int32_t *__restrict__ src, int nsrc) {
     int t = threadIdx.x;
-    if (t < nsrc) {
-        int64_t idx = (int64_t)src[t] * kWave + t;
-        D[idx] = 0;
-        CODE[idx] = bit_codes ? (1u << t) : 1u;
-    }
+    if (t < nsrc) D[(int64_t)src[t] * kWave + t] = 0;
 }
 
 // ------------------------------------------------------------ relaxation
 // One pull round over (chunk, vertex) items, item i = (chunk i / n, vertex
 // i % n).  Every item whose neighbour's label dropped in the previous round
-// is flagged (a byte in F_cur); round 0 (`dense`) takes every item.  A round
+// is flagged (a word in F_cur); round 0 (`dense`) takes every item.  A round
 // takes its items in one of two ways, decided on the device from the
 // previous round's counters:
 //   worklist : the flagged items were also appended to Q_cur (cnt[r&3] of
 //              them, at most qcap) -- a thin frontier costs only its size;
 //   flag scan: too many were flagged for the list, so the wave reads 64
 //              consecutive flags at once and walks the set ones.
-// Per item the wave recomputes, for its 64 rows, the lexicographic minimum of
-// its own (label, code) and (neighbour label + w, 2^k + neighbour) over the
-// whole adjacency and writes what changed.  If any label dropped it flags
-// every neighbour in F_next (a writer that finds the flag clear also appends
-// the neighbour to Q_next while the list has room) and raises any[(r+1)&3].
-// Each flagged item is processed by exactly one wave per round: the flag
-// scan reads each flag once, and a listed item is taken only by the wave
-// whose atomic exchange clears its flag, so duplicate list entries are
-// skipped.  One writer per cell per round keeps the masked (label, code)
-// updates free of races between waves.  A
-// round clears the flags it consumed, so F_next is clean when the round
-// starts, and clears any/cnt[(r+2)&3] for the round after next.  Pulling the
-// full adjacency after every neighbour change makes the final (label, code)
-// of each cell the exact lexicographic minimum, independent of the order
-// waves run in.
+// Per item the wave recomputes, for its 64 rows, the minimum of its own label
+// and (neighbour label + w) over the whole adjacency and writes the rows that
+// dropped.  If any label dropped it flags every neighbour in F_next (a writer
+// that finds the flag clear also appends the neighbour to Q_next while the
+// list has room) and raises any[(r+1)&3].  Each flagged item is processed by
+// exactly one wave per round: the flag scan reads each flag once, and a
+// listed item is taken only by the wave whose atomic exchange clears its
+// flag, so duplicate list entries are skipped.  A round clears the flags it
+// consumed, so F_next is clean when the round starts, and clears
+// any/cnt[(r+2)&3] for the round after next.  Labels only decrease and every
+// drop re-flags the neighbours, so the rounds end at the shortest-path
+// fixed point whatever order the waves run in.
 //
 // Adjacency: DMAX > 0 uses the padded fixed-degree copy ell[v * DMAX + j]
 // (missing neighbours have v = -1).  The 64 items a wave holds load their
@@ -97,16 +93,12 @@ __global__ void k_seed_v(DT *__restrict__ D, uint32_t *__restrict__ CODE,
 // are gathered with no dependent load in front, and two items are processed
 // together to keep 2 * DMAX gathers in flight.  DMAX == 0 walks the CSR.
 template <typename DT>
-__device__ __forceinline__ void relax_fold(DT du, int u, int w, uint32_t pred_base, DT INF,
-                                           DT &best, uint32_t &bc) {
+__device__ __forceinline__ DT relax_min(DT du, int u, int w, DT INF, DT best) {
     if (u >= 0 && du != INF) {
         const DT nd = du + (DT)w;
-        const uint32_t nc = pred_base + (uint32_t)u;
-        if (nd < best || (nd == best && nc < bc)) {
-            best = nd;
-            bc = nc;
-        }
+        best = nd < best ? nd : best;
     }
+    return best;
 }
 
 // Wave-aggregated append of `entry` to a list of capacity cap (one atomic per
@@ -151,11 +143,11 @@ __device__ __forceinline__ void flag_neighbours(int64_t c, int64_t np, int n, in
 
 template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_relax_v(
-    DT *__restrict__ D, uint32_t *__restrict__ CODE, int64_t np, int n,
-    const int32_t *__restrict__ rp, const Edge *__restrict__ adj, const Edge *__restrict__ ell,
-    uint32_t pred_base, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
-    uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur, uint32_t *__restrict__ Qnext,
-    uint32_t qcap, uint32_t *__restrict__ any, uint32_t *__restrict__ cnt, int r, int nxcd) {
+    DT *__restrict__ D, int64_t np, int n, const int32_t *__restrict__ rp,
+    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int64_t nitems, int dense,
+    uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
+    uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
+    uint32_t *__restrict__ cnt, int r, int nxcd) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         any[(r + 2) & 3] = 0;
         cnt[(r + 2) & 3] = 0;
@@ -222,8 +214,6 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
                 const DT cura = Da[ia];
                 const DT curb = two ? Db[ib] : INF;
-                const uint32_t cca = CODE[ca * np * kWave + ia];
-                const uint32_t ccb = two ? CODE[cb * np * kWave + ib] : 0u;
                 int ua[DMAX], wa[DMAX], ub[DMAX], wb[DMAX];
                 DT da[DMAX], db[DMAX];
 #pragma unroll
@@ -239,17 +229,14 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                     db[q] = (two && ub[q] >= 0) ? Db[(int64_t)ub[q] * kWave + lane] : INF;
                 }
                 DT besta = cura, bestb = curb;
-                uint32_t bca = cca, bcb = ccb;
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    relax_fold(da[q], ua[q], wa[q], pred_base, INF, besta, bca);
-                    relax_fold(db[q], ub[q], wb[q], pred_base, INF, bestb, bcb);
+                    besta = relax_min(da[q], ua[q], wa[q], INF, besta);
+                    bestb = relax_min(db[q], ub[q], wb[q], INF, bestb);
                 }
                 const bool chga = besta < cura, chgb = two && bestb < curb;
                 if (chga) Da[ia] = besta;
-                if (bca != cca) CODE[ca * np * kWave + ia] = bca;
                 if (chgb) Db[ib] = bestb;
-                if (two && bcb != ccb) CODE[cb * np * kWave + ib] = bcb;
                 const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
                 if (anya || anyb) {
                     // lane q < DMAX takes neighbour q
@@ -278,12 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 const int64_t c = __builtin_amdgcn_readlane(mc, t);
                 const int v = __builtin_amdgcn_readlane(mv, t);
                 DT *Dc = D + c * np * kWave;
-                uint32_t *Cc = CODE + c * np * kWave;
                 const int64_t idx = (int64_t)v * kWave + lane;
                 const DT cur = Dc[idx];
-                const uint32_t cc = Cc[idx];
                 DT best = cur;
-                uint32_t bc = cc;
                 const int e0 = __builtin_amdgcn_readlane(me0, t);
                 const int e1 = __builtin_amdgcn_readlane(me1, t);
                 for (int eb = e0; eb < e1; eb += kWave) {
@@ -303,12 +287,11 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                         }
 #pragma unroll
                         for (int q = 0; q < 4; q++)
-                            if (t0 + q < cnt_e) relax_fold(du[q], uu[q], ww[q], pred_base, INF, best, bc);
+                            if (t0 + q < cnt_e) best = relax_min(du[q], uu[q], ww[q], INF, best);
                     }
                 }
                 const bool dchg = best < cur;
                 if (dchg) Dc[idx] = best;
-                if (bc != cc) Cc[idx] = bc;
                 if (__ballot(dchg) != 0) {
                     for (int eb = e0; eb < e1; eb += kWave) {
                         const int e = eb + lane;
@@ -369,15 +352,14 @@ __device__ __forceinline__ P merge_one(const T *__restrict__ fv, int64_t np, uin
 // tile-major and dealt to XCDs in contiguous ranges, so the waves of an XCD
 // sweep the rows of one vertex tile together and the tile's lower-layer rows
 // stay in that XCD's L2.  Results go straight to the vertex-major staging:
-// each lane writes its XG labels (and codes) as one contiguous run.  Rows
-// past nx (the tail of the last chunk) are written as INF.
+// each lane writes its XG labels (and packed words) as one contiguous run.
+// Rows past nx (the tail of the last chunk) are written as INF.
 template <typename T, typename P, typename DT, int XG>
 __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int64_t np,
                                                     const uint32_t *__restrict__ xs, int nx,
                                                     int nsub, int64_t ntiles, int k,
-                                                    DT *__restrict__ D,
-                                                    uint32_t *__restrict__ CODE, int nxcd,
-                                                    int64_t per_xcd) {
+                                                    DT *__restrict__ D, P *__restrict__ CODE,
+                                                    int nxcd, int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
@@ -388,152 +370,80 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
     const int64_t v = tile * kWave + lane;
     const T *fv = f + v;
     const DT INF = (DT)~(DT)0;
-    const P ymask = ((P)1 << k) - 1;
     DT dv[XG];
-    uint32_t yv[XG];
+    P pv[XG];
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         const int xi = sub * XG + jx;
         dv[jx] = INF;
-        yv[jx] = 0;
+        pv[jx] = ~(P)0;
         if (xi < nx) {
-            const P acc = merge_one<T, P>(fv, np, xs[xi], k);
-            dv[jx] = (DT)(acc >> k);
-            yv[jx] = (uint32_t)(acc & ymask);
+            pv[jx] = merge_one<T, P>(fv, np, xs[xi], k);
+            dv[jx] = (DT)(pv[jx] >> k);
         }
     }
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
     DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
-    uint32_t *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
+    P *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
-        cd[jx] = yv[jx];
+        cd[jx] = pv[jx];
     }
 }
 
-// Persistent merge: each XCD owns the vertex tiles tile = xcd, xcd + nxcd, ...
-// and all waves resident on that XCD split one tile's iteration space
-// (row xi of the batch, pair index p) evenly, so the XCD reads the lower
-// layers of a single 64-vertex tile at a time -- at most 2^k rows of 256 B,
-// which its 4 MB L2 holds.  A wave's slice covers whole or partial rows; each
-// finished row segment is folded into the packed accumulator M[xi][v] with an
-// unsigned atomicMin (the min is order-independent, so the split changes
-// nothing).  M must start at all ones.  Pairs of a row are enumerated in
-// increasing Y: pair p is Y = deposit(p + 1, Xp) and the next subset of Xp is
-// ((Y | ~Xp) + 1) & Xp.
-__device__ __forceinline__ uint32_t deposit_bits(uint32_t x, uint32_t mask) {
-    uint32_t r = 0;
-    for (uint32_t bb = 1; mask != 0; bb <<= 1) {
-        const uint32_t low = mask & (~mask + 1u);
-        if (x & bb) r |= low;
-        mask ^= low;
-    }
-    return r;
-}
-
-template <typename T, typename P>
-__global__ __launch_bounds__(kBlock) void k_merge_p(const T *__restrict__ f, int64_t np,
-                                                    const uint32_t *__restrict__ xs, int nx,
-                                                    int k, int64_t ntiles, P *__restrict__ M,
-                                                    int nxcd) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
-    const int xcd = blockIdx.x % nxcd;
-    const int64_t waves_x = (int64_t)(gridDim.x / nxcd) * (kBlock / kWave);
-    const int64_t w = (int64_t)(blockIdx.x / nxcd) * (kBlock / kWave) + wid;
-    const uint32_t npairs = (1u << (__popc(xs[0]) - 1)) - 1u;
-    const int64_t W = (int64_t)nx * npairs;
-    const int64_t lo = W * w / waves_x, hi = W * (w + 1) / waves_x;
-    if (lo >= hi) return;
-    for (int64_t tile = xcd; tile < ntiles; tile += nxcd) {
-        const int64_t v = tile * kWave + lane;
-        const T *fv = f + v;
-        int64_t i = lo;
-        while (i < hi) {
-            const int xi = (int)(i / npairs);
-            const uint32_t p = (uint32_t)(i - (int64_t)xi * npairs);
-            const uint32_t cnt = (uint32_t)min((int64_t)(npairs - p), hi - i);
-            const uint32_t X = xs[xi];
-            const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
-            const uint32_t nXp = ~Xp;
-            uint32_t Y = deposit_bits(p + 1u, Xp);
-            P acc = ~(P)0;
-            uint32_t j = 0;
-            for (; j + 8 <= cnt; j += 8) {
-                uint32_t Ys[8];
-                Ys[0] = Y;
+// ----------------------------------------------------------------- records
+// Record of one converged cell, lane = row of chunk c at vertex v (wave-
+// uniform): the merge split Y when the label still equals the merged value
+// (packed word pk = (sum << k) | Y), else 2^k + the smallest neighbour u with
+// label(u) + w == label(v).  Neighbour entries are read as wave-uniform
+// (scalar) loads, eight gathers in flight.
+template <typename DT, typename P>
+__device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, P pk, int k,
+                                               bool real, int v, const int32_t *__restrict__ rp,
+                                               const Edge *__restrict__ adj) {
+    const int lane = __lane_id();
+    const bool edge = real && (P)L < (pk >> k);
+    int32_t code = (int32_t)(pk & (((P)1 << k) - 1));
+    if (__ballot(edge) == 0) return code;
+    const DT INF = (DT)~(DT)0;
+    uint32_t bu = 0xFFFFFFFFu;
+    const int e0 = rp[v], e1 = rp[v + 1];
+    for (int eb = e0; eb < e1; eb += 8) {
+        int uu[8], ww[8];
+        DT du[8];
 #pragma unroll
-                for (int q = 1; q < 8; q++) Ys[q] = ((Ys[q - 1] | nXp) + 1u) & Xp;
-                T a[8], c[8];
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    a[q] = fv[(int64_t)Ys[q] * np];
-                    c[q] = fv[(int64_t)(X ^ Ys[q]) * np];
-                }
-#pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const P s = (((P)a[q] + (P)c[q]) << k) | (P)Ys[q];
-                    acc = s < acc ? s : acc;
-                }
-                Y = ((Ys[7] | nXp) + 1u) & Xp;
-            }
-            for (; j < cnt; j++) {
-                const P s = (((P)fv[(int64_t)Y * np] + (P)fv[(int64_t)(X ^ Y) * np]) << k) | (P)Y;
-                acc = s < acc ? s : acc;
-                Y = ((Y | nXp) + 1u) & Xp;
-            }
-            atomicMin(&M[(int64_t)xi * np + v], acc);
-            i += cnt;
+        for (int q = 0; q < 8; q++) {
+            const int e = min(eb + q, e1 - 1);
+            uu[q] = adj[e].v;
+            ww[q] = adj[e].w;
+            du[q] = Dc[(int64_t)uu[q] * kWave + lane];
         }
-    }
-}
-
-// Packed accumulators M[xi][v] -> vertex-major staging (label, Y) through an
-// LDS transpose of 64 rows x 64 vertices per workgroup; rows past nx are INF.
-template <typename P, typename DT>
-__global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int64_t np, int nx,
-                                                   int k, DT *__restrict__ D,
-                                                   uint32_t *__restrict__ CODE, int64_t vtiles) {
-    __shared__ P sm[kWave][kWave + 1];
-    const int64_t chunk = blockIdx.x / vtiles;
-    const int64_t vt = blockIdx.x - chunk * vtiles;
-    const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int grp = t / kWave;
-    const int64_t v0 = vt * kWave;
 #pragma unroll
-    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
-        const int xl = grp + r * (kBlock / kWave);
-        const int64_t xi = chunk * kWave + xl;
-        sm[xl][lane] = (xi < nx) ? M[xi * np + v0 + lane] : ~(P)0;
+        for (int q = 0; q < 8; q++)
+            if (eb + q < e1 && du[q] != INF && (DT)(du[q] + (DT)ww[q]) == L &&
+                (uint32_t)uu[q] < bu)
+                bu = (uint32_t)uu[q];
     }
-    __syncthreads();
-    const P ymask = ((P)1 << k) - 1;
-#pragma unroll
-    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
-        const int vv = grp + r * (kBlock / kWave);
-        const P a = sm[lane][vv];
-        const int64_t idx = (chunk * np + v0 + vv) * kWave + lane;
-        const bool real = (chunk * kWave + lane) < nx;
-        D[idx] = real ? (DT)(a >> k) : ~(DT)0;
-        CODE[idx] = real ? (uint32_t)(a & ymask) : 0u;
-    }
+    if (edge) code = (int32_t)((1u << k) + bu);
+    return code;
 }
 
 // ----------------------------------------------------------------- unpack
 // Vertex-major staging -> table rows f[X], b[X] through an LDS transpose of
-// 64 vertices x 64 rows per workgroup.
-template <typename T, typename DT>
+// 64 vertices x 64 rows per workgroup; each wave decides the records of 16
+// vertices (cell_record) on the way into LDS.
+template <typename T, typename DT, typename P>
 __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
-                                                     const uint32_t *__restrict__ CODE,
-                                                     int64_t np, const uint32_t *__restrict__ xs,
-                                                     int nx, int n, T *__restrict__ f,
-                                                     int32_t *__restrict__ b, int nxcd,
-                                                     int64_t per_xcd, int64_t vtiles) {
+                                                     const P *__restrict__ CODE, int64_t np,
+                                                     const uint32_t *__restrict__ xs, int nx,
+                                                     int n, int k, const int32_t *__restrict__ rp,
+                                                     const Edge *__restrict__ adj,
+                                                     T *__restrict__ f, int32_t *__restrict__ b,
+                                                     int nxcd, int64_t per_xcd, int64_t vtiles) {
     __shared__ DT sd[kWave][kWave + 1];
-    __shared__ uint32_t sc[kWave][kWave + 1];
+    __shared__ int32_t sc[kWave][kWave + 1];
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     const int64_t item = xcd_item(nxcd, per_xcd);
     const int64_t chunk = item / vtiles;
@@ -541,14 +451,20 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
     if (chunk * kWave >= nx) return;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    const int grp = t / kWave;
+    const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
     const int64_t v0 = vt * kWave;
-#pragma unroll
+    const DT *Dc = D + chunk * np * kWave;
+    const bool real = chunk * kWave + lane < nx;
+#pragma unroll 1
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int vv = grp + r * (kBlock / kWave);
-        const int64_t idx = ((chunk * np) + v0 + vv) * kWave + lane;
-        sd[vv][lane] = D[idx];
-        sc[vv][lane] = CODE[idx];
+        const int64_t v = v0 + vv;
+        const int64_t idx = ((chunk * np) + v) * kWave + lane;
+        const DT L = D[idx];
+        int32_t code = 0;
+        if (v < n) code = cell_record<DT, P>(Dc, L, CODE[idx], k, real, (int)v, rp, adj);
+        sd[vv][lane] = L;
+        sc[vv][lane] = code;
     }
     __syncthreads();
 #pragma unroll
@@ -559,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
         if (xi < nx && v < n) {
             const int64_t X = xs[xi];
             f[X * np + v] = (T)sd[lane][xl];
-            b[X * np + v] = (int32_t)sc[lane][xl];
+            b[X * np + v] = sc[lane][xl];
         }
     }
 }
@@ -590,16 +506,27 @@ __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
     if (__lane_id() == 0) atomicMax(&rowmax[t], d);
 }
 
-// Distances and predecessors of one shortest-path query (chunk 0, lane 0).
-[[maybe_unused]] static __global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, const uint32_t *__restrict__ CODE,
-                                int n, uint32_t pred_base, int64_t *__restrict__ d,
-                                int64_t *__restrict__ pred) {
+// Distances and predecessors of one shortest-path query (chunk 0, lane 0):
+// the predecessor is the smallest neighbour u with d(u) + w == d(v); the
+// source (distance 0, weights >= 1) and unreached vertices have none.
+[[maybe_unused]] static __global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, int n,
+                                                        const int32_t *__restrict__ rp,
+                                                        const Edge *__restrict__ adj,
+                                                        int64_t *__restrict__ d,
+                                                        int64_t *__restrict__ pred) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    uint64_t dd = D[(int64_t)v * kWave];
-    uint32_t code = CODE[(int64_t)v * kWave];
+    const uint64_t dd = D[(int64_t)v * kWave];
+    int64_t p = -1;
+    if (dd != 0 && dd != ~0ull) {
+        for (int e = rp[v]; e < rp[v + 1]; e++) {
+            const int u = adj[e].v;
+            const uint64_t du = D[(int64_t)u * kWave];
+            if (du != ~0ull && du + (uint64_t)adj[e].w == dd && (p < 0 || u < p)) p = u;
+        }
+    }
     d[v] = (dd == ~0ull) ? INT64_MAX : (int64_t)dd;
-    pred[v] = (code >= pred_base) ? (int64_t)(code - pred_base) : -1;
+    pred[v] = p;
 }
 
 // -------------------------------------------------------------- traceback
