@@ -72,6 +72,7 @@ typedef struct {
     int64_t device_bytes;    /* device memory held during the solve */
     int64_t npairs;          /* traceback: number of (v,u) pairs */
     int64_t *pairs;          /* 2*npairs, 0-based, reference DFS order (reader-el.c:1907-1944) */
+    int64_t relax_blocks;    /* vertex blocks of the block-local relaxation; 0 = vertex-wise rounds */
 } emv_result;
 
 /* Context on HIP device `device`. */

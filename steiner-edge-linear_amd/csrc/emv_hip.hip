@@ -42,7 +42,19 @@ struct emv_ctx {
     Edge *d_adj = nullptr;
     Edge *d_ell = nullptr;  // padded fixed-degree copy (ell_deg > 0)
     int ell_deg = 0;
+    // vertex blocks of the block-local relaxation (nblk > 0 when the graph
+    // has the locality for it)
+    int nblk = 0;
+    int32_t *d_boff = nullptr, *d_bvert = nullptr, *d_brp = nullptr, *d_bext = nullptr,
+            *d_vblk = nullptr;
+    Edge *d_badj = nullptr;
+    size_t lds_block = 65536, lds_cu = 65536;  // LDS limits of the device
+    bool blk_attr[3] = {false, false, false};  // dynamic-LDS attribute set, per label width
     int64_t maxw = 0;
+    int sssp_bits = 64;  // label width of the last shortest-path chunk
+    // EMV_TRACE: per-relaxation counters of the block kernel on stderr
+    // (visits, sweeps, swept slots)
+    unsigned long long *d_stats = nullptr;
     // table; the allocation is reused across solves
     void *d_f = nullptr;
     int32_t *d_b = nullptr;
@@ -241,22 +253,74 @@ void relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
     }
 }
 
+// The block-local kernel for labels of type DT, if the device gives a
+// workgroup enough LDS for one block of labels.
+template <typename DT>
+int blk_slot() {
+    return sizeof(DT) == 2 ? 0 : sizeof(DT) == 4 ? 1 : 2;
+}
+
+template <typename DT>
+bool use_blocks(emv_ctx *c) {
+    const size_t bytes = blk_lds_bytes<DT>();
+    if (c->nblk == 0 || bytes > c->lds_block) return false;
+    if (bytes > 65536 && !c->blk_attr[blk_slot<DT>()]) {
+        if (hipFuncSetAttribute((const void *)k_relax_blk<DT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        c->blk_attr[blk_slot<DT>()] = true;
+    }
+    return true;
+}
+
+bool use_blocks_any(emv_ctx *c, int dbits) {
+    if (dbits == 16) return use_blocks<uint16_t>(c);
+    if (dbits == 32) return use_blocks<uint32_t>(c);
+    return use_blocks<uint64_t>(c);
+}
+
+template <typename DT>
+void blk_round(emv_ctx *c, int64_t nitems, int dense, int r) {
+    const size_t bytes = blk_lds_bytes<DT>();
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_cu / bytes));
+    hipLaunchKernelGGL(k_relax_blk<DT>, dim3(c->num_cu * per_cu), dim3(kBT), bytes,
+                       c->stream, (DT *)c->d_D, c->np, c->nblk, c->d_boff, c->d_bvert, c->d_brp,
+                       c->d_bext, c->d_badj, c->d_vblk, nitems, dense, c->d_flag[r & 1],
+                       c->d_flag[(r + 1) & 1], c->d_any, r, c->d_stats);
+}
+
 template <typename DT>
 int relax_chunks(emv_ctx *c, int64_t nchunks, int64_t *rounds_out) {
-    const int64_t nitems = nchunks * c->n;
+    const bool blocks = use_blocks<DT>(c);
+    const int64_t nitems = nchunks * (blocks ? (int64_t)c->nblk : c->n);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_any, (int64_t)4,
                        0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, c->stream, c->d_cnt, (int64_t)4,
                        0u);
     int r = 0;
-    relax_round<DT>(c, nitems, 1, r);
+    if (blocks) {
+        blk_round<DT>(c, nitems, 1, r);
+        if (hipGetLastError() != hipSuccess) {
+            // the device refused the block kernel's LDS: vertex-wise rounds
+            c->lds_block = std::min(c->lds_block, blk_lds_bytes<DT>() - 1);
+            return relax_chunks<DT>(c, nchunks, rounds_out);
+        }
+    } else {
+        relax_round<DT>(c, nitems, 1, r);
+    }
     HIPCHK(c, hipGetLastError());
     r = 1;
     int64_t rounds = 1;
     int group = 2;
     for (;;) {
         for (int g = 0; g < group; g++, r++) {
-            relax_round<DT>(c, nitems, 0, r);
+            if (blocks)
+                blk_round<DT>(c, nitems, 0, r);
+            else
+                relax_round<DT>(c, nitems, 0, r);
             rounds++;
         }
         HIPCHK(c, hipGetLastError());
@@ -271,6 +335,20 @@ int relax_chunks(emv_ctx *c, int64_t nchunks, int64_t *rounds_out) {
         }
     }
     *rounds_out += rounds;
+    if (c->d_stats && blocks) {
+        unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        HIPCHK(c, hipMemcpy(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemset(c->d_stats, 0, sizeof(st)));
+        const double v = st[0] ? (double)st[0] : 1.0;
+        fprintf(stderr,
+                "emv-trace: relax chunks=%lld items=%lld rounds=%lld visits=%llu sweeps=%llu "
+                "swept_slots=%llu cycles/visit load=%.0f sweep=%.0f write=%.0f lds=%zu/%zu "
+                "grid=%d\n",
+                (long long)nchunks, (long long)nitems, (long long)rounds, st[0], st[1], st[2],
+                st[3] / v, st[4] / v, st[5] / v, c->lds_block, c->lds_cu,
+                c->num_cu * (int)std::max<size_t>(
+                                1, std::min<size_t>(2, c->lds_cu / blk_lds_bytes<DT>())));
+    }
     return EMV_OK;
 }
 
@@ -377,6 +455,83 @@ int run_layers_t(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result 
     return run_layers<T, uint64_t, uint64_t>(c, st, f, b, res, ev);
 }
 
+void free_blocks(emv_ctx *c) {
+    dfree(c->d_boff);
+    dfree(c->d_bvert);
+    dfree(c->d_brp);
+    dfree(c->d_bext);
+    dfree(c->d_vblk);
+    dfree(c->d_badj);
+    c->nblk = 0;
+}
+
+// Blocks of at most kBV vertices grown breadth-first over unassigned
+// vertices from the lowest unassigned id; slots follow the BFS order.
+// Returns the share of adjacency entries that stay inside their block.
+struct BlockPart {
+    int nblk = 0;
+    std::vector<int32_t> boff, bvert, brp, bext, vblk;
+    std::vector<Edge> badj;
+};
+
+double build_blocks(int64_t n, const std::vector<int32_t> &rp, const std::vector<Edge> &adj,
+                    BlockPart &P) {
+    P.vblk.assign((size_t)n, -1);
+    std::vector<int32_t> slot((size_t)n, -1), queue;
+    queue.reserve(kBV);
+    P.boff.assign(1, 0);
+    P.bvert.clear();
+    P.bvert.reserve((size_t)n);
+    for (int64_t seed = 0; seed < n; seed++) {
+        if (P.vblk[seed] >= 0) continue;
+        const int b = P.nblk++;
+        queue.clear();
+        queue.push_back((int32_t)seed);
+        P.vblk[seed] = b;
+        for (size_t head = 0; head < queue.size() && queue.size() < (size_t)kBV; head++) {
+            const int32_t u = queue[head];
+            for (int32_t e = rp[u]; e < rp[u + 1] && queue.size() < (size_t)kBV; e++) {
+                const int32_t v = adj[e].v;
+                if (P.vblk[v] < 0) {
+                    P.vblk[v] = b;
+                    queue.push_back(v);
+                }
+            }
+        }
+        for (int32_t u : queue) {
+            slot[u] = (int32_t)P.bvert.size();
+            P.bvert.push_back(u);
+        }
+        P.boff.push_back((int32_t)P.bvert.size());
+    }
+    P.brp.assign((size_t)n + 1, 0);
+    P.bext.assign((size_t)n, 0);
+    P.badj.clear();
+    P.badj.reserve(adj.size());
+    int64_t inside = 0;
+    for (int64_t s = 0; s < n; s++) {
+        const int32_t u = P.bvert[s];
+        const int32_t b = P.vblk[u];
+        const int32_t s0 = P.boff[b];
+        P.brp[s] = (int32_t)P.badj.size();
+        for (int32_t e = rp[u]; e < rp[u + 1]; e++)
+            if (P.vblk[adj[e].v] == b) P.badj.push_back(Edge{slot[adj[e].v] - s0, adj[e].w});
+        P.bext[s] = (int32_t)P.badj.size();
+        inside += P.bext[s] - P.brp[s];
+        for (int32_t e = rp[u]; e < rp[u + 1]; e++)
+            if (P.vblk[adj[e].v] != b) P.badj.push_back(adj[e]);
+    }
+    P.brp[n] = (int32_t)P.badj.size();
+    return adj.empty() ? 0.0 : (double)inside / (double)adj.size();
+}
+
+template <typename T>
+int upload(emv_ctx *c, T *&d, const std::vector<T> &h) {
+    HIPCHK(c, hipMalloc((void **)&d, sizeof(T) * std::max<size_t>(1, h.size())));
+    if (!h.empty()) HIPCHK(c, hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return EMV_OK;
+}
+
 int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
              const std::vector<Edge> &adj, int64_t maxw) {
     HIPCHK(c, hipSetDevice(c->device));
@@ -405,6 +560,26 @@ int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
         }
     if (ell && (int64_t)ell * n > 2 * std::max<int64_t>(1, 2 * m)) ell = 0;
     if (getenv("EMV_NO_ELL")) ell = 0;
+    free_blocks(c);
+    // Block-local relaxation when most edges stay inside their block (grids,
+    // meshes).  EMV_BLOCKS=0 turns it off, EMV_BLOCKS=1 forces it.
+    {
+        const char *eb = getenv("EMV_BLOCKS");
+        const int mode = eb ? atoi(eb) : -1;
+        if (mode != 0 && n >= 2 && !adj.empty()) {
+            BlockPart P;
+            const double inside = build_blocks(n, rp, adj, P);
+            if (mode == 1 || (inside >= 0.6 && n >= 4 * kBV)) {
+                CHK(upload(c, c->d_boff, P.boff));
+                CHK(upload(c, c->d_bvert, P.bvert));
+                CHK(upload(c, c->d_brp, P.brp));
+                CHK(upload(c, c->d_bext, P.bext));
+                CHK(upload(c, c->d_vblk, P.vblk));
+                CHK(upload(c, c->d_badj, P.badj));
+                c->nblk = P.nblk;
+            }
+        }
+    }
     if (ell) {
         std::vector<Edge> e((size_t)(n * ell), Edge{-1, 0});
         for (int64_t u = 0; u < n; u++)
@@ -434,7 +609,8 @@ int check_weight(emv_ctx *c, int64_t w) {
     return EMV_OK;
 }
 
-// Shortest paths from up to 64 sources in staging chunk 0 (64-bit labels).
+// Shortest paths from up to 64 sources in staging chunk 0: 32-bit labels
+// when n * max w stays below the all-ones label, else 64-bit (sssp_bits).
 int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, int64_t *rounds) {
     const int ns = (int)src.size();
     if (ns > kWave) {
@@ -444,16 +620,39 @@ int sssp_chunk(emv_ctx *c, const std::vector<int32_t> &src, int64_t *rounds) {
     CHK(ensure_ws(c, 1));
     const int64_t cells = chunk_cells(c);
     const int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 65536);
-    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
-                       (uint64_t *)c->d_D, cells, ~0ull);
     dfree(c->d_src);
     HIPCHK(c, hipMalloc((void **)&c->d_src, sizeof(int32_t) * std::max(ns, 1)));
     HIPCHK(c, hipMemcpyAsync(c->d_src, src.data(), sizeof(int32_t) * ns, hipMemcpyHostToDevice,
                              c->stream));
+    c->sssp_bits = ((uint64_t)c->n * (uint64_t)c->maxw < 0xFFFFFFFFull) ? 32 : 64;
+    if (c->sssp_bits == 32) {
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
+                           (uint32_t *)c->d_D, cells, ~0u);
+        hipLaunchKernelGGL((k_seed_v<uint32_t>), dim3(1), dim3(kWave), 0, c->stream,
+                           (uint32_t *)c->d_D, c->d_src, ns);
+        HIPCHK(c, hipGetLastError());
+        return relax_chunks<uint32_t>(c, 1, rounds);
+    }
+    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
+                       (uint64_t *)c->d_D, cells, ~0ull);
     hipLaunchKernelGGL((k_seed_v<uint64_t>), dim3(1), dim3(kWave), 0, c->stream,
                        (uint64_t *)c->d_D, c->d_src, ns);
     HIPCHK(c, hipGetLastError());
     return relax_chunks<uint64_t>(c, 1, rounds);
+}
+
+// Singleton rows (f == nullptr: only the per-row maxima) from chunk 0.
+template <typename T>
+void unpack_singletons(emv_ctx *c, int64_t k, T *f, int32_t *b) {
+    const dim3 grid((unsigned)((c->n + kBlock - 1) / kBlock), (unsigned)k);
+    if (c->sssp_bits == 32)
+        hipLaunchKernelGGL((k_unpack_singletons_v<T, uint32_t>), grid, dim3(kBlock), 0, c->stream,
+                           (const uint32_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, b,
+                           c->d_rowmax);
+    else
+        hipLaunchKernelGGL((k_unpack_singletons_v<T, uint64_t>), grid, dim3(kBlock), 0, c->stream,
+                           (const uint64_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, b,
+                           c->d_rowmax);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -488,10 +687,19 @@ extern "C" int emv_open(emv_ctx **out, int device) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = EMV_ERR_HIP; break; }
         c->num_cu = prop.multiProcessorCount;
+        c->lds_block = prop.sharedMemPerBlock;
+        c->lds_cu = std::max<size_t>(prop.maxSharedMemoryPerMultiProcessor, prop.sharedMemPerBlock);
         // MI355X: 8 XCDs of 32 CUs; workgroups are dealt round-robin over XCDs.
         c->nxcd = (c->num_cu % 32 == 0 && c->num_cu >= 64) ? c->num_cu / 32 : 1;
         const char *rx = getenv("EMV_RELAX_XCD");
         c->relax_nxcd = (rx && atoi(rx) > 0) ? c->nxcd : 1;
+        if (getenv("EMV_TRACE") && atoi(getenv("EMV_TRACE")) > 0) {
+            if (hipMalloc((void **)&c->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
+                hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+                rc = EMV_ERR_NOMEM;
+                break;
+            }
+        }
         if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = EMV_ERR_HIP;
             break;
@@ -523,6 +731,8 @@ extern "C" void emv_close(emv_ctx *c) {
     dfree(c->d_rp);
     dfree(c->d_adj);
     dfree(c->d_ell);
+    free_blocks(c);
+    dfree(c->d_stats);
     dfree(c->d_any);
     dfree(c->d_cnt);
     dfree(c->d_rowmax);
@@ -675,9 +885,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // ---- 2. bound from the per-row maxima
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, c->stream, c->d_rowmax,
                        (int64_t)64, 0ull);
-    hipLaunchKernelGGL((k_unpack_singletons_v<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
-                       dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src, (int)k,
-                       (int)n, (int32_t *)nullptr, (int32_t *)nullptr, c->d_rowmax);
+    unpack_singletons<int32_t>(c, k, nullptr, nullptr);
     HIPCHK(c, hipGetLastError());
     std::vector<unsigned long long> rowmax((size_t)k);
     HIPCHK(c, hipMemcpyAsync(rowmax.data(), c->d_rowmax, sizeof(unsigned long long) * k,
@@ -736,13 +944,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     }
     HIPCHK(c, hipEventRecord(e_zero1, c->stream));
     if (st.tbytes == 4)
-        hipLaunchKernelGGL((k_unpack_singletons_v<int32_t>), dim3((n + kBlock - 1) / kBlock, k),
-                           dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src,
-                           (int)k, (int)n, (int32_t *)c->d_f, c->d_b, c->d_rowmax);
+        unpack_singletons<int32_t>(c, k, (int32_t *)c->d_f, c->d_b);
     else
-        hipLaunchKernelGGL((k_unpack_singletons_v<int64_t>), dim3((n + kBlock - 1) / kBlock, k),
-                           dim3(kBlock), 0, c->stream, (const uint64_t *)c->d_D, np, c->d_src,
-                           (int)k, (int)n, (int64_t *)c->d_f, c->d_b, c->d_rowmax);
+        unpack_singletons<int64_t>(c, k, (int64_t *)c->d_f, c->d_b);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(e_single1, c->stream));
     c->tbytes = st.tbytes;
@@ -822,6 +1026,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->table_bytes = st.tbytes;
     res->merge_bits = st.pbits;
     res->label_bits = st.dbits;
+    res->relax_blocks = use_blocks_any(c, st.dbits) ? c->nblk : 0;
     res->dp_cells = st.S2 * n;
     res->zero_ms = ev_ms(e_zero0, e_zero1);
     res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;
@@ -915,8 +1120,14 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
         set_err(c, "allocation failed");
         return EMV_ERR_NOMEM;
     }
-    hipLaunchKernelGGL(k_unpack_sssp_v, dim3((c->n + 255) / 256), dim3(256), 0, c->stream,
-                       (const uint64_t *)c->d_D, (int)c->n, c->d_rp, c->d_adj, dd, dp);
+    if (c->sssp_bits == 32)
+        hipLaunchKernelGGL((k_unpack_sssp_v<uint32_t>), dim3((c->n + 255) / 256), dim3(256), 0,
+                           c->stream, (const uint32_t *)c->d_D, (int)c->n, c->d_rp, c->d_adj, dd,
+                           dp);
+    else
+        hipLaunchKernelGGL((k_unpack_sssp_v<uint64_t>), dim3((c->n + 255) / 256), dim3(256), 0,
+                           c->stream, (const uint64_t *)c->d_D, (int)c->n, c->d_rp, c->d_adj, dd,
+                           dp);
     hipError_t e0 = hipEventRecord(b, c->stream);
     hipError_t e1 = hipStreamSynchronize(c->stream);
     if (ms) *ms = ev_ms(a, b);

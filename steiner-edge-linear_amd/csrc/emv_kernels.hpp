@@ -306,6 +306,403 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
     if (raised && lane == 0) any[(r + 1) & 3] = 1;
 }
 
+// ------------------------------------------------------ block relaxation
+// Graphs with locality (grids, meshes) are cut into blocks of at most kBV
+// vertices grown breadth-first.  Slots s0 .. s0+cnt-1 of block b (s0 =
+// boff[b]) hold its vertices bvert[s]; the adjacency of slot s lists the
+// neighbours inside the block first, as local slots (badj[brp[s] ..
+// bext[s])), then the others by vertex id (badj[bext[s] .. brp[s+1])).
+// One workgroup takes one (chunk, block) item per visit:
+//   1. loads the block's labels of the chunk's 64 rows into LDS and folds in
+//      every neighbour outside the block (label + w, read once);
+//   2. sweeps the block in LDS, forward and backward in turn; a sweep takes
+//      the slots marked dirty (all of them in the first sweep), lowers each
+//      label to the minimum over its block neighbours of (label + w), and
+//      marks the block neighbours of every slot that dropped for the next
+//      sweep -- until a sweep lowers nothing: the block is then at its fixed
+//      point for the outside labels it read;
+//   3. writes back the labels that dropped and, for a dropped vertex with
+//      neighbours outside the block, flags those neighbours' blocks for the
+//      next round.
+// A round visits every flagged item once (round 0 every item); workgroup g
+// takes items g, g + G, g + 2G, ... so a cluster of flagged blocks spreads
+// over the grid.  Labels only decrease; a drop inside a block is settled
+// within the visit and a drop on its border re-flags the blocks it touches,
+// so the rounds end at the same fixed point as the vertex-wise rounds, after
+// about (hop length of the longest path) / (block diameter) rounds instead of
+// the hop length.  Each item is visited by one workgroup per round, which is
+// the only writer of its labels.
+constexpr int kBV = 248;  // vertices per block: 32-bit labels of a block fit 64 KB of LDS
+constexpr int kBD = 4;    // adjacency entries per vertex and kind held in registers
+constexpr int kBW = 16;   // waves per block workgroup (one workgroup fills a CU's LDS)
+constexpr int kBT = kBW * kWave;
+constexpr int kBPiece = 256;  // flags read per collection step
+
+template <typename DT>
+__host__ __device__ constexpr size_t blk_lds_bytes() {
+    return sizeof(DT) * kBV * kWave + 1024;
+}
+
+// A 64-bit value the wave holds in every lane, made explicitly wave-uniform
+// (scalar registers), so that masks built with cross-lane reductions keep the
+// loops that walk them scalar.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Lowest (or highest) set bit of a wave-uniform mask, as a scalar.
+__device__ __forceinline__ int pick_bit(uint64_t m, bool lowest) {
+    const int j = lowest ? __ffsll((unsigned long long)m) - 1 : 63 - __clzll((unsigned long long)m);
+    return __builtin_amdgcn_readfirstlane(j);
+}
+
+// Puts the block neighbours of slot j (wave-local index, registers ic of its
+// wave; the part of its list past the register copy, badj[ea..eb), one lane
+// per entry) on the worklists: the slots of the calling wave's band [i0,
+// i0+nv) into its mask, the others as LDS marks for their wave.
+__device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD], int i0, int nv,
+                                                      int ea, int eb,
+                                                      const Edge *__restrict__ badj,
+                                                      uint8_t *marks, uint64_t &wmask,
+                                                      bool &cross) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int q = 0; q < kBD; q++) {
+        const int sl = __builtin_amdgcn_readlane(ic[q], j);
+        if (sl >= 0) {
+            if (sl >= i0 && sl < i0 + nv) {
+                wmask |= 1ull << (sl - i0);
+            } else {
+                if (lane == 0) marks[sl] = 1;
+                cross = true;
+            }
+        }
+    }
+    if (ea < eb) {
+        uint64_t add = 0;
+        bool xo = false;
+        for (int e = ea + lane; e < eb; e += kWave) {
+            const int sl = badj[e].v;
+            if (sl >= i0 && sl < i0 + nv) {
+                add |= 1ull << (sl - i0);
+            } else {
+                marks[sl] = 1;
+                xo = true;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) add |= __shfl_xor(add, off);
+        wmask = uniform64(wmask | add);
+        cross = cross || (__ballot(xo) != 0);
+    }
+}
+
+template <typename DT>
+__global__ __launch_bounds__(kBT) void k_relax_blk(
+    DT *__restrict__ D, int64_t np, int nblk, const int32_t *__restrict__ boff,
+    const int32_t *__restrict__ bvert, const int32_t *__restrict__ brp,
+    const int32_t *__restrict__ bext, const Edge *__restrict__ badj,
+    const int32_t *__restrict__ vblk, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
+    uint32_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r,
+    unsigned long long *__restrict__ stats) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    DT *L = reinterpret_cast<DT *>(lds_raw);  // L[slot * 64 + row]
+    uint32_t *s_misc = reinterpret_cast<uint32_t *>(lds_raw + sizeof(DT) * kBV * kWave);
+    uint32_t *s_flag = s_misc;     // 3 rotating "sweep lowered something" words
+    uint32_t *s_cnt = s_misc + 4;  // flagged items of the current piece
+    uint8_t *s_list = reinterpret_cast<uint8_t *>(s_misc + 8);      // 256 entries
+    uint8_t *s_dirty = reinterpret_cast<uint8_t *>(s_misc + 8 + 64);  // 2 x kBV marks
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
+    if (blockIdx.x == 0 && tid == 0) any[(r + 2) & 3] = 0;
+    if (!dense && any[r & 3] == 0) return;
+    const DT INF = (DT)~(DT)0;
+    const int64_t G = gridDim.x;
+    const int64_t mine = (nitems > (int64_t)blockIdx.x) ? (nitems - 1 - blockIdx.x) / G + 1 : 0;
+    // the marks start clear and are clear again whenever a visit's sweeps end
+    for (int i = tid; i < 2 * kBV; i += kBT) s_dirty[i] = 0;
+    bool raised = false;
+    for (int64_t p0 = 0; p0 < mine; p0 += kBPiece) {
+        if (tid == 0) *s_cnt = 0;
+        __syncthreads();
+        if (tid < kBPiece && p0 + tid < mine) {
+            const int64_t it = blockIdx.x + G * (p0 + tid);
+            bool f = dense != 0;
+            if (!dense && Fcur[it] != 0) {
+                f = true;
+                Fcur[it] = 0;
+            }
+            if (f) s_list[atomicAdd(s_cnt, 1u)] = (uint8_t)tid;
+        }
+        __syncthreads();
+        const int nl = (int)*s_cnt;
+        for (int qi = 0; qi < nl; qi++) {
+            const int64_t item = blockIdx.x + G * (p0 + s_list[qi]);
+            const long long t_start = stats ? clock64() : 0;
+            const int c = (int)(item / nblk);
+            const int b = (int)(item - (int64_t)c * nblk);
+            const int s0 = boff[b];
+            const int cnt = boff[b + 1] - s0;
+            DT *Dc = D + (int64_t)c * np * kWave;
+            // wave w owns the contiguous slots [i0, i0 + nv); lane j holds slot i0 + j
+            const int qw = (cnt + kBW - 1) / kBW;
+            const int i0 = min(cnt, wid * qw);
+            const int nv = min(cnt, i0 + qw) - i0;
+            int vj = 0, e0 = 0, em = 0, e1 = 0, ovf = 0;
+            int ic[kBD], iw[kBD], xu[kBD], xw[kBD], xb[kBD];
+#pragma unroll
+            for (int q = 0; q < kBD; q++) {
+                ic[q] = -1;
+                iw[q] = 0;
+                xu[q] = -1;
+                xw[q] = 0;
+                xb[q] = -1;
+            }
+            if (lane < nv) {
+                const int s = s0 + i0 + lane;
+                vj = bvert[s];
+                e0 = brp[s];
+                em = bext[s];
+                e1 = brp[s + 1];
+                ovf = (em - e0 > kBD || e1 - em > kBD) ? 1 : 0;
+#pragma unroll
+                for (int q = 0; q < kBD; q++) {
+                    if (e0 + q < em) {
+                        const Edge ed = badj[e0 + q];
+                        ic[q] = ed.v;
+                        iw[q] = ed.w;
+                    }
+                    if (em + q < e1) {
+                        const Edge ed = badj[em + q];
+                        xu[q] = ed.v;
+                        xw[q] = ed.w;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kBD; q++)
+                    if (xu[q] >= 0) xb[q] = vblk[xu[q]];
+            }
+            const bool any_ovf = __ballot(ovf != 0) != 0;
+            if (tid < 3) s_flag[tid] = 0;
+            const uint64_t own_mask = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+            // 1a. the block's labels, eight slots per step
+            for (int j0 = 0; j0 < nv; j0 += 8) {
+                DT own[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const int j = min(j0 + t, nv - 1);
+                    own[t] = Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane];
+                }
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (j0 + t < nv) L[(i0 + j0 + t) * kWave + lane] = own[t];
+            }
+            // 1b. fold in the outside neighbours of the border slots, four per
+            //     step; a slot that drops puts its block neighbours on the
+            //     worklists of the first sweep
+            uint64_t chg_mask = 0;  // slots whose label dropped during this visit
+            uint64_t wmask = dense ? own_mask : 0ull;
+            bool cross0 = false;
+            {
+                uint64_t bmask = uniform64(__ballot(lane < nv && e1 > em));
+                while (bmask) {
+                    int jj[4];
+                    int nt = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (bmask) {
+                            const int j = pick_bit(bmask, true);
+                            bmask &= bmask - 1;
+                            jj[t] = j;
+                            nt++;
+                        } else {
+                            jj[t] = jj[0];
+                        }
+                    }
+                    DT ext[4][kBD];
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int u = __builtin_amdgcn_readlane(xu[q], jj[t]);
+                            ext[t][q] = u >= 0 ? Dc[(int64_t)u * kWave + lane] : INF;
+                        }
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (t >= nt) break;
+                        const int j = jj[t];
+                        const DT x0 = L[(i0 + j) * kWave + lane];
+                        DT x = x0;
+#pragma unroll
+                        for (int q = 0; q < kBD; q++)
+                            x = relax_min(ext[t][q], __builtin_amdgcn_readlane(xu[q], j),
+                                          __builtin_amdgcn_readlane(xw[q], j), INF, x);
+                        if (any_ovf && __builtin_amdgcn_readlane(ovf, j)) {
+                            const int ea = __builtin_amdgcn_readlane(em, j) + kBD;
+                            const int eb = __builtin_amdgcn_readlane(e1, j);
+                            for (int e = ea; e < eb; e++) {
+                                const Edge ed = badj[e];
+                                x = relax_min(Dc[(int64_t)ed.v * kWave + lane], ed.v, ed.w, INF, x);
+                            }
+                        }
+                        const bool lower = x < x0;
+                        if (lower) L[(i0 + j) * kWave + lane] = x;
+                        if (__ballot(lower) != 0) {
+                            chg_mask |= 1ull << j;
+                            if (!dense) {
+                                const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
+                                mark_block_neighbours(
+                                    j, ic, i0, nv, ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0,
+                                    ov ? __builtin_amdgcn_readlane(em, j) : 0, badj, s_dirty, wmask,
+                                    cross0);
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const long long t_loaded = stats ? clock64() : 0;
+            // 2. sweeps.  Each wave drains a worklist of its own slots (all of
+            //    them on a dense round, else the block neighbours of the slots
+            //    the outside lowered): a slot whose label drops puts its block
+            //    neighbours on the list -- its own wave's at once, another
+            //    wave's through the LDS marks, which that wave takes up after
+            //    the barrier.  The visit ends when a sweep leaves no mark for
+            //    another wave.
+            unsigned long long processed = 0;
+            for (int it = 0;; it++) {
+                uint8_t *dcur = s_dirty + (it & 1) * kBV;
+                uint8_t *dnext = s_dirty + ((it + 1) & 1) * kBV;
+                if (tid == 0) s_flag[(it + 1) % 3] = 0;
+                {
+                    const bool dl = lane < nv && dcur[i0 + lane] != 0;
+                    if (dl) dcur[i0 + lane] = 0;
+                    wmask = uniform64(wmask | __ballot(dl));
+                }
+                bool cross = false;
+                while (wmask) {
+                    int jj[4];
+                    int nt = 0;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (wmask) {
+                            const int j = pick_bit(wmask, true);
+                            wmask &= wmask - 1;
+                            jj[t] = j;
+                            nt++;
+                        } else {
+                            jj[t] = jj[0];
+                        }
+                    }
+                    processed += nt;
+                    DT x[4], nb[4][kBD];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        x[t] = L[(i0 + jj[t]) * kWave + lane];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
+                            nb[t][q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (t >= nt) break;
+                        const int j = jj[t];
+                        DT best = x[t];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++)
+                            best = relax_min(nb[t][q], __builtin_amdgcn_readlane(ic[q], j),
+                                             __builtin_amdgcn_readlane(iw[q], j), INF, best);
+                        const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
+                        const int ea = ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0;
+                        const int eb = ov ? __builtin_amdgcn_readlane(em, j) : 0;
+                        for (int e = ea; e < eb; e++) {
+                            const Edge ed = badj[e];
+                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                        }
+                        const bool lower = best < x[t];
+                        if (lower) L[(i0 + j) * kWave + lane] = best;
+                        if (__ballot(lower) != 0) {
+                            chg_mask |= 1ull << j;
+                            mark_block_neighbours(j, ic, i0, nv, ea, eb, badj, dnext, wmask, cross);
+                        }
+                    }
+                }
+                if (cross && lane == 0) s_flag[it % 3] = 1;
+                __syncthreads();
+                if (s_flag[it % 3] == 0) {
+                    if (stats && lane == 0) {
+                        if (tid == 0) {
+                            atomicAdd(&stats[0], 1ull);
+                            atomicAdd(&stats[1], (unsigned long long)(it + 1));
+                        }
+                        atomicAdd(&stats[2], processed);
+                    }
+                    break;
+                }
+            }
+            const long long t_swept = stats ? clock64() : 0;
+            // 3. write back the slots that dropped, eight per step; flag the
+            //    blocks beyond a dropped border slot (lane q takes its outside
+            //    neighbour q, the rest of the list one lane each)
+            chg_mask = uniform64(chg_mask);
+            while (chg_mask) {
+                int jj[8];
+                int nt = 0;
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    if (chg_mask) {
+                        const int j = pick_bit(chg_mask, true);
+                        chg_mask &= chg_mask - 1;
+                        jj[t] = j;
+                        nt++;
+                    } else {
+                        jj[t] = jj[0];
+                    }
+                }
+                DT x[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++) x[t] = L[(i0 + jj[t]) * kWave + lane];
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    if (t >= nt) break;
+                    const int j = jj[t];
+                    Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane] = x[t];
+                    const int ea = __builtin_amdgcn_readlane(em, j);
+                    const int eb = __builtin_amdgcn_readlane(e1, j);
+                    if (ea < eb) {
+                        int fb = -1;
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int bq = __builtin_amdgcn_readlane(xb[q], j);
+                            if (lane == q) fb = bq;
+                        }
+                        uint32_t *Fc = Fnext + (int64_t)c * nblk;
+                        if (fb >= 0 && Fc[fb] == 0) Fc[fb] = 1;
+                        for (int e = ea + kBD + lane; e < eb; e += kWave) {
+                            const int ob = vblk[badj[e].v];
+                            if (Fc[ob] == 0) Fc[ob] = 1;
+                        }
+                        raised = true;
+                    }
+                }
+            }
+            if (stats && tid == 0) {
+                const long long t_end = clock64();
+                atomicAdd(&stats[3], (unsigned long long)(t_loaded - t_start));
+                atomicAdd(&stats[4], (unsigned long long)(t_swept - t_loaded));
+                atomicAdd(&stats[5], (unsigned long long)(t_end - t_swept));
+            }
+            __syncthreads();  // LDS is reused by the next item
+        }
+        __syncthreads();  // s_cnt is reset by the next piece
+    }
+    if (raised && lane == 0) any[(r + 1) & 3] = 1;
+}
+
 // ------------------------------------------------------------------ merge
 // f[X][v] = min over nonempty Y strictly inside X of f[Y][v] + f[X\Y][v]
 // (reader-el.c:2068-2088).  Only Y without the top bit of X is enumerated:
@@ -482,17 +879,18 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
 
 // Singleton rows: f[1<<t][v] = dist(kk[t], v) from chunk 0 lane t; record
 // (kk[t], {t}) for every v (reader-el.c:2015-2027).  Also the per-row
-// maximum distance, which bounds the table: f[X][v] <= sum over t in X of
-// dist(kk[t], v).
-template <typename T>
+// maximum distance (all ones if a vertex is unreached), which bounds the
+// table: f[X][v] <= sum over t in X of dist(kk[t], v).
+template <typename T, typename DT>
 __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
-    const uint64_t *__restrict__ D, int64_t np, const int32_t *__restrict__ kk, int k, int n,
+    const DT *__restrict__ D, int64_t np, const int32_t *__restrict__ kk, int k, int n,
     T *__restrict__ f, int32_t *__restrict__ b, unsigned long long *__restrict__ rowmax) {
     const int t = blockIdx.y;
     const int v = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long d = 0;
     if (v < n) {
-        d = D[(int64_t)v * kWave + t];
+        const DT x = D[(int64_t)v * kWave + t];
+        d = (x == (DT)~(DT)0) ? ~0ull : (unsigned long long)x;
         if (f) {
             const int64_t X = 1ll << t;
             f[X * np + v] = (T)d;
@@ -509,23 +907,24 @@ __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
 // Distances and predecessors of one shortest-path query (chunk 0, lane 0):
 // the predecessor is the smallest neighbour u with d(u) + w == d(v); the
 // source (distance 0, weights >= 1) and unreached vertices have none.
-[[maybe_unused]] static __global__ void k_unpack_sssp_v(const uint64_t *__restrict__ D, int n,
-                                                        const int32_t *__restrict__ rp,
-                                                        const Edge *__restrict__ adj,
-                                                        int64_t *__restrict__ d,
-                                                        int64_t *__restrict__ pred) {
+template <typename DT>
+__global__ void k_unpack_sssp_v(const DT *__restrict__ D, int n, const int32_t *__restrict__ rp,
+                                const Edge *__restrict__ adj, int64_t *__restrict__ d,
+                                int64_t *__restrict__ pred) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const uint64_t dd = D[(int64_t)v * kWave];
+    const DT INF = (DT)~(DT)0;
+    const DT dd = D[(int64_t)v * kWave];
     int64_t p = -1;
-    if (dd != 0 && dd != ~0ull) {
+    if (dd != 0 && dd != INF) {
         for (int e = rp[v]; e < rp[v + 1]; e++) {
             const int u = adj[e].v;
-            const uint64_t du = D[(int64_t)u * kWave];
-            if (du != ~0ull && du + (uint64_t)adj[e].w == dd && (p < 0 || u < p)) p = u;
+            const DT du = D[(int64_t)u * kWave];
+            if (du != INF && (uint64_t)du + (uint64_t)adj[e].w == (uint64_t)dd && (p < 0 || u < p))
+                p = u;
         }
     }
-    d[v] = (dd == ~0ull) ? INT64_MAX : (int64_t)dd;
+    d[v] = (dd == INF) ? INT64_MAX : (int64_t)dd;
     pred[v] = p;
 }
 

@@ -40,6 +40,7 @@ class EmvResult(ctypes.Structure):
         ("merge_alg_bytes", ctypes.c_double), ("relax_alg_bytes", ctypes.c_double),
         ("merge_launches", ctypes.c_int64), ("relax_rounds", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64), ("npairs", ctypes.c_int64), ("pairs", _i64p),
+        ("relax_blocks", ctypes.c_int64),
     ]
 
 
@@ -103,6 +104,7 @@ class Solution:
         self.merge_launches = r.merge_launches
         self.relax_rounds = r.relax_rounds
         self.device_bytes = r.device_bytes
+        self.relax_blocks = r.relax_blocks
         self.pairs = pairs
 
 

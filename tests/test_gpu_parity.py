@@ -304,6 +304,7 @@ def test_worklist_rounds_agree(gpu_ctx, monkeypatch, qcap):
     """Rounds taken from the worklist (EMV_QCAP > 0) give the flag-scan
     result cell for cell; 64 forces list overflow back to the flag scan."""
     inst = Instance(3600, emv.grid_graph(60, 60, 100, 12), emv.terminals(3600, 7, 12))
+    monkeypatch.setenv("EMV_BLOCKS", "0")  # vertex-wise rounds, where the worklist lives
     _, f1, c1 = solve_full(gpu_ctx, inst)
     monkeypatch.setenv("EMV_QCAP", qcap)
     ctx = emv.Context(0)
@@ -313,3 +314,81 @@ def test_worklist_rounds_agree(gpu_ctx, monkeypatch, qcap):
         ctx.close()
     np.testing.assert_array_equal(f1, f2)
     np.testing.assert_array_equal(c1, c2)
+
+
+@pytest.mark.parametrize("flags", [0, emv.WIDE_LABELS, emv.FORCE_P64])
+def test_block_relax_grid_vs_oracle(gpu_ctx, flags):
+    """Grids take the block-local relaxation (LDS sweeps inside vertex
+    blocks); every cell and record equal the oracle's at every label width."""
+    R, C = 50, 60
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 21)
+    kk = emv.terminals(n, 8, 21)
+    print(f"case grid {R}x{C} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    if flags != emv.FORCE_P64:
+        assert sol.relax_blocks > 0
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
+@pytest.mark.parametrize("n,d,seed", [(1000, 4, 1), (300, 20, 2), (1200, 16, 3)])
+def test_block_relax_forced_vs_oracle(gpu_ctx, monkeypatch, n, d, seed):
+    """EMV_BLOCKS=1 forces blocks on graphs without locality: most neighbours
+    outside the block (d=16: more outside neighbours than the kernel holds in
+    registers) or more than eight inside it (n=300, d=20) take the overflow
+    walks of the kernel."""
+    e = emv.regular_graph(n, d, 1000, 700 + seed)
+    kk = emv.terminals(n, 7, 700 + seed)
+    monkeypatch.setenv("EMV_BLOCKS", "1")
+    print(f"case forced blocks n={n} d={d}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
+    assert sol.relax_blocks > 0
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
+def test_block_and_vertex_rounds_agree_on_group_instance(gpu_ctx, monkeypatch):
+    """A group-reduced grid (virtual terminals of degree 12, joined by weight-M
+    edges) solved with blocks and with vertex-wise rounds: same table, same
+    records, the oracle's cost."""
+    import workloads
+
+    n, e, grp, nn, red, kk, M = workloads.group_instance(36, 36, 5, 12, 100, 31)
+    inst = Instance(nn, red, kk)
+    print(f"case group grid nn={nn} k={len(kk)}", flush=True)
+    s1, f1, c1 = solve_full(gpu_ctx, inst)
+    assert s1.relax_blocks > 0
+    monkeypatch.setenv("EMV_BLOCKS", "0")
+    s2, f2, c2 = solve_full(gpu_ctx, inst)
+    assert s2.relax_blocks == 0
+    np.testing.assert_array_equal(f1, f2)
+    np.testing.assert_array_equal(c1, c2)
+    assert s1.cost == oracle.emv(nn, red, kk)[0]
+
+
+@pytest.mark.parametrize("shape", ["regular", "grid"])
+def test_large_weights_64bit_labels(gpu_ctx, shape):
+    """Weights near 2^30: the shortest-path chunk runs on 64-bit labels
+    (n * max w >= 2^32), the table is int64, and cells, records and cost
+    still equal the oracle's (vertex-wise rounds and blocks)."""
+    if shape == "regular":
+        n = 1500
+        e = emv.regular_graph(n, 5, 1 << 30, 41)
+    else:
+        n = 40 * 45
+        e = emv.grid_graph(40, 45, 1 << 30, 41)
+    kk = emv.terminals(n, 6, 41)
+    print(f"case large weights {shape} n={n}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
+    assert sol.table_bytes == 8
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+    d, p, _ = gpu_ctx.sssp(int(kk[0]))
+    np.testing.assert_array_equal(d, of[1])
