@@ -72,7 +72,7 @@ __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict
 }
 
 // ------------------------------------------------------------------ merge
-// Same recurrence and packing as k_merge_v (reader-el.c:2068-2088), per
+// Same recurrence and packing as merge_one (reader-el.c:2068-2088), per
 // instance: f rows of the instance's table, shift by the instance's k.
 template <typename P, int XG>
 __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f,

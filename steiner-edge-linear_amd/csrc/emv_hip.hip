@@ -69,6 +69,8 @@ struct emv_ctx {
     uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
     uint32_t qcap = 0;
     uint32_t *d_cnt = nullptr;                // 4 rotating worklist counters
+    void *d_M = nullptr;                      // packed accumulators of a split merge
+    size_t m_cap = 0;
     int64_t ws_chunks = 0;
     uint32_t *d_any = nullptr;               // 4 rotating "round has work" words
     unsigned long long *d_rowmax = nullptr;  // per singleton row
@@ -177,6 +179,8 @@ int ensure_table(emv_ctx *c, size_t fbytes, size_t bbytes) {
 int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
 
 void free_ws(emv_ctx *c) {
+    dfree(c->d_M);
+    c->m_cap = 0;
     dfree(c->d_q[0]);
     dfree(c->d_q[1]);
     c->qcap = 0;
@@ -360,8 +364,8 @@ struct SolveState {
     int dbits;
 };
 
-// Rows per merge wave: few rows per wave when the batch has few rows (the
-// top layers), so that every tile still spreads over many waves.
+// Rows per merge wave: few rows per wave when the batch has few rows, so
+// that every tile still spreads over many waves.
 int merge_rows_per_wave(int nx) {
     if (nx >= 1024) return 8;
     if (nx >= 256) return 4;
@@ -382,8 +386,42 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     return EMV_OK;
 }
 
+// Split merge for layers whose rows have >= kSplitPairs pairs: fill the
+// packed accumulators, fold the parts, pack into the staging.
+constexpr uint32_t kSplitPairs = 128;
+
 template <typename T, typename P, typename DT>
-int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
+int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
+    const size_t bytes = sizeof(P) * (size_t)nx * (size_t)c->np;
+    if (c->m_cap < bytes) {
+        dfree(c->d_M);
+        c->m_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_M, bytes));
+        c->m_cap = bytes;
+    }
+    const int64_t cells = (int64_t)nx * c->np;
+    const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL(k_fill<P>, dim3(fb), dim3(kBlock), 0, c->stream, (P *)c->d_M, cells, ~(P)0);
+    // enough items per tile to occupy an XCD's waves; parts of >= 4 outer steps
+    const int64_t steps = (m - 1 <= 3) ? 1 : (1ll << (m - 1 - 3));
+    const int64_t S = std::max<int64_t>(1, std::min<int64_t>((512 + nx - 1) / nx, steps / 4));
+    const int64_t ntiles = c->np / kWave;
+    int64_t per;
+    const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
+    hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
+                       c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per);
+    const int64_t vtiles = c->np / kWave;
+    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
+                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
+                       vtiles);
+    HIPCHK(c, hipGetLastError());
+    return EMV_OK;
+}
+
+template <typename T, typename P, typename DT>
+int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
+    if (((1u << (m - 1)) - 1u) >= kSplitPairs)
+        return launch_merge_split<T, P, DT>(c, f, nx, nchunks, k, m);
     switch (merge_rows_per_wave(nx)) {
     case 8: return launch_merge_xg<T, P, DT, 8>(c, f, nx, nchunks, k);
     case 4: return launch_merge_xg<T, P, DT, 4>(c, f, nx, nchunks, k);
@@ -423,7 +461,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             ev.push_back(e2);
             HIPCHK(c, hipEventRecord(e0, c->stream));
             // merge -> staging
-            CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k)));
+            CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             // relax
@@ -693,6 +731,8 @@ extern "C" int emv_open(emv_ctx **out, int device) {
         c->nxcd = (c->num_cu % 32 == 0 && c->num_cu >= 64) ? c->num_cu / 32 : 1;
         const char *rx = getenv("EMV_RELAX_XCD");
         c->relax_nxcd = (rx && atoi(rx) > 0) ? c->nxcd : 1;
+
+
         if (getenv("EMV_TRACE") && atoi(getenv("EMV_TRACE")) > 0) {
             if (hipMalloc((void **)&c->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
                 hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {

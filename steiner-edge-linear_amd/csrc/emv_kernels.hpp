@@ -710,38 +710,103 @@ __global__ __launch_bounds__(kBT) void k_relax_blk(
 // accumulator packs (sum << k) | Y, so an unsigned min keeps the smallest
 // sum and, among equal sums, the smallest Y -- the reference's first strict
 // minimum over increasing Y (strict '<' at 2078).
-template <typename T, typename P>
-__device__ __forceinline__ P merge_one(const T *__restrict__ fv, int64_t np, uint32_t X, int k) {
-    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
-    const uint32_t npairs = (1u << __popc(Xp)) - 1u;
-    P acc = ~(P)0;
-    uint32_t Y = Xp;  // subsets of Xp in decreasing order: Y <- (Y - 1) & Xp
-    uint32_t i = 0;
-    // eight pairs per trip, sixteen row loads in flight, no guards
-    for (; i + 8 <= npairs; i += 8) {
-        uint32_t Ys[8];
-        Ys[0] = Y;
-#pragma unroll
-        for (int q = 1; q < 8; q++) Ys[q] = (Ys[q - 1] - 1) & Xp;
-        T a[8], c[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            a[q] = fv[(int64_t)Ys[q] * np];
-            c[q] = fv[(int64_t)(X ^ Ys[q]) * np];
-        }
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const P s = (((P)a[q] + (P)c[q]) << k) | (P)Ys[q];
-            acc = s < acc ? s : acc;
-        }
-        Y = (Ys[7] - 1) & Xp;
+// The subset of `mask` whose bits, read from the lowest, spell x.
+__device__ __forceinline__ uint32_t deposit_bits(uint32_t x, uint32_t mask) {
+    uint32_t r = 0;
+    for (uint32_t bb = 1; mask != 0; bb <<= 1) {
+        const uint32_t low = mask & (~mask + 1u);
+        if (x & bb) r |= low;
+        mask ^= low;
     }
-    for (; i < npairs; i++) {
+    return r;
+}
+
+// Pairs of X, enumerated for cheap addressing.  Y runs over the nonempty
+// subsets of Xp = X without its top bit (each unordered pair once, Y the side
+// without the top bit, which is the smaller one).  Row offsets of disjoint
+// sets add up, off(Y | Z) = off(Y) + off(Z), so with Xp split into L (its
+// lowest set bits, at most three) and H = Xp \ L, Y = Yh | Yl is walked with
+// Yh over the subsets of H ("outer" steps, highest first: Yh = deposit(nH - 1
+// - o, H)) and Yl over the subsets of L unrolled: one 64-bit multiply serves
+// 2^|L| pairs, and the partner row X \ Y sits at off(X) - off(Y).  The packed
+// minimum does not depend on the order, so any slice [o0, o1) of the outer
+// steps can be folded separately.
+template <typename T, typename P>
+__device__ __forceinline__ P merge_outer(const T *__restrict__ fv, int64_t np, uint32_t X, int k,
+                                         uint32_t o0, uint32_t o1) {
+    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+    uint32_t L = 0, H = Xp;
+    for (int i = 0; i < 3 && H; i++) {
+        const uint32_t lo = H & (~H + 1u);
+        L |= lo;
+        H ^= lo;
+    }
+    const int nl = __popc(L);
+    const uint32_t nH = 1u << __popc(H);
+    const int64_t oX = (int64_t)X * np;
+    P acc = ~(P)0;
+    uint32_t Yh = deposit_bits(nH - 1u - o0, H);
+    if (nl == 3) {
+        uint32_t yl[8];
+        int64_t ol[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            yl[j] = deposit_bits((uint32_t)j, L);
+            ol[j] = (int64_t)yl[j] * np;
+        }
+        const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Yh == 0 last, apart
+        for (uint32_t o = o0; o < oend; o++) {
+            const int64_t oh = (int64_t)Yh * np;
+            const T *pa = fv + oh;
+            const T *pc = fv + (oX - oh);
+            T a[8], c[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                a[j] = pa[ol[j]];
+                c[j] = pc[-ol[j]];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const P s = (((P)a[j] + (P)c[j]) << k) | (P)(Yh | yl[j]);
+                acc = s < acc ? s : acc;
+            }
+            Yh = (Yh - 1u) & H;
+        }
+        if (o1 == nH) {  // Yh == 0: the seven nonempty subsets of L
+            const T *pc = fv + oX;
+            T a[8], c[8];
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                a[j] = fv[ol[j]];
+                c[j] = pc[-ol[j]];
+            }
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                const P s = (((P)a[j] + (P)c[j]) << k) | (P)yl[j];
+                acc = s < acc ? s : acc;
+            }
+        }
+        return acc;
+    }
+    // |Xp| <= 2 (layers 2 and 3): H is empty, one outer step over L
+    for (uint32_t Y = L; Y != 0; Y = (Y - 1u) & L) {
         const P s = (((P)fv[(int64_t)Y * np] + (P)fv[(int64_t)(X ^ Y) * np]) << k) | (P)Y;
         acc = s < acc ? s : acc;
-        Y = (Y - 1) & Xp;
     }
     return acc;
+}
+
+// Number of outer steps of X in merge_outer.
+__device__ __forceinline__ uint32_t merge_steps(uint32_t X) {
+    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+    const int r = __popc(Xp);
+    return r <= 3 ? 1u : (1u << (r - 3));
+}
+
+// All pairs of X.
+template <typename T, typename P>
+__device__ __forceinline__ P merge_one(const T *__restrict__ fv, int64_t np, uint32_t X, int k) {
+    return merge_outer<T, P>(fv, np, X, k, 0u, merge_steps(X));
 }
 
 // Work item = one wave: 64 consecutive vertices (one per lane, 256 B
@@ -787,6 +852,72 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
         cd[jx] = pv[jx];
+    }
+}
+
+// Split merge for the upper layers (few rows, many pairs per row): the outer
+// steps of row xi are cut into S parts, and item = (tile, xi, part) is
+// numbered tile-major and dealt to XCDs in contiguous ranges, so the waves
+// of an XCD work through one 64-vertex tile at a time and read its
+// lower-layer rows from that XCD's L2.  Each item folds its part into the
+// packed accumulator M[xi * np + v] with an unsigned atomicMin; the minimum
+// does not depend on the order, so the split leaves every result unchanged.
+// M starts at all ones; k_pack_v moves it to the staging.
+template <typename T, typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f, int64_t np,
+                                                        const uint32_t *__restrict__ xs, int nx,
+                                                        int S, int64_t ntiles, int k,
+                                                        P *__restrict__ M, int nxcd,
+                                                        int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)nx * S;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int xi = rem / S;
+    const int part = rem - xi * S;
+    const uint32_t X = xs[xi];
+    const uint32_t steps = merge_steps(X);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / S);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / S);
+    if (o0 >= o1) return;
+    const int64_t v = tile * kWave + lane;
+    const P acc = merge_outer<T, P>(f + v, np, X, k, o0, o1);
+    atomicMin(&M[(int64_t)xi * np + v], acc);
+}
+
+// Packed accumulators M[xi][v] -> vertex-major staging (label, packed word)
+// through an LDS transpose of 64 rows x 64 vertices per workgroup; rows past
+// nx are INF.
+template <typename P, typename DT>
+__global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int64_t np, int nx,
+                                                   int k, DT *__restrict__ D,
+                                                   P *__restrict__ CODE, int64_t vtiles) {
+    __shared__ P sm[kWave][kWave + 1];
+    const int64_t chunk = blockIdx.x / vtiles;
+    const int64_t vt = blockIdx.x - chunk * vtiles;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int grp = t / kWave;
+    const int64_t v0 = vt * kWave;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int xl = grp + r * (kBlock / kWave);
+        const int64_t xi = chunk * kWave + xl;
+        sm[xl][lane] = (xi < nx) ? M[xi * np + v0 + lane] : ~(P)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int vv = grp + r * (kBlock / kWave);
+        const P a = sm[lane][vv];
+        const int64_t idx = (chunk * np + v0 + vv) * kWave + lane;
+        const bool real = (chunk * kWave + lane) < nx;
+        D[idx] = real ? (DT)(a >> k) : ~(DT)0;
+        CODE[idx] = real ? a : ~(P)0;
     }
 }
 

@@ -392,3 +392,21 @@ def test_large_weights_64bit_labels(gpu_ctx, shape):
     np.testing.assert_array_equal(code, ocode)
     d, p, _ = gpu_ctx.sssp(int(kk[0]))
     np.testing.assert_array_equal(d, of[1])
+
+
+
+
+@pytest.mark.parametrize("flags", [0, emv.FORCE_I64, emv.FORCE_P64])
+def test_merge_k11_vs_oracle(gpu_ctx, flags):
+    """The merge at k=11 -- whole rows on the lower layers, rows split into
+    parts folded with atomicMin on the upper ones: every cell and record
+    equal the oracle's, for int32/int64 tables and 32/64-bit accumulators."""
+    n, d, k = 3000, 5, 11
+    e = emv.regular_graph(n, d, 1000, 89)
+    kk = emv.terminals(n, k, 89)
+    print(f"case merge k=11 flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
