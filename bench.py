@@ -9,7 +9,9 @@ table of 2^14 x 2^18 cells.  The graph is resident in HBM before timing.
 Prints one JSON line.  `value` = DP cells (2^k * n) per second of wall time
 over the timed steps.  `roofline` covers the dominant kernel (largest share
 of device time), with achieved = algorithmic bytes / kernel time from HIP
-events on the library's stream.  `cpu_baseline` times the reference
+events on the library's stream, and traffic = HBM bytes per launch measured
+by two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled against a
+plain copy of known size.  `cpu_baseline` times the reference
 reader-el (oracle/_ref, OpenMP) on a bounded sample of the same family on
 this host's cores.
 
@@ -27,7 +29,7 @@ the plain one-instance-at-a-time GPU path (Context.solve):
              replay the same arrival times through a FIFO single server
              whose service times are the measured plain solve times.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-traffic]
                     [--legs plain,batch,serve] [--queries 24]
 """
 import argparse
@@ -92,6 +94,76 @@ def cpu_baseline():
                 sample=sample, kernel_ms=ms)
 
 
+def _pmc_avg(db, prefix):
+    """Average counter value per dispatch (KiB) of the kernel instance whose
+    name starts with `prefix` and that takes the most time, its name and its
+    number of dispatches."""
+    import collections
+    import sqlite3
+
+    cur = sqlite3.connect(db).cursor()
+    rows = cur.execute("select name, counter_value, duration from pmc_events").fetchall()
+    by = collections.defaultdict(lambda: [0.0, 0.0, 0])
+    for name, v, dur in rows:
+        short = name.replace("void ", "", 1).lstrip()
+        if short.startswith(prefix):
+            t = by[short.split("(")[0]]
+            t[0] += v
+            t[1] += dur
+            t[2] += 1
+    if not by:
+        return None, None, 0
+    name, (tot, _, cnt) = max(by.items(), key=lambda kv: kv[1][1])
+    return tot / cnt, name, cnt
+
+
+def measure_traffic(args, kernel_prefix):
+    """HBM bytes per launch of the dominant kernel from two rocprofv3 counter
+    passes (FETCH_SIZE, then WRITE_SIZE) over one solve of the same workload,
+    each scaled by what the counter reads for a plain copy of known size
+    (emv-copyprobe, 1 GiB read + 1 GiB written per launch).  None when
+    rocprofv3 is missing or a pass fails."""
+    import shutil
+
+    prof = shutil.which("rocprofv3")
+    probe = os.path.join(PKG, "emv-copyprobe")
+    if not prof or not os.path.exists(probe):
+        return None
+    copy_bytes = 1 << 30
+    out = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        env = dict(os.environ, TMPDIR="/tmp")
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            runs = {
+                "copy": [probe, str(copy_bytes), "3"],
+                "solve": [sys.executable, os.path.abspath(__file__), "--legs", "plain", "--steps",
+                          "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--n", str(args.n),
+                          "--k", str(args.k)],
+            }
+            for tag, cmd in runs.items():
+                name = f"{tag}_{ctr}"
+                log(f"traffic: rocprofv3 --pmc {ctr} ({tag})")
+                r = subprocess.run([prof, "--pmc", ctr, "-d", td, "-o", name, "--"] + cmd, cwd=td,
+                                   env=env, capture_output=True, text=True, timeout=600)
+                db = os.path.join(td, f"{name}_results.db")
+                if r.returncode != 0 or not os.path.exists(db):
+                    log(f"traffic: {name} failed (rc={r.returncode})")
+                    return None
+                prefix = "k_copy16" if tag == "copy" else kernel_prefix
+                out[name] = _pmc_avg(db, prefix)
+    fetch_copy = out["copy_FETCH_SIZE"][0]
+    write_copy = out["copy_WRITE_SIZE"][0]
+    fetch, kname, nd = out["solve_FETCH_SIZE"]
+    write = out["solve_WRITE_SIZE"][0]
+    if not (fetch_copy and write_copy and fetch is not None and write is not None):
+        return None
+    fs = copy_bytes / (fetch_copy * 1024.0)
+    ws = copy_bytes / (write_copy * 1024.0)
+    return dict(kernel=kname, bytes=fetch * 1024.0 * fs + write * 1024.0 * ws,
+                fetch_bytes=fetch * 1024.0 * fs, write_bytes=write * 1024.0 * ws,
+                fetch_scale=fs, write_scale=ws, dispatches=nd)
+
+
 def plain_leg(args):
     import emv
 
@@ -115,7 +187,8 @@ def plain_leg(args):
         sol = ctx.solve(kk)  # returns after the library's stream has drained
         sols.append(sol)
         log(f"step {i} kernel={sol.kernel_ms:.1f} ms merge={sol.merge_ms:.1f} "
-            f"relax={sol.relax_ms:.1f} single={sol.singleton_ms:.1f} rounds={sol.relax_rounds}")
+            f"relax={sol.relax_ms:.1f} records={sol.unpack_ms:.1f} single={sol.singleton_ms:.1f} "
+            f"rounds={sol.relax_rounds}")
     wall = time.perf_counter() - t0
     ctx.close()
     for s in sols:
@@ -125,11 +198,15 @@ def plain_leg(args):
     cells = sols[-1].dp_cells
     merge_ms = sum(s.merge_ms for s in sols) / len(sols)
     relax_ms = sum(s.relax_ms for s in sols) / len(sols)
+    unpack_ms = sum(s.unpack_ms for s in sols) / len(sols)
+    # relax: the layer rounds (the singleton rounds are in singleton_ms);
+    # its algorithmic bytes are the per-subset model without the k singleton rows
+    k = sols[-1].k
+    relax_bytes = sols[-1].relax_alg_bytes * ((1 << k) - 1 - k) / ((1 << k) - 1)
     kern = {
         "merge": dict(bytes=sols[-1].merge_alg_bytes, ms=merge_ms,
                       launches=sols[-1].merge_launches),
-        "relax": dict(bytes=sols[-1].relax_alg_bytes, ms=relax_ms,
-                      launches=sols[-1].relax_rounds),
+        "relax": dict(bytes=relax_bytes, ms=relax_ms, launches=sols[-1].relax_rounds),
     }
     for v in kern.values():
         v["achieved_GBs"] = v["bytes"] / (v["ms"] * 1e6) if v["ms"] > 0 else 0.0
@@ -141,6 +218,16 @@ def plain_leg(args):
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=round(kern[dom]["frac"], 4), traffic=None,
                 merge_relax_frac=round((kern["merge"]["bytes"] + kern["relax"]["bytes"]) /
                                        ((merge_ms + relax_ms) * 1e6) / HBM_PEAK_GBS, 4))
+    if not args.no_traffic:
+        tr = measure_traffic(args, "emv::k_relax" if dom == "relax" else "emv::k_merge")
+        if tr and dom == "merge":
+            tr = None  # the merge runs as several kernels per layer; no single per-launch figure
+        if tr:
+            launch_s = kern[dom]["ms"] / 1e3 / max(kern[dom]["launches"], 1)
+            roof["traffic"] = tr["bytes"]
+            roof["traffic_detail"] = tr
+            roof["traffic_GBs"] = round(tr["bytes"] / launch_s / 1e9, 1)
+            roof["traffic_frac"] = round(tr["bytes"] / launch_s / 1e9 / HBM_PEAK_GBS, 4)
     return dict(
         value=cells * args.steps / wall,
         ms_per_step=wall * 1e3 / args.steps,
@@ -153,7 +240,7 @@ def plain_leg(args):
         cost=cost,
         kernel_ms=sum(s.kernel_ms for s in sols) / len(sols),
         singleton_ms=sum(s.singleton_ms for s in sols) / len(sols),
-        merge_ms=merge_ms, relax_ms=relax_ms, kernels=kern, roofline=roof)
+        merge_ms=merge_ms, relax_ms=relax_ms, unpack_ms=unpack_ms, kernels=kern, roofline=roof)
 
 
 def batch_leg(args):
@@ -327,6 +414,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--queries", type=int, default=24)
     ap.add_argument("--rates", default="0.5,1,2")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes behind roofline.traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "

@@ -60,19 +60,21 @@ typedef struct {
     double zero_ms;          /* table allocation + initial fill */
     double singleton_ms;     /* k single-source shortest paths */
     double merge_ms;         /* sum over layers of the subset-merge kernels */
-    double relax_ms;         /* sum over layers of the relaxation (all rounds + unpack) */
-    double kernel_ms;        /* singleton + merge + relax, as the reference's [kernel:] */
+    double relax_ms;         /* sum over layers of the relaxation rounds */
+    double kernel_ms;        /* singleton + merge + relax + unpack, as the reference's [kernel:] */
     double traceback_ms;
     double total_ms;
     /* algorithmic traffic models (bytes) */
     double merge_alg_bytes;  /* cb*n*(3^k-2^(k+1)+1) + (cb+ab)*n*(2^k-k-1) */
     double relax_alg_bytes;  /* (2^k-1)*(cb*n + cb*2m + (cb+ab)*n) + k*(4(n+1)+16m) */
     int64_t merge_launches;
-    int64_t relax_rounds;    /* relaxation rounds launched, all layers */
+    int64_t relax_rounds;    /* relaxation rounds launched, layers 2..k */
     int64_t device_bytes;    /* device memory held during the solve */
     int64_t npairs;          /* traceback: number of (v,u) pairs */
     int64_t *pairs;          /* 2*npairs, 0-based, reference DFS order (reader-el.c:1907-1944) */
     int64_t relax_blocks;    /* vertex blocks of the block-local relaxation; 0 = vertex-wise rounds */
+    double unpack_ms;        /* sum over layers of the record decisions + table writes */
+    int64_t singleton_rounds;/* relaxation rounds of the singleton shortest paths */
 } emv_result;
 
 /* Context on HIP device `device`. */

@@ -452,13 +452,11 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             const int64_t nchunks = (nx + kWave - 1) / kWave;
             HIPCHK(c, hipMemcpyAsync(c->d_xs, xs.data() + off, sizeof(uint32_t) * nx,
                                      hipMemcpyHostToDevice, c->stream));
-            hipEvent_t e0, e1, e2;
-            HIPCHK(c, hipEventCreate(&e0));
-            ev.push_back(e0);
-            HIPCHK(c, hipEventCreate(&e1));
-            ev.push_back(e1);
-            HIPCHK(c, hipEventCreate(&e2));
-            ev.push_back(e2);
+            hipEvent_t e0, e1, e2, e3;
+            for (hipEvent_t *pe : {&e0, &e1, &e2, &e3}) {
+                HIPCHK(c, hipEventCreate(pe));
+                ev.push_back(*pe);
+            }
             HIPCHK(c, hipEventRecord(e0, c->stream));
             // merge -> staging
             CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
@@ -466,7 +464,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             HIPCHK(c, hipEventRecord(e1, c->stream));
             // relax
             CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
-            // unpack
+            HIPCHK(c, hipEventRecord(e2, c->stream));
+            // records + unpack
             const int64_t vtiles = (n + kWave - 1) / kWave;
             int64_t per;
             const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
@@ -474,7 +473,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                                (const DT *)c->d_D, (const P *)c->d_code, np, c->d_xs, nx, (int)n,
                                k, c->d_rp, c->d_adj, f, b, c->nxcd, per, vtiles);
             HIPCHK(c, hipGetLastError());
-            HIPCHK(c, hipEventRecord(e2, c->stream));
+            HIPCHK(c, hipEventRecord(e3, c->stream));
         }
     }
     return EMV_OK;
@@ -921,7 +920,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // ---- 1. singleton shortest paths in chunk 0, lane t = kk[t]
     std::vector<int32_t> src((size_t)k);
     for (int64_t t = 0; t < k; t++) src[t] = (int32_t)kk[t];
-    CHK(sssp_chunk(c, src, &res->relax_rounds));
+    CHK(sssp_chunk(c, src, &res->singleton_rounds));
     // ---- 2. bound from the per-row maxima
     hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, c->stream, c->d_rowmax,
                        (int64_t)64, 0ull);
@@ -1070,9 +1069,10 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->dp_cells = st.S2 * n;
     res->zero_ms = ev_ms(e_zero0, e_zero1);
     res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;
-    for (size_t i = ev_layers0; i + 2 < ev.size(); i += 3) {
+    for (size_t i = ev_layers0; i + 3 < ev.size(); i += 4) {
         res->merge_ms += ev_ms(ev[i], ev[i + 1]);
         res->relax_ms += ev_ms(ev[i + 1], ev[i + 2]);
+        res->unpack_ms += ev_ms(ev[i + 2], ev[i + 3]);
     }
     res->kernel_ms = ev_ms(e_begin, e_end) - res->zero_ms;
     res->traceback_ms = ev_ms(e_tb0, e_tb1);

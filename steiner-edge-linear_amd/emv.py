@@ -40,7 +40,8 @@ class EmvResult(ctypes.Structure):
         ("merge_alg_bytes", ctypes.c_double), ("relax_alg_bytes", ctypes.c_double),
         ("merge_launches", ctypes.c_int64), ("relax_rounds", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64), ("npairs", ctypes.c_int64), ("pairs", _i64p),
-        ("relax_blocks", ctypes.c_int64),
+        ("relax_blocks", ctypes.c_int64), ("unpack_ms", ctypes.c_double),
+        ("singleton_rounds", ctypes.c_int64),
     ]
 
 
@@ -105,6 +106,8 @@ class Solution:
         self.relax_rounds = r.relax_rounds
         self.device_bytes = r.device_bytes
         self.relax_blocks = r.relax_blocks
+        self.unpack_ms = r.unpack_ms
+        self.singleton_rounds = r.singleton_rounds
         self.pairs = pairs
 
 

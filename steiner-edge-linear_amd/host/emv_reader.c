@@ -132,11 +132,11 @@ static el_out erickson_gpu(const stp_query *q, int list)
     double cells = (double)res.dp_cells;
     printf("gpu: [device: %d] [init: %.2lf ms] [upload: %.2lf ms] [table: int%d] [merge acc: %d-bit] "
            "[singleton: %.2lf ms] [merge: %.2lf ms %.2lf GB/s] [relax: %.2lf ms %.2lf GB/s %ld rounds] "
-           "[dp cells: %.0lf] [cells/s: %.4e] [device mem: %.2lfGiB]\n",
+           "[records: %.2lf ms] [dp cells: %.0lf] [cells/s: %.4e] [device mem: %.2lfGiB]\n",
            device_id, ctx_init_ms, upload_ms, 8 * res.table_bytes, res.merge_bits, res.singleton_ms,
            res.merge_ms, res.merge_ms > 0 ? res.merge_alg_bytes / (res.merge_ms * 1e6) : 0.0,
            res.relax_ms, res.relax_ms > 0 ? res.relax_alg_bytes / (res.relax_ms * 1e6) : 0.0,
-           (long)res.relax_rounds, cells, cells / (res.kernel_ms / 1e3),
+           (long)res.relax_rounds, res.unpack_ms, cells, cells / (res.kernel_ms / 1e3),
            (double)res.device_bytes / (double)(1 << 30));
     fflush(stdout);
     emv_result_free(&res);
