@@ -81,9 +81,13 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
                                                    const int64_t *__restrict__ item_pre, int nmg,
                                                    const uint32_t *__restrict__ xs,
                                                    uint32_t *__restrict__ D,
-                                                   uint64_t *__restrict__ CODE) {
+                                                   uint64_t *__restrict__ CODE, int nxcd,
+                                                   int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int64_t item = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    // items dealt to XCDs in contiguous ranges: the waves of an XCD sweep the
+    // rows of one tile together, and its lower-layer rows stay in that L2
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
     if (item >= item_pre[nmg]) return;
     const int s = find_seg(item_pre, nmg, item);
     const GMerge M = mg[s];
@@ -207,34 +211,73 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 me1 = rp[gv + 1];
             }
         }
-        while (act) {
-            const int t = __ffsll((unsigned long long)act) - 1;
-            act &= act - 1;
-            const int g = __builtin_amdgcn_readlane(mg, t);
-            const int v = __builtin_amdgcn_readlane(mv, t);
-            const GChunk c = ch[g];
-            uint32_t *Dc = D + c.dbase;
-            const int64_t idx = (int64_t)v * kWave + lane;
-            const uint32_t cur = Dc[idx];
-            uint32_t best = cur;
-            int nb = -1;  // lane q < degree holds neighbour q (for the flags)
-            if constexpr (DMAX > 0) {
-                int uu[DMAX], ww[DMAX];
-                uint32_t du[DMAX];
+        if constexpr (DMAX > 0) {
+            // two items per step: 2 * DMAX neighbour gathers in flight
+            while (act) {
+                const int t1 = __ffsll((unsigned long long)act) - 1;
+                act &= act - 1;
+                const bool two = act != 0;
+                const int t2 = two ? __ffsll((unsigned long long)act) - 1 : t1;
+                if (two) act &= act - 1;
+                const int ga = __builtin_amdgcn_readlane(mg, t1);
+                const int gb = __builtin_amdgcn_readlane(mg, t2);
+                const int va = __builtin_amdgcn_readlane(mv, t1);
+                const int vb = __builtin_amdgcn_readlane(mv, t2);
+                const GChunk c1 = ch[ga];
+                const GChunk c2 = ch[gb];
+                uint32_t *Da = D + c1.dbase;
+                uint32_t *Db = D + c2.dbase;
+                const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
+                const uint32_t cura = Da[ia];
+                const uint32_t curb = two ? Db[ib] : INF;
+                int ua[DMAX], wa[DMAX], ub[DMAX], wb[DMAX];
+                uint32_t da[DMAX], db[DMAX];
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    uu[q] = __builtin_amdgcn_readlane(e[q].v, t);
-                    ww[q] = __builtin_amdgcn_readlane(e[q].w, t);
+                    ua[q] = __builtin_amdgcn_readlane(e[q].v, t1);
+                    wa[q] = __builtin_amdgcn_readlane(e[q].w, t1);
+                    ub[q] = __builtin_amdgcn_readlane(e[q].v, t2);
+                    wb[q] = __builtin_amdgcn_readlane(e[q].w, t2);
                 }
-#pragma unroll
-                for (int q = 0; q < DMAX; q++)
-                    du[q] = uu[q] >= 0 ? Dc[(int64_t)uu[q] * kWave + lane] : INF;
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    best = relax_min(du[q], uu[q], ww[q], INF, best);
-                    if (lane == q) nb = uu[q];
+                    da[q] = ua[q] >= 0 ? Da[(int64_t)ua[q] * kWave + lane] : INF;
+                    db[q] = (two && ub[q] >= 0) ? Db[(int64_t)ub[q] * kWave + lane] : INF;
                 }
-            } else {
+                uint32_t besta = cura, bestb = curb;
+                int na = -1, nb = -1;  // lane q < DMAX holds neighbour q (for the flags)
+#pragma unroll
+                for (int q = 0; q < DMAX; q++) {
+                    besta = relax_min(da[q], ua[q], wa[q], INF, besta);
+                    bestb = relax_min(db[q], ub[q], wb[q], INF, bestb);
+                    if (lane == q) {
+                        na = ua[q];
+                        nb = ub[q];
+                    }
+                }
+                const bool chga = besta < cura, chgb = two && bestb < curb;
+                if (chga) Da[ia] = besta;
+                if (chgb) Db[ib] = bestb;
+                if (__ballot(chga) != 0) {
+                    gflag(c1, item_pre[ga], na, Fnext, Qnext, cnext, qcap, list_open);
+                    raised = true;
+                }
+                if (__ballot(chgb) != 0) {
+                    gflag(c2, item_pre[gb], nb, Fnext, Qnext, cnext, qcap, list_open);
+                    raised = true;
+                }
+            }
+        } else {
+            while (act) {
+                const int t = __ffsll((unsigned long long)act) - 1;
+                act &= act - 1;
+                const int g = __builtin_amdgcn_readlane(mg, t);
+                const int v = __builtin_amdgcn_readlane(mv, t);
+                const GChunk c = ch[g];
+                uint32_t *Dc = D + c.dbase;
+                const int64_t idx = (int64_t)v * kWave + lane;
+                const uint32_t cur = Dc[idx];
+                uint32_t best = cur;
                 const int e0 = __builtin_amdgcn_readlane(me0, t);
                 const int e1 = __builtin_amdgcn_readlane(me1, t);
                 for (int eb = e0; eb < e1; eb += kWave) {
@@ -247,23 +290,17 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                         best = relax_min(Dc[(int64_t)u * kWave + lane], u, w, INF, best);
                     }
                 }
-            }
-            const bool dchg = best < cur;
-            if (dchg) Dc[idx] = best;
-            if (__ballot(dchg) != 0) {
-                const int64_t base_item = item_pre[g];
-                if constexpr (DMAX > 0) {
-                    gflag(c, base_item, nb, Fnext, Qnext, cnext, qcap, list_open);
-                } else {
-                    const int e0 = __builtin_amdgcn_readlane(me0, t);
-                    const int e1 = __builtin_amdgcn_readlane(me1, t);
+                const bool dchg = best < cur;
+                if (dchg) Dc[idx] = best;
+                if (__ballot(dchg) != 0) {
+                    const int64_t base_item = item_pre[g];
                     for (int eb = e0; eb < e1; eb += kWave) {
                         const int e2 = eb + lane;
                         gflag(c, base_item, e2 < e1 ? adj[e2].v : -1, Fnext, Qnext, cnext, qcap,
                               list_open);
                     }
+                    raised = true;
                 }
-                raised = true;
             }
         }
     }

@@ -364,7 +364,7 @@ struct StepPlan {
 void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P) {
     P = StepPlan();
     int64_t dcur = 0, fcur = 0;
-    int maxrows = 0;
+    int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
     for (int64_t t : tickets) {
         Inst &I = e->inst[t];
@@ -394,6 +394,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         subsets_of_size(I.k, I.stage, rows);
         const int nrows = (int)rows.size();
         maxrows = std::max(maxrows, nrows);
+        maxstage = std::max(maxstage, I.stage);
         if (I.pbits == 64) P.pbits = 64;
         GMerge M;
         M.chunk0 = (int)P.ch.size();
@@ -426,6 +427,9 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         P.xg = e->prm.merge_rows_per_wave;
     else
         P.xg = maxrows >= 1024 ? 8 : maxrows >= 256 ? 4 : maxrows >= 64 ? 2 : 1;
+    // rows of >= 128 pairs (layer >= 9): one row per wave, so that a tile's
+    // items fill an XCD and its lower rows are read from L2
+    if (e->prm.merge_rows_per_wave == 0 && maxstage >= 9) P.xg = 1;
     P.relax_pre.push_back(0);
     P.unpack_pre.push_back(0);
     for (const GChunk &c : P.ch) {
@@ -512,10 +516,13 @@ template <typename P_, int XG>
 void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P) {
     const int64_t items = P.merge_pre.back();
     const int64_t blocks = (items + (kBlock / kWave) - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((k_gmerge<P_, XG>), dim3((unsigned)blocks), dim3(kBlock), 0, e->stream,
-                       e->d_arena, (const GChunk *)(db + P.off_ch), (const GMerge *)(db + P.off_mg),
-                       (const int64_t *)(db + P.off_mp), (int)P.mg.size(),
-                       (const uint32_t *)(db + P.off_xs), e->d_D, e->d_code);
+    const int nxcd = (e->num_cu % 32 == 0 && e->num_cu >= 64) ? e->num_cu / 32 : 1;
+    const int64_t per = (blocks + nxcd - 1) / nxcd;
+    hipLaunchKernelGGL((k_gmerge<P_, XG>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
+                       e->stream, e->d_arena, (const GChunk *)(db + P.off_ch),
+                       (const GMerge *)(db + P.off_mg), (const int64_t *)(db + P.off_mp),
+                       (int)P.mg.size(), (const uint32_t *)(db + P.off_xs), e->d_D, e->d_code,
+                       nxcd, per);
 }
 
 template <typename P_>

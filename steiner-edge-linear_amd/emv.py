@@ -94,6 +94,7 @@ class Solution:
         self.merge_bits = r.merge_bits
         self.label_bits = r.label_bits
         self.dp_cells = r.dp_cells
+        self.zero_ms = r.zero_ms
         self.singleton_ms = r.singleton_ms
         self.merge_ms = r.merge_ms
         self.relax_ms = r.relax_ms
