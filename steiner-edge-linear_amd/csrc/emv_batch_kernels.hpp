@@ -9,9 +9,10 @@
 // The per-instance table keeps the single-instance layout in one arena
 // region: costs f[tbase + X * np + v] (int32), then the record codes
 // b[tbase + 2^k * np + X * np + v] (int32).
-// Staging chunks keep the vertex-major layout D[dbase + v * 64 + lane] with
-// 32-bit labels, and the merge's packed word (sum << k) | Y (64-bit) in
-// CODE[dbase + v * 64 + lane].  Work items are located through prefix arrays
+// Staging chunks keep the vertex-major layout, labels of type DT (16 or 32
+// bits, per chunk group) at ((DT *)(D + 4 * dbase))[v * 64 + lane] -- dbase
+// counts 4-byte cells -- and the merge's packed word (sum << k) | Y (64-bit)
+// in CODE[dbase + v * 64 + lane].  Work items are located through prefix arrays
 // over the descriptors (binary search), so instances of any size share one
 // grid.
 #pragma once
@@ -24,7 +25,7 @@ namespace emv {
 
 // One staging chunk: up to 64 rows of one instance.
 struct GChunk {
-    int64_t dbase;  // element offset of the chunk in D / CODE (np * 64 cells)
+    int64_t dbase;  // cell offset of the chunk in the staging (np * 64 cells)
     int64_t fbase;  // offset of the chunk's activity flags (np words)
     int64_t tbase;  // element offset of the instance's table
     int32_t n, np;  // instance vertices, table row stride
@@ -44,6 +45,11 @@ struct GMerge {
     int32_t nsub;    // nchunks * 64 / XG
 };
 
+template <typename DT>
+__device__ __forceinline__ DT *chunk_labels(void *D, int64_t dbase) {
+    return reinterpret_cast<DT *>(reinterpret_cast<char *>(D) + 4 * dbase);
+}
+
 __device__ __forceinline__ int find_seg(const int64_t *__restrict__ pre, int nseg, int64_t i) {
     // largest s with pre[s] <= i, pre[0] = 0, pre[nseg] = total
     int lo = 0, hi = nseg;
@@ -58,7 +64,7 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ pre, int nse
 // Every cell of every singleton chunk: INF label; then label 0 at the sources.
 __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict__ cell_pre,
                         int nch, const int32_t *__restrict__ kk_all,
-                        const int32_t *__restrict__ kk_off, uint32_t *__restrict__ D) {
+                        const int32_t *__restrict__ kk_off, void *__restrict__ D) {
     const int64_t total = cell_pre[nch];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
@@ -67,20 +73,21 @@ __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict
         const int64_t off = i - cell_pre[g];  // v * 64 + lane
         const int lane = (int)(off & (kWave - 1));
         const int v = (int)(off / kWave);
-        D[c.dbase + off] = (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) ? 0u : ~0u;
+        chunk_labels<uint32_t>(D, c.dbase)[off] =
+            (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) ? 0u : ~0u;
     }
 }
 
 // ------------------------------------------------------------------ merge
 // Same recurrence and packing as merge_one (reader-el.c:2068-2088), per
 // instance: f rows of the instance's table, shift by the instance's k.
-template <typename P, int XG>
+template <typename P, int XG, typename DT>
 __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f,
                                                    const GChunk *__restrict__ ch,
                                                    const GMerge *__restrict__ mg,
                                                    const int64_t *__restrict__ item_pre, int nmg,
                                                    const uint32_t *__restrict__ xs,
-                                                   uint32_t *__restrict__ D,
+                                                   void *__restrict__ D,
                                                    uint64_t *__restrict__ CODE, int nxcd,
                                                    int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -100,23 +107,23 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     const int k = c0.k;
     int nrows = 0;
     for (int q = 0; q < M.nchunks; q++) nrows += ch[M.chunk0 + q].nx;
-    uint32_t dv[XG];
+    DT dv[XG];
     uint64_t pv[XG];
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         const int xi = sub * XG + jx;
-        dv[jx] = ~0u;
+        dv[jx] = (DT)~(DT)0;
         pv[jx] = ~0ull;
         if (xi < nrows) {
             const P acc = merge_one<int32_t, P>(fv, (int64_t)c0.np, xs[c0.x0 + xi], k);
-            dv[jx] = (uint32_t)(acc >> k);
+            dv[jx] = (DT)(acc >> k);
             pv[jx] = (uint64_t)acc;
         }
     }
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
     const GChunk cc = ch[M.chunk0 + chunk];
-    uint32_t *dd = D + cc.dbase + v * kWave + loff;
+    DT *dd = chunk_labels<DT>(D, cc.dbase) + v * kWave + loff;
     uint64_t *cd = CODE + cc.dbase + v * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
@@ -146,12 +153,13 @@ __device__ __forceinline__ void gflag(const GChunk &c, int64_t base_item, int nb
 // ------------------------------------------------------------ relaxation
 // The pull round of k_relax_v (worklist or flag scan, same rules) over the
 // items of many chunks: item i of chunk g is vertex i - item_pre[g] of that
-// chunk's instance.  Adjacency is the shared padded fixed-degree copy
+// chunk's instance; blk_chunk[b] names the chunk of item 64 * b, so a wave
+// finds its items' chunks with one load.  Adjacency is the shared padded fixed-degree copy
 // (DMAX > 0, neighbour ids local to the instance's graph) or the shared CSR.
-template <int DMAX>
+template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_grelax(
-    uint32_t *__restrict__ D, const GChunk *__restrict__ ch,
-    const int64_t *__restrict__ item_pre, int nch, const int32_t *__restrict__ rp,
+    void *__restrict__ D, const GChunk *__restrict__ ch,
+    const int64_t *__restrict__ item_pre, const int32_t *__restrict__ blk_chunk, int nch, const int32_t *__restrict__ rp,
     const Edge *__restrict__ adj, const Edge *__restrict__ ell, int dense,
     uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
     uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
@@ -170,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
     const int64_t nitems = item_pre[nch];
     const int64_t span = use_list ? (int64_t)qn : nitems;
     const int64_t nblk = (span + kWave - 1) / kWave;
-    const uint32_t INF = ~0u;
+    const DT INF = (DT)~(DT)0;
     bool raised = false;
     for (int64_t blk = wave0; blk < nblk; blk += wstride) {
         const int64_t j = blk * kWave + lane;
@@ -179,7 +187,14 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
         int mg = 0, mv = 0;
         bool fl = false;
         if (mi >= 0) {
-            mg = find_seg(item_pre, nch, mi);
+            if (use_list) {
+                mg = find_seg(item_pre, nch, mi);
+            } else {
+                // the block's chunk from the table; a chunk boundary inside
+                // the block moves the lanes past it on
+                mg = blk_chunk[blk];
+                while (item_pre[mg + 1] <= mi) mg++;
+            }
             mv = (int)(mi - item_pre[mg]);
             const int64_t mf = ch[mg].fbase + mv;
             if (dense) {
@@ -225,13 +240,13 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const int vb = __builtin_amdgcn_readlane(mv, t2);
                 const GChunk c1 = ch[ga];
                 const GChunk c2 = ch[gb];
-                uint32_t *Da = D + c1.dbase;
-                uint32_t *Db = D + c2.dbase;
+                DT *Da = chunk_labels<DT>(D, c1.dbase);
+                DT *Db = chunk_labels<DT>(D, c2.dbase);
                 const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
-                const uint32_t cura = Da[ia];
-                const uint32_t curb = two ? Db[ib] : INF;
+                const DT cura = Da[ia];
+                const DT curb = two ? Db[ib] : INF;
                 int ua[DMAX], wa[DMAX], ub[DMAX], wb[DMAX];
-                uint32_t da[DMAX], db[DMAX];
+                DT da[DMAX], db[DMAX];
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
                     ua[q] = __builtin_amdgcn_readlane(e[q].v, t1);
@@ -244,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     da[q] = ua[q] >= 0 ? Da[(int64_t)ua[q] * kWave + lane] : INF;
                     db[q] = (two && ub[q] >= 0) ? Db[(int64_t)ub[q] * kWave + lane] : INF;
                 }
-                uint32_t besta = cura, bestb = curb;
+                DT besta = cura, bestb = curb;
                 int na = -1, nb = -1;  // lane q < DMAX holds neighbour q (for the flags)
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
@@ -274,10 +289,10 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const int g = __builtin_amdgcn_readlane(mg, t);
                 const int v = __builtin_amdgcn_readlane(mv, t);
                 const GChunk c = ch[g];
-                uint32_t *Dc = D + c.dbase;
+                DT *Dc = chunk_labels<DT>(D, c.dbase);
                 const int64_t idx = (int64_t)v * kWave + lane;
-                const uint32_t cur = Dc[idx];
-                uint32_t best = cur;
+                const DT cur = Dc[idx];
+                DT best = cur;
                 const int e0 = __builtin_amdgcn_readlane(me0, t);
                 const int e1 = __builtin_amdgcn_readlane(me1, t);
                 for (int eb = e0; eb < e1; eb += kWave) {
@@ -314,8 +329,9 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
 // Singleton chunks write f[1<<t][v] and the record (kk[t], {t})
 // (reader-el.c:2015-2027), and fold the per-row maximum into
 // rowmax[inst * 64 + t].
+template <typename DT>
 __global__ __launch_bounds__(kBlock) void k_gunpack(
-    const uint32_t *__restrict__ D, const uint64_t *__restrict__ CODE,
+    const void *__restrict__ D, const uint64_t *__restrict__ CODE,
     const GChunk *__restrict__ ch, const int64_t *__restrict__ blk_pre, int nch,
     const uint32_t *__restrict__ xs, const int32_t *__restrict__ kk_all,
     const int32_t *__restrict__ kk_off, const int32_t *__restrict__ rp,
@@ -331,19 +347,19 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
-    const uint32_t *Dc = D + c.dbase;
+    const DT *Dc = chunk_labels<DT>(const_cast<void *>(D), c.dbase);
     const bool real = lane < c.nx;
 #pragma unroll 1
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int vv = grp + r * (kBlock / kWave);
         const int64_t v = v0 + vv;
         const int64_t idx = v * kWave + lane;
-        const uint32_t L = Dc[idx];
+        const DT L = Dc[idx];
         int32_t code = 0;
         if (c.kind == 0 && v < c.n)
-            code = cell_record<uint32_t, uint64_t>(Dc, L, CODE[c.dbase + idx], c.k, real,
-                                                   c.voff + (int)v, rp, adj);
-        sd[vv][lane] = L;
+            code = cell_record<DT, uint64_t>(Dc, L, CODE[c.dbase + idx], c.k, real,
+                                             c.voff + (int)v, rp, adj);
+        sd[vv][lane] = (L == (DT)~(DT)0) ? ~0u : (uint32_t)L;
         sc[vv][lane] = code;
     }
     __syncthreads();

@@ -121,6 +121,7 @@ struct Inst {
     int state = ST_QUEUED;
     int stage = 1;  // next stage: 1 = singletons, m >= 2 = layer m
     int pbits = 32;
+    int dbits = 32;  // relaxation label width of the layer stages (16 when the bound allows)
     int status = EMV_OK;
     int64_t abase = -1, acells = 0;  // arena region: f then b, S * np cells each
     int64_t cost = 0;
@@ -343,11 +344,26 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
     return EMV_OK;
 }
 
-// Host-side plan of one step, packed into one descriptor upload.
+// Host-side plan of one step, packed into one descriptor upload.  The
+// step's chunks form two groups by relaxation label width: group 0 holds the
+// 32-bit chunks (every singleton stage, and the layer stages whose bound
+// needs 32 bits), group 1 the 16-bit ones; each group's kernels get the
+// group's slice of the chunk table and prefix arrays of their own.  Chunk
+// dbase counts cells of 4 bytes in the staging; a 16-bit chunk uses the
+// first half of its region.
+struct StepGroup {
+    int ch0 = 0, nch = 0;  // chunk slice [ch0, ch0 + nch)
+    int mg0 = 0, nmg = 0;  // merge slice (GMerge.chunk0 is relative to ch0)
+    std::vector<int64_t> relax_pre, merge_pre, unpack_pre;
+    std::vector<int32_t> relax_blk;  // chunk (relative) of relaxation item 64 * b
+    size_t off_rp = 0, off_mp = 0, off_up = 0, off_rb = 0;
+};
+
 struct StepPlan {
     std::vector<GChunk> ch;
     std::vector<GMerge> mg;
-    std::vector<int64_t> relax_pre, merge_pre, unpack_pre, seed_pre;
+    StepGroup grp[2];
+    std::vector<int64_t> seed_pre;
     std::vector<uint32_t> xs;
     std::vector<int> seed_chunks;  // indices into ch
     std::vector<int64_t> insts;    // tickets stepped
@@ -355,73 +371,73 @@ struct StepPlan {
     int pbits = 32;
     int64_t cells = 0, flagbytes = 0;
     // device layout (byte offsets into the descriptor buffer)
-    size_t off_ch = 0, off_mg = 0, off_rp = 0, off_mp = 0, off_up = 0, off_sp = 0, off_xs = 0,
-           off_seedch = 0, bytes = 0;
+    size_t off_ch = 0, off_mg = 0, off_sp = 0, off_xs = 0, off_seedch = 0, bytes = 0;
     std::vector<char> blob;
     std::vector<GChunk> seed_ch;
 };
+
+int inst_group(const Inst &I) { return (I.stage >= 2 && I.dbits == 16) ? 1 : 0; }
 
 void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P) {
     P = StepPlan();
     int64_t dcur = 0, fcur = 0;
     int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
-    for (int64_t t : tickets) {
-        Inst &I = e->inst[t];
-        const GraphRec &G = e->graphs[I.graph];
-        const int64_t S2 = 1ll << I.k;
-        P.insts.push_back(t);
-        if (I.stage == 1) {
+    for (int gsel = 0; gsel < 2; gsel++) {
+        StepGroup &SG = P.grp[gsel];
+        SG.ch0 = (int)P.ch.size();
+        SG.mg0 = (int)P.mg.size();
+        for (int64_t t : tickets) {
+            Inst &I = e->inst[t];
+            if (inst_group(I) != gsel) continue;
+            const GraphRec &G = e->graphs[I.graph];
+            P.insts.push_back(t);
             GChunk c;
-            c.dbase = dcur;
-            c.fbase = fcur;
             c.tbase = I.abase;
             c.n = (int32_t)G.n;
             c.np = (int32_t)G.np;
             c.voff = (int32_t)G.voff;
             c.k = I.k;
-            c.x0 = 0;
-            c.nx = I.k;
-            c.kind = 1;
             c.inst = I.slot;
-            P.seed_chunks.push_back((int)P.ch.size());
-            P.ch.push_back(c);
-            dcur += G.np * kWave;
-            fcur += G.np;
-            continue;
+            if (I.stage == 1) {
+                c.dbase = dcur;
+                c.fbase = fcur;
+                c.x0 = 0;
+                c.nx = I.k;
+                c.kind = 1;
+                P.seed_chunks.push_back((int)P.ch.size());
+                P.ch.push_back(c);
+                dcur += G.np * kWave;
+                fcur += G.np;
+                continue;
+            }
+            rows.clear();
+            subsets_of_size(I.k, I.stage, rows);
+            const int nrows = (int)rows.size();
+            maxrows = std::max(maxrows, nrows);
+            maxstage = std::max(maxstage, I.stage);
+            if (I.pbits == 64) P.pbits = 64;
+            GMerge M;
+            M.chunk0 = (int)P.ch.size() - SG.ch0;
+            M.nchunks = (nrows + kWave - 1) / kWave;
+            M.ntiles = (int32_t)(G.np / kWave);
+            M.nsub = 0;  // set once the step's rows per wave is known
+            const int x0 = (int)P.xs.size();
+            P.xs.insert(P.xs.end(), rows.begin(), rows.end());
+            for (int q = 0; q < M.nchunks; q++) {
+                c.dbase = dcur;
+                c.fbase = fcur;
+                c.x0 = x0 + q * kWave;
+                c.nx = std::min(kWave, nrows - q * kWave);
+                c.kind = 0;
+                P.ch.push_back(c);
+                dcur += G.np * kWave;
+                fcur += G.np;
+            }
+            P.mg.push_back(M);
         }
-        rows.clear();
-        subsets_of_size(I.k, I.stage, rows);
-        const int nrows = (int)rows.size();
-        maxrows = std::max(maxrows, nrows);
-        maxstage = std::max(maxstage, I.stage);
-        if (I.pbits == 64) P.pbits = 64;
-        GMerge M;
-        M.chunk0 = (int)P.ch.size();
-        M.nchunks = (nrows + kWave - 1) / kWave;
-        M.ntiles = (int32_t)(G.np / kWave);
-        M.nsub = 0;  // set once the step's rows per wave is known
-        const int x0 = (int)P.xs.size();
-        P.xs.insert(P.xs.end(), rows.begin(), rows.end());
-        for (int q = 0; q < M.nchunks; q++) {
-            GChunk c;
-            c.dbase = dcur;
-            c.fbase = fcur;
-            c.tbase = I.abase;
-            c.n = (int32_t)G.n;
-            c.np = (int32_t)G.np;
-            c.voff = (int32_t)G.voff;
-            c.k = I.k;
-            c.x0 = x0 + q * kWave;
-            c.nx = std::min(kWave, nrows - q * kWave);
-            c.kind = 0;
-            c.inst = I.slot;
-            P.ch.push_back(c);
-            dcur += G.np * kWave;
-            fcur += G.np;
-        }
-        P.mg.push_back(M);
-        (void)S2;
+        SG.nch = (int)P.ch.size() - SG.ch0;
+        SG.nmg = (int)P.mg.size() - SG.mg0;
     }
     if (e->prm.merge_rows_per_wave > 0)
         P.xg = e->prm.merge_rows_per_wave;
@@ -430,16 +446,28 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     // rows of >= 128 pairs (layer >= 9): one row per wave, so that a tile's
     // items fill an XCD and its lower rows are read from L2
     if (e->prm.merge_rows_per_wave == 0 && maxstage >= 9) P.xg = 1;
-    P.relax_pre.push_back(0);
-    P.unpack_pre.push_back(0);
-    for (const GChunk &c : P.ch) {
-        P.relax_pre.push_back(P.relax_pre.back() + c.n);
-        P.unpack_pre.push_back(P.unpack_pre.back() + (c.n + kWave - 1) / kWave);
-    }
-    P.merge_pre.push_back(0);
-    for (GMerge &M : P.mg) {
-        M.nsub = M.nchunks * (kWave / P.xg);
-        P.merge_pre.push_back(P.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
+    for (StepGroup &SG : P.grp) {
+        SG.relax_pre.assign(1, 0);
+        SG.unpack_pre.assign(1, 0);
+        for (int i = 0; i < SG.nch; i++) {
+            const GChunk &c = P.ch[SG.ch0 + i];
+            SG.relax_pre.push_back(SG.relax_pre.back() + c.n);
+            SG.unpack_pre.push_back(SG.unpack_pre.back() + (c.n + kWave - 1) / kWave);
+        }
+        SG.merge_pre.assign(1, 0);
+        for (int i = 0; i < SG.nmg; i++) {
+            GMerge &M = P.mg[SG.mg0 + i];
+            M.nsub = M.nchunks * (kWave / P.xg);
+            SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
+        }
+        const int64_t nitems = SG.relax_pre.back();
+        SG.relax_blk.assign((size_t)((nitems + kWave - 1) / kWave), 0);
+        int g = 0;
+        for (size_t b = 0; b < SG.relax_blk.size(); b++) {
+            const int64_t it = (int64_t)b * kWave;
+            while (SG.relax_pre[g + 1] <= it) g++;
+            SG.relax_blk[b] = g;
+        }
     }
     P.seed_pre.push_back(0);
     for (int g : P.seed_chunks) {
@@ -451,29 +479,35 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     // pack
     auto place = [&](size_t bytes) {
         size_t o = (P.bytes + 255) / 256 * 256;
-        P.bytes = o + bytes;
+        P.bytes = o + std::max<size_t>(bytes, 8);
         return o;
     };
     P.off_ch = place(sizeof(GChunk) * P.ch.size());
     P.off_mg = place(sizeof(GMerge) * P.mg.size());
-    P.off_rp = place(sizeof(int64_t) * P.relax_pre.size());
-    P.off_mp = place(sizeof(int64_t) * P.merge_pre.size());
-    P.off_up = place(sizeof(int64_t) * P.unpack_pre.size());
     P.off_sp = place(sizeof(int64_t) * P.seed_pre.size());
-    P.off_xs = place(sizeof(uint32_t) * std::max<size_t>(P.xs.size(), 1));
+    P.off_xs = place(sizeof(uint32_t) * P.xs.size());
     P.off_seedch = place(sizeof(GChunk) * P.seed_ch.size());
+    for (StepGroup &SG : P.grp) {
+        SG.off_rp = place(sizeof(int64_t) * SG.relax_pre.size());
+        SG.off_mp = place(sizeof(int64_t) * SG.merge_pre.size());
+        SG.off_up = place(sizeof(int64_t) * SG.unpack_pre.size());
+        SG.off_rb = place(sizeof(int32_t) * SG.relax_blk.size());
+    }
     P.blob.assign(P.bytes, 0);
     auto put = [&](size_t off, const void *src, size_t n) {
         if (n) memcpy(P.blob.data() + off, src, n);
     };
     put(P.off_ch, P.ch.data(), sizeof(GChunk) * P.ch.size());
     put(P.off_mg, P.mg.data(), sizeof(GMerge) * P.mg.size());
-    put(P.off_rp, P.relax_pre.data(), sizeof(int64_t) * P.relax_pre.size());
-    put(P.off_mp, P.merge_pre.data(), sizeof(int64_t) * P.merge_pre.size());
-    put(P.off_up, P.unpack_pre.data(), sizeof(int64_t) * P.unpack_pre.size());
     put(P.off_sp, P.seed_pre.data(), sizeof(int64_t) * P.seed_pre.size());
     put(P.off_xs, P.xs.data(), sizeof(uint32_t) * P.xs.size());
     put(P.off_seedch, P.seed_ch.data(), sizeof(GChunk) * P.seed_ch.size());
+    for (StepGroup &SG : P.grp) {
+        put(SG.off_rp, SG.relax_pre.data(), sizeof(int64_t) * SG.relax_pre.size());
+        put(SG.off_mp, SG.merge_pre.data(), sizeof(int64_t) * SG.merge_pre.size());
+        put(SG.off_up, SG.unpack_pre.data(), sizeof(int64_t) * SG.unpack_pre.size());
+        put(SG.off_rb, SG.relax_blk.data(), sizeof(int32_t) * SG.relax_blk.size());
+    }
 }
 
 int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
@@ -492,47 +526,69 @@ int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
     return EMV_OK;
 }
 
-template <int DMAX>
-void launch_grelax(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
+template <typename DT, int DMAX>
+void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
+                   int r) {
     const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
-    hipLaunchKernelGGL((k_grelax<DMAX>), dim3(grid), dim3(kBlock), 0, e->stream, e->d_D,
-                       (const GChunk *)(db + P.off_ch),
-                       (const int64_t *)(db + P.off_rp), (int)P.ch.size(), e->d_rp, e->d_adj,
-                       e->d_ell, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
-                       (const uint32_t *)e->d_q[r & 1], e->d_q[(r + 1) & 1], e->qcap, e->d_any,
-                       e->d_cnt, r);
+    // the worklist numbers items within one group: only for single-group steps
+    const uint32_t qcap = (P.grp[0].nch > 0 && P.grp[1].nch > 0) ? 0u : e->qcap;
+    hipLaunchKernelGGL((k_grelax<DT, DMAX>), dim3(grid), dim3(kBlock), 0, e->stream,
+                       (void *)e->d_D, (const GChunk *)(db + P.off_ch) + SG.ch0,
+                       (const int64_t *)(db + SG.off_rp), (const int32_t *)(db + SG.off_rb),
+                       SG.nch, e->d_rp, e->d_adj, e->d_ell, dense, e->d_flag[r & 1],
+                       e->d_flag[(r + 1) & 1], (const uint32_t *)e->d_q[r & 1],
+                       e->d_q[(r + 1) & 1], qcap, e->d_any, e->d_cnt, r);
 }
 
-void grelax_round(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
+template <typename DT>
+void grelax_group(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
+                  int r) {
     switch (e->ell_deg) {
-    case 4: launch_grelax<4>(e, db, P, dense, r); break;
-    case 8: launch_grelax<8>(e, db, P, dense, r); break;
-    case 16: launch_grelax<16>(e, db, P, dense, r); break;
-    default: launch_grelax<0>(e, db, P, dense, r); break;
+    case 4: launch_grelax<DT, 4>(e, db, P, SG, dense, r); break;
+    case 8: launch_grelax<DT, 8>(e, db, P, SG, dense, r); break;
+    case 16: launch_grelax<DT, 16>(e, db, P, SG, dense, r); break;
+    default: launch_grelax<DT, 0>(e, db, P, SG, dense, r); break;
     }
 }
 
-template <typename P_, int XG>
-void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P) {
-    const int64_t items = P.merge_pre.back();
+// One round over both groups (two launches sharing the round's flags and
+// "round has work" words).
+void grelax_round(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
+    if (P.grp[0].nch > 0) grelax_group<uint32_t>(e, db, P, P.grp[0], dense, r);
+    if (P.grp[1].nch > 0) grelax_group<uint16_t>(e, db, P, P.grp[1], dense, r);
+}
+
+template <typename P_, int XG, typename DT>
+void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
+    const int64_t items = SG.merge_pre.back();
+    if (items == 0) return;
     const int64_t blocks = (items + (kBlock / kWave) - 1) / (kBlock / kWave);
     const int nxcd = (e->num_cu % 32 == 0 && e->num_cu >= 64) ? e->num_cu / 32 : 1;
     const int64_t per = (blocks + nxcd - 1) / nxcd;
-    hipLaunchKernelGGL((k_gmerge<P_, XG>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
-                       e->stream, e->d_arena, (const GChunk *)(db + P.off_ch),
-                       (const GMerge *)(db + P.off_mg), (const int64_t *)(db + P.off_mp),
-                       (int)P.mg.size(), (const uint32_t *)(db + P.off_xs), e->d_D, e->d_code,
-                       nxcd, per);
+    hipLaunchKernelGGL((k_gmerge<P_, XG, DT>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
+                       e->stream, e->d_arena, (const GChunk *)(db + P.off_ch) + SG.ch0,
+                       (const GMerge *)(db + P.off_mg) + SG.mg0, (const int64_t *)(db + SG.off_mp),
+                       SG.nmg, (const uint32_t *)(db + P.off_xs), (void *)e->d_D, e->d_code, nxcd,
+                       per);
 }
 
-template <typename P_>
-void launch_gmerge_p(emv_engine *e, char *db, const StepPlan &P) {
+template <typename P_, typename DT>
+void launch_gmerge_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
     switch (P.xg) {
-    case 8: launch_gmerge_xg<P_, 8>(e, db, P); break;
-    case 4: launch_gmerge_xg<P_, 4>(e, db, P); break;
-    case 2: launch_gmerge_xg<P_, 2>(e, db, P); break;
-    default: launch_gmerge_xg<P_, 1>(e, db, P); break;
+    case 8: launch_gmerge_xg<P_, 8, DT>(e, db, P, SG); break;
+    case 4: launch_gmerge_xg<P_, 4, DT>(e, db, P, SG); break;
+    case 2: launch_gmerge_xg<P_, 2, DT>(e, db, P, SG); break;
+    default: launch_gmerge_xg<P_, 1, DT>(e, db, P, SG); break;
     }
+}
+
+template <typename DT>
+void launch_gmerge_group(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
+    if (SG.nmg == 0) return;
+    if (P.pbits == 64)
+        launch_gmerge_p<uint64_t, DT>(e, db, P, SG);
+    else
+        launch_gmerge_p<uint32_t, DT>(e, db, P, SG);
 }
 
 // Launches of one step up to (not including) the relaxation rounds.
@@ -544,12 +600,8 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
                            (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
     }
-    if (!P.mg.empty() && P.merge_pre.back() > 0) {
-        if (P.pbits == 64)
-            launch_gmerge_p<uint64_t>(e, db, P);
-        else
-            launch_gmerge_p<uint32_t>(e, db, P);
-    }
+    launch_gmerge_group<uint32_t>(e, db, P, P.grp[0]);
+    launch_gmerge_group<uint16_t>(e, db, P, P.grp[1]);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_any, (int64_t)4,
                        0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_cnt, (int64_t)4,
@@ -558,14 +610,21 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
     return EMV_OK;
 }
 
-int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
-    const int64_t blocks = P.unpack_pre.back();
+template <typename DT>
+void launch_gunpack(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
+    const int64_t blocks = SG.unpack_pre.back();
     if (blocks > 0)
-        hipLaunchKernelGGL(k_gunpack, dim3((unsigned)blocks), dim3(kBlock), 0, e->stream, e->d_D,
-                           e->d_code, (const GChunk *)(db + P.off_ch),
-                           (const int64_t *)(db + P.off_up), (int)P.ch.size(),
+        hipLaunchKernelGGL(k_gunpack<DT>, dim3((unsigned)blocks), dim3(kBlock), 0, e->stream,
+                           (const void *)e->d_D, e->d_code,
+                           (const GChunk *)(db + P.off_ch) + SG.ch0,
+                           (const int64_t *)(db + SG.off_up), SG.nch,
                            (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_rp,
                            e->d_adj, e->d_arena, e->d_arena, e->d_rowmax);
+}
+
+int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
+    launch_gunpack<uint32_t>(e, db, P, P.grp[0]);
+    launch_gunpack<uint16_t>(e, db, P, P.grp[1]);
     EHIP(e, hipGetLastError());
     return EMV_OK;
 }
@@ -628,6 +687,11 @@ int advance(emv_engine *e, const StepPlan &P) {
                 continue;
             }
             I.pbits = (bits_for(B) + I.k <= 32) ? 32 : 64;
+            // labels are at most B, candidates at most B + max w
+            I.dbits = (B + (unsigned long long)e->graphs[I.graph].maxw < 0xFFFFull &&
+                       !(I.flags & EMV_WIDE_LABELS))
+                          ? 16
+                          : 32;
             if (B + (unsigned long long)e->graphs[I.graph].maxw >= 0xFFFFFFFFull ||
                 B >= (unsigned long long)INT32_MAX) {
                 I.status = EMV_ERR_RANGE;

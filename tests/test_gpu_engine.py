@@ -138,3 +138,27 @@ def test_server_admission_by_footprint():
             q = emv.distinct(n, 8, seed, 2)
             assert s.poll(t).cost == oracle.emv(n, e, q)[0]
         assert s.poll(t2).queue_ms > 0.0
+
+
+def test_batch_label_widths_agree():
+    """Instances whose bound allows it relax on 16-bit labels, the others on
+    32-bit ones, in the same step (two launches per round).  Forcing 32-bit
+    labels everywhere (WIDE_LABELS) gives the same tables and records."""
+    insts = mixed_instances(6, 3)
+    # one instance with large weights, whose bound needs 32-bit labels
+    n = 700
+    e = emv.regular_graph(n, 4, 20000, 77)
+    insts.append((n, e, emv.terminals(n, 6, 77)))
+    with emv.Batch(insts) as b:
+        c1, _ = b.solve(emv.KEEP_TABLE)
+        t1 = [b.read_rows(i, 0, 1 << len(kk), n_) for i, (n_, _, kk) in enumerate(insts)]
+        c2, _ = b.solve(emv.KEEP_TABLE | emv.WIDE_LABELS)
+        np.testing.assert_array_equal(c1, c2)
+        for i, (n_, e_, kk) in enumerate(insts):
+            f2, code2 = b.read_rows(i, 0, 1 << len(kk), n_)
+            np.testing.assert_array_equal(t1[i][0], f2)
+            np.testing.assert_array_equal(t1[i][1], code2)
+            ocost, of, ocode = oracle.emv(n_, e_, kk)
+            assert c1[i] == ocost
+            np.testing.assert_array_equal(f2, of)
+            np.testing.assert_array_equal(code2, ocode)
