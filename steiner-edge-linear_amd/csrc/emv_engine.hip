@@ -797,15 +797,13 @@ int admit(emv_engine *e) {
         for (int j = 0; j < I.k; j++) k64[j] = I.kk[j];
         EHIP(e, hipMemcpyAsync(e->d_kk + (int64_t)I.slot * kWave, k64.data(),
                                sizeof(int32_t) * kWave, hipMemcpyHostToDevice, e->stream));
-        // table: f = 0 (padding stays 0), row 0 INF, b = -1; rowmax = 0
+        // table: row 0 INF with record -1 (every other real cell is written by
+        // the solve; padding columns only feed dropped lanes); rowmax = 0
         const int64_t S2 = 1ll << I.k;
-        const int blocks = (int)std::min<int64_t>((S2 * G.np + kBlock - 1) / kBlock, 1 << 18);
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
-                           e->d_arena + off, S2 * G.np, 0);
         hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream, e->d_arena + off,
                            G.np, INT32_MAX);
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
-                           e->d_arena + off + S2 * G.np, S2 * G.np, -1);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream,
+                           e->d_arena + off + S2 * G.np, G.np, -1);
         hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, e->stream,
                            e->d_rowmax + (int64_t)I.slot * kWave, (int64_t)kWave, 0ull);
         EHIP(e, hipGetLastError());

@@ -965,20 +965,17 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     HIPCHK(c, hipEventRecord(e_zero0, c->stream));
     CHK(ensure_table(c, (size_t)cells * st.tbytes, (size_t)cells * 4));
     {
-        int blocks = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
-        if (st.tbytes == 4) {
-            hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
-                               (int32_t *)c->d_f, cells, 0);
+        // Every cell of rows 1 .. 2^k-1 at v < n is written by the solve, and
+        // the padding columns v >= n only feed lanes whose results are
+        // dropped, so only row 0 (never written; INF and record -1 as the
+        // reference's initialisation, reader-el.c:2244-2259) is filled.
+        if (st.tbytes == 4)
             hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (int32_t *)c->d_f, np, INT32_MAX);
-        } else {
-            hipLaunchKernelGGL(k_fill<int64_t>, dim3(blocks), dim3(kBlock), 0, c->stream,
-                               (int64_t *)c->d_f, cells, (int64_t)0);
+        else
             hipLaunchKernelGGL(k_fill<int64_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (int64_t *)c->d_f, np, INT64_MAX);
-        }
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(kBlock), 0, c->stream, c->d_b,
-                           cells, -1);
+        hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, c->stream, c->d_b, np, -1);
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(e_zero1, c->stream));

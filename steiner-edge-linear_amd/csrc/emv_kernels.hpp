@@ -1,7 +1,8 @@
 // emv_kernels.hpp -- gfx950 device code of the EMV Steiner DP.
 //
 // Table in HBM (subset-major as reader-el.c:690-692, rows padded to np, an
-// odd number of 64-vertex tiles; padding cells stay 0):
+// odd number of 64-vertex tiles; the padding columns v >= n hold whatever
+// the allocation held and only feed lanes whose results are dropped):
 //   f[X * np + v]   cost, int32 or int64
 //   b[X * np + v]   int32 record code: Y for a merge split (Y == X stops the
 //                   traceback), 2^k + u for an edge from predecessor u
