@@ -43,6 +43,11 @@ struct GMerge {
     int32_t chunk0, nchunks;
     int32_t ntiles;  // np / 64
     int32_t nsub;    // nchunks * 64 / XG
+    int32_t nrows;   // rows of the layer
+    int32_t x0;      // first row in the step's row list
+    int32_t k, np;
+    int64_t tbase;   // the instance's table
+    int64_t dbase0;  // staging cell of its first chunk; chunk q at dbase0 + q * np * 64
 };
 
 template <typename DT>
@@ -83,7 +88,6 @@ __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict
 // instance: f rows of the instance's table, shift by the instance's k.
 template <typename P, int XG, typename DT>
 __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f,
-                                                   const GChunk *__restrict__ ch,
                                                    const GMerge *__restrict__ mg,
                                                    const int64_t *__restrict__ item_pre, int nmg,
                                                    const uint32_t *__restrict__ xs,
@@ -98,15 +102,13 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     if (item >= item_pre[nmg]) return;
     const int s = find_seg(item_pre, nmg, item);
     const GMerge M = mg[s];
-    const GChunk c0 = ch[M.chunk0];
     const int64_t local = item - item_pre[s];
     const int64_t tile = local / M.nsub;
     const int sub = (int)(local - tile * M.nsub);
     const int64_t v = tile * kWave + lane;
-    const int32_t *fv = f + c0.tbase + v;
-    const int k = c0.k;
-    int nrows = 0;
-    for (int q = 0; q < M.nchunks; q++) nrows += ch[M.chunk0 + q].nx;
+    const int32_t *fv = f + M.tbase + v;
+    const int k = M.k;
+    const int nrows = M.nrows;
     DT dv[XG];
     uint64_t pv[XG];
 #pragma unroll
@@ -115,16 +117,16 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
         dv[jx] = (DT)~(DT)0;
         pv[jx] = ~0ull;
         if (xi < nrows) {
-            const P acc = merge_one<int32_t, P>(fv, (int64_t)c0.np, xs[c0.x0 + xi], k);
+            const P acc = merge_one<int32_t, P>(fv, (int64_t)M.np, xs[M.x0 + xi], k);
             dv[jx] = (DT)(acc >> k);
             pv[jx] = (uint64_t)acc;
         }
     }
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
-    const GChunk cc = ch[M.chunk0 + chunk];
-    DT *dd = chunk_labels<DT>(D, cc.dbase) + v * kWave + loff;
-    uint64_t *cd = CODE + cc.dbase + v * kWave + loff;
+    const int64_t dbase = M.dbase0 + (int64_t)chunk * M.np * kWave;
+    DT *dd = chunk_labels<DT>(D, dbase) + v * kWave + loff;
+    uint64_t *cd = CODE + dbase + v * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];

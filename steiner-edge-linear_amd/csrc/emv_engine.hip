@@ -316,6 +316,15 @@ int ensure_slots(emv_engine *e, int slots) {
     return EMV_OK;
 }
 
+// Clears the item flags.
+void clear_flags(emv_engine *e) {
+    const int64_t c = e->flag_bytes;
+    const int blocks = (int)std::min<int64_t>((c + kBlock - 1) / kBlock, 65536);
+    for (int i = 0; i < 2; i++)
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
+                           e->d_flag[i], c, 0u);
+}
+
 int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
     if (cells > e->stage_cells) {
         dfree(e->d_D);
@@ -333,13 +342,9 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
         const int64_t c = flags + flags / 4;
         EHIP(e, hipMalloc((void **)&e->d_flag[0], sizeof(uint32_t) * c));
         EHIP(e, hipMalloc((void **)&e->d_flag[1], sizeof(uint32_t) * c));
-        const int blocks = (int)std::min<int64_t>((c + kBlock - 1) / kBlock, 65536);
-        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
-                           e->d_flag[0], c, 0u);
-        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
-                           e->d_flag[1], c, 0u);
-        EHIP(e, hipGetLastError());
         e->flag_bytes = c;
+        clear_flags(e);
+        EHIP(e, hipGetLastError());
     }
     return EMV_OK;
 }
@@ -422,6 +427,12 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.nchunks = (nrows + kWave - 1) / kWave;
             M.ntiles = (int32_t)(G.np / kWave);
             M.nsub = 0;  // set once the step's rows per wave is known
+            M.nrows = nrows;
+            M.x0 = (int32_t)P.xs.size();
+            M.k = I.k;
+            M.np = (int32_t)G.np;
+            M.tbase = I.abase;
+            M.dbase0 = dcur;
             const int x0 = (int)P.xs.size();
             P.xs.insert(P.xs.end(), rows.begin(), rows.end());
             for (int q = 0; q < M.nchunks; q++) {
@@ -439,13 +450,18 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         SG.nch = (int)P.ch.size() - SG.ch0;
         SG.nmg = (int)P.mg.size() - SG.mg0;
     }
+    // rows per merge wave: many rows while rows have few pairs (a wave's
+    // fixed costs are spread over more work), one row per wave on the layers
+    // whose rows have >= 128 pairs, so that a tile's items fill an XCD and its
+    // lower rows are read from L2
     if (e->prm.merge_rows_per_wave > 0)
         P.xg = e->prm.merge_rows_per_wave;
+    else if (maxstage >= 9)
+        P.xg = 1;
+    else if (maxstage <= 5)
+        P.xg = 8;
     else
         P.xg = maxrows >= 1024 ? 8 : maxrows >= 256 ? 4 : maxrows >= 64 ? 2 : 1;
-    // rows of >= 128 pairs (layer >= 9): one row per wave, so that a tile's
-    // items fill an XCD and its lower rows are read from L2
-    if (e->prm.merge_rows_per_wave == 0 && maxstage >= 9) P.xg = 1;
     for (StepGroup &SG : P.grp) {
         SG.relax_pre.assign(1, 0);
         SG.unpack_pre.assign(1, 0);
@@ -566,8 +582,8 @@ void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P, const StepGrou
     const int nxcd = (e->num_cu % 32 == 0 && e->num_cu >= 64) ? e->num_cu / 32 : 1;
     const int64_t per = (blocks + nxcd - 1) / nxcd;
     hipLaunchKernelGGL((k_gmerge<P_, XG, DT>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
-                       e->stream, e->d_arena, (const GChunk *)(db + P.off_ch) + SG.ch0,
-                       (const GMerge *)(db + P.off_mg) + SG.mg0, (const int64_t *)(db + SG.off_mp),
+                       e->stream, e->d_arena, (const GMerge *)(db + P.off_mg) + SG.mg0,
+                       (const int64_t *)(db + SG.off_mp),
                        SG.nmg, (const uint32_t *)(db + P.off_xs), (void *)e->d_D, e->d_code, nxcd,
                        per);
 }
@@ -1148,12 +1164,7 @@ extern "C" int emv_batch_solve(emv_engine *e, uint32_t flags, int64_t *costs,
     }
     if (!done) {
         // flags may be dirty after a replay that ran out of rounds
-        if (graph_fallback && e->flag_bytes > 0) {
-            const int blocks = (int)std::min<int64_t>((e->flag_bytes + kBlock - 1) / kBlock, 65536);
-            for (int i = 0; i < 2; i++)
-                hipLaunchKernelGGL(k_fill<uint32_t>, dim3(blocks), dim3(kBlock), 0, e->stream,
-                                   e->d_flag[i], e->flag_bytes, 0u);
-        }
+        if (graph_fallback && e->flag_bytes > 0) clear_flags(e);
         ECHK(batch_solve_eager(e, flags, st));
     }
     int rc = EMV_OK;
