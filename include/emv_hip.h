@@ -75,6 +75,8 @@ typedef struct {
     int64_t relax_blocks;    /* vertex blocks of the block-local relaxation; 0 = vertex-wise rounds */
     double unpack_ms;        /* sum over layers of the record decisions + table writes */
     int64_t singleton_rounds;/* relaxation rounds of the singleton shortest paths */
+    int64_t block_dist_bits; /* 16 / 32: block visits after the first apply in-block distances
+                                from the lowered border slots; 0: they sweep */
 } emv_result;
 
 /* Context on HIP device `device`. */

@@ -35,6 +35,8 @@ struct GChunk {
     int32_t nx;     // rows in the chunk (lane j < nx is row x0 + j)
     int32_t kind;   // 0: layer rows, 1: singleton rows (row j = terminal j)
     int32_t inst;   // engine slot of the instance
+    int32_t b0;     // block relaxation: first block of the graph in the shared block arrays
+    int32_t nblk;   //   and its blocks (0: vertex-wise relaxation)
 };
 
 // One instance at a layer: its chunks are consecutive, its merge items are
@@ -320,6 +322,68 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 }
             }
         }
+    }
+    if (raised && lane == 0) any[(r + 1) & 3] = 1;
+}
+
+// The block-local round of k_relax_blk over the blocks of many chunks: item i
+// of chunk g (item_pre[g] <= i < item_pre[g+1]) is block i - item_pre[g] of
+// that chunk's graph, whose slots start at boff[b0 + block].  The block
+// arrays of all graphs share one slot and adjacency numbering (bvert,
+// brp/bext, badj); vblk and the outside entries of badj hold vertex ids local
+// to the graph.  The flags of item i are Fcur / Fnext[F0 + i], chunk g's
+// from F0 + item_pre[g] = its fbase.  Dm / bdoff: the in-block distance
+// matrices of all block graphs (Dm null: sweeps only).
+template <typename DT, typename DW>
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<DT>()))) void k_gblk(
+    void *__restrict__ D, const GChunk *__restrict__ ch, const int64_t *__restrict__ item_pre,
+    int nch, const int32_t *__restrict__ boff, const int32_t *__restrict__ bvert,
+    const int32_t *__restrict__ brp, const int32_t *__restrict__ bext,
+    const Edge *__restrict__ badj, const int32_t *__restrict__ vblk, const DW *__restrict__ Dm,
+    const int64_t *__restrict__ bdoff, int64_t F0, int dense, uint32_t *__restrict__ Fcur,
+    uint32_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    DT *L = reinterpret_cast<DT *>(lds_raw);  // L[slot * 64 + row]
+    uint32_t *s_misc = reinterpret_cast<uint32_t *>(lds_raw + sizeof(DT) * kBV * kWave);
+    uint32_t *s_flag = s_misc;     // 3 rotating "sweep lowered something" words
+    uint32_t *s_cnt = s_misc + 4;  // flagged items of the current piece
+    uint8_t *s_list = reinterpret_cast<uint8_t *>(s_misc + 8);        // 256 entries
+    uint8_t *s_dirty = reinterpret_cast<uint8_t *>(s_misc + 8 + 64);  // 2 x kBV marks
+    uint64_t *s_chg = reinterpret_cast<uint64_t *>(s_misc + kBMiscChg);  // kBW masks
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    if (blockIdx.x == 0 && tid == 0) any[(r + 2) & 3] = 0;
+    if (!dense && any[r & 3] == 0) return;
+    const int64_t nitems = item_pre[nch];
+    const int64_t G = gridDim.x;
+    const int64_t mine = (nitems > (int64_t)blockIdx.x) ? (nitems - 1 - blockIdx.x) / G + 1 : 0;
+    for (int i = tid; i < 2 * kBV; i += kBT) s_dirty[i] = 0;
+    bool raised = false;
+    for (int64_t p0 = 0; p0 < mine; p0 += kBPiece) {
+        if (tid == 0) *s_cnt = 0;
+        __syncthreads();
+        if (tid < kBPiece && p0 + tid < mine) {
+            const int64_t it = blockIdx.x + G * (p0 + tid);
+            bool f = dense != 0;
+            if (!dense && Fcur[F0 + it] != 0) {
+                f = true;
+                Fcur[F0 + it] = 0;
+            }
+            if (f) s_list[atomicAdd(s_cnt, 1u)] = (uint8_t)tid;
+        }
+        __syncthreads();
+        const int nl = (int)*s_cnt;
+        for (int qi = 0; qi < nl; qi++) {
+            const int64_t item = blockIdx.x + G * (p0 + s_list[qi]);
+            const int g = find_seg(item_pre, nch, item);
+            const GChunk c = ch[g];
+            const int b = c.b0 + (int)(item - item_pre[g]);
+            blk_visit<DT, DW>(chunk_labels<DT>(D, c.dbase), boff[b], boff[b + 1] - boff[b],
+                              bvert, brp, bext, badj, vblk + c.voff, Dm ? Dm + bdoff[b] : nullptr,
+                              Fnext + c.fbase, dense, L, s_flag, s_dirty, s_chg, nullptr, raised);
+            __syncthreads();  // LDS is reused by the next item
+        }
+        __syncthreads();  // s_cnt is reset by the next piece
     }
     if (raised && lane == 0) any[(r + 1) & 3] = 1;
 }

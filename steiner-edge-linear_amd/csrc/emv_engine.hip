@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "emv_batch_kernels.hpp"
+#include "emv_blocks.hpp"
 #include "emv_engine.h"
 #include "emv_hip.h"
 
@@ -108,6 +109,7 @@ struct Arena {
 struct GraphRec {
     int64_t voff, n, np, maxw;
     int maxdeg;
+    int32_t b0 = 0, nblk = 0;  // blocks of block-local relaxation (nblk 0: vertex-wise)
 };
 
 enum { ST_QUEUED = 0, ST_RUNNING = 1, ST_DONE = 2 };
@@ -144,6 +146,15 @@ struct emv_engine {
     Edge *d_ell = nullptr;
     int ell_deg = 0;
     int64_t gvert = 0;
+    // blocks of the graphs that take block-local relaxation, in one numbering
+    int32_t *d_boff = nullptr, *d_bvert = nullptr, *d_brp = nullptr, *d_bext = nullptr;
+    int32_t *d_vblk = nullptr;  // gvert entries: block of each vertex within its graph
+    Edge *d_badj = nullptr;
+    // in-block distances from border slots, all block graphs (bdw 16 / 32 bits; 0: sweeps only)
+    void *d_bdist = nullptr;
+    int64_t *d_bdoff = nullptr;
+    int bdw = 0;
+    size_t lds_cu = 65536;      // LDS per CU
     // table arena (int32 cells)
     int32_t *d_arena = nullptr;
     Arena arena;
@@ -213,6 +224,75 @@ void subsets_of_size(int k, int m, std::vector<uint32_t> &out) {
         const uint64_t r = X + lo;
         X = (((r ^ X) >> 2) / lo) | r;
     }
+}
+
+template <typename T>
+int upload_vec(emv_engine *e, T *&d, const std::vector<T> &h) {
+    EHIP(e, hipMalloc((void **)&d, sizeof(T) * std::max<size_t>(1, h.size())));
+    if (!h.empty()) EHIP(e, hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return EMV_OK;
+}
+
+// Cuts the graphs that suit block-local relaxation into blocks (the rule of
+// the single-instance path) and uploads the blocks of all of them in one
+// numbering: graph g's blocks are b0 .. b0+nblk-1, their slots follow on
+// from the previous graph's, brp / bext index the shared badj.
+int upload_blocks(emv_engine *e, const std::vector<int32_t> &rp, const std::vector<Edge> &adj) {
+    const int mode = blocks_mode();
+    if (mode == 0) return EMV_OK;
+    std::vector<int32_t> boff(1, 0), bvert, brp, bext, vblk((size_t)e->gvert, -1);
+    std::vector<Edge> badj;
+    std::vector<int64_t> bdoff(1, 0);
+    std::vector<uint64_t> bdist;
+    uint64_t dmax = 0;
+    const bool closure = closure_mode() != 0;
+    int32_t nb = 0;
+    for (GraphRec &G : e->graphs) {
+        if (G.n < 2 || rp[G.voff + G.n] == rp[G.voff]) continue;
+        BlockPart P;
+        const double inside = build_blocks(G.n, rp.data() + G.voff, adj.data(), P);
+        if (!blocks_wanted(mode, G.n, inside)) continue;
+        const int32_t slot0 = (int32_t)bvert.size(), e0 = (int32_t)badj.size();
+        G.b0 = nb;
+        G.nblk = P.nblk;
+        for (int i = 1; i <= P.nblk; i++) boff.push_back(slot0 + P.boff[i]);
+        bvert.insert(bvert.end(), P.bvert.begin(), P.bvert.end());
+        for (int64_t s = 0; s < G.n; s++) {
+            brp.push_back(e0 + P.brp[s]);
+            bext.push_back(e0 + P.bext[s]);
+        }
+        badj.insert(badj.end(), P.badj.begin(), P.badj.end());
+        for (int64_t u = 0; u < G.n; u++) vblk[(size_t)(G.voff + u)] = P.vblk[u];
+        nb += P.nblk;
+        if (closure) {
+            BlockDist B;
+            block_distances(P, B);
+            const int64_t d0 = (int64_t)bdist.size();
+            for (int i = 1; i <= P.nblk; i++) bdoff.push_back(d0 + B.bdoff[i]);
+            bdist.insert(bdist.end(), B.d.begin(), B.d.end());
+            dmax = std::max(dmax, B.dmax);
+        }
+    }
+    if (nb == 0) return EMV_OK;
+    brp.push_back((int32_t)badj.size());
+    e->bdw = closure ? dist_width(dmax) : 0;
+    if (e->bdw == 16) {
+        uint16_t *p = nullptr;
+        ECHK(upload_vec(e, p, narrow<uint16_t>(bdist)));
+        e->d_bdist = p;
+    } else if (e->bdw == 32) {
+        uint32_t *p = nullptr;
+        ECHK(upload_vec(e, p, narrow<uint32_t>(bdist)));
+        e->d_bdist = p;
+    }
+    if (e->bdw) ECHK(upload_vec(e, e->d_bdoff, bdoff));
+    ECHK(upload_vec(e, e->d_boff, boff));
+    ECHK(upload_vec(e, e->d_bvert, bvert));
+    ECHK(upload_vec(e, e->d_brp, brp));
+    ECHK(upload_vec(e, e->d_bext, bext));
+    ECHK(upload_vec(e, e->d_vblk, vblk));
+    ECHK(upload_vec(e, e->d_badj, badj));
+    return EMV_OK;
 }
 
 int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
@@ -296,7 +376,7 @@ int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
         EHIP(e, hipMemcpy(e->d_ell, el.data(), sizeof(Edge) * el.size(), hipMemcpyHostToDevice));
     }
     e->ell_deg = ell;
-    return EMV_OK;
+    return upload_blocks(e, rp, adj);
 }
 
 int ensure_slots(emv_engine *e, int slots) {
@@ -367,7 +447,7 @@ struct StepGroup {
 struct StepPlan {
     std::vector<GChunk> ch;
     std::vector<GMerge> mg;
-    StepGroup grp[2];
+    StepGroup grp[4];  // label width (u32, u16) x relaxation (vertex-wise, block-local)
     std::vector<int64_t> seed_pre;
     std::vector<uint32_t> xs;
     std::vector<int> seed_chunks;  // indices into ch
@@ -381,20 +461,26 @@ struct StepPlan {
     std::vector<GChunk> seed_ch;
 };
 
-int inst_group(const Inst &I) { return (I.stage >= 2 && I.dbits == 16) ? 1 : 0; }
+// Step group of an instance: bit 0 its label width (set: 16-bit layer labels),
+// bit 1 block-local relaxation on its graph.
+int inst_group(const emv_engine *e, const Inst &I) {
+    return ((I.stage >= 2 && I.dbits == 16) ? 1 : 0) + (e->graphs[I.graph].nblk > 0 ? 2 : 0);
+}
+
+bool group_blocks(int gsel) { return (gsel & 2) != 0; }
 
 void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P) {
     P = StepPlan();
     int64_t dcur = 0, fcur = 0;
     int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
-    for (int gsel = 0; gsel < 2; gsel++) {
+    for (int gsel = 0; gsel < 4; gsel++) {
         StepGroup &SG = P.grp[gsel];
         SG.ch0 = (int)P.ch.size();
         SG.mg0 = (int)P.mg.size();
         for (int64_t t : tickets) {
             Inst &I = e->inst[t];
-            if (inst_group(I) != gsel) continue;
+            if (inst_group(e, I) != gsel) continue;
             const GraphRec &G = e->graphs[I.graph];
             P.insts.push_back(t);
             GChunk c;
@@ -404,6 +490,10 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             c.voff = (int32_t)G.voff;
             c.k = I.k;
             c.inst = I.slot;
+            c.b0 = G.b0;
+            c.nblk = G.nblk;
+            // activity flags: one per vertex, or one per block
+            const int64_t nflags = G.nblk > 0 ? G.nblk : G.np;
             if (I.stage == 1) {
                 c.dbase = dcur;
                 c.fbase = fcur;
@@ -413,7 +503,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 P.seed_chunks.push_back((int)P.ch.size());
                 P.ch.push_back(c);
                 dcur += G.np * kWave;
-                fcur += G.np;
+                fcur += nflags;
                 continue;
             }
             rows.clear();
@@ -443,7 +533,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 c.kind = 0;
                 P.ch.push_back(c);
                 dcur += G.np * kWave;
-                fcur += G.np;
+                fcur += nflags;
             }
             P.mg.push_back(M);
         }
@@ -462,12 +552,13 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         P.xg = 8;
     else
         P.xg = maxrows >= 1024 ? 8 : maxrows >= 256 ? 4 : maxrows >= 64 ? 2 : 1;
-    for (StepGroup &SG : P.grp) {
+    for (int gsel = 0; gsel < 4; gsel++) {
+        StepGroup &SG = P.grp[gsel];
         SG.relax_pre.assign(1, 0);
         SG.unpack_pre.assign(1, 0);
         for (int i = 0; i < SG.nch; i++) {
             const GChunk &c = P.ch[SG.ch0 + i];
-            SG.relax_pre.push_back(SG.relax_pre.back() + c.n);
+            SG.relax_pre.push_back(SG.relax_pre.back() + (group_blocks(gsel) ? c.nblk : c.n));
             SG.unpack_pre.push_back(SG.unpack_pre.back() + (c.n + kWave - 1) / kWave);
         }
         SG.merge_pre.assign(1, 0);
@@ -476,7 +567,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.nsub = M.nchunks * (kWave / P.xg);
             SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
         }
-        const int64_t nitems = SG.relax_pre.back();
+        const int64_t nitems = group_blocks(gsel) ? 0 : SG.relax_pre.back();
         SG.relax_blk.assign((size_t)((nitems + kWave - 1) / kWave), 0);
         int g = 0;
         for (size_t b = 0; b < SG.relax_blk.size(); b++) {
@@ -547,7 +638,9 @@ void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
                    int r) {
     const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
     // the worklist numbers items within one group: only for single-group steps
-    const uint32_t qcap = (P.grp[0].nch > 0 && P.grp[1].nch > 0) ? 0u : e->qcap;
+    int groups = 0;
+    for (const StepGroup &G : P.grp) groups += G.nch > 0 ? 1 : 0;
+    const uint32_t qcap = groups > 1 ? 0u : e->qcap;
     hipLaunchKernelGGL((k_grelax<DT, DMAX>), dim3(grid), dim3(kBlock), 0, e->stream,
                        (void *)e->d_D, (const GChunk *)(db + P.off_ch) + SG.ch0,
                        (const int64_t *)(db + SG.off_rp), (const int32_t *)(db + SG.off_rb),
@@ -567,11 +660,38 @@ void grelax_group(emv_engine *e, char *db, const StepPlan &P, const StepGroup &S
     }
 }
 
-// One round over both groups (two launches sharing the round's flags and
-// "round has work" words).
+// Block-local round over a block group: one workgroup per (chunk, block)
+// visit, the block's labels in LDS; as many workgroups per CU as its LDS
+// holds (at most two of 1024 threads).
+template <typename DT, typename DW>
+void launch_gblk_w(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
+                   int r) {
+    const size_t bytes = blk_lds_bytes<DT>();
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, e->lds_cu / bytes));
+    hipLaunchKernelGGL((k_gblk<DT, DW>), dim3(e->num_cu * per_cu), dim3(kBT), bytes, e->stream,
+                       (void *)e->d_D, (const GChunk *)(db + P.off_ch) + SG.ch0,
+                       (const int64_t *)(db + SG.off_rp), SG.nch, e->d_boff, e->d_bvert, e->d_brp,
+                       e->d_bext, e->d_badj, e->d_vblk, (const DW *)e->d_bdist, e->d_bdoff,
+                       P.ch[SG.ch0].fbase, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
+                       e->d_any, r);
+}
+
+template <typename DT>
+void launch_gblk(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
+                 int r) {
+    if (e->bdw == 32)
+        launch_gblk_w<DT, uint32_t>(e, db, P, SG, dense, r);
+    else
+        launch_gblk_w<DT, uint16_t>(e, db, P, SG, dense, r);
+}
+
+// One round over the step's groups (one launch each, sharing the round's
+// flags and "round has work" words).
 void grelax_round(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
     if (P.grp[0].nch > 0) grelax_group<uint32_t>(e, db, P, P.grp[0], dense, r);
     if (P.grp[1].nch > 0) grelax_group<uint16_t>(e, db, P, P.grp[1], dense, r);
+    if (P.grp[2].nch > 0) launch_gblk<uint32_t>(e, db, P, P.grp[2], dense, r);
+    if (P.grp[3].nch > 0) launch_gblk<uint16_t>(e, db, P, P.grp[3], dense, r);
 }
 
 template <typename P_, int XG, typename DT>
@@ -616,8 +736,12 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
                            (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
     }
-    launch_gmerge_group<uint32_t>(e, db, P, P.grp[0]);
-    launch_gmerge_group<uint16_t>(e, db, P, P.grp[1]);
+    for (int gsel = 0; gsel < 4; gsel++) {
+        if (gsel & 1)
+            launch_gmerge_group<uint16_t>(e, db, P, P.grp[gsel]);
+        else
+            launch_gmerge_group<uint32_t>(e, db, P, P.grp[gsel]);
+    }
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_any, (int64_t)4,
                        0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_cnt, (int64_t)4,
@@ -639,8 +763,12 @@ void launch_gunpack(emv_engine *e, char *db, const StepPlan &P, const StepGroup 
 }
 
 int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
-    launch_gunpack<uint32_t>(e, db, P, P.grp[0]);
-    launch_gunpack<uint16_t>(e, db, P, P.grp[1]);
+    for (int gsel = 0; gsel < 4; gsel++) {
+        if (gsel & 1)
+            launch_gunpack<uint16_t>(e, db, P, P.grp[gsel]);
+        else
+            launch_gunpack<uint32_t>(e, db, P, P.grp[gsel]);
+    }
     EHIP(e, hipGetLastError());
     return EMV_OK;
 }
@@ -861,6 +989,7 @@ int engine_open(int device, emv_engine **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EMV_ERR_HIP;
     e->num_cu = prop.multiProcessorCount;
+    if (prop.maxSharedMemoryPerMultiProcessor > 0) e->lds_cu = prop.maxSharedMemoryPerMultiProcessor;
     EHIP(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     EHIP(e, hipMalloc((void **)&e->d_any, sizeof(uint32_t) * 4));
     EHIP(e, hipMalloc((void **)&e->d_cnt, sizeof(uint32_t) * 4));
@@ -909,6 +1038,14 @@ extern "C" void emv_engine_destroy(emv_engine *e) {
     dfree(e->d_rp);
     dfree(e->d_adj);
     dfree(e->d_ell);
+    dfree(e->d_boff);
+    dfree(e->d_bvert);
+    dfree(e->d_brp);
+    dfree(e->d_bext);
+    dfree(e->d_vblk);
+    dfree(e->d_badj);
+    dfree(e->d_bdist);
+    dfree(e->d_bdoff);
     dfree(e->d_arena);
     dfree(e->d_kk);
     dfree(e->d_kkoff);

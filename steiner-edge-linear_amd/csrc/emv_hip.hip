@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "emv_hip.h"
+#include "emv_blocks.hpp"
 #include "emv_kernels.hpp"
 
 using namespace emv;
@@ -48,6 +49,10 @@ struct emv_ctx {
     int32_t *d_boff = nullptr, *d_bvert = nullptr, *d_brp = nullptr, *d_bext = nullptr,
             *d_vblk = nullptr;
     Edge *d_badj = nullptr;
+    // in-block distances from border slots (bdw 16 / 32 bits; 0: none, sweeps only)
+    void *d_bdist = nullptr;
+    int64_t *d_bdoff = nullptr;
+    int bdw = 0;
     size_t lds_block = 65536, lds_cu = 65536;  // LDS limits of the device
     bool blk_attr[3] = {false, false, false};  // dynamic-LDS attribute set, per label width
     int64_t maxw = 0;
@@ -269,7 +274,10 @@ bool use_blocks(emv_ctx *c) {
     const size_t bytes = blk_lds_bytes<DT>();
     if (c->nblk == 0 || bytes > c->lds_block) return false;
     if (bytes > 65536 && !c->blk_attr[blk_slot<DT>()]) {
-        if (hipFuncSetAttribute((const void *)k_relax_blk<DT>,
+        if (hipFuncSetAttribute((const void *)k_relax_blk<DT, uint16_t>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes) != hipSuccess ||
+            hipFuncSetAttribute((const void *)k_relax_blk<DT, uint32_t>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)bytes) != hipSuccess) {
             (void)hipGetLastError();
@@ -286,14 +294,23 @@ bool use_blocks_any(emv_ctx *c, int dbits) {
     return use_blocks<uint64_t>(c);
 }
 
-template <typename DT>
-void blk_round(emv_ctx *c, int64_t nitems, int dense, int r) {
+template <typename DT, typename DW>
+void blk_round_w(emv_ctx *c, int64_t nitems, int dense, int r) {
     const size_t bytes = blk_lds_bytes<DT>();
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_cu / bytes));
-    hipLaunchKernelGGL(k_relax_blk<DT>, dim3(c->num_cu * per_cu), dim3(kBT), bytes,
+    hipLaunchKernelGGL((k_relax_blk<DT, DW>), dim3(c->num_cu * per_cu), dim3(kBT), bytes,
                        c->stream, (DT *)c->d_D, c->np, c->nblk, c->d_boff, c->d_bvert, c->d_brp,
-                       c->d_bext, c->d_badj, c->d_vblk, nitems, dense, c->d_flag[r & 1],
-                       c->d_flag[(r + 1) & 1], c->d_any, r, c->d_stats);
+                       c->d_bext, c->d_badj, c->d_vblk, (const DW *)c->d_bdist, c->d_bdoff,
+                       nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r,
+                       c->d_stats);
+}
+
+template <typename DT>
+void blk_round(emv_ctx *c, int64_t nitems, int dense, int r) {
+    if (c->bdw == 32)
+        blk_round_w<DT, uint32_t>(c, nitems, dense, r);
+    else
+        blk_round_w<DT, uint16_t>(c, nitems, dense, r);
 }
 
 template <typename DT>
@@ -499,67 +516,10 @@ void free_blocks(emv_ctx *c) {
     dfree(c->d_bext);
     dfree(c->d_vblk);
     dfree(c->d_badj);
+    dfree(c->d_bdist);
+    dfree(c->d_bdoff);
+    c->bdw = 0;
     c->nblk = 0;
-}
-
-// Blocks of at most kBV vertices grown breadth-first over unassigned
-// vertices from the lowest unassigned id; slots follow the BFS order.
-// Returns the share of adjacency entries that stay inside their block.
-struct BlockPart {
-    int nblk = 0;
-    std::vector<int32_t> boff, bvert, brp, bext, vblk;
-    std::vector<Edge> badj;
-};
-
-double build_blocks(int64_t n, const std::vector<int32_t> &rp, const std::vector<Edge> &adj,
-                    BlockPart &P) {
-    P.vblk.assign((size_t)n, -1);
-    std::vector<int32_t> slot((size_t)n, -1), queue;
-    queue.reserve(kBV);
-    P.boff.assign(1, 0);
-    P.bvert.clear();
-    P.bvert.reserve((size_t)n);
-    for (int64_t seed = 0; seed < n; seed++) {
-        if (P.vblk[seed] >= 0) continue;
-        const int b = P.nblk++;
-        queue.clear();
-        queue.push_back((int32_t)seed);
-        P.vblk[seed] = b;
-        for (size_t head = 0; head < queue.size() && queue.size() < (size_t)kBV; head++) {
-            const int32_t u = queue[head];
-            for (int32_t e = rp[u]; e < rp[u + 1] && queue.size() < (size_t)kBV; e++) {
-                const int32_t v = adj[e].v;
-                if (P.vblk[v] < 0) {
-                    P.vblk[v] = b;
-                    queue.push_back(v);
-                }
-            }
-        }
-        for (int32_t u : queue) {
-            slot[u] = (int32_t)P.bvert.size();
-            P.bvert.push_back(u);
-        }
-        P.boff.push_back((int32_t)P.bvert.size());
-    }
-    P.brp.assign((size_t)n + 1, 0);
-    P.bext.assign((size_t)n, 0);
-    P.badj.clear();
-    P.badj.reserve(adj.size());
-    int64_t inside = 0;
-    for (int64_t s = 0; s < n; s++) {
-        const int32_t u = P.bvert[s];
-        const int32_t b = P.vblk[u];
-        const int32_t s0 = P.boff[b];
-        P.brp[s] = (int32_t)P.badj.size();
-        for (int32_t e = rp[u]; e < rp[u + 1]; e++)
-            if (P.vblk[adj[e].v] == b) P.badj.push_back(Edge{slot[adj[e].v] - s0, adj[e].w});
-        P.bext[s] = (int32_t)P.badj.size();
-        inside += P.bext[s] - P.brp[s];
-        for (int32_t e = rp[u]; e < rp[u + 1]; e++)
-            if (P.vblk[adj[e].v] != b) P.badj.push_back(adj[e]);
-    }
-    P.brp[n] = (int32_t)P.badj.size();
-    return adj.empty() ? 0.0 : (double)inside / (double)adj.size();
 }
 
 template <typename T>
@@ -601,12 +561,11 @@ int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
     // Block-local relaxation when most edges stay inside their block (grids,
     // meshes).  EMV_BLOCKS=0 turns it off, EMV_BLOCKS=1 forces it.
     {
-        const char *eb = getenv("EMV_BLOCKS");
-        const int mode = eb ? atoi(eb) : -1;
+        const int mode = blocks_mode();
         if (mode != 0 && n >= 2 && !adj.empty()) {
             BlockPart P;
-            const double inside = build_blocks(n, rp, adj, P);
-            if (mode == 1 || (inside >= 0.6 && n >= 4 * kBV)) {
+            const double inside = build_blocks(n, rp.data(), adj.data(), P);
+            if (blocks_wanted(mode, n, inside)) {
                 CHK(upload(c, c->d_boff, P.boff));
                 CHK(upload(c, c->d_bvert, P.bvert));
                 CHK(upload(c, c->d_brp, P.brp));
@@ -614,6 +573,21 @@ int load_csr(emv_ctx *c, int64_t n, int64_t m, const std::vector<int32_t> &rp,
                 CHK(upload(c, c->d_vblk, P.vblk));
                 CHK(upload(c, c->d_badj, P.badj));
                 c->nblk = P.nblk;
+                if (closure_mode() != 0) {
+                    BlockDist B;
+                    block_distances(P, B);
+                    c->bdw = dist_width(B.dmax);
+                    if (c->bdw == 16) {
+                        uint16_t *p = nullptr;
+                        CHK(upload(c, p, narrow<uint16_t>(B.d)));
+                        c->d_bdist = p;
+                    } else if (c->bdw == 32) {
+                        uint32_t *p = nullptr;
+                        CHK(upload(c, p, narrow<uint32_t>(B.d)));
+                        c->d_bdist = p;
+                    }
+                    if (c->bdw) CHK(upload(c, c->d_bdoff, B.bdoff));
+                }
             }
         }
     }
@@ -1063,6 +1037,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->merge_bits = st.pbits;
     res->label_bits = st.dbits;
     res->relax_blocks = use_blocks_any(c, st.dbits) ? c->nblk : 0;
+    res->block_dist_bits = res->relax_blocks ? c->bdw : 0;
     res->dp_cells = st.S2 * n;
     res->zero_ms = ev_ms(e_zero0, e_zero1);
     res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;

@@ -338,10 +338,27 @@ constexpr int kBD = 4;    // adjacency entries per vertex and kind held in regis
 constexpr int kBW = 16;   // waves per block workgroup (one workgroup fills a CU's LDS)
 constexpr int kBT = kBW * kWave;
 constexpr int kBPiece = 256;  // flags read per collection step
+constexpr int kBQ = (kBV + kBW - 1) / kBW;  // slots per wave of a block, at most
+constexpr int kBRow = kBW * kBQ;             // entries per row of an in-block distance matrix
+
+// LDS of a block workgroup: the labels, then 1 KB of words: 3 sweep flags,
+// the piece counter, the piece list (256 B), 2 x kBV sweep marks, and from
+// word kBMiscChg the kBW per-wave masks of lowered border slots.
+constexpr int kBMiscChg = 200;
+static_assert(8 + 64 + (2 * kBV + 3) / 4 <= kBMiscChg && kBMiscChg * 4 + 8 * kBW <= 1024,
+              "block LDS words overlap");
 
 template <typename DT>
 __host__ __device__ constexpr size_t blk_lds_bytes() {
     return sizeof(DT) * kBV * kWave + 1024;
+}
+
+// Waves per SIMD a block workgroup's registers must leave room for: two
+// workgroups of kBW waves per CU (8 per SIMD) while two fit the CU's LDS
+// (labels up to 32 bits), else one.
+template <typename DT>
+constexpr int kBlkWaves() {
+    return sizeof(DT) <= 4 ? 2 * kBW / 4 : kBW / 4;
 }
 
 // A 64-bit value the wave holds in every lane, made explicitly wave-uniform
@@ -399,12 +416,372 @@ __device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD
     }
 }
 
-template <typename DT>
-__global__ __launch_bounds__(kBT) void k_relax_blk(
+// min(x, lb + d) for a finite label lb and an in-block distance d, without
+// wrapping: 16-bit labels add in 32 bits, 32-bit ones saturate (a saturated
+// sum is INF and lowers nothing), 64-bit labels stay far below 2^63.
+template <typename DT, typename DW>
+__device__ __forceinline__ DT closure_min(DT x, DT lb, DW d) {
+    if constexpr (sizeof(DT) == 2) {
+        const uint32_t c = (uint32_t)lb + (uint32_t)d;
+        return c < (uint32_t)x ? (DT)c : x;
+    } else if constexpr (sizeof(DT) == 4) {
+        const uint32_t c = __builtin_elementwise_add_sat((uint32_t)lb, (uint32_t)d);
+        return c < x ? c : x;
+    } else {
+        const uint64_t c = lb + (uint64_t)d;
+        return c < x ? c : x;
+    }
+}
+
+// A wave's kBQ entries of a distance row, read as whole dwords (one scalar
+// load of 32 or 64 bytes: the address is wave-uniform and the row padded).
+template <typename DW>
+__device__ __forceinline__ void band_row(const DW *__restrict__ p, DW (&dv)[kBQ]) {
+    // the matrices are read-only while any kernel runs: constant address space
+    typedef __attribute__((address_space(4))) const uint32_t *const_u32p;
+    const const_u32p q = (const_u32p)(p);
+    if constexpr (sizeof(DW) == 2) {
+#pragma unroll
+        for (int t = 0; t < kBQ / 2; t++) {
+            const uint32_t w = q[t];
+            dv[2 * t] = (DW)(w & 0xFFFFu);
+            dv[2 * t + 1] = (DW)(w >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < kBQ; t++) dv[t] = q[t];
+    }
+}
+
+// One visit of a (chunk, block) item: Dc the chunk's labels, s0 .. s0+cnt-1
+// the block's slots, vblk the block of each vertex of the graph and Fc the
+// chunk's block flags of the next round.  All threads of the workgroup take
+// part; raised is set when a block beyond this one was flagged.
+//
+// Db (optional) is the block's matrix of in-block distances from each border
+// slot (the first slots of a block) to every slot: row b holds, for wave w's
+// band of slots, its qw entries from kBQ * w on (kBRow entries a row).  With it, a visit of a
+// later round -- the block sits at its in-block fixed point since its last
+// visit, only border labels can drop when the outside is folded in -- takes
+// every slot to min over the lowered border slots b of (label of b + Db[b][
+// slot]) in one pass, instead of sweeping: that is the in-block fixed point
+// again, since in-block distances obey the triangle inequality over every
+// in-block edge.
+template <typename DT, typename DW>
+__device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
+                                          const int32_t *__restrict__ bvert,
+                                          const int32_t *__restrict__ brp,
+                                          const int32_t *__restrict__ bext,
+                                          const Edge *__restrict__ badj,
+                                          const int32_t *__restrict__ vblk,
+                                          const DW *__restrict__ Db,
+                                          uint32_t *__restrict__ Fc, int dense, DT *L,
+                                          uint32_t *s_flag, uint8_t *s_dirty, uint64_t *s_chg,
+                                          unsigned long long *__restrict__ stats, bool &raised) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const DT INF = (DT)~(DT)0;
+    const long long t_start = stats ? clock64() : 0;
+    // wave w owns the contiguous slots [i0, i0 + nv); lane j holds slot i0 + j
+    const int qw = (cnt + kBW - 1) / kBW;
+    const int i0 = min(cnt, wid * qw);
+    const int nv = min(cnt, i0 + qw) - i0;
+    int vj = 0, e0 = 0, em = 0, e1 = 0, ovf = 0;
+    int ic[kBD], iw[kBD], xu[kBD], xw[kBD], xb[kBD];
+#pragma unroll
+    for (int q = 0; q < kBD; q++) {
+        ic[q] = -1;
+        iw[q] = 0;
+        xu[q] = -1;
+        xw[q] = 0;
+        xb[q] = -1;
+    }
+    if (lane < nv) {
+        const int s = s0 + i0 + lane;
+        vj = bvert[s];
+        e0 = brp[s];
+        em = bext[s];
+        e1 = brp[s + 1];
+        ovf = (em - e0 > kBD || e1 - em > kBD) ? 1 : 0;
+#pragma unroll
+        for (int q = 0; q < kBD; q++) {
+            if (e0 + q < em) {
+                const Edge ed = badj[e0 + q];
+                ic[q] = ed.v;
+                iw[q] = ed.w;
+            }
+            if (em + q < e1) {
+                const Edge ed = badj[em + q];
+                xu[q] = ed.v;
+                xw[q] = ed.w;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBD; q++)
+            if (xu[q] >= 0) xb[q] = vblk[xu[q]];
+    }
+    const bool any_ovf = __ballot(ovf != 0) != 0;
+    if (tid < 3) s_flag[tid] = 0;
+    const uint64_t own_mask = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
+    // 1a. the block's labels, eight slots per step
+    for (int j0 = 0; j0 < nv; j0 += 8) {
+        DT own[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int j = min(j0 + t, nv - 1);
+            own[t] = Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if (j0 + t < nv) L[(i0 + j0 + t) * kWave + lane] = own[t];
+    }
+    // 1b. fold in the outside neighbours of the border slots, four per
+    //     step; a slot that drops puts its block neighbours on the
+    //     worklists of the first sweep
+    uint64_t chg_mask = 0;  // slots whose label dropped during this visit
+    uint64_t wmask = dense ? own_mask : 0ull;
+    bool cross0 = false;
+    {
+        uint64_t bmask = uniform64(__ballot(lane < nv && e1 > em));
+        while (bmask) {
+            int jj[4];
+            int nt = 0;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (bmask) {
+                    const int j = pick_bit(bmask, true);
+                    bmask &= bmask - 1;
+                    jj[t] = j;
+                    nt++;
+                } else {
+                    jj[t] = jj[0];
+                }
+            }
+            DT ext[4][kBD];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+#pragma unroll
+                for (int q = 0; q < kBD; q++) {
+                    const int u = __builtin_amdgcn_readlane(xu[q], jj[t]);
+                    ext[t][q] = u >= 0 ? Dc[(int64_t)u * kWave + lane] : INF;
+                }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                if (t >= nt) break;
+                const int j = jj[t];
+                const DT x0 = L[(i0 + j) * kWave + lane];
+                DT x = x0;
+#pragma unroll
+                for (int q = 0; q < kBD; q++)
+                    x = relax_min(ext[t][q], __builtin_amdgcn_readlane(xu[q], j),
+                                  __builtin_amdgcn_readlane(xw[q], j), INF, x);
+                if (any_ovf && __builtin_amdgcn_readlane(ovf, j)) {
+                    const int ea = __builtin_amdgcn_readlane(em, j) + kBD;
+                    const int eb = __builtin_amdgcn_readlane(e1, j);
+                    for (int e = ea; e < eb; e++) {
+                        const Edge ed = badj[e];
+                        x = relax_min(Dc[(int64_t)ed.v * kWave + lane], ed.v, ed.w, INF, x);
+                    }
+                }
+                const bool lower = x < x0;
+                if (lower) L[(i0 + j) * kWave + lane] = x;
+                if (__ballot(lower) != 0) {
+                    chg_mask |= 1ull << j;
+                    if (!dense && Db == nullptr) {
+                        const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
+                        mark_block_neighbours(
+                            j, ic, i0, nv, ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0,
+                            ov ? __builtin_amdgcn_readlane(em, j) : 0, badj, s_dirty, wmask,
+                            cross0);
+                    }
+                }
+            }
+        }
+    }
+    const bool closure = !dense && Db != nullptr;
+    if (closure && lane == 0) s_chg[wid] = chg_mask;
+    __syncthreads();
+    const long long t_loaded = stats ? clock64() : 0;
+    if (closure) {
+        // 2'. closure from the border slots the fold lowered (each wave's
+        //     in s_chg), one distance row per lowered slot, scalar loads of
+        //     the wave's band of it
+        DT x[kBQ], x0[kBQ];
+#pragma unroll
+        for (int t = 0; t < kBQ; t++) x0[t] = x[t] = t < nv ? L[(i0 + t) * kWave + lane] : INF;
+        // two lowered slots per step (a lone last one is taken twice: min
+        // is idempotent); entries past the band (t >= nv) are padding and
+        // their results are dropped below
+        for (int w2 = 0; w2 < kBW; w2++) {
+            uint64_t m = uniform64(s_chg[w2]);
+            const int j0 = min(cnt, w2 * qw);
+            while (m) {
+                const int b1 = j0 + pick_bit(m, true);
+                m &= m - 1;
+                const int b2 = m ? j0 + pick_bit(m, true) : b1;
+                m &= m - 1;
+                const DT l1 = L[b1 * kWave + lane];
+                const DT l2 = L[b2 * kWave + lane];
+                DW d1[kBQ], d2[kBQ];
+                band_row<DW>(Db + (int64_t)b1 * kBRow + wid * kBQ, d1);
+                band_row<DW>(Db + (int64_t)b2 * kBRow + wid * kBQ, d2);
+#pragma unroll
+                for (int t = 0; t < kBQ; t++)
+                    x[t] = closure_min<DT, DW>(closure_min<DT, DW>(x[t], l1, d1[t]), l2, d2[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kBQ; t++) {
+            const bool lower = t < nv && x[t] < x0[t];
+            if (lower) L[(i0 + t) * kWave + lane] = x[t];
+            if (__ballot(lower) != 0) chg_mask |= 1ull << t;
+        }
+        if (stats && tid == 0) atomicAdd(&stats[0], 1ull);
+    } else {
+        // 2. sweeps.  Each wave drains a worklist of its own slots (all of
+        //    them on a dense round, else the block neighbours of the slots
+        //    the outside lowered): a slot whose label drops puts its block
+        //    neighbours on the list -- its own wave's at once, another
+        //    wave's through the LDS marks, which that wave takes up after
+        //    the barrier.  The visit ends when a sweep leaves no mark for
+        //    another wave.
+        unsigned long long processed = 0;
+        for (int it = 0;; it++) {
+            uint8_t *dcur = s_dirty + (it & 1) * kBV;
+            uint8_t *dnext = s_dirty + ((it + 1) & 1) * kBV;
+            if (tid == 0) s_flag[(it + 1) % 3] = 0;
+            {
+                const bool dl = lane < nv && dcur[i0 + lane] != 0;
+                if (dl) dcur[i0 + lane] = 0;
+                wmask = uniform64(wmask | __ballot(dl));
+            }
+            bool cross = false;
+            while (wmask) {
+                int jj[4];
+                int nt = 0;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (wmask) {
+                        const int j = pick_bit(wmask, true);
+                        wmask &= wmask - 1;
+                        jj[t] = j;
+                        nt++;
+                    } else {
+                        jj[t] = jj[0];
+                    }
+                }
+                processed += nt;
+                DT x[4], nb[4][kBD];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    x[t] = L[(i0 + jj[t]) * kWave + lane];
+#pragma unroll
+                    for (int q = 0; q < kBD; q++) {
+                        const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
+                        nb[t][q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (t >= nt) break;
+                    const int j = jj[t];
+                    DT best = x[t];
+#pragma unroll
+                    for (int q = 0; q < kBD; q++)
+                        best = relax_min(nb[t][q], __builtin_amdgcn_readlane(ic[q], j),
+                                         __builtin_amdgcn_readlane(iw[q], j), INF, best);
+                    const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
+                    const int ea = ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0;
+                    const int eb = ov ? __builtin_amdgcn_readlane(em, j) : 0;
+                    for (int e = ea; e < eb; e++) {
+                        const Edge ed = badj[e];
+                        best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                    }
+                    const bool lower = best < x[t];
+                    if (lower) L[(i0 + j) * kWave + lane] = best;
+                    if (__ballot(lower) != 0) {
+                        chg_mask |= 1ull << j;
+                        mark_block_neighbours(j, ic, i0, nv, ea, eb, badj, dnext, wmask, cross);
+                    }
+                }
+            }
+            if (cross && lane == 0) s_flag[it % 3] = 1;
+            __syncthreads();
+            if (s_flag[it % 3] == 0) {
+                if (stats && lane == 0) {
+                    if (tid == 0) {
+                        atomicAdd(&stats[0], 1ull);
+                        atomicAdd(&stats[1], (unsigned long long)(it + 1));
+                    }
+                    atomicAdd(&stats[2], processed);
+                }
+                break;
+            }
+        }
+    }
+    const long long t_swept = stats ? clock64() : 0;
+    // 3. write back the slots that dropped, eight per step; flag the
+    //    blocks beyond a dropped border slot (lane q takes its outside
+    //    neighbour q, the rest of the list one lane each)
+    chg_mask = uniform64(chg_mask);
+    while (chg_mask) {
+        int jj[8];
+        int nt = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if (chg_mask) {
+                const int j = pick_bit(chg_mask, true);
+                chg_mask &= chg_mask - 1;
+                jj[t] = j;
+                nt++;
+            } else {
+                jj[t] = jj[0];
+            }
+        }
+        DT x[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) x[t] = L[(i0 + jj[t]) * kWave + lane];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if (t >= nt) break;
+            const int j = jj[t];
+            Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane] = x[t];
+            const int ea = __builtin_amdgcn_readlane(em, j);
+            const int eb = __builtin_amdgcn_readlane(e1, j);
+            if (ea < eb) {
+                int fb = -1;
+#pragma unroll
+                for (int q = 0; q < kBD; q++) {
+                    const int bq = __builtin_amdgcn_readlane(xb[q], j);
+                    if (lane == q) fb = bq;
+                }
+                if (fb >= 0 && Fc[fb] == 0) Fc[fb] = 1;
+                for (int e = ea + kBD + lane; e < eb; e += kWave) {
+                    const int ob = vblk[badj[e].v];
+                    if (Fc[ob] == 0) Fc[ob] = 1;
+                }
+                raised = true;
+            }
+        }
+    }
+    if (stats && tid == 0) {
+        const long long t_end = clock64();
+        atomicAdd(&stats[3], (unsigned long long)(t_loaded - t_start));
+        atomicAdd(&stats[4], (unsigned long long)(t_swept - t_loaded));
+        atomicAdd(&stats[5], (unsigned long long)(t_end - t_swept));
+    }
+}
+
+// Dm / bdoff (optional, Dm null: sweeps only): the in-block distance
+// matrices, block b's at Dm + bdoff[b].
+template <typename DT, typename DW>
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<DT>()))) void k_relax_blk(
     DT *__restrict__ D, int64_t np, int nblk, const int32_t *__restrict__ boff,
     const int32_t *__restrict__ bvert, const int32_t *__restrict__ brp,
     const int32_t *__restrict__ bext, const Edge *__restrict__ badj,
-    const int32_t *__restrict__ vblk, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
+    const int32_t *__restrict__ vblk, const DW *__restrict__ Dm,
+    const int64_t *__restrict__ bdoff, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
     uint32_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r,
     unsigned long long *__restrict__ stats) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -414,12 +791,11 @@ __global__ __launch_bounds__(kBT) void k_relax_blk(
     uint32_t *s_cnt = s_misc + 4;  // flagged items of the current piece
     uint8_t *s_list = reinterpret_cast<uint8_t *>(s_misc + 8);      // 256 entries
     uint8_t *s_dirty = reinterpret_cast<uint8_t *>(s_misc + 8 + 64);  // 2 x kBV marks
+    uint64_t *s_chg = reinterpret_cast<uint64_t *>(s_misc + kBMiscChg);  // kBW masks
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
-    const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
     if (blockIdx.x == 0 && tid == 0) any[(r + 2) & 3] = 0;
     if (!dense && any[r & 3] == 0) return;
-    const DT INF = (DT)~(DT)0;
     const int64_t G = gridDim.x;
     const int64_t mine = (nitems > (int64_t)blockIdx.x) ? (nitems - 1 - blockIdx.x) / G + 1 : 0;
     // the marks start clear and are clear again whenever a visit's sweeps end
@@ -441,262 +817,12 @@ __global__ __launch_bounds__(kBT) void k_relax_blk(
         const int nl = (int)*s_cnt;
         for (int qi = 0; qi < nl; qi++) {
             const int64_t item = blockIdx.x + G * (p0 + s_list[qi]);
-            const long long t_start = stats ? clock64() : 0;
             const int c = (int)(item / nblk);
             const int b = (int)(item - (int64_t)c * nblk);
-            const int s0 = boff[b];
-            const int cnt = boff[b + 1] - s0;
-            DT *Dc = D + (int64_t)c * np * kWave;
-            // wave w owns the contiguous slots [i0, i0 + nv); lane j holds slot i0 + j
-            const int qw = (cnt + kBW - 1) / kBW;
-            const int i0 = min(cnt, wid * qw);
-            const int nv = min(cnt, i0 + qw) - i0;
-            int vj = 0, e0 = 0, em = 0, e1 = 0, ovf = 0;
-            int ic[kBD], iw[kBD], xu[kBD], xw[kBD], xb[kBD];
-#pragma unroll
-            for (int q = 0; q < kBD; q++) {
-                ic[q] = -1;
-                iw[q] = 0;
-                xu[q] = -1;
-                xw[q] = 0;
-                xb[q] = -1;
-            }
-            if (lane < nv) {
-                const int s = s0 + i0 + lane;
-                vj = bvert[s];
-                e0 = brp[s];
-                em = bext[s];
-                e1 = brp[s + 1];
-                ovf = (em - e0 > kBD || e1 - em > kBD) ? 1 : 0;
-#pragma unroll
-                for (int q = 0; q < kBD; q++) {
-                    if (e0 + q < em) {
-                        const Edge ed = badj[e0 + q];
-                        ic[q] = ed.v;
-                        iw[q] = ed.w;
-                    }
-                    if (em + q < e1) {
-                        const Edge ed = badj[em + q];
-                        xu[q] = ed.v;
-                        xw[q] = ed.w;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kBD; q++)
-                    if (xu[q] >= 0) xb[q] = vblk[xu[q]];
-            }
-            const bool any_ovf = __ballot(ovf != 0) != 0;
-            if (tid < 3) s_flag[tid] = 0;
-            const uint64_t own_mask = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
-            // 1a. the block's labels, eight slots per step
-            for (int j0 = 0; j0 < nv; j0 += 8) {
-                DT own[8];
-#pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    const int j = min(j0 + t, nv - 1);
-                    own[t] = Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane];
-                }
-#pragma unroll
-                for (int t = 0; t < 8; t++)
-                    if (j0 + t < nv) L[(i0 + j0 + t) * kWave + lane] = own[t];
-            }
-            // 1b. fold in the outside neighbours of the border slots, four per
-            //     step; a slot that drops puts its block neighbours on the
-            //     worklists of the first sweep
-            uint64_t chg_mask = 0;  // slots whose label dropped during this visit
-            uint64_t wmask = dense ? own_mask : 0ull;
-            bool cross0 = false;
-            {
-                uint64_t bmask = uniform64(__ballot(lane < nv && e1 > em));
-                while (bmask) {
-                    int jj[4];
-                    int nt = 0;
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if (bmask) {
-                            const int j = pick_bit(bmask, true);
-                            bmask &= bmask - 1;
-                            jj[t] = j;
-                            nt++;
-                        } else {
-                            jj[t] = jj[0];
-                        }
-                    }
-                    DT ext[4][kBD];
-#pragma unroll
-                    for (int t = 0; t < 4; t++)
-#pragma unroll
-                        for (int q = 0; q < kBD; q++) {
-                            const int u = __builtin_amdgcn_readlane(xu[q], jj[t]);
-                            ext[t][q] = u >= 0 ? Dc[(int64_t)u * kWave + lane] : INF;
-                        }
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if (t >= nt) break;
-                        const int j = jj[t];
-                        const DT x0 = L[(i0 + j) * kWave + lane];
-                        DT x = x0;
-#pragma unroll
-                        for (int q = 0; q < kBD; q++)
-                            x = relax_min(ext[t][q], __builtin_amdgcn_readlane(xu[q], j),
-                                          __builtin_amdgcn_readlane(xw[q], j), INF, x);
-                        if (any_ovf && __builtin_amdgcn_readlane(ovf, j)) {
-                            const int ea = __builtin_amdgcn_readlane(em, j) + kBD;
-                            const int eb = __builtin_amdgcn_readlane(e1, j);
-                            for (int e = ea; e < eb; e++) {
-                                const Edge ed = badj[e];
-                                x = relax_min(Dc[(int64_t)ed.v * kWave + lane], ed.v, ed.w, INF, x);
-                            }
-                        }
-                        const bool lower = x < x0;
-                        if (lower) L[(i0 + j) * kWave + lane] = x;
-                        if (__ballot(lower) != 0) {
-                            chg_mask |= 1ull << j;
-                            if (!dense) {
-                                const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
-                                mark_block_neighbours(
-                                    j, ic, i0, nv, ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0,
-                                    ov ? __builtin_amdgcn_readlane(em, j) : 0, badj, s_dirty, wmask,
-                                    cross0);
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            const long long t_loaded = stats ? clock64() : 0;
-            // 2. sweeps.  Each wave drains a worklist of its own slots (all of
-            //    them on a dense round, else the block neighbours of the slots
-            //    the outside lowered): a slot whose label drops puts its block
-            //    neighbours on the list -- its own wave's at once, another
-            //    wave's through the LDS marks, which that wave takes up after
-            //    the barrier.  The visit ends when a sweep leaves no mark for
-            //    another wave.
-            unsigned long long processed = 0;
-            for (int it = 0;; it++) {
-                uint8_t *dcur = s_dirty + (it & 1) * kBV;
-                uint8_t *dnext = s_dirty + ((it + 1) & 1) * kBV;
-                if (tid == 0) s_flag[(it + 1) % 3] = 0;
-                {
-                    const bool dl = lane < nv && dcur[i0 + lane] != 0;
-                    if (dl) dcur[i0 + lane] = 0;
-                    wmask = uniform64(wmask | __ballot(dl));
-                }
-                bool cross = false;
-                while (wmask) {
-                    int jj[4];
-                    int nt = 0;
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if (wmask) {
-                            const int j = pick_bit(wmask, true);
-                            wmask &= wmask - 1;
-                            jj[t] = j;
-                            nt++;
-                        } else {
-                            jj[t] = jj[0];
-                        }
-                    }
-                    processed += nt;
-                    DT x[4], nb[4][kBD];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        x[t] = L[(i0 + jj[t]) * kWave + lane];
-#pragma unroll
-                        for (int q = 0; q < kBD; q++) {
-                            const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
-                            nb[t][q] = sl >= 0 ? L[sl * kWave + lane] : INF;
-                        }
-                    }
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if (t >= nt) break;
-                        const int j = jj[t];
-                        DT best = x[t];
-#pragma unroll
-                        for (int q = 0; q < kBD; q++)
-                            best = relax_min(nb[t][q], __builtin_amdgcn_readlane(ic[q], j),
-                                             __builtin_amdgcn_readlane(iw[q], j), INF, best);
-                        const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
-                        const int ea = ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0;
-                        const int eb = ov ? __builtin_amdgcn_readlane(em, j) : 0;
-                        for (int e = ea; e < eb; e++) {
-                            const Edge ed = badj[e];
-                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
-                        }
-                        const bool lower = best < x[t];
-                        if (lower) L[(i0 + j) * kWave + lane] = best;
-                        if (__ballot(lower) != 0) {
-                            chg_mask |= 1ull << j;
-                            mark_block_neighbours(j, ic, i0, nv, ea, eb, badj, dnext, wmask, cross);
-                        }
-                    }
-                }
-                if (cross && lane == 0) s_flag[it % 3] = 1;
-                __syncthreads();
-                if (s_flag[it % 3] == 0) {
-                    if (stats && lane == 0) {
-                        if (tid == 0) {
-                            atomicAdd(&stats[0], 1ull);
-                            atomicAdd(&stats[1], (unsigned long long)(it + 1));
-                        }
-                        atomicAdd(&stats[2], processed);
-                    }
-                    break;
-                }
-            }
-            const long long t_swept = stats ? clock64() : 0;
-            // 3. write back the slots that dropped, eight per step; flag the
-            //    blocks beyond a dropped border slot (lane q takes its outside
-            //    neighbour q, the rest of the list one lane each)
-            chg_mask = uniform64(chg_mask);
-            while (chg_mask) {
-                int jj[8];
-                int nt = 0;
-#pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    if (chg_mask) {
-                        const int j = pick_bit(chg_mask, true);
-                        chg_mask &= chg_mask - 1;
-                        jj[t] = j;
-                        nt++;
-                    } else {
-                        jj[t] = jj[0];
-                    }
-                }
-                DT x[8];
-#pragma unroll
-                for (int t = 0; t < 8; t++) x[t] = L[(i0 + jj[t]) * kWave + lane];
-#pragma unroll
-                for (int t = 0; t < 8; t++) {
-                    if (t >= nt) break;
-                    const int j = jj[t];
-                    Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane] = x[t];
-                    const int ea = __builtin_amdgcn_readlane(em, j);
-                    const int eb = __builtin_amdgcn_readlane(e1, j);
-                    if (ea < eb) {
-                        int fb = -1;
-#pragma unroll
-                        for (int q = 0; q < kBD; q++) {
-                            const int bq = __builtin_amdgcn_readlane(xb[q], j);
-                            if (lane == q) fb = bq;
-                        }
-                        uint32_t *Fc = Fnext + (int64_t)c * nblk;
-                        if (fb >= 0 && Fc[fb] == 0) Fc[fb] = 1;
-                        for (int e = ea + kBD + lane; e < eb; e += kWave) {
-                            const int ob = vblk[badj[e].v];
-                            if (Fc[ob] == 0) Fc[ob] = 1;
-                        }
-                        raised = true;
-                    }
-                }
-            }
-            if (stats && tid == 0) {
-                const long long t_end = clock64();
-                atomicAdd(&stats[3], (unsigned long long)(t_loaded - t_start));
-                atomicAdd(&stats[4], (unsigned long long)(t_swept - t_loaded));
-                atomicAdd(&stats[5], (unsigned long long)(t_end - t_swept));
-            }
+            blk_visit<DT, DW>(D + (int64_t)c * np * kWave, boff[b], boff[b + 1] - boff[b], bvert,
+                              brp, bext, badj, vblk, Dm ? Dm + bdoff[b] : nullptr,
+                              Fnext + (int64_t)c * nblk, dense, L, s_flag, s_dirty, s_chg, stats,
+                              raised);
             __syncthreads();  // LDS is reused by the next item
         }
         __syncthreads();  // s_cnt is reset by the next piece

@@ -41,7 +41,7 @@ class EmvResult(ctypes.Structure):
         ("merge_launches", ctypes.c_int64), ("relax_rounds", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64), ("npairs", ctypes.c_int64), ("pairs", _i64p),
         ("relax_blocks", ctypes.c_int64), ("unpack_ms", ctypes.c_double),
-        ("singleton_rounds", ctypes.c_int64),
+        ("singleton_rounds", ctypes.c_int64), ("block_dist_bits", ctypes.c_int64),
     ]
 
 
@@ -109,6 +109,7 @@ class Solution:
         self.relax_blocks = r.relax_blocks
         self.unpack_ms = r.unpack_ms
         self.singleton_rounds = r.singleton_rounds
+        self.block_dist_bits = r.block_dist_bits
         self.pairs = pairs
 
 

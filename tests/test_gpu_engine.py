@@ -162,3 +162,49 @@ def test_batch_label_widths_agree():
             assert c1[i] == ocost
             np.testing.assert_array_equal(f2, of)
             np.testing.assert_array_equal(code2, ocode)
+
+
+def _check_batch_vs_oracle(insts, costs, b):
+    for i, (n, e, kk) in enumerate(insts):
+        ocost, of, ocode = oracle.emv(n, e, kk)
+        assert costs[i] == ocost
+        f, code = b.read_rows(i, 0, 1 << len(kk), n)
+        np.testing.assert_array_equal(f, of)
+        np.testing.assert_array_equal(code, ocode)
+
+
+def test_batch_block_relaxation_forced(monkeypatch):
+    """EMV_BLOCKS=1 puts every graph on block-local relaxation (random regular
+    graphs: blocks with few inside edges, many border flags): the tables and
+    records equal the oracle's, eagerly and from a graph replay."""
+    monkeypatch.setenv("EMV_BLOCKS", "1")
+    insts = mixed_instances(6, 4)
+    n = 700
+    insts.append((n, emv.regular_graph(n, 4, 20000, 78), emv.terminals(n, 6, 78)))
+    with emv.Batch(insts) as b:
+        c1, s1 = b.solve(emv.KEEP_TABLE)
+        _check_batch_vs_oracle(insts, c1, b)
+        c2, s2 = b.solve(emv.KEEP_TABLE)
+        assert s2.graph_used == 1 or s2.graph_fallback == 1
+        _check_batch_vs_oracle(insts, c2, b)
+
+
+@pytest.mark.parametrize("closure", ["1", "0"])
+def test_batch_grids_and_regular_graphs_in_one_step(monkeypatch, closure):
+    """Grids (block-local relaxation by the locality rule) and random regular
+    graphs (vertex-wise) step together: four launch groups per round (two
+    label widths x two relaxation kinds) sharing the round's flags.  Block
+    visits after the first apply in-block distances (32-bit rows here: one
+    grid has w up to 30000) or, with EMV_BLOCK_CLOSURE=0, sweep."""
+    monkeypatch.setenv("EMV_BLOCK_CLOSURE", closure)
+    insts = []
+    for i, (R, C, w) in enumerate([(40, 30, 100), (33, 47, 100), (36, 36, 30000)]):
+        n = R * C
+        insts.append((n, emv.grid_graph(R, C, w, 60 + i), emv.terminals(n, 5 + i, 60 + i)))
+    insts += mixed_instances(4, 5)
+    with emv.Batch(insts) as b:
+        c1, _ = b.solve(emv.KEEP_TABLE)
+        _check_batch_vs_oracle(insts, c1, b)
+        c2, _ = b.solve(emv.KEEP_TABLE | emv.WIDE_LABELS)
+        np.testing.assert_array_equal(c1, c2)
+        _check_batch_vs_oracle(insts, c2, b)

@@ -316,18 +316,24 @@ def test_worklist_rounds_agree(gpu_ctx, monkeypatch, qcap):
     np.testing.assert_array_equal(c1, c2)
 
 
+@pytest.mark.parametrize("closure", ["1", "0"])
 @pytest.mark.parametrize("flags", [0, emv.WIDE_LABELS, emv.FORCE_P64])
-def test_block_relax_grid_vs_oracle(gpu_ctx, flags):
+def test_block_relax_grid_vs_oracle(gpu_ctx, monkeypatch, flags, closure):
     """Grids take the block-local relaxation (LDS sweeps inside vertex
-    blocks); every cell and record equal the oracle's at every label width."""
+    blocks, then in-block distances from the lowered border slots on later
+    visits, or sweeps on every visit with EMV_BLOCK_CLOSURE=0); every cell
+    and record equal the oracle's at every label width."""
     R, C = 50, 60
     n = R * C
     e = emv.grid_graph(R, C, 100, 21)
     kk = emv.terminals(n, 8, 21)
-    print(f"case grid {R}x{C} flags={flags}", flush=True)
+    monkeypatch.setenv("EMV_BLOCK_CLOSURE", closure)
+    print(f"case grid {R}x{C} flags={flags} closure={closure}", flush=True)
     sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
     if flags != emv.FORCE_P64:
         assert sol.relax_blocks > 0
+        # in-block distances of a w <= 100 grid fit 16 bits
+        assert sol.block_dist_bits == (16 if closure == "1" else 0)
     ocost, of, ocode = oracle.emv(n, e, kk)
     assert sol.cost == ocost
     np.testing.assert_array_equal(f, of)
@@ -345,7 +351,7 @@ def test_block_relax_forced_vs_oracle(gpu_ctx, monkeypatch, n, d, seed):
     monkeypatch.setenv("EMV_BLOCKS", "1")
     print(f"case forced blocks n={n} d={d}", flush=True)
     sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
-    assert sol.relax_blocks > 0
+    assert sol.relax_blocks > 0 and sol.block_dist_bits in (16, 32)
     ocost, of, ocode = oracle.emv(n, e, kk)
     assert sol.cost == ocost
     np.testing.assert_array_equal(f, of)
@@ -406,6 +412,22 @@ def test_merge_k11_vs_oracle(gpu_ctx, flags):
     kk = emv.terminals(n, k, 89)
     print(f"case merge k=11 flags={flags}", flush=True)
     sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
+def test_block_distances_32bit(gpu_ctx):
+    """A grid whose in-block distances need 32 bits (w up to 20000): the
+    closure reads 32-bit rows; cells and records equal the oracle's."""
+    R, C = 40, 45
+    n = R * C
+    e = emv.grid_graph(R, C, 20000, 23)
+    kk = emv.terminals(n, 7, 23)
+    print(f"case grid {R}x{C} w<=20000", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
+    assert sol.relax_blocks > 0 and sol.block_dist_bits == 32
     ocost, of, ocode = oracle.emv(n, e, kk)
     assert sol.cost == ocost
     np.testing.assert_array_equal(f, of)
