@@ -973,6 +973,16 @@ int step_eager(emv_engine *e, int *rounds_used) {
     ECHK(run_relax_eager(e, db, P, rounds_used));
     ECHK(enqueue_back(e, db, P));
     e->steps++;
+    if (getenv("EMV_TRACE")) {
+        int64_t items[4];
+        for (int g = 0; g < 4; g++) items[g] = P.grp[g].relax_pre.back();
+        fprintf(stderr,
+                "emv-trace: step %lld instances=%zu chunks=%zu rounds=%d items u32/u16 vertex "
+                "%lld/%lld block %lld/%lld\n",
+                (long long)e->steps, P.insts.size(), P.ch.size(), *rounds_used,
+                (long long)items[0], (long long)items[1], (long long)items[2],
+                (long long)items[3]);
+    }
     return advance(e, P);
 }
 
