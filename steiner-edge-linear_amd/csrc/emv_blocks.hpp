@@ -93,8 +93,9 @@ inline double build_blocks(int64_t n, const int32_t *rp, const Edge *adj, BlockP
 // Distances inside each block (over the block's own edges only) from every
 // border slot to every slot of the block: block b's matrix starts at
 // bdoff[b], row i (border slot s0 + i) has kBRow entries, slot s0 + j at
-// kBQ * (j / qw) + j % qw with qw = ceil(cnt / kBW) -- the bands of the
-// block workgroup's waves, each padded to kBQ entries (padding 0).  Blocks
+// kBQ * (j % kBW) + j / kBW -- the bands of the block workgroup's waves
+// (wave w holds slots w, w + kBW, ...), each padded to kBQ entries (padding
+// 0).  Blocks
 // are connected (grown breadth-first), so every entry is finite.  dmax is
 // the largest entry.
 struct BlockDist {
@@ -113,7 +114,6 @@ inline void block_distances(const BlockPart &P, BlockDist &B) {
     std::vector<uint64_t> row;
     for (int b = 0; b < P.nblk; b++) {
         const int32_t s0 = P.boff[b], cnt = P.boff[b + 1] - s0;
-        const int32_t qw = (cnt + kBW - 1) / kBW;
         for (int i = 0; i < P.bnb[b]; i++) {
             row.assign((size_t)cnt, ~0ull);
             heap.clear();
@@ -136,7 +136,7 @@ inline void block_distances(const BlockPart &P, BlockDist &B) {
             }
             uint64_t *out = B.d.data() + B.bdoff[b] + (int64_t)i * kBRow;
             for (int32_t j = 0; j < cnt; j++) {
-                out[kBQ * (j / qw) + j % qw] = row[j];
+                out[kBQ * (j % kBW) + j / kBW] = row[j];
                 B.dmax = std::max(B.dmax, row[j]);
             }
         }

@@ -363,10 +363,10 @@ int relax_chunks(emv_ctx *c, int64_t nchunks, int64_t *rounds_out) {
         const double v = st[0] ? (double)st[0] : 1.0;
         fprintf(stderr,
                 "emv-trace: relax chunks=%lld items=%lld rounds=%lld visits=%llu sweeps=%llu "
-                "swept_slots=%llu cycles/visit load=%.0f sweep=%.0f write=%.0f lds=%zu/%zu "
-                "grid=%d\n",
+                "swept_slots=%llu idle_visits=%llu lowered_border=%llu cycles/visit load=%.0f "
+                "sweep=%.0f write=%.0f lds=%zu/%zu grid=%d\n",
                 (long long)nchunks, (long long)nitems, (long long)rounds, st[0], st[1], st[2],
-                st[3] / v, st[4] / v, st[5] / v, c->lds_block, c->lds_cu,
+                st[6], st[7], st[3] / v, st[4] / v, st[5] / v, c->lds_block, c->lds_cu,
                 c->num_cu * (int)std::max<size_t>(
                                 1, std::min<size_t>(2, c->lds_cu / blk_lds_bytes<DT>())));
     }

@@ -378,9 +378,9 @@ __device__ __forceinline__ int pick_bit(uint64_t m, bool lowest) {
 
 // Puts the block neighbours of slot j (wave-local index, registers ic of its
 // wave; the part of its list past the register copy, badj[ea..eb), one lane
-// per entry) on the worklists: the slots of the calling wave's band [i0,
-// i0+nv) into its mask, the others as LDS marks for their wave.
-__device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD], int i0, int nv,
+// per entry) on the worklists: the slots of the calling wave's band (slots
+// wid, wid + kBW, ...) into its mask, the others as LDS marks for their wave.
+__device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD], int wid,
                                                       int ea, int eb,
                                                       const Edge *__restrict__ badj,
                                                       uint8_t *marks, uint64_t &wmask,
@@ -390,8 +390,8 @@ __device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD
     for (int q = 0; q < kBD; q++) {
         const int sl = __builtin_amdgcn_readlane(ic[q], j);
         if (sl >= 0) {
-            if (sl >= i0 && sl < i0 + nv) {
-                wmask |= 1ull << (sl - i0);
+            if (sl % kBW == wid) {
+                wmask |= 1ull << (sl / kBW);
             } else {
                 if (lane == 0) marks[sl] = 1;
                 cross = true;
@@ -403,8 +403,8 @@ __device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD
         bool xo = false;
         for (int e = ea + lane; e < eb; e += kWave) {
             const int sl = badj[e].v;
-            if (sl >= i0 && sl < i0 + nv) {
-                add |= 1ull << (sl - i0);
+            if (sl % kBW == wid) {
+                add |= 1ull << (sl / kBW);
             } else {
                 marks[sl] = 1;
                 xo = true;
@@ -459,8 +459,8 @@ __device__ __forceinline__ void band_row(const DW *__restrict__ p, DW (&dv)[kBQ]
 // part; raised is set when a block beyond this one was flagged.
 //
 // Db (optional) is the block's matrix of in-block distances from each border
-// slot (the first slots of a block) to every slot: row b holds, for wave w's
-// band of slots, its qw entries from kBQ * w on (kBRow entries a row).  With it, a visit of a
+// slot (the first slots of a block) to every slot: row b holds the entry of
+// slot j * kBW + w (wave w's band) at kBQ * w + j (kBRow entries a row).  With it, a visit of a
 // later round -- the block sits at its in-block fixed point since its last
 // visit, only border labels can drop when the outside is folded in -- takes
 // every slot to min over the lowered border slots b of (label of b + Db[b][
@@ -483,10 +483,10 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
     const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
     const DT INF = (DT)~(DT)0;
     const long long t_start = stats ? clock64() : 0;
-    // wave w owns the contiguous slots [i0, i0 + nv); lane j holds slot i0 + j
-    const int qw = (cnt + kBW - 1) / kBW;
-    const int i0 = min(cnt, wid * qw);
-    const int nv = min(cnt, i0 + qw) - i0;
+    // wave w owns the slots w, w + kBW, w + 2 kBW, ... (its band, strided so
+    // that the border slots, which come first, spread over all waves); lane
+    // j holds slot j * kBW + w
+    const int nv = wid < cnt ? (cnt - 1 - wid) / kBW + 1 : 0;
     int vj = 0, e0 = 0, em = 0, e1 = 0, ovf = 0;
     int ic[kBD], iw[kBD], xu[kBD], xw[kBD], xb[kBD];
 #pragma unroll
@@ -498,7 +498,7 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         xb[q] = -1;
     }
     if (lane < nv) {
-        const int s = s0 + i0 + lane;
+        const int s = s0 + lane * kBW + wid;
         vj = bvert[s];
         e0 = brp[s];
         em = bext[s];
@@ -534,7 +534,7 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         }
 #pragma unroll
         for (int t = 0; t < 8; t++)
-            if (j0 + t < nv) L[(i0 + j0 + t) * kWave + lane] = own[t];
+            if (j0 + t < nv) L[((j0 + t) * kBW + wid) * kWave + lane] = own[t];
     }
     // 1b. fold in the outside neighbours of the border slots, four per
     //     step; a slot that drops puts its block neighbours on the
@@ -570,7 +570,7 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
             for (int t = 0; t < 4; t++) {
                 if (t >= nt) break;
                 const int j = jj[t];
-                const DT x0 = L[(i0 + j) * kWave + lane];
+                const DT x0 = L[(j * kBW + wid) * kWave + lane];
                 DT x = x0;
 #pragma unroll
                 for (int q = 0; q < kBD; q++)
@@ -585,13 +585,13 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
                     }
                 }
                 const bool lower = x < x0;
-                if (lower) L[(i0 + j) * kWave + lane] = x;
+                if (lower) L[(j * kBW + wid) * kWave + lane] = x;
                 if (__ballot(lower) != 0) {
                     chg_mask |= 1ull << j;
                     if (!dense && Db == nullptr) {
                         const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
                         mark_block_neighbours(
-                            j, ic, i0, nv, ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0,
+                            j, ic, wid, ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0,
                             ov ? __builtin_amdgcn_readlane(em, j) : 0, badj, s_dirty, wmask,
                             cross0);
                     }
@@ -609,17 +609,16 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         //     the wave's band of it
         DT x[kBQ], x0[kBQ];
 #pragma unroll
-        for (int t = 0; t < kBQ; t++) x0[t] = x[t] = t < nv ? L[(i0 + t) * kWave + lane] : INF;
+        for (int t = 0; t < kBQ; t++) x0[t] = x[t] = t < nv ? L[(t * kBW + wid) * kWave + lane] : INF;
         // two lowered slots per step (a lone last one is taken twice: min
         // is idempotent); entries past the band (t >= nv) are padding and
         // their results are dropped below
         for (int w2 = 0; w2 < kBW; w2++) {
             uint64_t m = uniform64(s_chg[w2]);
-            const int j0 = min(cnt, w2 * qw);
             while (m) {
-                const int b1 = j0 + pick_bit(m, true);
+                const int b1 = pick_bit(m, true) * kBW + w2;
                 m &= m - 1;
-                const int b2 = m ? j0 + pick_bit(m, true) : b1;
+                const int b2 = m ? pick_bit(m, true) * kBW + w2 : b1;
                 m &= m - 1;
                 const DT l1 = L[b1 * kWave + lane];
                 const DT l2 = L[b2 * kWave + lane];
@@ -634,10 +633,17 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
 #pragma unroll
         for (int t = 0; t < kBQ; t++) {
             const bool lower = t < nv && x[t] < x0[t];
-            if (lower) L[(i0 + t) * kWave + lane] = x[t];
+            if (lower) L[(t * kBW + wid) * kWave + lane] = x[t];
             if (__ballot(lower) != 0) chg_mask |= 1ull << t;
         }
-        if (stats && tid == 0) atomicAdd(&stats[0], 1ull);
+        if (stats && tid == 0) {
+            // visits, visits whose fold lowered nothing, lowered border slots
+            unsigned long long low = 0;
+            for (int w2 = 0; w2 < kBW; w2++) low += __popcll(s_chg[w2]);
+            atomicAdd(&stats[0], 1ull);
+            if (low == 0) atomicAdd(&stats[6], 1ull);
+            atomicAdd(&stats[7], low);
+        }
     } else {
         // 2. sweeps.  Each wave drains a worklist of its own slots (all of
         //    them on a dense round, else the block neighbours of the slots
@@ -652,8 +658,8 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
             uint8_t *dnext = s_dirty + ((it + 1) & 1) * kBV;
             if (tid == 0) s_flag[(it + 1) % 3] = 0;
             {
-                const bool dl = lane < nv && dcur[i0 + lane] != 0;
-                if (dl) dcur[i0 + lane] = 0;
+                const bool dl = lane < nv && dcur[lane * kBW + wid] != 0;
+                if (dl) dcur[lane * kBW + wid] = 0;
                 wmask = uniform64(wmask | __ballot(dl));
             }
             bool cross = false;
@@ -675,7 +681,7 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
                 DT x[4], nb[4][kBD];
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    x[t] = L[(i0 + jj[t]) * kWave + lane];
+                    x[t] = L[(jj[t] * kBW + wid) * kWave + lane];
 #pragma unroll
                     for (int q = 0; q < kBD; q++) {
                         const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
@@ -699,10 +705,10 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
                         best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
                     }
                     const bool lower = best < x[t];
-                    if (lower) L[(i0 + j) * kWave + lane] = best;
+                    if (lower) L[(j * kBW + wid) * kWave + lane] = best;
                     if (__ballot(lower) != 0) {
                         chg_mask |= 1ull << j;
-                        mark_block_neighbours(j, ic, i0, nv, ea, eb, badj, dnext, wmask, cross);
+                        mark_block_neighbours(j, ic, wid, ea, eb, badj, dnext, wmask, cross);
                     }
                 }
             }
@@ -741,7 +747,7 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         }
         DT x[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) x[t] = L[(i0 + jj[t]) * kWave + lane];
+        for (int t = 0; t < 8; t++) x[t] = L[(jj[t] * kBW + wid) * kWave + lane];
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             if (t >= nt) break;
