@@ -416,9 +416,11 @@ __device__ __forceinline__ void mark_block_neighbours(int j, const int (&ic)[kBD
     }
 }
 
-// min(x, lb + d) for a finite label lb and an in-block distance d, without
-// wrapping: 16-bit labels add in 32 bits, 32-bit ones saturate (a saturated
-// sum is INF and lowers nothing), 64-bit labels stay far below 2^63.
+// min(x, lb + d) for a label lb and an in-block distance d.  lb is INF in
+// the lanes (rows) where the border slot holds no path yet, and the sum must
+// then lower nothing: 16-bit labels add in 32 bits (INF + d >= INF), 32-bit
+// ones saturate to INF, and a 64-bit sum that wraps (only INF + d can:
+// finite labels stay far below 2^63) is dropped.
 template <typename DT, typename DW>
 __device__ __forceinline__ DT closure_min(DT x, DT lb, DW d) {
     if constexpr (sizeof(DT) == 2) {
@@ -429,7 +431,7 @@ __device__ __forceinline__ DT closure_min(DT x, DT lb, DW d) {
         return c < x ? c : x;
     } else {
         const uint64_t c = lb + (uint64_t)d;
-        return c < x ? c : x;
+        return (c >= lb && c < x) ? c : x;
     }
 }
 

@@ -432,3 +432,22 @@ def test_block_distances_32bit(gpu_ctx):
     assert sol.cost == ocost
     np.testing.assert_array_equal(f, of)
     np.testing.assert_array_equal(code, ocode)
+
+
+def test_block_closure_with_64bit_singleton_labels(gpu_ctx):
+    """Group-reduced grid whose singleton rows need 64-bit labels (n * M >
+    2^32) while its blocks take in-block distances: a border slot that holds
+    no path yet in some row (INF) must lower nothing there -- INF + d wraps
+    in 64 bits.  Cells, records and cost equal the oracle's."""
+    import workloads
+
+    n, e, grp, nn, red, kk, M = workloads.group_instance(40, 40, 6, 8, 3000, 33)
+    assert nn * M > 2 ** 32
+    inst = Instance(nn, red, kk)
+    print(f"case group grid nn={nn} k={len(kk)} M={M}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    assert sol.relax_blocks > 0 and sol.block_dist_bits == 32
+    ocost, of, ocode = oracle.emv(nn, red, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
