@@ -50,6 +50,9 @@ struct GMerge {
     int32_t k, np;
     int64_t tbase;   // the instance's table
     int64_t dbase0;  // staging cell of its first chunk; chunk q at dbase0 + q * np * 64
+    int32_t S;       // split merge: parts per row (0: rows merged whole by k_gmerge)
+    int32_t pad_;
+    int64_t mbase;   // split merge: accumulator of row xi at Macc[mbase + xi * np + v]
 };
 
 template <typename DT>
@@ -133,6 +136,122 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
         cd[jx] = pv[jx];
+    }
+}
+
+// Split merge of the instances whose rows have many pairs (the upper
+// layers): as k_merge_split, item = (tile, row, part) numbered tile-major per
+// instance, each part folded into the row-major accumulator with an
+// unsigned atomicMin (the minimum does not depend on the order).  Macc starts
+// at all ones; k_gpack moves it to the staging.
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_gmerge_split(
+    const int32_t *__restrict__ f, const GMerge *__restrict__ mg,
+    const int64_t *__restrict__ item_pre, int nmg, const uint32_t *__restrict__ xs,
+    P *__restrict__ Macc, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
+    if (item >= item_pre[nmg]) return;
+    const int s = find_seg(item_pre, nmg, item);
+    const GMerge M = mg[s];
+    const int64_t local = item - item_pre[s];
+    const int64_t per_tile = (int64_t)M.nrows * M.S;
+    const int64_t tile = local / per_tile;
+    const int rem = (int)(local - tile * per_tile);
+    const int xi = rem / M.S;
+    const int part = rem - xi * M.S;
+    const uint32_t X = xs[M.x0 + xi];
+    const uint32_t steps = merge_steps(X);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / M.S);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / M.S);
+    if (o0 >= o1) return;
+    const int64_t v = tile * kWave + lane;
+    const P acc = merge_outer<int32_t, P>(f + M.tbase + v, (int64_t)M.np, X, M.k, o0, o1);
+    atomicMin(&Macc[M.mbase + (int64_t)xi * M.np + v], acc);
+}
+
+// Accumulators of the split instances -> vertex-major staging (label, packed
+// word) through an LDS transpose; workgroup = (instance, chunk, 64-vertex
+// tile) numbered by blk_pre.  Lanes past the instance's rows are INF.
+template <typename P, typename DT>
+__global__ __launch_bounds__(kBlock) void k_gpack(const P *__restrict__ Macc,
+                                                  const GMerge *__restrict__ mg,
+                                                  const int64_t *__restrict__ blk_pre, int nmg,
+                                                  void *__restrict__ D,
+                                                  uint64_t *__restrict__ CODE) {
+    __shared__ P sm[kWave][kWave + 1];
+    const int64_t b = blockIdx.x;
+    if (b >= blk_pre[nmg]) return;
+    const int s = find_seg(blk_pre, nmg, b);
+    const GMerge M = mg[s];
+    const int64_t local = b - blk_pre[s];
+    const int64_t chunk = local / M.ntiles;
+    const int64_t v0 = (local - chunk * M.ntiles) * kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int grp = threadIdx.x / kWave;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int xl = grp + r * (kBlock / kWave);
+        const int64_t xi = chunk * kWave + xl;
+        sm[xl][lane] = (xi < M.nrows) ? Macc[M.mbase + xi * M.np + v0 + lane] : ~(P)0;
+    }
+    __syncthreads();
+    const int64_t dbase = M.dbase0 + chunk * M.np * kWave;
+    DT *Dd = chunk_labels<DT>(D, dbase);
+    const bool real = chunk * kWave + lane < M.nrows;
+#pragma unroll
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int vv = grp + r * (kBlock / kWave);
+        const P a = sm[lane][vv];
+        const int64_t idx = (v0 + vv) * kWave + lane;
+        Dd[idx] = real ? (DT)(a >> M.k) : (DT)~(DT)0;
+        CODE[dbase + idx] = real ? (uint64_t)a : ~0ull;
+    }
+}
+
+// ------------------------------------------------------------ lane packing
+// Partial chunks of instances on one graph share relaxation chunks: a packed
+// chunk ("bin") at dbase takes, in lane j, row src[j].lane of the staging
+// chunk at src[j].dbase (src[j].dbase < 0: an unused lane, INF).  Rows relax
+// independently over the same graph, so a bin relaxes exactly as its source
+// chunks would.  k_glanes<DT, true> copies the rows in before the rounds,
+// k_glanes<DT, false> copies them back after.  Workgroup = (bin, 64-vertex
+// tile), numbered by blk_pre.
+struct GLane {
+    int64_t dbase;
+    int32_t lane;
+    int32_t pad_;
+};
+
+struct GBin {
+    int64_t dbase;  // the bin's staging cells
+    int32_t np;
+    int32_t map0;   // its 64 GLane entries
+};
+
+template <typename DT, bool IN>
+__global__ __launch_bounds__(kBlock) void k_glanes(void *__restrict__ D,
+                                                   const GBin *__restrict__ bins,
+                                                   const int64_t *__restrict__ blk_pre, int nbin,
+                                                   const GLane *__restrict__ lanes) {
+    const int64_t b = blockIdx.x;
+    if (b >= blk_pre[nbin]) return;
+    const int s = find_seg(blk_pre, nbin, b);
+    const GBin B = bins[s];
+    const int64_t v0 = (b - blk_pre[s]) * kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int grp = threadIdx.x / kWave;
+    const GLane src = lanes[B.map0 + lane];
+    DT *Db = chunk_labels<DT>(D, B.dbase);
+    DT *Ds = src.dbase >= 0 ? chunk_labels<DT>(D, src.dbase) + src.lane : nullptr;
+#pragma unroll 4
+    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
+        const int64_t v = v0 + grp + r * (kBlock / kWave);
+        if constexpr (IN)
+            Db[v * kWave + lane] = Ds ? Ds[v * kWave] : (DT)~(DT)0;
+        else if (Ds)
+            Ds[v * kWave] = Db[v * kWave + lane];
     }
 }
 

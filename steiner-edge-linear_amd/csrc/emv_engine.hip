@@ -173,6 +173,10 @@ struct emv_engine {
     int64_t stage_cells = 0;
     uint32_t *d_flag[2] = {nullptr, nullptr};
     int64_t flag_bytes = 0;  // flag words held per array
+    void *d_macc = nullptr;  // split-merge accumulators (packed words)
+    int64_t macc_bytes = 0;
+    bool no_split = false;   // EMV_NO_SPLIT: every layer through k_gmerge
+    bool no_pack = false;    // EMV_NO_PACK: every staging chunk relaxes in place
     uint32_t *d_any = nullptr;
     uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
     uint32_t *d_q[2] = {nullptr, nullptr};
@@ -405,6 +409,16 @@ void clear_flags(emv_engine *e) {
                            e->d_flag[i], c, 0u);
 }
 
+// Split-merge accumulators of a step: bytes = cells * (P bits / 8).
+int ensure_macc(emv_engine *e, int64_t bytes) {
+    if (bytes <= e->macc_bytes) return EMV_OK;
+    dfree(e->d_macc);
+    e->macc_bytes = 0;
+    EHIP(e, hipMalloc(&e->d_macc, (size_t)bytes));
+    e->macc_bytes = bytes;
+    return EMV_OK;
+}
+
 int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
     if (cells > e->stage_cells) {
         dfree(e->d_D);
@@ -439,13 +453,19 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
 struct StepGroup {
     int ch0 = 0, nch = 0;  // chunk slice [ch0, ch0 + nch)
     int mg0 = 0, nmg = 0;  // merge slice (GMerge.chunk0 is relative to ch0)
-    std::vector<int64_t> relax_pre, merge_pre, unpack_pre;
-    std::vector<int32_t> relax_blk;  // chunk (relative) of relaxation item 64 * b
-    size_t off_rp = 0, off_mp = 0, off_up = 0, off_rb = 0;
+    int ms0 = 0, nms = 0;  // split-merge slice, right after the merge slice
+    int rch0 = 0, nrch = 0;  // relaxation chunks (staging chunks in place, or packed bins)
+    int bin0 = 0, nbin = 0;  // packed bins
+    std::vector<int64_t> relax_pre, merge_pre, unpack_pre, split_pre, pack_pre, bin_pre;
+    std::vector<int32_t> relax_blk;  // relaxation chunk (relative) of relaxation item 64 * b
+    size_t off_rp = 0, off_mp = 0, off_up = 0, off_rb = 0, off_sp = 0, off_pp = 0, off_bp = 0;
 };
 
 struct StepPlan {
     std::vector<GChunk> ch;
+    std::vector<GChunk> rch;  // relaxation chunks of all groups
+    std::vector<GBin> bins;
+    std::vector<GLane> lanes;
     std::vector<GMerge> mg;
     StepGroup grp[4];  // label width (u32, u16) x relaxation (vertex-wise, block-local)
     std::vector<int64_t> seed_pre;
@@ -455,8 +475,10 @@ struct StepPlan {
     int xg = 1;
     int pbits = 32;
     int64_t cells = 0, flagbytes = 0;
+    int64_t macc_cells = 0;  // split-merge accumulators of the step
     // device layout (byte offsets into the descriptor buffer)
     size_t off_ch = 0, off_mg = 0, off_sp = 0, off_xs = 0, off_seedch = 0, bytes = 0;
+    size_t off_rch = 0, off_bins = 0, off_lanes = 0;
     std::vector<char> blob;
     std::vector<GChunk> seed_ch;
 };
@@ -474,10 +496,12 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     int64_t dcur = 0, fcur = 0;
     int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
+    std::vector<GMerge> split;
     for (int gsel = 0; gsel < 4; gsel++) {
         StepGroup &SG = P.grp[gsel];
         SG.ch0 = (int)P.ch.size();
         SG.mg0 = (int)P.mg.size();
+        split.clear();
         for (int64_t t : tickets) {
             Inst &I = e->inst[t];
             if (inst_group(e, I) != gsel) continue;
@@ -509,8 +533,12 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             rows.clear();
             subsets_of_size(I.k, I.stage, rows);
             const int nrows = (int)rows.size();
-            maxrows = std::max(maxrows, nrows);
-            maxstage = std::max(maxstage, I.stage);
+            // rows of >= kSplitPairs pairs take the split merge
+            const bool splits = ((1u << (I.stage - 1)) - 1u) >= kSplitPairs && !e->no_split;
+            if (!splits) {
+                maxrows = std::max(maxrows, nrows);
+                maxstage = std::max(maxstage, I.stage);
+            }
             if (I.pbits == 64) P.pbits = 64;
             GMerge M;
             M.chunk0 = (int)P.ch.size() - SG.ch0;
@@ -523,6 +551,17 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.np = (int32_t)G.np;
             M.tbase = I.abase;
             M.dbase0 = dcur;
+            M.S = 0;
+            M.pad_ = 0;
+            M.mbase = 0;
+            if (splits) {
+                // enough items per tile to occupy an XCD; parts of >= 4 outer steps
+                const int64_t steps = (I.stage - 1 <= 3) ? 1 : (1ll << (I.stage - 1 - 3));
+                M.S = (int32_t)std::max<int64_t>(
+                    1, std::min<int64_t>((512 + nrows - 1) / nrows, steps / 4));
+                M.mbase = P.macc_cells;
+                P.macc_cells += (int64_t)nrows * G.np;
+            }
             const int x0 = (int)P.xs.size();
             P.xs.insert(P.xs.end(), rows.begin(), rows.end());
             for (int q = 0; q < M.nchunks; q++) {
@@ -535,10 +574,16 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 dcur += G.np * kWave;
                 fcur += nflags;
             }
-            P.mg.push_back(M);
+            if (splits)
+                split.push_back(M);
+            else
+                P.mg.push_back(M);
         }
         SG.nch = (int)P.ch.size() - SG.ch0;
         SG.nmg = (int)P.mg.size() - SG.mg0;
+        SG.ms0 = (int)P.mg.size();
+        SG.nms = (int)split.size();
+        P.mg.insert(P.mg.end(), split.begin(), split.end());
     }
     // rows per merge wave: many rows while rows have few pairs (a wave's
     // fixed costs are spread over more work), one row per wave on the layers
@@ -552,20 +597,92 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         P.xg = 8;
     else
         P.xg = maxrows >= 1024 ? 8 : maxrows >= 256 ? 4 : maxrows >= 64 ? 2 : 1;
+    // relaxation chunks: full staging chunks relax in place; the partial ones
+    // of instances on one graph are packed into bins (first fit by rows,
+    // largest first, a chunk's rows kept together); a bin of one chunk
+    // relaxes that chunk in place
+    fcur = 0;
+    for (int gsel = 0; gsel < 4; gsel++) {
+        StepGroup &SG = P.grp[gsel];
+        SG.rch0 = (int)P.rch.size();
+        SG.bin0 = (int)P.bins.size();
+        auto add_relax = [&](GChunk c) {
+            c.fbase = fcur;
+            fcur += c.nblk > 0 ? c.nblk : c.np;
+            P.rch.push_back(c);
+        };
+        std::map<int32_t, std::vector<int>> partial;  // graph (voff) -> chunks
+        for (int i = 0; i < SG.nch; i++) {
+            const GChunk &c = P.ch[SG.ch0 + i];
+            if (c.nx >= kWave || e->no_pack)
+                add_relax(c);
+            else
+                partial[c.voff].push_back(SG.ch0 + i);
+        }
+        for (auto &kv : partial) {
+            std::vector<int> &lst = kv.second;
+            std::stable_sort(lst.begin(), lst.end(),
+                             [&](int a, int b) { return P.ch[a].nx > P.ch[b].nx; });
+            std::vector<std::pair<int, std::vector<int>>> bins;  // rows used, chunks
+            for (int ci : lst) {
+                size_t j = 0;
+                while (j < bins.size() && bins[j].first + P.ch[ci].nx > kWave) j++;
+                if (j == bins.size()) bins.push_back({0, {}});
+                bins[j].first += P.ch[ci].nx;
+                bins[j].second.push_back(ci);
+            }
+            for (auto &bn : bins) {
+                if (bn.second.size() == 1) {
+                    add_relax(P.ch[bn.second[0]]);
+                    continue;
+                }
+                GChunk c = P.ch[bn.second[0]];
+                GBin B;
+                B.dbase = dcur;
+                B.np = c.np;
+                B.map0 = (int32_t)P.lanes.size();
+                for (int ci : bn.second)
+                    for (int j = 0; j < P.ch[ci].nx; j++)
+                        P.lanes.push_back(GLane{P.ch[ci].dbase, j, 0});
+                while ((int)P.lanes.size() < B.map0 + kWave) P.lanes.push_back(GLane{-1, 0, 0});
+                P.bins.push_back(B);
+                c.dbase = dcur;
+                c.nx = bn.first;
+                dcur += (int64_t)c.np * kWave;
+                add_relax(c);
+            }
+        }
+        SG.nrch = (int)P.rch.size() - SG.rch0;
+        SG.nbin = (int)P.bins.size() - SG.bin0;
+    }
     for (int gsel = 0; gsel < 4; gsel++) {
         StepGroup &SG = P.grp[gsel];
         SG.relax_pre.assign(1, 0);
         SG.unpack_pre.assign(1, 0);
+        for (int i = 0; i < SG.nrch; i++) {
+            const GChunk &c = P.rch[SG.rch0 + i];
+            SG.relax_pre.push_back(SG.relax_pre.back() + (group_blocks(gsel) ? c.nblk : c.n));
+        }
         for (int i = 0; i < SG.nch; i++) {
             const GChunk &c = P.ch[SG.ch0 + i];
-            SG.relax_pre.push_back(SG.relax_pre.back() + (group_blocks(gsel) ? c.nblk : c.n));
             SG.unpack_pre.push_back(SG.unpack_pre.back() + (c.n + kWave - 1) / kWave);
         }
+        SG.bin_pre.assign(1, 0);
+        for (int i = 0; i < SG.nbin; i++)
+            SG.bin_pre.push_back(SG.bin_pre.back() + P.bins[SG.bin0 + i].np / kWave);
         SG.merge_pre.assign(1, 0);
         for (int i = 0; i < SG.nmg; i++) {
             GMerge &M = P.mg[SG.mg0 + i];
             M.nsub = M.nchunks * (kWave / P.xg);
             SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
+        }
+        SG.split_pre.assign(1, 0);
+        SG.pack_pre.assign(1, 0);
+        for (int i = 0; i < SG.nms; i++) {
+            GMerge &M = P.mg[SG.ms0 + i];
+            M.nsub = M.nchunks * kWave;
+            SG.split_pre.push_back(SG.split_pre.back() + (int64_t)M.ntiles * M.nrows * M.S);
+            SG.pack_pre.push_back(SG.pack_pre.back() + (int64_t)M.ntiles * M.nchunks);
         }
         const int64_t nitems = group_blocks(gsel) ? 0 : SG.relax_pre.back();
         SG.relax_blk.assign((size_t)((nitems + kWave - 1) / kWave), 0);
@@ -594,11 +711,17 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     P.off_sp = place(sizeof(int64_t) * P.seed_pre.size());
     P.off_xs = place(sizeof(uint32_t) * P.xs.size());
     P.off_seedch = place(sizeof(GChunk) * P.seed_ch.size());
+    P.off_rch = place(sizeof(GChunk) * P.rch.size());
+    P.off_bins = place(sizeof(GBin) * P.bins.size());
+    P.off_lanes = place(sizeof(GLane) * P.lanes.size());
     for (StepGroup &SG : P.grp) {
         SG.off_rp = place(sizeof(int64_t) * SG.relax_pre.size());
         SG.off_mp = place(sizeof(int64_t) * SG.merge_pre.size());
         SG.off_up = place(sizeof(int64_t) * SG.unpack_pre.size());
         SG.off_rb = place(sizeof(int32_t) * SG.relax_blk.size());
+        SG.off_sp = place(sizeof(int64_t) * SG.split_pre.size());
+        SG.off_pp = place(sizeof(int64_t) * SG.pack_pre.size());
+        SG.off_bp = place(sizeof(int64_t) * SG.bin_pre.size());
     }
     P.blob.assign(P.bytes, 0);
     auto put = [&](size_t off, const void *src, size_t n) {
@@ -609,11 +732,17 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     put(P.off_sp, P.seed_pre.data(), sizeof(int64_t) * P.seed_pre.size());
     put(P.off_xs, P.xs.data(), sizeof(uint32_t) * P.xs.size());
     put(P.off_seedch, P.seed_ch.data(), sizeof(GChunk) * P.seed_ch.size());
+    put(P.off_rch, P.rch.data(), sizeof(GChunk) * P.rch.size());
+    put(P.off_bins, P.bins.data(), sizeof(GBin) * P.bins.size());
+    put(P.off_lanes, P.lanes.data(), sizeof(GLane) * P.lanes.size());
     for (StepGroup &SG : P.grp) {
         put(SG.off_rp, SG.relax_pre.data(), sizeof(int64_t) * SG.relax_pre.size());
         put(SG.off_mp, SG.merge_pre.data(), sizeof(int64_t) * SG.merge_pre.size());
         put(SG.off_up, SG.unpack_pre.data(), sizeof(int64_t) * SG.unpack_pre.size());
         put(SG.off_rb, SG.relax_blk.data(), sizeof(int32_t) * SG.relax_blk.size());
+        put(SG.off_sp, SG.split_pre.data(), sizeof(int64_t) * SG.split_pre.size());
+        put(SG.off_pp, SG.pack_pre.data(), sizeof(int64_t) * SG.pack_pre.size());
+        put(SG.off_bp, SG.bin_pre.data(), sizeof(int64_t) * SG.bin_pre.size());
     }
 }
 
@@ -639,12 +768,12 @@ void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
     const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
     // the worklist numbers items within one group: only for single-group steps
     int groups = 0;
-    for (const StepGroup &G : P.grp) groups += G.nch > 0 ? 1 : 0;
+    for (const StepGroup &G : P.grp) groups += G.nrch > 0 ? 1 : 0;
     const uint32_t qcap = groups > 1 ? 0u : e->qcap;
     hipLaunchKernelGGL((k_grelax<DT, DMAX>), dim3(grid), dim3(kBlock), 0, e->stream,
-                       (void *)e->d_D, (const GChunk *)(db + P.off_ch) + SG.ch0,
+                       (void *)e->d_D, (const GChunk *)(db + P.off_rch) + SG.rch0,
                        (const int64_t *)(db + SG.off_rp), (const int32_t *)(db + SG.off_rb),
-                       SG.nch, e->d_rp, e->d_adj, e->d_ell, dense, e->d_flag[r & 1],
+                       SG.nrch, e->d_rp, e->d_adj, e->d_ell, dense, e->d_flag[r & 1],
                        e->d_flag[(r + 1) & 1], (const uint32_t *)e->d_q[r & 1],
                        e->d_q[(r + 1) & 1], qcap, e->d_any, e->d_cnt, r);
 }
@@ -669,10 +798,10 @@ void launch_gblk_w(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
     const size_t bytes = blk_lds_bytes<DT>();
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, e->lds_cu / bytes));
     hipLaunchKernelGGL((k_gblk<DT, DW>), dim3(e->num_cu * per_cu), dim3(kBT), bytes, e->stream,
-                       (void *)e->d_D, (const GChunk *)(db + P.off_ch) + SG.ch0,
-                       (const int64_t *)(db + SG.off_rp), SG.nch, e->d_boff, e->d_bvert, e->d_brp,
-                       e->d_bext, e->d_badj, e->d_vblk, (const DW *)e->d_bdist, e->d_bdoff,
-                       P.ch[SG.ch0].fbase, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
+                       (void *)e->d_D, (const GChunk *)(db + P.off_rch) + SG.rch0,
+                       (const int64_t *)(db + SG.off_rp), SG.nrch, e->d_boff, e->d_bvert,
+                       e->d_brp, e->d_bext, e->d_badj, e->d_vblk, (const DW *)e->d_bdist,
+                       e->d_bdoff, P.rch[SG.rch0].fbase, dense, e->d_flag[r & 1], e->d_flag[(r + 1) & 1],
                        e->d_any, r);
 }
 
@@ -688,10 +817,10 @@ void launch_gblk(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG
 // One round over the step's groups (one launch each, sharing the round's
 // flags and "round has work" words).
 void grelax_round(emv_engine *e, char *db, const StepPlan &P, int dense, int r) {
-    if (P.grp[0].nch > 0) grelax_group<uint32_t>(e, db, P, P.grp[0], dense, r);
-    if (P.grp[1].nch > 0) grelax_group<uint16_t>(e, db, P, P.grp[1], dense, r);
-    if (P.grp[2].nch > 0) launch_gblk<uint32_t>(e, db, P, P.grp[2], dense, r);
-    if (P.grp[3].nch > 0) launch_gblk<uint16_t>(e, db, P, P.grp[3], dense, r);
+    if (P.grp[0].nrch > 0) grelax_group<uint32_t>(e, db, P, P.grp[0], dense, r);
+    if (P.grp[1].nrch > 0) grelax_group<uint16_t>(e, db, P, P.grp[1], dense, r);
+    if (P.grp[2].nrch > 0) launch_gblk<uint32_t>(e, db, P, P.grp[2], dense, r);
+    if (P.grp[3].nrch > 0) launch_gblk<uint16_t>(e, db, P, P.grp[3], dense, r);
 }
 
 template <typename P_, int XG, typename DT>
@@ -718,13 +847,56 @@ void launch_gmerge_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup
     }
 }
 
+template <typename P_, typename DT>
+void launch_gsplit_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
+    const int64_t items = SG.split_pre.back();
+    if (items == 0) return;
+    const int64_t blocks = (items + (kBlock / kWave) - 1) / (kBlock / kWave);
+    const int nxcd = (e->num_cu % 32 == 0 && e->num_cu >= 64) ? e->num_cu / 32 : 1;
+    const int64_t per = (blocks + nxcd - 1) / nxcd;
+    const GMerge *mg = (const GMerge *)(db + P.off_mg) + SG.ms0;
+    hipLaunchKernelGGL((k_gmerge_split<P_>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
+                       e->stream, e->d_arena, mg, (const int64_t *)(db + SG.off_sp), SG.nms,
+                       (const uint32_t *)(db + P.off_xs), (P_ *)e->d_macc, nxcd, per);
+    hipLaunchKernelGGL((k_gpack<P_, DT>), dim3((unsigned)SG.pack_pre.back()), dim3(kBlock), 0,
+                       e->stream, (const P_ *)e->d_macc, mg, (const int64_t *)(db + SG.off_pp),
+                       SG.nms, (void *)e->d_D, e->d_code);
+}
+
 template <typename DT>
 void launch_gmerge_group(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
-    if (SG.nmg == 0) return;
-    if (P.pbits == 64)
-        launch_gmerge_p<uint64_t, DT>(e, db, P, SG);
-    else
-        launch_gmerge_p<uint32_t, DT>(e, db, P, SG);
+    if (SG.nmg > 0) {
+        if (P.pbits == 64)
+            launch_gmerge_p<uint64_t, DT>(e, db, P, SG);
+        else
+            launch_gmerge_p<uint32_t, DT>(e, db, P, SG);
+    }
+    if (SG.nms > 0) {
+        if (P.pbits == 64)
+            launch_gsplit_p<uint64_t, DT>(e, db, P, SG);
+        else
+            launch_gsplit_p<uint32_t, DT>(e, db, P, SG);
+    }
+}
+
+// Rows of partial chunks into their bins (in) or back (out).
+template <typename DT, bool IN>
+void launch_glanes(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
+    if (SG.nbin == 0) return;
+    hipLaunchKernelGGL((k_glanes<DT, IN>), dim3((unsigned)SG.bin_pre.back()), dim3(kBlock), 0,
+                       e->stream, (void *)e->d_D, (const GBin *)(db + P.off_bins) + SG.bin0,
+                       (const int64_t *)(db + SG.off_bp), SG.nbin,
+                       (const GLane *)(db + P.off_lanes));
+}
+
+template <bool IN>
+void glanes_all(emv_engine *e, char *db, const StepPlan &P) {
+    for (int gsel = 0; gsel < 4; gsel++) {
+        if (gsel & 1)
+            launch_glanes<uint16_t, IN>(e, db, P, P.grp[gsel]);
+        else
+            launch_glanes<uint32_t, IN>(e, db, P, P.grp[gsel]);
+    }
 }
 
 // Launches of one step up to (not including) the relaxation rounds.
@@ -736,12 +908,20 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
                            (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
     }
+    if (P.macc_cells > 0) {
+        // the split accumulators start at all ones (a fill kernel: graph-safe)
+        const int64_t words = P.macc_cells * (P.pbits / 32);
+        hipLaunchKernelGGL(k_fill<uint32_t>,
+                           dim3((unsigned)std::min<int64_t>((words + kBlock - 1) / kBlock, 1 << 20)),
+                           dim3(kBlock), 0, e->stream, (uint32_t *)e->d_macc, words, ~0u);
+    }
     for (int gsel = 0; gsel < 4; gsel++) {
         if (gsel & 1)
             launch_gmerge_group<uint16_t>(e, db, P, P.grp[gsel]);
         else
             launch_gmerge_group<uint32_t>(e, db, P, P.grp[gsel]);
     }
+    glanes_all<true>(e, db, P);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_any, (int64_t)4,
                        0u);
     hipLaunchKernelGGL(k_fill<uint32_t>, dim3(1), dim3(64), 0, e->stream, e->d_cnt, (int64_t)4,
@@ -763,6 +943,7 @@ void launch_gunpack(emv_engine *e, char *db, const StepPlan &P, const StepGroup 
 }
 
 int enqueue_back(emv_engine *e, char *db, const StepPlan &P) {
+    glanes_all<false>(e, db, P);
     for (int gsel = 0; gsel < 4; gsel++) {
         if (gsel & 1)
             launch_gunpack<uint16_t>(e, db, P, P.grp[gsel]);
@@ -966,6 +1147,7 @@ int step_eager(emv_engine *e, int *rounds_used) {
     StepPlan P;
     build_plan(e, run, P);
     ECHK(ensure_stage(e, P.cells, P.flagbytes));
+    ECHK(ensure_macc(e, P.macc_cells * (P.pbits / 8)));
     char *db = nullptr;
     ECHK(desc_buffer(e, 0, P.bytes, &db));
     EHIP(e, hipMemcpyAsync(db, P.blob.data(), P.bytes, hipMemcpyHostToDevice, e->stream));
@@ -977,9 +1159,10 @@ int step_eager(emv_engine *e, int *rounds_used) {
         int64_t items[4];
         for (int g = 0; g < 4; g++) items[g] = P.grp[g].relax_pre.back();
         fprintf(stderr,
-                "emv-trace: step %lld instances=%zu chunks=%zu rounds=%d items u32/u16 vertex "
-                "%lld/%lld block %lld/%lld\n",
-                (long long)e->steps, P.insts.size(), P.ch.size(), *rounds_used,
+                "emv-trace: step %lld instances=%zu chunks=%zu relax_chunks=%zu bins=%zu "
+                "rounds=%d items u32/u16 vertex %lld/%lld block %lld/%lld\n",
+                (long long)e->steps, P.insts.size(), P.ch.size(), P.rch.size(), P.bins.size(),
+                *rounds_used,
                 (long long)items[0], (long long)items[1], (long long)items[2],
                 (long long)items[3]);
     }
@@ -1004,6 +1187,8 @@ int engine_open(int device, emv_engine **out) {
     EHIP(e, hipMalloc((void **)&e->d_any, sizeof(uint32_t) * 4));
     EHIP(e, hipMalloc((void **)&e->d_cnt, sizeof(uint32_t) * 4));
     if (const char *qc = getenv("EMV_QCAP")) e->qcap = (uint32_t)std::max(0, atoi(qc));
+    e->no_split = getenv("EMV_NO_SPLIT") != nullptr;
+    e->no_pack = getenv("EMV_NO_PACK") != nullptr;
     EHIP(e, hipMalloc((void **)&e->d_q[0], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipMalloc((void **)&e->d_q[1], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipHostMalloc((void **)&e->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault));
@@ -1064,6 +1249,7 @@ extern "C" void emv_engine_destroy(emv_engine *e) {
     dfree(e->d_code);
     dfree(e->d_flag[0]);
     dfree(e->d_flag[1]);
+    dfree(e->d_macc);
     dfree(e->d_any);
     dfree(e->d_cnt);
     dfree(e->d_q[0]);
@@ -1217,7 +1403,7 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
     // rounds per step from the last eager solve (+ margin)
     if ((int)e->rounds_per_step.size() < kmax) return EMV_OK;
     plans.resize(runs.size());
-    int64_t max_cells = 0, max_flags = 0;
+    int64_t max_cells = 0, max_flags = 0, max_macc = 0;
     // build with the stage each instance will have at that step
     std::vector<int> saved_stage(e->inst.size());
     for (size_t t = 0; t < e->inst.size(); t++) saved_stage[t] = e->inst[t].stage;
@@ -1226,9 +1412,11 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
         build_plan(e, runs[s], plans[s]);
         max_cells = std::max(max_cells, plans[s].cells);
         max_flags = std::max(max_flags, plans[s].flagbytes);
+        max_macc = std::max(max_macc, plans[s].macc_cells * (plans[s].pbits / 8));
     }
     for (size_t t = 0; t < e->inst.size(); t++) e->inst[t].stage = saved_stage[t];
     ECHK(ensure_stage(e, max_cells, max_flags));
+    ECHK(ensure_macc(e, max_macc));
     std::vector<char *> dbs(plans.size());
     for (size_t s = 0; s < plans.size(); s++) {
         ECHK(desc_buffer(e, 1 + s, plans[s].bytes, &dbs[s]));

@@ -405,7 +405,6 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
 
 // Split merge for layers whose rows have >= kSplitPairs pairs: fill the
 // packed accumulators, fold the parts, pack into the staging.
-constexpr uint32_t kSplitPairs = 128;
 
 template <typename T, typename P, typename DT>
 int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {

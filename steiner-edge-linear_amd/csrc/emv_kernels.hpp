@@ -931,6 +931,9 @@ __device__ __forceinline__ P merge_outer(const T *__restrict__ fv, int64_t np, u
     return acc;
 }
 
+// Rows with at least this many pairs (the upper layers) take the split merge.
+constexpr uint32_t kSplitPairs = 128;
+
 // Number of outer steps of X in merge_outer.
 __device__ __forceinline__ uint32_t merge_steps(uint32_t X) {
     const uint32_t Xp = X ^ (1u << (31 - __clz(X)));

@@ -208,3 +208,54 @@ def test_batch_grids_and_regular_graphs_in_one_step(monkeypatch, closure):
         c2, _ = b.solve(emv.KEEP_TABLE | emv.WIDE_LABELS)
         np.testing.assert_array_equal(c1, c2)
         _check_batch_vs_oracle(insts, c2, b)
+
+
+@pytest.mark.parametrize("no_split", [False, True])
+def test_batch_split_merge_upper_layers(monkeypatch, no_split):
+    """Instances with k >= 9 reach layers whose rows have >= 128 pairs: the
+    engine cuts those rows into parts folded with atomicMin (16/32-bit labels,
+    32- and 64-bit packed words: the last instance's bound needs 64) or, with
+    EMV_NO_SPLIT, merges them whole.  Tables and records equal the oracle's."""
+    if no_split:
+        monkeypatch.setenv("EMV_NO_SPLIT", "1")
+    insts = []
+    for i, (n, d, k, w) in enumerate([(300, 4, 9, 1000), (260, 3, 10, 1000), (200, 5, 11, 100),
+                                      (240, 4, 10, 800000)]):
+        e = emv.regular_graph(n, d, w, 950 + i)
+        insts.append((n, e, emv.terminals(n, k, 950 + i)))
+    with emv.Batch(insts) as b:
+        c1, _ = b.solve(emv.KEEP_TABLE)
+        _check_batch_vs_oracle(insts, c1, b)
+        c2, s2 = b.solve(emv.KEEP_TABLE)
+        assert s2.graph_used == 1 or s2.graph_fallback == 1
+        _check_batch_vs_oracle(insts, c2, b)
+
+
+@pytest.mark.parametrize("no_pack", [False, True])
+@pytest.mark.parametrize("shape", ["regular", "grid"])
+def test_server_packed_relaxation(monkeypatch, shape, no_pack):
+    """Requests on one resident graph whose layers leave chunks partly
+    filled share packed relaxation chunks (rows copied into a bin before the
+    rounds and back after), vertex-wise (random graph) or block-local (grid);
+    EMV_NO_PACK relaxes every chunk in place.  Tables and records equal the
+    oracle's either way."""
+    if no_pack:
+        monkeypatch.setenv("EMV_NO_PACK", "1")
+    if shape == "grid":
+        n = 45 * 40
+        e = emv.grid_graph(45, 40, 100, 71)
+    else:
+        n = 600
+        e = emv.regular_graph(n, 5, 1000, 71)
+    qs = [emv.distinct(n, k, 7100 + i, 2) for i, k in enumerate([3, 4, 5, 6, 7, 4, 8, 5])]
+    with emv.Server(n, e, arena_bytes=256 << 20, max_inflight=16) as s:
+        tickets = [s.submit(q, emv.KEEP_TABLE) for q in qs]
+        while any(s.poll(t).state != 2 for t in tickets):
+            s.step()
+        for t, q in zip(tickets, qs):
+            info = s.poll(t)
+            ocost, of, ocode = oracle.emv(n, e, q)
+            assert info.status == 0 and info.cost == ocost
+            f, code = s.read_rows(t, 0, 1 << len(q), n)
+            np.testing.assert_array_equal(f, of)
+            np.testing.assert_array_equal(code, ocode)
