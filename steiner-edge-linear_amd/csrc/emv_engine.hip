@@ -185,6 +185,13 @@ struct emv_engine {
     // per-step descriptor buffer (eager steps reuse one; graph steps own one each)
     std::vector<void *> desc_bufs;
     std::vector<int64_t> desc_caps;
+    // the last batch replay, kept while its key (descriptors, rounds, buffers,
+    // launch shape) repeats, so that a repeated solve relaunches it
+    hipGraph_t ggraph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<char> gkey;
+    uint32_t *d_conv = nullptr;  // per replayed step: 1 if it ran out of rounds
+    size_t conv_cap = 0;
     // tunables
     emv_engine_params prm;
     // counters
@@ -508,6 +515,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             const GraphRec &G = e->graphs[I.graph];
             P.insts.push_back(t);
             GChunk c;
+            memset(&c, 0, sizeof(c));  // padding too: descriptors are compared bytewise
             c.tbase = I.abase;
             c.n = (int32_t)G.n;
             c.np = (int32_t)G.np;
@@ -541,6 +549,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             }
             if (I.pbits == 64) P.pbits = 64;
             GMerge M;
+            memset(&M, 0, sizeof(M));
             M.chunk0 = (int)P.ch.size() - SG.ch0;
             M.nchunks = (nrows + kWave - 1) / kWave;
             M.ntiles = (int32_t)(G.np / kWave);
@@ -638,6 +647,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 }
                 GChunk c = P.ch[bn.second[0]];
                 GBin B;
+                memset(&B, 0, sizeof(B));
                 B.dbase = dcur;
                 B.np = c.np;
                 B.map0 = (int32_t)P.lanes.size();
@@ -1254,6 +1264,9 @@ extern "C" void emv_engine_destroy(emv_engine *e) {
     dfree(e->d_cnt);
     dfree(e->d_q[0]);
     dfree(e->d_q[1]);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->ggraph) (void)hipGraphDestroy(e->ggraph);
+    dfree(e->d_conv);
     for (void *p : e->desc_bufs)
         if (p) (void)hipFree(p);
     if (e->h_pin) (void)hipHostFree(e->h_pin);
@@ -1423,45 +1436,74 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
         EHIP(e, hipMemcpyAsync(dbs[s], plans[s].blob.data(), plans[s].bytes,
                                hipMemcpyHostToDevice, e->stream));
     }
-    uint32_t *d_conv = nullptr;
-    EHIP(e, hipMalloc((void **)&d_conv, sizeof(uint32_t) * std::max<size_t>(plans.size(), 1)));
+    if (e->conv_cap < plans.size()) {
+        dfree(e->d_conv);
+        e->conv_cap = 0;
+        EHIP(e, hipMalloc((void **)&e->d_conv, sizeof(uint32_t) * std::max<size_t>(plans.size(), 1)));
+        e->conv_cap = plans.size();
+    }
+    uint32_t *d_conv = e->d_conv;
     EHIP(e, hipStreamSynchronize(e->stream));
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int rc = EMV_OK;
-    if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipFree(d_conv);
-        eset(e, "hipStreamBeginCapture failed");
-        return EMV_ERR_HIP;
-    }
-    for (size_t s = 0; s < plans.size() && rc == EMV_OK; s++) {
-        rc = enqueue_front(e, dbs[s], plans[s]);
+    // the replay's key: everything its nodes depend on
+    std::vector<char> key;
+    auto add_key = [&](const void *p, size_t n) {
+        key.insert(key.end(), (const char *)p, (const char *)p + n);
+    };
+    const void *bufs[] = {e->d_D,    e->d_code, e->d_flag[0], e->d_flag[1], e->d_macc,
+                          e->d_arena, e->d_any, e->d_cnt,     d_conv};
+    add_key(bufs, sizeof(bufs));
+    add_key(&e->prm, sizeof(e->prm));
+    for (size_t s = 0; s < plans.size(); s++) {
         const int R = std::max(1, e->rounds_per_step[s + 1] + e->prm.graph_round_margin);
-        grelax_round(e, dbs[s], plans[s], 1, 0);
-        for (int r = 1; r < R; r++) grelax_round(e, dbs[s], plans[s], 0, r);
-        hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, e->stream, e->d_any, R, d_conv,
-                           (int)s);
-        if (rc == EMV_OK) rc = enqueue_back(e, dbs[s], plans[s]);
+        add_key(&R, sizeof(R));
+        add_key(&dbs[s], sizeof(dbs[s]));
+        add_key(plans[s].blob.data(), plans[s].blob.size());
     }
-    hipError_t ce = hipStreamEndCapture(e->stream, &graph);
-    if (rc != EMV_OK || ce != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipFree(d_conv);
-        if (rc == EMV_OK) {
-            eset(e, "graph capture failed: %s", hipGetErrorString(ce));
-            rc = EMV_ERR_HIP;
+    if (!(e->gexec && key == e->gkey)) {
+        if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+        if (e->ggraph) (void)hipGraphDestroy(e->ggraph);
+        e->gexec = nullptr;
+        e->ggraph = nullptr;
+        e->gkey.clear();
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            eset(e, "hipStreamBeginCapture failed");
+            return EMV_ERR_HIP;
         }
-        return rc;
+        int rc = EMV_OK;
+        for (size_t s = 0; s < plans.size() && rc == EMV_OK; s++) {
+            rc = enqueue_front(e, dbs[s], plans[s]);
+            const int R = std::max(1, e->rounds_per_step[s + 1] + e->prm.graph_round_margin);
+            grelax_round(e, dbs[s], plans[s], 1, 0);
+            for (int r = 1; r < R; r++) grelax_round(e, dbs[s], plans[s], 0, r);
+            hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, e->stream, e->d_any, R, d_conv,
+                               (int)s);
+            if (rc == EMV_OK) rc = enqueue_back(e, dbs[s], plans[s]);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+        if (rc != EMV_OK || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (rc == EMV_OK) {
+                eset(e, "graph capture failed: %s", hipGetErrorString(ce));
+                rc = EMV_ERR_HIP;
+            }
+            return rc;
+        }
+        hipGraphExec_t exec = nullptr;
+        if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            eset(e, "hipGraphInstantiate failed");
+            return EMV_ERR_HIP;
+        }
+        e->ggraph = graph;
+        e->gexec = exec;
+        e->gkey.swap(key);
     }
-    hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (ie == hipSuccess) ie = hipGraphLaunch(exec, e->stream);
+    hipError_t ie = hipGraphLaunch(e->gexec, e->stream);
     if (ie == hipSuccess) ie = hipStreamSynchronize(e->stream);
     std::vector<uint32_t> conv(plans.size(), 1);
     if (ie == hipSuccess)
         ie = hipMemcpy(conv.data(), d_conv, sizeof(uint32_t) * plans.size(), hipMemcpyDeviceToHost);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    (void)hipFree(d_conv);
     if (ie != hipSuccess) {
         eset(e, "graph replay failed: %s", hipGetErrorString(ie));
         return EMV_ERR_HIP;
