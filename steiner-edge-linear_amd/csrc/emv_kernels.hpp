@@ -646,6 +646,56 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
             if (low == 0) atomicAdd(&stats[6], 1ull);
             atomicAdd(&stats[7], low);
         }
+    } else if (dense) {
+        // 2''. the first visit: every wave relaxes all its slots over their
+        //      block neighbours, in place, until an iteration lowers nothing
+        //      anywhere in the block (a barrier per iteration; labels only
+        //      drop, so at most cnt iterations).  No worklists: a visit of
+        //      a round that starts every slot dirty is cheaper this way.
+        for (int it = 0;; it++) {
+            if (tid == 0) s_flag[(it + 1) % 3] = 0;
+            bool any_low = false;
+#pragma unroll
+            for (int t = 0; t < kBQ; t++) {
+                if (t >= nv) break;
+                const int sl0 = t * kBW + wid;
+                const DT x0 = L[sl0 * kWave + lane];
+                DT nb[kBD];
+#pragma unroll
+                for (int q = 0; q < kBD; q++) {
+                    const int sl = __builtin_amdgcn_readlane(ic[q], t);
+                    nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                }
+                DT best = x0;
+#pragma unroll
+                for (int q = 0; q < kBD; q++)
+                    best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
+                                     __builtin_amdgcn_readlane(iw[q], t), INF, best);
+                if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
+                    const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
+                    const int eb = __builtin_amdgcn_readlane(em, t);
+                    for (int e = ea; e < eb; e++) {
+                        const Edge ed = badj[e];
+                        best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                    }
+                }
+                const bool lower = best < x0;
+                if (lower) L[sl0 * kWave + lane] = best;
+                if (__ballot(lower) != 0) {
+                    chg_mask |= 1ull << t;
+                    any_low = true;
+                }
+            }
+            if (any_low && lane == 0) s_flag[it % 3] = 1;
+            __syncthreads();
+            if (s_flag[it % 3] == 0) {
+                if (stats && tid == 0) {
+                    atomicAdd(&stats[0], 1ull);
+                    atomicAdd(&stats[1], (unsigned long long)(it + 1));
+                }
+                break;
+            }
+        }
     } else {
         // 2. sweeps.  Each wave drains a worklist of its own slots (all of
         //    them on a dense round, else the block neighbours of the slots
