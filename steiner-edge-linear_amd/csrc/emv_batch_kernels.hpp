@@ -70,6 +70,19 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ pre, int nse
     return lo;
 }
 
+// The prefix array copied into LDS by the whole workgroup (when it has at
+// most kSegCache entries), so that each wave's search costs LDS reads instead
+// of dependent global loads.  Every thread of the workgroup must call it.
+constexpr int kSegCache = 512;
+
+__device__ __forceinline__ const int64_t *cache_pre(const int64_t *__restrict__ pre, int nseg,
+                                                    int64_t *sh) {
+    if (nseg + 1 > kSegCache) return pre;
+    for (int i = threadIdx.x; i <= nseg; i += blockDim.x) sh[i] = pre[i];
+    __syncthreads();
+    return sh;
+}
+
 // ------------------------------------------------------ singleton seeding
 // Every cell of every singleton chunk: INF label; then label 0 at the sources.
 __global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict__ cell_pre,
@@ -103,11 +116,13 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     // items dealt to XCDs in contiguous ranges: the waves of an XCD sweep the
     // rows of one tile together, and its lower-layer rows stay in that L2
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    __shared__ int64_t sh_pre[kSegCache];
+    const int64_t *pre = cache_pre(item_pre, nmg, sh_pre);
     const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
-    if (item >= item_pre[nmg]) return;
-    const int s = find_seg(item_pre, nmg, item);
+    if (item >= pre[nmg]) return;
+    const int s = find_seg(pre, nmg, item);
     const GMerge M = mg[s];
-    const int64_t local = item - item_pre[s];
+    const int64_t local = item - pre[s];
     const int64_t tile = local / M.nsub;
     const int sub = (int)(local - tile * M.nsub);
     const int64_t v = tile * kWave + lane;
@@ -151,11 +166,13 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_split(
     P *__restrict__ Macc, int nxcd, int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    __shared__ int64_t sh_pre[kSegCache];
+    const int64_t *pre = cache_pre(item_pre, nmg, sh_pre);
     const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
-    if (item >= item_pre[nmg]) return;
-    const int s = find_seg(item_pre, nmg, item);
+    if (item >= pre[nmg]) return;
+    const int s = find_seg(pre, nmg, item);
     const GMerge M = mg[s];
-    const int64_t local = item - item_pre[s];
+    const int64_t local = item - pre[s];
     const int64_t per_tile = (int64_t)M.nrows * M.S;
     const int64_t tile = local / per_tile;
     const int rem = (int)(local - tile * per_tile);
