@@ -551,18 +551,30 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
     const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
     const DT *Dc = chunk_labels<DT>(const_cast<void *>(D), c.dbase);
     const bool real = lane < c.nx;
+    // the labels and packed words of kPre vertices are loaded together, then
+    // their records decided one vertex at a time
+    constexpr int kPre = 4;
 #pragma unroll 1
-    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
-        const int vv = grp + r * (kBlock / kWave);
-        const int64_t v = v0 + vv;
-        const int64_t idx = v * kWave + lane;
-        const DT L = Dc[idx];
-        int32_t code = 0;
-        if (c.kind == 0 && v < c.n)
-            code = cell_record<DT, uint64_t>(Dc, L, CODE[c.dbase + idx], c.k, real,
-                                             c.voff + (int)v, rp, adj);
-        sd[vv][lane] = (L == (DT)~(DT)0) ? ~0u : (uint32_t)L;
-        sc[vv][lane] = code;
+    for (int r0 = 0; r0 < kWave / (kBlock / kWave); r0 += kPre) {
+        DT Ls[kPre];
+        uint64_t Ps[kPre];
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int64_t idx = (v0 + grp + (r0 + u) * (kBlock / kWave)) * kWave + lane;
+            Ls[u] = Dc[idx];
+            Ps[u] = c.kind == 0 ? CODE[c.dbase + idx] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int vv = grp + (r0 + u) * (kBlock / kWave);
+            const int64_t v = v0 + vv;
+            int32_t code = 0;
+            if (c.kind == 0 && v < c.n)
+                code = cell_record<DT, uint64_t>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v,
+                                                 rp, adj);
+            sd[vv][lane] = (Ls[u] == (DT)~(DT)0) ? ~0u : (uint32_t)Ls[u];
+            sc[vv][lane] = code;
+        }
     }
     __syncthreads();
     const int64_t S2 = 1ll << c.k;

@@ -1171,16 +1171,29 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
     const int64_t v0 = vt * kWave;
     const DT *Dc = D + chunk * np * kWave;
     const bool real = chunk * kWave + lane < nx;
+    // the labels and packed words of kPre vertices are loaded together, then
+    // their records decided one vertex at a time
+    constexpr int kPre = 4;
 #pragma unroll 1
-    for (int r = 0; r < kWave / (kBlock / kWave); r++) {
-        const int vv = grp + r * (kBlock / kWave);
-        const int64_t v = v0 + vv;
-        const int64_t idx = ((chunk * np) + v) * kWave + lane;
-        const DT L = D[idx];
-        int32_t code = 0;
-        if (v < n) code = cell_record<DT, P>(Dc, L, CODE[idx], k, real, (int)v, rp, adj);
-        sd[vv][lane] = L;
-        sc[vv][lane] = code;
+    for (int r0 = 0; r0 < kWave / (kBlock / kWave); r0 += kPre) {
+        DT Ls[kPre];
+        P Ps[kPre];
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int64_t idx =
+                ((chunk * np) + v0 + grp + (r0 + u) * (kBlock / kWave)) * kWave + lane;
+            Ls[u] = D[idx];
+            Ps[u] = CODE[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int vv = grp + (r0 + u) * (kBlock / kWave);
+            const int64_t v = v0 + vv;
+            int32_t code = 0;
+            if (v < n) code = cell_record<DT, P>(Dc, Ls[u], Ps[u], k, real, (int)v, rp, adj);
+            sd[vv][lane] = Ls[u];
+            sc[vv][lane] = code;
+        }
     }
     __syncthreads();
 #pragma unroll
