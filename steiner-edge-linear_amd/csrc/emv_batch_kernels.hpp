@@ -11,8 +11,8 @@
 // b[tbase + 2^k * np + X * np + v] (int32).
 // Staging chunks keep the vertex-major layout, labels of type DT (16 or 32
 // bits, per chunk group) at ((DT *)(D + 4 * dbase))[v * 64 + lane] -- dbase
-// counts 4-byte cells -- and the merge's packed word (sum << k) | Y (64-bit)
-// in CODE[dbase + v * 64 + lane].  Work items are located through prefix arrays
+// counts 4-byte cells -- and the merge's packed word (sum << k) | Y in
+// CODE[dbase + v * 64 + lane], 32 or 64 bits wide (the step's P type).  Work items are located through prefix arrays
 // over the descriptors (binary search), so instances of any size share one
 // grid.
 #pragma once
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
                                                    const int64_t *__restrict__ item_pre, int nmg,
                                                    const uint32_t *__restrict__ xs,
                                                    void *__restrict__ D,
-                                                   uint64_t *__restrict__ CODE, int nxcd,
+                                                   P *__restrict__ CODE, int nxcd,
                                                    int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
     // items dealt to XCDs in contiguous ranges: the waves of an XCD sweep the
@@ -130,23 +130,23 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     const int k = M.k;
     const int nrows = M.nrows;
     DT dv[XG];
-    uint64_t pv[XG];
+    P pv[XG];
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         const int xi = sub * XG + jx;
         dv[jx] = (DT)~(DT)0;
-        pv[jx] = ~0ull;
+        pv[jx] = ~(P)0;
         if (xi < nrows) {
             const P acc = merge_one<int32_t, P>(fv, (int64_t)M.np, xs[M.x0 + xi], k);
             dv[jx] = (DT)(acc >> k);
-            pv[jx] = (uint64_t)acc;
+            pv[jx] = acc;
         }
     }
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
     const int64_t dbase = M.dbase0 + (int64_t)chunk * M.np * kWave;
     DT *dd = chunk_labels<DT>(D, dbase) + v * kWave + loff;
-    uint64_t *cd = CODE + dbase + v * kWave + loff;
+    P *cd = CODE + dbase + v * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_gpack(const P *__restrict__ Macc,
                                                   const GMerge *__restrict__ mg,
                                                   const int64_t *__restrict__ blk_pre, int nmg,
                                                   void *__restrict__ D,
-                                                  uint64_t *__restrict__ CODE) {
+                                                  P *__restrict__ CODE) {
     __shared__ P sm[kWave][kWave + 1];
     const int64_t b = blockIdx.x;
     if (b >= blk_pre[nmg]) return;
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_gpack(const P *__restrict__ Macc,
         const P a = sm[lane][vv];
         const int64_t idx = (v0 + vv) * kWave + lane;
         Dd[idx] = real ? (DT)(a >> M.k) : (DT)~(DT)0;
-        CODE[dbase + idx] = real ? (uint64_t)a : ~0ull;
+        CODE[dbase + idx] = real ? a : ~(P)0;
     }
 }
 
@@ -531,15 +531,15 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<D
 // Singleton chunks write f[1<<t][v] and the record (kk[t], {t})
 // (reader-el.c:2015-2027), and fold the per-row maximum into
 // rowmax[inst * 64 + t].
-template <typename DT>
+template <typename DT, typename P>
 __global__ __launch_bounds__(kBlock) void k_gunpack(
-    const void *__restrict__ D, const uint64_t *__restrict__ CODE,
+    const void *__restrict__ D, const P *__restrict__ CODE,
     const GChunk *__restrict__ ch, const int64_t *__restrict__ blk_pre, int nch,
     const uint32_t *__restrict__ xs, const int32_t *__restrict__ kk_all,
     const int32_t *__restrict__ kk_off, const int32_t *__restrict__ rp,
     const Edge *__restrict__ adj, int32_t *__restrict__ f, int32_t *__restrict__ b,
     unsigned long long *__restrict__ rowmax) {
-    __shared__ uint32_t sd[kWave][kWave + 1];
+    __shared__ DT sd[kWave][kWave + 1];  // label type: 16-bit labels halve this LDS
     __shared__ int32_t sc[kWave][kWave + 1];
     const int64_t item = blockIdx.x;
     if (item >= blk_pre[nch]) return;
@@ -557,12 +557,12 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
 #pragma unroll 1
     for (int r0 = 0; r0 < kWave / (kBlock / kWave); r0 += kPre) {
         DT Ls[kPre];
-        uint64_t Ps[kPre];
+        P Ps[kPre];
 #pragma unroll
         for (int u = 0; u < kPre; u++) {
             const int64_t idx = (v0 + grp + (r0 + u) * (kBlock / kWave)) * kWave + lane;
             Ls[u] = Dc[idx];
-            Ps[u] = c.kind == 0 ? CODE[c.dbase + idx] : 0ull;
+            Ps[u] = c.kind == 0 ? CODE[c.dbase + idx] : (P)0;
         }
 #pragma unroll
         for (int u = 0; u < kPre; u++) {
@@ -570,9 +570,8 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
             const int64_t v = v0 + vv;
             int32_t code = 0;
             if (c.kind == 0 && v < c.n)
-                code = cell_record<DT, uint64_t>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v,
-                                                 rp, adj);
-            sd[vv][lane] = (Ls[u] == (DT)~(DT)0) ? ~0u : (uint32_t)Ls[u];
+                code = cell_record<DT, P>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v, rp, adj);
+            sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }
     }
@@ -585,7 +584,8 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
         const int64_t v = v0 + lane;
         unsigned long long dmax = 0;
         if (xl < c.nx && v < c.n) {
-            const uint32_t d = sd[lane][xl];
+            const DT dl = sd[lane][xl];
+            const uint32_t d = (dl == (DT)~(DT)0) ? ~0u : (uint32_t)dl;
             if (c.kind == 0) {
                 const int64_t X = xs[c.x0 + xl];
                 f[c.tbase + X * c.np + v] = (int32_t)d;

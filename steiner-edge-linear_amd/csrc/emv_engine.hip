@@ -169,7 +169,7 @@ struct emv_engine {
     unsigned long long *d_rowmax = nullptr;  // max_slots * 64
     // staging
     uint32_t *d_D = nullptr;
-    uint64_t *d_code = nullptr;  // merge's packed words
+    uint64_t *d_code = nullptr;  // merge's packed words (a step with 32-bit words uses half)
     int64_t stage_cells = 0;
     uint32_t *d_flag[2] = {nullptr, nullptr};
     int64_t flag_bytes = 0;  // flag words held per array
@@ -843,7 +843,7 @@ void launch_gmerge_xg(emv_engine *e, char *db, const StepPlan &P, const StepGrou
     hipLaunchKernelGGL((k_gmerge<P_, XG, DT>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
                        e->stream, e->d_arena, (const GMerge *)(db + P.off_mg) + SG.mg0,
                        (const int64_t *)(db + SG.off_mp),
-                       SG.nmg, (const uint32_t *)(db + P.off_xs), (void *)e->d_D, e->d_code, nxcd,
+                       SG.nmg, (const uint32_t *)(db + P.off_xs), (void *)e->d_D, (P_ *)e->d_code, nxcd,
                        per);
 }
 
@@ -870,7 +870,7 @@ void launch_gsplit_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup
                        (const uint32_t *)(db + P.off_xs), (P_ *)e->d_macc, nxcd, per);
     hipLaunchKernelGGL((k_gpack<P_, DT>), dim3((unsigned)SG.pack_pre.back()), dim3(kBlock), 0,
                        e->stream, (const P_ *)e->d_macc, mg, (const int64_t *)(db + SG.off_pp),
-                       SG.nms, (void *)e->d_D, e->d_code);
+                       SG.nms, (void *)e->d_D, (P_ *)e->d_code);
 }
 
 template <typename DT>
@@ -943,9 +943,17 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
 template <typename DT>
 void launch_gunpack(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
     const int64_t blocks = SG.unpack_pre.back();
-    if (blocks > 0)
-        hipLaunchKernelGGL(k_gunpack<DT>, dim3((unsigned)blocks), dim3(kBlock), 0, e->stream,
-                           (const void *)e->d_D, e->d_code,
+    if (blocks == 0) return;
+    if (P.pbits == 64)
+        hipLaunchKernelGGL((k_gunpack<DT, uint64_t>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           e->stream, (const void *)e->d_D, (const uint64_t *)e->d_code,
+                           (const GChunk *)(db + P.off_ch) + SG.ch0,
+                           (const int64_t *)(db + SG.off_up), SG.nch,
+                           (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_rp,
+                           e->d_adj, e->d_arena, e->d_arena, e->d_rowmax);
+    else
+        hipLaunchKernelGGL((k_gunpack<DT, uint32_t>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           e->stream, (const void *)e->d_D, (const uint32_t *)e->d_code,
                            (const GChunk *)(db + P.off_ch) + SG.ch0,
                            (const int64_t *)(db + SG.off_up), SG.nch,
                            (const uint32_t *)(db + P.off_xs), e->d_kk, e->d_kkoff, e->d_rp,
