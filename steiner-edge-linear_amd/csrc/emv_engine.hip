@@ -1169,18 +1169,31 @@ int step_eager(emv_engine *e, int *rounds_used) {
     char *db = nullptr;
     ECHK(desc_buffer(e, 0, P.bytes, &db));
     EHIP(e, hipMemcpyAsync(db, P.blob.data(), P.bytes, hipMemcpyHostToDevice, e->stream));
+    const bool trace = getenv("EMV_TRACE") != nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (trace)
+        for (auto &x : ev) EHIP(e, hipEventCreate(&x));
+    if (trace) EHIP(e, hipEventRecord(ev[0], e->stream));
     ECHK(enqueue_front(e, db, P));
+    if (trace) EHIP(e, hipEventRecord(ev[1], e->stream));
     ECHK(run_relax_eager(e, db, P, rounds_used));
+    if (trace) EHIP(e, hipEventRecord(ev[2], e->stream));
     ECHK(enqueue_back(e, db, P));
     e->steps++;
-    if (getenv("EMV_TRACE")) {
+    if (trace) {
+        EHIP(e, hipEventRecord(ev[3], e->stream));
+        EHIP(e, hipEventSynchronize(ev[3]));
+        float ms[3] = {0, 0, 0};
+        for (int i = 0; i < 3; i++) EHIP(e, hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (auto &x : ev) (void)hipEventDestroy(x);
         int64_t items[4];
         for (int g = 0; g < 4; g++) items[g] = P.grp[g].relax_pre.back();
         fprintf(stderr,
                 "emv-trace: step %lld instances=%zu chunks=%zu relax_chunks=%zu bins=%zu "
+                "ms merge %.2f relax %.2f unpack %.2f "
                 "rounds=%d items u32/u16 vertex %lld/%lld block %lld/%lld\n",
                 (long long)e->steps, P.insts.size(), P.ch.size(), P.rch.size(), P.bins.size(),
-                *rounds_used,
+                ms[0], ms[1], ms[2], *rounds_used,
                 (long long)items[0], (long long)items[1], (long long)items[2],
                 (long long)items[3]);
     }
