@@ -38,7 +38,7 @@ typedef struct {
 
 /* Tunables (emv_engine_default_params fills the defaults). */
 typedef struct {
-    int32_t relax_blocks_per_cu;  /* relaxation grid = CUs * this (1..32), default 8 */
+    int32_t relax_blocks_per_cu;  /* relaxation grid = CUs * this (1..256), default 64 */
     int32_t merge_rows_per_wave;  /* 1, 2, 4, 8, or 0 = by layer size */
     int32_t use_graph;            /* batch: replay steps 2..kmax from a hipGraph */
     int32_t graph_round_margin;   /* rounds added to each replayed step's recorded count

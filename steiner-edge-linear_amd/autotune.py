@@ -15,7 +15,7 @@ import time
 
 import numpy as np
 
-CANDIDATES = [(rb, rows) for rb in (4, 8, 16) for rows in (0, 1, 2)]
+CANDIDATES = [(rb, rows) for rb in (16, 32, 64, 128) for rows in (0,)]
 
 
 def cache_path():

@@ -1204,7 +1204,7 @@ int engine_open(int device, emv_engine **out) {
     emv_engine *e = new (std::nothrow) emv_engine();
     if (!e) return EMV_ERR_NOMEM;
     e->device = device;
-    e->prm.relax_blocks_per_cu = 8;
+    e->prm.relax_blocks_per_cu = 64;
     e->prm.merge_rows_per_wave = 0;
     e->prm.use_graph = 1;
     e->prm.graph_round_margin = 2;
@@ -1232,7 +1232,7 @@ int engine_open(int device, emv_engine **out) {
 
 extern "C" int emv_engine_default_params(emv_engine_params *p) {
     if (!p) return EMV_ERR_ARG;
-    p->relax_blocks_per_cu = 8;
+    p->relax_blocks_per_cu = 64;
     p->merge_rows_per_wave = 0;
     p->use_graph = 1;
     p->graph_round_margin = 2;
@@ -1241,7 +1241,7 @@ extern "C" int emv_engine_default_params(emv_engine_params *p) {
 
 extern "C" int emv_engine_set_params(emv_engine *e, const emv_engine_params *p) {
     if (!e || !p) return EMV_ERR_ARG;
-    if (p->relax_blocks_per_cu < 1 || p->relax_blocks_per_cu > 32 ||
+    if (p->relax_blocks_per_cu < 1 || p->relax_blocks_per_cu > 256 ||
         (p->merge_rows_per_wave != 0 && p->merge_rows_per_wave != 1 &&
          p->merge_rows_per_wave != 2 && p->merge_rows_per_wave != 4 &&
          p->merge_rows_per_wave != 8) ||

@@ -37,6 +37,7 @@ struct emv_ctx {
     int nxcd = 8;
     int num_cu = 256;
     int relax_nxcd = 1;  // relaxation sweep: 1 = grid-wide order, nxcd = per-XCD ranges
+    int relax_wg = 64;   // vertex-wise rounds: workgroups per CU (EMV_RELAX_WG)
     // graph (CSR on the device)
     int64_t n = 0, m = 0;
     int32_t *d_rp = nullptr;
@@ -244,7 +245,7 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // after convergence return at once.
 template <typename DT, int DMAX>
 void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
-    const int grid = c->num_cu * 8;
+    const int grid = c->num_cu * c->relax_wg;
     hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
                        (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
@@ -703,8 +704,8 @@ extern "C" int emv_open(emv_ctx **out, int device) {
         c->nxcd = (c->num_cu % 32 == 0 && c->num_cu >= 64) ? c->num_cu / 32 : 1;
         const char *rx = getenv("EMV_RELAX_XCD");
         c->relax_nxcd = (rx && atoi(rx) > 0) ? c->nxcd : 1;
-
-
+        if (const char *rw = getenv("EMV_RELAX_WG"))
+            c->relax_wg = std::max(1, std::min(256, atoi(rw)));
         if (getenv("EMV_TRACE") && atoi(getenv("EMV_TRACE")) > 0) {
             if (hipMalloc((void **)&c->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
                 hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {

@@ -344,7 +344,7 @@ class _Engine:
     def __exit__(self, *a):
         self.close()
 
-    def set_params(self, relax_blocks_per_cu=8, merge_rows_per_wave=0, use_graph=1,
+    def set_params(self, relax_blocks_per_cu=64, merge_rows_per_wave=0, use_graph=1,
                    graph_round_margin=2):
         p = EngineParams(relax_blocks_per_cu, merge_rows_per_wave, use_graph, graph_round_margin)
         self._check(_bind_engine().emv_engine_set_params(self._h, ctypes.byref(p)))
