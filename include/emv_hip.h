@@ -77,6 +77,7 @@ typedef struct {
     int64_t singleton_rounds;/* relaxation rounds of the singleton shortest paths */
     int64_t block_dist_bits; /* 16 / 32: block visits after the first apply in-block distances
                                 from the lowered border slots; 0: they sweep */
+    double merge_split_ms;   /* part of merge_ms in the layers merged in parts (rows of >= 128 pairs) */
 } emv_result;
 
 /* Context on HIP device `device`. */

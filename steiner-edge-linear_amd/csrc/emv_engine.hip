@@ -775,7 +775,10 @@ int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
 template <typename DT, int DMAX>
 void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
                    int r) {
-    const int grid = e->num_cu * e->prm.relax_blocks_per_cu;
+    // no more workgroups than 64-item blocks of waves
+    const int64_t need = (SG.relax_pre.back() + (int64_t)kBlock - 1) / kBlock;
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)e->num_cu * e->prm.relax_blocks_per_cu, need));
     // the worklist numbers items within one group: only for single-group steps
     int groups = 0;
     for (const StepGroup &G : P.grp) groups += G.nrch > 0 ? 1 : 0;
@@ -807,7 +810,9 @@ void launch_gblk_w(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
                    int r) {
     const size_t bytes = blk_lds_bytes<DT>();
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, e->lds_cu / bytes));
-    hipLaunchKernelGGL((k_gblk<DT, DW>), dim3(e->num_cu * per_cu), dim3(kBT), bytes, e->stream,
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>((int64_t)e->num_cu * per_cu, SG.relax_pre.back()));
+    hipLaunchKernelGGL((k_gblk<DT, DW>), dim3(grid), dim3(kBT), bytes, e->stream,
                        (void *)e->d_D, (const GChunk *)(db + P.off_rch) + SG.rch0,
                        (const int64_t *)(db + SG.off_rp), SG.nrch, e->d_boff, e->d_bvert,
                        e->d_brp, e->d_bext, e->d_badj, e->d_vblk, (const DW *)e->d_bdist,

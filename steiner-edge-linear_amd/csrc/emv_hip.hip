@@ -38,6 +38,7 @@ struct emv_ctx {
     int num_cu = 256;
     int relax_nxcd = 1;  // relaxation sweep: 1 = grid-wide order, nxcd = per-XCD ranges
     int relax_wg = 64;   // vertex-wise rounds: workgroups per CU (EMV_RELAX_WG)
+    std::vector<char> batch_split;  // per layer batch of a solve: merged by the split kernels
     // graph (CSR on the device)
     int64_t n = 0, m = 0;
     int32_t *d_rp = nullptr;
@@ -245,7 +246,10 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // after convergence return at once.
 template <typename DT, int DMAX>
 void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
-    const int grid = c->num_cu * c->relax_wg;
+    // no more workgroups than 64-item blocks of waves (small instances)
+    const int64_t need = (nitems + (int64_t)kBlock - 1) / kBlock;
+    int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * c->relax_wg, need));
+    if (c->relax_nxcd > 1) grid = (grid + c->relax_nxcd - 1) / c->relax_nxcd * c->relax_nxcd;
     hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
                        (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
@@ -299,7 +303,8 @@ template <typename DT, typename DW>
 void blk_round_w(emv_ctx *c, int64_t nitems, int dense, int r) {
     const size_t bytes = blk_lds_bytes<DT>();
     const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, c->lds_cu / bytes));
-    hipLaunchKernelGGL((k_relax_blk<DT, DW>), dim3(c->num_cu * per_cu), dim3(kBT), bytes,
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * per_cu, nitems));
+    hipLaunchKernelGGL((k_relax_blk<DT, DW>), dim3(grid), dim3(kBT), bytes,
                        c->stream, (DT *)c->d_D, c->np, c->nblk, c->d_boff, c->d_bvert, c->d_brp,
                        c->d_bext, c->d_badj, c->d_vblk, (const DW *)c->d_bdist, c->d_bdoff,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r,
@@ -474,6 +479,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 HIPCHK(c, hipEventCreate(pe));
                 ev.push_back(*pe);
             }
+            c->batch_split.push_back(((1u << (m - 1)) - 1u) >= kSplitPairs ? 1 : 0);
             HIPCHK(c, hipEventRecord(e0, c->stream));
             // merge -> staging
             CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
@@ -964,6 +970,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
 
     // ---- 3. layers
     const size_t ev_layers0 = ev.size();
+    c->batch_split.clear();
     if (st.tbytes == 4)
         CHK(run_layers_t<int32_t>(c, st, (int32_t *)c->d_f, c->d_b, res, ev));
     else
@@ -1043,6 +1050,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     res->singleton_ms = ev_ms(e_begin, e_single1) - res->zero_ms;
     for (size_t i = ev_layers0; i + 3 < ev.size(); i += 4) {
         res->merge_ms += ev_ms(ev[i], ev[i + 1]);
+        const size_t bi = (i - ev_layers0) / 4;
+        if (bi < c->batch_split.size() && c->batch_split[bi])
+            res->merge_split_ms += ev_ms(ev[i], ev[i + 1]);
         res->relax_ms += ev_ms(ev[i + 1], ev[i + 2]);
         res->unpack_ms += ev_ms(ev[i + 2], ev[i + 3]);
     }
