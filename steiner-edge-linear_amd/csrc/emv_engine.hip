@@ -177,6 +177,7 @@ struct emv_engine {
     int64_t macc_bytes = 0;
     bool no_split = false;   // EMV_NO_SPLIT: every layer through k_gmerge
     bool no_pack = false;    // EMV_NO_PACK: every staging chunk relaxes in place
+    int split_items = 32;    // split merge: items per tile aimed at (EMV_SPLIT_ITEMS)
     uint32_t *d_any = nullptr;
     uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
     uint32_t *d_q[2] = {nullptr, nullptr};
@@ -564,10 +565,12 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.pad_ = 0;
             M.mbase = 0;
             if (splits) {
-                // enough items per tile to occupy an XCD; parts of >= 4 outer steps
+                // parts so that a tile has about split_items items (a step
+                // holds many instances' tiles, so few parts pay: fewer
+                // atomics), each part >= 4 outer steps
                 const int64_t steps = (I.stage - 1 <= 3) ? 1 : (1ll << (I.stage - 1 - 3));
                 M.S = (int32_t)std::max<int64_t>(
-                    1, std::min<int64_t>((512 + nrows - 1) / nrows, steps / 4));
+                    1, std::min<int64_t>((e->split_items + nrows - 1) / nrows, steps / 4));
                 M.mbase = P.macc_cells;
                 P.macc_cells += (int64_t)nrows * G.np;
             }
@@ -1225,6 +1228,7 @@ int engine_open(int device, emv_engine **out) {
     if (const char *qc = getenv("EMV_QCAP")) e->qcap = (uint32_t)std::max(0, atoi(qc));
     e->no_split = getenv("EMV_NO_SPLIT") != nullptr;
     e->no_pack = getenv("EMV_NO_PACK") != nullptr;
+    if (const char *si = getenv("EMV_SPLIT_ITEMS")) e->split_items = std::max(1, atoi(si));
     EHIP(e, hipMalloc((void **)&e->d_q[0], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipMalloc((void **)&e->d_q[1], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipHostMalloc((void **)&e->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault));
