@@ -85,19 +85,29 @@ __device__ __forceinline__ const int64_t *cache_pre(const int64_t *__restrict__ 
 
 // ------------------------------------------------------ singleton seeding
 // Every cell of every singleton chunk: INF label; then label 0 at the sources.
-__global__ void k_gseed(const GChunk *__restrict__ ch, const int64_t *__restrict__ cell_pre,
-                        int nch, const int32_t *__restrict__ kk_all,
-                        const int32_t *__restrict__ kk_off, void *__restrict__ D) {
-    const int64_t total = cell_pre[nch];
+__global__ __launch_bounds__(kBlock) void k_gseed(const GChunk *__restrict__ ch,
+                                                  const int64_t *__restrict__ cell_pre, int nch,
+                                                  const int32_t *__restrict__ kk_all,
+                                                  const int32_t *__restrict__ kk_off,
+                                                  void *__restrict__ D) {
+    __shared__ int64_t sh_pre[kSegCache];
+    const int64_t *pre = cache_pre(cell_pre, nch, sh_pre);
+    const int64_t total = pre[nch];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // a thread's cells increase, so its chunk is searched once and then
+    // advanced across chunk boundaries
+    int g = -1;
+    int32_t src = -1;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int g = find_seg(cell_pre, nch, i);
-        const GChunk c = ch[g];
-        const int64_t off = i - cell_pre[g];  // v * 64 + lane
-        const int lane = (int)(off & (kWave - 1));
-        const int v = (int)(off / kWave);
-        chunk_labels<uint32_t>(D, c.dbase)[off] =
-            (lane < c.nx && kk_all[kk_off[c.inst] + lane] == v) ? 0u : ~0u;
+        const int lane = (int)(i & (kWave - 1));  // chunk cell counts are multiples of 64
+        if (g < 0 || pre[g + 1] <= i) {
+            g = (g < 0) ? find_seg(pre, nch, i) : g;
+            while (pre[g + 1] <= i) g++;
+            const GChunk c = ch[g];
+            src = lane < c.nx ? kk_all[kk_off[c.inst] + lane] : -1;
+        }
+        const int64_t off = i - pre[g];  // v * 64 + lane
+        chunk_labels<uint32_t>(D, ch[g].dbase)[off] = ((int)(off / kWave) == src) ? 0u : ~0u;
     }
 }
 
