@@ -134,6 +134,10 @@ struct Inst {
 
 }  // namespace
 
+namespace {
+struct StepPlan;
+}  // namespace
+
 struct emv_engine {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -193,6 +197,8 @@ struct emv_engine {
     std::vector<char> gkey;
     uint32_t *d_conv = nullptr;  // per replayed step: 1 if it ran out of rounds
     size_t conv_cap = 0;
+    std::vector<StepPlan> gplans;     // plans of the replayed steps, and the instance
+    std::vector<int64_t> gplan_sig;   // states and settings they were built from
     // tunables
     emv_engine_params prm;
     // counters
@@ -1433,7 +1439,6 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
     for (const Inst &I : e->inst)
         if (I.state == ST_DONE && I.status != EMV_OK) return EMV_OK;  // let eager report it
     // plans of the remaining lockstep steps
-    std::vector<StepPlan> plans;
     std::vector<std::vector<int64_t>> runs;
     int kmax = 0;
     for (const Inst &I : e->inst) kmax = std::max(kmax, I.k);
@@ -1445,26 +1450,42 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
     }
     // rounds per step from the last eager solve (+ margin)
     if ((int)e->rounds_per_step.size() < kmax) return EMV_OK;
-    plans.resize(runs.size());
-    int64_t max_cells = 0, max_flags = 0, max_macc = 0;
-    // build with the stage each instance will have at that step
-    std::vector<int> saved_stage(e->inst.size());
-    for (size_t t = 0; t < e->inst.size(); t++) saved_stage[t] = e->inst[t].stage;
-    for (size_t s = 0; s < runs.size(); s++) {
-        for (int64_t t : runs[s]) e->inst[t].stage = (int)s + 2;
-        build_plan(e, runs[s], plans[s]);
-        max_cells = std::max(max_cells, plans[s].cells);
-        max_flags = std::max(max_flags, plans[s].flagbytes);
-        max_macc = std::max(max_macc, plans[s].macc_cells * (plans[s].pbits / 8));
-    }
-    for (size_t t = 0; t < e->inst.size(); t++) e->inst[t].stage = saved_stage[t];
-    ECHK(ensure_stage(e, max_cells, max_flags));
-    ECHK(ensure_macc(e, max_macc));
-    std::vector<char *> dbs(plans.size());
-    for (size_t s = 0; s < plans.size(); s++) {
-        ECHK(desc_buffer(e, 1 + s, plans[s].bytes, &dbs[s]));
-        EHIP(e, hipMemcpyAsync(dbs[s], plans[s].blob.data(), plans[s].bytes,
-                               hipMemcpyHostToDevice, e->stream));
+    // the plans are a function of the instances' states after the singleton
+    // step and of the engine's settings: a repeated solve reuses them (and
+    // the descriptors already on the device)
+    std::vector<int64_t> sig = {kmax, e->prm.relax_blocks_per_cu, e->prm.merge_rows_per_wave,
+                                e->split_items, e->no_split ? 1 : 0, e->no_pack ? 1 : 0};
+    for (const Inst &I : e->inst)
+        sig.insert(sig.end(), {(int64_t)I.state, (int64_t)I.stage, (int64_t)I.k,
+                               (int64_t)I.graph, (int64_t)I.dbits, (int64_t)I.pbits, I.abase,
+                               (int64_t)I.slot});
+    std::vector<StepPlan> &plans = e->gplans;
+    std::vector<char *> dbs(runs.size());
+    if (!(sig == e->gplan_sig && plans.size() == runs.size())) {
+        e->gplan_sig.clear();
+        plans.assign(runs.size(), StepPlan());
+        int64_t max_cells = 0, max_flags = 0, max_macc = 0;
+        // build with the stage each instance will have at that step
+        std::vector<int> saved_stage(e->inst.size());
+        for (size_t t = 0; t < e->inst.size(); t++) saved_stage[t] = e->inst[t].stage;
+        for (size_t s = 0; s < runs.size(); s++) {
+            for (int64_t t : runs[s]) e->inst[t].stage = (int)s + 2;
+            build_plan(e, runs[s], plans[s]);
+            max_cells = std::max(max_cells, plans[s].cells);
+            max_flags = std::max(max_flags, plans[s].flagbytes);
+            max_macc = std::max(max_macc, plans[s].macc_cells * (plans[s].pbits / 8));
+        }
+        for (size_t t = 0; t < e->inst.size(); t++) e->inst[t].stage = saved_stage[t];
+        ECHK(ensure_stage(e, max_cells, max_flags));
+        ECHK(ensure_macc(e, max_macc));
+        for (size_t s = 0; s < plans.size(); s++) {
+            ECHK(desc_buffer(e, 1 + s, plans[s].bytes, &dbs[s]));
+            EHIP(e, hipMemcpyAsync(dbs[s], plans[s].blob.data(), plans[s].bytes,
+                                   hipMemcpyHostToDevice, e->stream));
+        }
+        e->gplan_sig = sig;
+    } else {
+        for (size_t s = 0; s < plans.size(); s++) dbs[s] = (char *)e->desc_bufs[1 + s];
     }
     if (e->conv_cap < plans.size()) {
         dfree(e->d_conv);
@@ -1483,11 +1504,11 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
                           e->d_arena, e->d_any, e->d_cnt,     d_conv};
     add_key(bufs, sizeof(bufs));
     add_key(&e->prm, sizeof(e->prm));
+    add_key(sig.data(), sizeof(int64_t) * sig.size());
     for (size_t s = 0; s < plans.size(); s++) {
         const int R = std::max(1, e->rounds_per_step[s + 1] + e->prm.graph_round_margin);
         add_key(&R, sizeof(R));
         add_key(&dbs[s], sizeof(dbs[s]));
-        add_key(plans[s].blob.data(), plans[s].blob.size());
     }
     if (!(e->gexec && key == e->gkey)) {
         if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
