@@ -246,11 +246,12 @@ def plain_leg(args):
     return dict(
         value=cells * args.steps / wall,
         ms_per_step=wall * 1e3 / args.steps,
-        dtype="int32" if sols[-1].table_bytes == 4 else "int64",
+        # table cells: 16-bit when the distance bound allows, else int32 / int64
+        dtype={2: "uint16", 4: "int32", 8: "int64"}[sols[-1].table_bytes],
         config={"workload": wl["workload"] if (args.n, args.k) == (262144, 14)
                 else f"8-regular n={args.n} k={args.k} seed 2",
                 "n": wl["n"], "m": int(len(e)), "k": wl["k"], "dp_cells": cells,
-                "table": f"int{8 * sols[-1].table_bytes}",
+                "table": {2: "uint16", 4: "int32", 8: "int64"}[sols[-1].table_bytes],
                 "merge_acc_bits": sols[-1].merge_bits, "label_bits": sols[-1].label_bits},
         cost=cost,
         kernel_ms=sum(s.kernel_ms for s in sols) / len(sols),

@@ -45,6 +45,7 @@ extern "C" {
 #define EMV_KEEP_TABLE  0x4u  /* keep the table on the device for emv_read_rows */
 #define EMV_FORCE_P64   0x8u  /* 64-bit merge packing and relaxation labels */
 #define EMV_WIDE_LABELS 0x10u /* no 16-bit relaxation labels even when they fit */
+#define EMV_TABLE32     0x20u /* int32 table even when the bound allows 16-bit cells */
 
 typedef struct emv_ctx emv_ctx;
 

@@ -931,8 +931,11 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         set_err(c, "distance bound %llu exceeds the 64-bit packing", B);
         return EMV_ERR_RANGE;
     }
+    // table cells: 16-bit while every finite cell stays below 0xFFFF (row 0
+    // holds 0xFFFF as INF), else int32, else int64
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
-    st.pbits = (st.tbytes == 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
+    if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
+    st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
     if (flags & EMV_FORCE_P64)
         st.dbits = 64;
     else if (top < 0xFFFFull && !(flags & EMV_WIDE_LABELS))
@@ -949,7 +952,10 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         // the padding columns v >= n only feed lanes whose results are
         // dropped, so only row 0 (never written; INF and record -1 as the
         // reference's initialisation, reader-el.c:2244-2259) is filled.
-        if (st.tbytes == 4)
+        if (st.tbytes == 2)
+            hipLaunchKernelGGL(k_fill<uint16_t>, dim3(4), dim3(kBlock), 0, c->stream,
+                               (uint16_t *)c->d_f, np, (uint16_t)0xFFFF);
+        else if (st.tbytes == 4)
             hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (int32_t *)c->d_f, np, INT32_MAX);
         else
@@ -959,7 +965,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipEventRecord(e_zero1, c->stream));
-    if (st.tbytes == 4)
+    if (st.tbytes == 2)
+        unpack_singletons<uint16_t>(c, k, (uint16_t *)c->d_f, c->d_b);
+    else if (st.tbytes == 4)
         unpack_singletons<int32_t>(c, k, (int32_t *)c->d_f, c->d_b);
     else
         unpack_singletons<int64_t>(c, k, (int64_t *)c->d_f, c->d_b);
@@ -971,7 +979,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // ---- 3. layers
     const size_t ev_layers0 = ev.size();
     c->batch_split.clear();
-    if (st.tbytes == 4)
+    if (st.tbytes == 2)
+        CHK(run_layers_t<uint16_t>(c, st, (uint16_t *)c->d_f, c->d_b, res, ev));
+    else if (st.tbytes == 4)
         CHK(run_layers_t<int32_t>(c, st, (int32_t *)c->d_f, c->d_b, res, ev));
     else
         CHK(run_layers_t<int64_t>(c, st, (int64_t *)c->d_f, c->d_b, res, ev));
@@ -983,6 +993,12 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     int64_t cost = 0;
     if (C == 0) {
         cost = INT64_MAX;  // row 0 is never written (reader-el.c:2137 with k == 1)
+    } else if (st.tbytes == 2) {
+        uint16_t v16;
+        HIPCHK(c, hipMemcpyAsync(&v16, (uint16_t *)c->d_f + C * np + q, 2, hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        cost = v16;
     } else if (st.tbytes == 4) {
         int32_t v32;
         HIPCHK(c, hipMemcpyAsync(&v32, (int32_t *)c->d_f + C * np + q, 4, hipMemcpyDeviceToHost,
@@ -1096,7 +1112,10 @@ extern "C" int emv_read_rows(emv_ctx *c, int64_t X0, int64_t nrows, int64_t *f_o
                                 (size_t)n * c->tbytes, hipMemcpyDeviceToHost));
             for (int64_t v = 0; v < n; v++) {
                 int64_t val;
-                if (c->tbytes == 4) {
+                if (c->tbytes == 2) {
+                    const uint16_t x = ((uint16_t *)buf.data())[v];
+                    val = (x == 0xFFFF) ? INT64_MAX : (int64_t)x;
+                } else if (c->tbytes == 4) {
                     int32_t x = ((int32_t *)buf.data())[v];
                     val = (x == INT32_MAX) ? INT64_MAX : (int64_t)x;
                 } else {

@@ -56,15 +56,17 @@ def test_regular_1000_full_table(gpu_ctx):
     np.testing.assert_array_equal(f, gf)
     np.testing.assert_array_equal(kinds(code, inst.k), kinds(gcode, inst.k))
     assert sol.cost == META[CFG1]["ref_cost"]
-    assert sol.table_bytes == 4 and sol.merge_bits == 32 and sol.label_bits == 16
+    # the bound fits 16-bit cells
+    assert sol.table_bytes == 2 and sol.merge_bits == 32 and sol.label_bits == 16
 
 
 @pytest.mark.parametrize("flags", [emv.WIDE_LABELS, emv.FORCE_P64, emv.FORCE_I64,
-                                   emv.FORCE_I64 | emv.FORCE_P64])
+                                   emv.FORCE_I64 | emv.FORCE_P64, emv.TABLE32,
+                                   emv.TABLE32 | emv.WIDE_LABELS])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
 def test_wide_paths_bit_exact(gpu_ctx, name, flags):
     """Every width combination (16/32/64-bit labels, 32/64-bit merge
-    accumulators, int32/int64 table) gives the reference's table."""
+    accumulators, 16-bit/int32/int64 table) gives the reference's table."""
     inst = load_instance(name)
     print(f"case {name} flags={flags}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst, flags)
@@ -76,6 +78,8 @@ def test_wide_paths_bit_exact(gpu_ctx, name, flags):
         assert sol.label_bits == 32
     if flags & emv.FORCE_I64:
         assert sol.table_bytes == 8 and sol.merge_bits == 64
+    elif flags & emv.TABLE32:
+        assert sol.table_bytes == 4
 
 
 @pytest.mark.parametrize("seed", range(4))
