@@ -7,8 +7,13 @@
 // or share one resident graph (serving), and sit at different layers.
 //
 // The per-instance table keeps the single-instance layout in one arena
-// region: costs f[tbase + X * np + v] (int32), then the record codes
-// b[tbase + 2^k * np + X * np + v] (int32).
+// region: costs f[tbase + X * np + v] (int32 cells, or 16-bit cells from the
+// same base when the instance's bound allows), then the record codes
+// b[tbase + 2^k * np + X * np + v] (int32).  The cell width is known only
+// after the singleton stage (the bound): with k >= 3 that stage writes its
+// rows as int32 at the end of the cost region (row t at cells
+// (2^k - k + t) * np, clear of every singleton row and of 16-bit rows), and
+// the first layer step places them (k_gplace) at the instance's width.
 // Staging chunks keep the vertex-major layout, labels of type DT (16 or 32
 // bits, per chunk group) at ((DT *)(D + 4 * dbase))[v * 64 + lane] -- dbase
 // counts 4-byte cells -- and the merge's packed word (sum << k) | Y in
@@ -37,6 +42,7 @@ struct GChunk {
     int32_t inst;   // engine slot of the instance
     int32_t b0;     // block relaxation: first block of the graph in the shared block arrays
     int32_t nblk;   //   and its blocks (0: vertex-wise relaxation)
+    int32_t tbits;  // table cells: 16 or 32 bits
 };
 
 // One instance at a layer: its chunks are consecutive, its merge items are
@@ -51,7 +57,7 @@ struct GMerge {
     int64_t tbase;   // the instance's table
     int64_t dbase0;  // staging cell of its first chunk; chunk q at dbase0 + q * np * 64
     int32_t S;       // split merge: parts per row (0: rows merged whole by k_gmerge)
-    int32_t pad_;
+    int32_t tbits;   // table cells: 16 or 32 bits (at the same tbase either way)
     int64_t mbase;   // split merge: accumulator of row xi at Macc[mbase + xi * np + v]
 };
 
@@ -136,7 +142,6 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     const int64_t tile = local / M.nsub;
     const int sub = (int)(local - tile * M.nsub);
     const int64_t v = tile * kWave + lane;
-    const int32_t *fv = f + M.tbase + v;
     const int k = M.k;
     const int nrows = M.nrows;
     DT dv[XG];
@@ -147,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
         dv[jx] = (DT)~(DT)0;
         pv[jx] = ~(P)0;
         if (xi < nrows) {
-            const P acc = merge_one<int32_t, P>(fv, (int64_t)M.np, xs[M.x0 + xi], k);
+            const uint32_t X = xs[M.x0 + xi];
+            const P acc = merge_cells<P>(f, M, v, X, 0u, merge_steps(X));
             dv[jx] = (DT)(acc >> k);
             pv[jx] = acc;
         }
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_split(
     const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / M.S);
     if (o0 >= o1) return;
     const int64_t v = tile * kWave + lane;
-    const P acc = merge_outer<int32_t, P>(f + M.tbase + v, (int64_t)M.np, X, M.k, o0, o1);
+    const P acc = merge_cells<P>(f, M, v, X, o0, o1);
     atomicMin(&Macc[M.mbase + (int64_t)xi * M.np + v], acc);
 }
 
@@ -234,6 +240,51 @@ __global__ __launch_bounds__(kBlock) void k_gpack(const P *__restrict__ Macc,
         const int64_t idx = (v0 + vv) * kWave + lane;
         Dd[idx] = real ? (DT)(a >> M.k) : (DT)~(DT)0;
         CODE[dbase + idx] = real ? a : ~(P)0;
+    }
+}
+
+// All pairs of row X at the instance's table cells (16 or 32 bits), from
+// parts o0 .. o1 of the outer steps.
+template <typename P>
+__device__ __forceinline__ P merge_cells(const int32_t *__restrict__ f, const GMerge &M, int64_t v,
+                                         uint32_t X, uint32_t o0, uint32_t o1) {
+    if (M.tbits == 16)
+        return merge_outer<uint16_t, P>(reinterpret_cast<const uint16_t *>(f + M.tbase) + v,
+                                        (int64_t)M.np, X, M.k, o0, o1);
+    return merge_outer<int32_t, P>(f + M.tbase + v, (int64_t)M.np, X, M.k, o0, o1);
+}
+
+// Singleton rows of instances at their first layer, from their int32 copies
+// at the end of the cost region to rows 2^t at the instance's cell width, and
+// row 0 (INF).  Workgroup = (instance, row r): r < k moves singleton row r,
+// r == k fills row 0.  For k >= 3 the copies lie above every place written
+// here, at either width, so the workgroups need no order.
+struct GPlace {
+    int64_t tbase;
+    int32_t np, k, tbits, pad_;
+};
+
+__global__ __launch_bounds__(kBlock) void k_gplace(int32_t *__restrict__ f,
+                                                   const GPlace *__restrict__ pl,
+                                                   const int64_t *__restrict__ row_pre, int npl) {
+    __shared__ int64_t sh_pre[kSegCache];
+    const int64_t *pre = cache_pre(row_pre, npl, sh_pre);
+    const int64_t blk = blockIdx.x;
+    if (blk >= pre[npl]) return;
+    const int s = find_seg(pre, npl, blk);
+    const GPlace G = pl[s];
+    const int r = (int)(blk - pre[s]);
+    const int64_t S2 = 1ll << G.k;
+    int32_t *fr = f + G.tbase;
+    const int32_t *copy = fr + (S2 - G.k + r) * G.np;
+    const int64_t X = r < G.k ? (1ll << r) : 0;
+    for (int64_t v = threadIdx.x; v < G.np; v += blockDim.x) {
+        const int32_t x = r < G.k ? copy[v] : INT32_MAX;
+        if (G.tbits == 16)
+            reinterpret_cast<uint16_t *>(fr)[X * G.np + v] =
+                (x == INT32_MAX) ? (uint16_t)0xFFFF : (uint16_t)x;
+        else
+            fr[X * G.np + v] = x;
     }
 }
 
@@ -596,14 +647,19 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
         if (xl < c.nx && v < c.n) {
             const DT dl = sd[lane][xl];
             const uint32_t d = (dl == (DT)~(DT)0) ? ~0u : (uint32_t)dl;
+            uint16_t *f16 = reinterpret_cast<uint16_t *>(f + c.tbase);
             if (c.kind == 0) {
                 const int64_t X = xs[c.x0 + xl];
-                f[c.tbase + X * c.np + v] = (int32_t)d;
+                if (c.tbits == 16)
+                    f16[X * c.np + v] = (uint16_t)d;
+                else
+                    f[c.tbase + X * c.np + v] = (int32_t)d;
                 b[bbase + X * c.np + v] = sc[lane][xl];
             } else {
                 const int64_t X = 1ll << xl;
                 const int32_t src = kk_all[kk_off[c.inst] + xl];
-                f[c.tbase + X * c.np + v] = (d == ~0u) ? INT32_MAX : (int32_t)d;
+                const int64_t row = c.k >= 3 ? S2 - c.k + xl : X;  // staged copy, or in place
+                f[c.tbase + row * c.np + v] = (d == ~0u) ? INT32_MAX : (int32_t)d;
                 b[bbase + X * c.np + v] = (v == src) ? (int32_t)X : (int32_t)(S2 + src);
                 dmax = d;
             }

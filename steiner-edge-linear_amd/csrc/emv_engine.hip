@@ -124,6 +124,7 @@ struct Inst {
     int stage = 1;  // next stage: 1 = singletons, m >= 2 = layer m
     int pbits = 32;
     int dbits = 32;  // relaxation label width of the layer stages (16 when the bound allows)
+    int tbits = 32;  // table cell width from layer 2 on: 16 when the bound B < 0xFFFF
     int status = EMV_OK;
     int64_t abase = -1, acells = 0;  // arena region: f then b, S * np cells each
     int64_t cost = 0;
@@ -477,6 +478,8 @@ struct StepGroup {
 
 struct StepPlan {
     std::vector<GChunk> ch;
+    std::vector<GPlace> places;      // instances at their first layer: singleton rows to place
+    std::vector<int64_t> place_pre;  // rows (k + 1) per place
     std::vector<GChunk> rch;  // relaxation chunks of all groups
     std::vector<GBin> bins;
     std::vector<GLane> lanes;
@@ -492,7 +495,7 @@ struct StepPlan {
     int64_t macc_cells = 0;  // split-merge accumulators of the step
     // device layout (byte offsets into the descriptor buffer)
     size_t off_ch = 0, off_mg = 0, off_sp = 0, off_xs = 0, off_seedch = 0, bytes = 0;
-    size_t off_rch = 0, off_bins = 0, off_lanes = 0;
+    size_t off_rch = 0, off_bins = 0, off_lanes = 0, off_pl = 0, off_plp = 0;
     std::vector<char> blob;
     std::vector<GChunk> seed_ch;
 };
@@ -531,6 +534,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             c.inst = I.slot;
             c.b0 = G.b0;
             c.nblk = G.nblk;
+            c.tbits = I.tbits;
             // activity flags: one per vertex, or one per block
             const int64_t nflags = G.nblk > 0 ? G.nblk : G.np;
             if (I.stage == 1) {
@@ -544,6 +548,15 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 dcur += G.np * kWave;
                 fcur += nflags;
                 continue;
+            }
+            if (I.stage == 2 && I.k >= 3) {
+                GPlace pl;
+                memset(&pl, 0, sizeof(pl));
+                pl.tbase = I.abase;
+                pl.np = (int32_t)G.np;
+                pl.k = I.k;
+                pl.tbits = I.tbits;
+                P.places.push_back(pl);
             }
             rows.clear();
             subsets_of_size(I.k, I.stage, rows);
@@ -568,7 +581,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.tbase = I.abase;
             M.dbase0 = dcur;
             M.S = 0;
-            M.pad_ = 0;
+            M.tbits = I.tbits;
             M.mbase = 0;
             if (splits) {
                 // parts so that a tile has about split_items items (a step
@@ -717,6 +730,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         P.seed_ch.push_back(P.ch[g]);
         P.seed_pre.push_back(P.seed_pre.back() + (int64_t)P.ch[g].np * kWave);
     }
+    P.place_pre.assign(1, 0);
+    for (const GPlace &pl : P.places) P.place_pre.push_back(P.place_pre.back() + pl.k + 1);
     P.cells = dcur;
     P.flagbytes = fcur;
     // pack
@@ -733,6 +748,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     P.off_rch = place(sizeof(GChunk) * P.rch.size());
     P.off_bins = place(sizeof(GBin) * P.bins.size());
     P.off_lanes = place(sizeof(GLane) * P.lanes.size());
+    P.off_pl = place(sizeof(GPlace) * P.places.size());
+    P.off_plp = place(sizeof(int64_t) * P.place_pre.size());
     for (StepGroup &SG : P.grp) {
         SG.off_rp = place(sizeof(int64_t) * SG.relax_pre.size());
         SG.off_mp = place(sizeof(int64_t) * SG.merge_pre.size());
@@ -754,6 +771,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     put(P.off_rch, P.rch.data(), sizeof(GChunk) * P.rch.size());
     put(P.off_bins, P.bins.data(), sizeof(GBin) * P.bins.size());
     put(P.off_lanes, P.lanes.data(), sizeof(GLane) * P.lanes.size());
+    put(P.off_pl, P.places.data(), sizeof(GPlace) * P.places.size());
+    put(P.off_plp, P.place_pre.data(), sizeof(int64_t) * P.place_pre.size());
     for (StepGroup &SG : P.grp) {
         put(SG.off_rp, SG.relax_pre.data(), sizeof(int64_t) * SG.relax_pre.size());
         put(SG.off_mp, SG.merge_pre.data(), sizeof(int64_t) * SG.merge_pre.size());
@@ -932,6 +951,10 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
                            (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
     }
+    if (!P.places.empty())
+        hipLaunchKernelGGL(k_gplace, dim3((unsigned)P.place_pre.back()), dim3(kBlock), 0,
+                           e->stream, e->d_arena, (const GPlace *)(db + P.off_pl),
+                           (const int64_t *)(db + P.off_plp), (int)P.places.size());
     if (P.macc_cells > 0) {
         // the split accumulators start at all ones (a fill kernel: graph-safe)
         const int64_t words = P.macc_cells * (P.pbits / 32);
@@ -1044,6 +1067,8 @@ int advance(emv_engine *e, const StepPlan &P) {
                 continue;
             }
             I.pbits = (bits_for(B) + I.k <= 32) ? 32 : 64;
+            // 16-bit cells hold every finite cell below 0xFFFF (row 0's INF)
+            I.tbits = (I.k >= 3 && B < 0xFFFFull && !(I.flags & EMV_TABLE32)) ? 16 : 32;
             // labels are at most B, candidates at most B + max w
             I.dbits = (B + (unsigned long long)e->graphs[I.graph].maxw < 0xFFFFull &&
                        !(I.flags & EMV_WIDE_LABELS))
@@ -1075,11 +1100,20 @@ int read_costs_and_finish(emv_engine *e, const std::vector<int64_t> &done) {
             if (C == 0) {
                 I.cost = INT64_MAX;
             } else {
-                int32_t v = 0;
-                EHIP(e, hipMemcpyAsync(&v, e->d_arena + I.abase + C * G.np + I.kk[I.k - 1],
-                                       sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
-                EHIP(e, hipStreamSynchronize(e->stream));
-                I.cost = (v == INT32_MAX) ? INT64_MAX : (int64_t)v;
+                const int64_t cell = C * G.np + I.kk[I.k - 1];
+                if (I.tbits == 16) {
+                    uint16_t v = 0;
+                    EHIP(e, hipMemcpyAsync(&v, (uint16_t *)(e->d_arena + I.abase) + cell,
+                                           sizeof(uint16_t), hipMemcpyDeviceToHost, e->stream));
+                    EHIP(e, hipStreamSynchronize(e->stream));
+                    I.cost = (v == 0xFFFF) ? INT64_MAX : (int64_t)v;
+                } else {
+                    int32_t v = 0;
+                    EHIP(e, hipMemcpyAsync(&v, e->d_arena + I.abase + cell, sizeof(int32_t),
+                                           hipMemcpyDeviceToHost, e->stream));
+                    EHIP(e, hipStreamSynchronize(e->stream));
+                    I.cost = (v == INT32_MAX) ? INT64_MAX : (int64_t)v;
+                }
                 if (I.flags & EMV_LIST) ECHK(traceback_inst(e, I));
             }
         }
@@ -1157,8 +1191,12 @@ int admit(emv_engine *e) {
         // table: row 0 INF with record -1 (every other real cell is written by
         // the solve; padding columns only feed dropped lanes); rowmax = 0
         const int64_t S2 = 1ll << I.k;
-        hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream, e->d_arena + off,
-                           G.np, INT32_MAX);
+        // the cell width is decided by the bound after the singleton stage;
+        // with k >= 3 the first layer step writes row 0 (k_gplace)
+        I.tbits = 32;
+        if (I.k < 3)
+            hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream,
+                               e->d_arena + off, G.np, INT32_MAX);
         hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, e->stream,
                            e->d_arena + off + S2 * G.np, G.np, -1);
         hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, e->stream,
@@ -1457,8 +1495,8 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
                                 e->split_items, e->no_split ? 1 : 0, e->no_pack ? 1 : 0};
     for (const Inst &I : e->inst)
         sig.insert(sig.end(), {(int64_t)I.state, (int64_t)I.stage, (int64_t)I.k,
-                               (int64_t)I.graph, (int64_t)I.dbits, (int64_t)I.pbits, I.abase,
-                               (int64_t)I.slot});
+                               (int64_t)I.graph, (int64_t)I.dbits, (int64_t)I.pbits,
+                               (int64_t)I.tbits, I.abase, (int64_t)I.slot});
     std::vector<StepPlan> &plans = e->gplans;
     std::vector<char *> dbs(runs.size());
     if (!(sig == e->gplan_sig && plans.size() == runs.size())) {
@@ -1632,7 +1670,13 @@ extern "C" int emv_engine_read_rows(emv_engine *e, int64_t ticket, int64_t X0, i
     std::vector<int32_t> buf((size_t)G.np);
     for (int64_t r = 0; r < nrows; r++) {
         const int64_t X = X0 + r;
-        if (f_out) {
+        if (f_out && I.tbits == 16) {
+            const uint16_t *b16 = reinterpret_cast<const uint16_t *>(buf.data());
+            EHIP(e, hipMemcpy(buf.data(), (const uint16_t *)(e->d_arena + I.abase) + X * G.np,
+                              sizeof(uint16_t) * G.n, hipMemcpyDeviceToHost));
+            for (int64_t v = 0; v < G.n; v++)
+                f_out[r * G.n + v] = b16[v] == 0xFFFF ? INT64_MAX : (int64_t)b16[v];
+        } else if (f_out) {
             EHIP(e, hipMemcpy(buf.data(), e->d_arena + I.abase + X * G.np, sizeof(int32_t) * G.n,
                               hipMemcpyDeviceToHost));
             for (int64_t v = 0; v < G.n; v++)

@@ -140,10 +140,12 @@ def test_server_admission_by_footprint():
         assert s.poll(t2).queue_ms > 0.0
 
 
-def test_batch_label_widths_agree():
-    """Instances whose bound allows it relax on 16-bit labels, the others on
-    32-bit ones, in the same step (two launches per round).  Forcing 32-bit
-    labels everywhere (WIDE_LABELS) gives the same tables and records."""
+@pytest.mark.parametrize("wide", [emv.WIDE_LABELS, emv.TABLE32, emv.WIDE_LABELS | emv.TABLE32])
+def test_batch_label_widths_agree(wide):
+    """Instances whose bound allows it relax on 16-bit labels (and keep
+    16-bit table cells), the others on 32-bit ones, in the same step (two
+    launches per round).  Forcing 32-bit labels (WIDE_LABELS) or int32 cells
+    (TABLE32) everywhere gives the same tables and records."""
     insts = mixed_instances(6, 3)
     # one instance with large weights, whose bound needs 32-bit labels
     n = 700
@@ -152,7 +154,7 @@ def test_batch_label_widths_agree():
     with emv.Batch(insts) as b:
         c1, _ = b.solve(emv.KEEP_TABLE)
         t1 = [b.read_rows(i, 0, 1 << len(kk), n_) for i, (n_, _, kk) in enumerate(insts)]
-        c2, _ = b.solve(emv.KEEP_TABLE | emv.WIDE_LABELS)
+        c2, _ = b.solve(emv.KEEP_TABLE | wide)
         np.testing.assert_array_equal(c1, c2)
         for i, (n_, e_, kk) in enumerate(insts):
             f2, code2 = b.read_rows(i, 0, 1 << len(kk), n_)
