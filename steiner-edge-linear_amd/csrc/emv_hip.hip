@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "emv_hip.h"
@@ -396,9 +397,42 @@ int merge_rows_per_wave(int nx) {
     return 1;
 }
 
+// 16-bit cells take the two-vertices-per-lane merges (keys need k <= 16);
+// EMV_MERGE_H2=0 keeps one vertex per lane.
+bool merge_h2_enabled() {
+    static const int v = [] {
+        const char *e = getenv("EMV_MERGE_H2");
+        return e ? atoi(e) : 1;
+    }();
+    return v != 0;
+}
+
+template <typename T>
+bool use_merge_h2(int k, int64_t np) {
+    // keys hold Y in 16 bits; lane offsets of the low rows fit 32 bits.  The
+    // low rows are the three lowest bits of X without its top bit, at most
+    // bits k-4..k-2 (or below 2^(k-1) for k < 4).
+    const uint64_t lmax = k >= 4 ? 7ull << (k - 4) : (1ull << (k - 1)) - 1ull;
+    const uint64_t span = 2ull * lmax * (uint64_t)np + 256ull;
+    return std::is_same<T, uint16_t>::value && k <= 16 && span < (1ull << 32) &&
+           merge_h2_enabled();
+}
+
 template <typename T, typename P, typename DT, int XG>
 int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     const int nsub = (int)(nchunks * (kWave / XG));
+    if constexpr (std::is_same<T, uint16_t>::value) {
+        if (use_merge_h2<T>(k, c->np)) {
+            const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+            int64_t per;
+            const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
+            hipLaunchKernelGGL((k_merge_h2<P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream,
+                               f, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D,
+                               (P *)c->d_code, c->nxcd, per);
+            HIPCHK(c, hipGetLastError());
+            return EMV_OK;
+        }
+    }
     const int64_t ntiles = c->np / kWave;
     int64_t per;
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
@@ -427,11 +461,25 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
     // enough items per tile to occupy an XCD's waves; parts of >= 4 outer steps
     const int64_t steps = (m - 1 <= 3) ? 1 : (1ll << (m - 1 - 3));
     const int64_t S = std::max<int64_t>(1, std::min<int64_t>((512 + nx - 1) / nx, steps / 4));
-    const int64_t ntiles = c->np / kWave;
-    int64_t per;
-    const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
-    hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
-                       c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per);
+    bool h2 = false;
+    if constexpr (std::is_same<T, uint16_t>::value) {
+        if (use_merge_h2<T>(k, c->np)) {
+            const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+            int64_t per;
+            const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
+            hipLaunchKernelGGL((k_merge_split_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+                               c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd,
+                               per);
+            h2 = true;
+        }
+    }
+    if (!h2) {
+        const int64_t ntiles = c->np / kWave;
+        int64_t per;
+        const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
+        hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+                           c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per);
+    }
     const int64_t vtiles = c->np / kWave;
     hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
                        c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,

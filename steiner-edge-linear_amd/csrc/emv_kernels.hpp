@@ -1077,6 +1077,201 @@ __global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f,
     atomicMin(&M[(int64_t)xi * np + v], acc);
 }
 
+// ----------------------------------------- merge, 16-bit cells, two vertices
+// With 16-bit table cells every pair sum is at most the distance bound, below
+// 0xFFFF, so one 32-bit load brings the cells of two adjacent vertices and one
+// packed 16-bit add gives both sums without a carry between the halves.  Each
+// half is keyed as (sum << 16) | Y (one shift-or, or one and-or for the high
+// half), which orders like (sum << k) | Y for k <= 16: the smallest sum, then
+// the smallest Y.  Half the load instructions of merge_outer per cell pair.
+typedef unsigned short emv_u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t add_u16x2(uint32_t a, uint32_t c) {
+    const emv_u16x2 s = __builtin_bit_cast(emv_u16x2, a) + __builtin_bit_cast(emv_u16x2, c);
+    return __builtin_bit_cast(uint32_t, s);
+}
+
+// Rows are read through raw buffer loads: a wave-uniform buffer resource on
+// the row base (SGPRs, num_records = 0xFFFFFFFF, the gfx9 raw-buffer word 3)
+// plus the lane's 32-bit byte offset, so a load needs no 64-bit vector
+// address arithmetic.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const char *p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(p), (short)0, -1, 0x00020000);
+}
+
+__device__ __forceinline__ uint32_t ld_pair(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, 0, 0);
+}
+
+// As merge_outer over the outer steps [o0, o1), for the vertex pair at byte
+// offset lb (4-aligned, < 256) from the wave-uniform byte base fb; klo/khi
+// fold the keys of the low / high vertex.  Row offsets of the three low bits
+// L stay in per-lane 32-bit offsets, which needs 2 * L * np + 256 < 2^32 for
+// the largest L of the solve (the host checks it).
+__device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint32_t lb,
+                                               int64_t np, uint32_t X, uint32_t o0, uint32_t o1,
+                                               uint32_t &klo, uint32_t &khi) {
+    // kept in a VGPR so that (sum & hmask) | Y is one v_and_or_b32 with Y in an SGPR
+    uint32_t hmask = 0xFFFF0000u;
+    asm volatile("" : "+v"(hmask));
+    const int64_t rb = 2 * np;  // bytes per row
+    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+    uint32_t L = 0, H = Xp;
+    for (int i = 0; i < 3 && H; i++) {
+        const uint32_t lo = H & (~H + 1u);
+        L |= lo;
+        H ^= lo;
+    }
+    const int nl = __popc(L);
+    const uint32_t nH = 1u << __popc(H);
+    const int64_t oX = (int64_t)X * rb;
+    if (nl == 3) {
+        uint32_t yl[8], va[8], vc[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) yl[j] = deposit_bits((uint32_t)j, L);
+        // a row Yh|yl[j] at (fb + Yh*rb) + va[j]; its partner X\Y at
+        // (fb + oX - Yh*rb - L*rb) + vc[j]
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            va[j] = lb + (uint32_t)((int64_t)yl[j] * rb);
+            vc[j] = lb + (uint32_t)((int64_t)(L - yl[j]) * rb);
+        }
+        const int64_t oL = (int64_t)L * rb;
+        uint32_t Yh = deposit_bits(nH - 1u - o0, H);
+        const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Yh == 0 last, apart
+        for (uint32_t o = o0; o < oend; o++) {
+            const int64_t oh = (int64_t)Yh * rb;
+            const __amdgpu_buffer_rsrc_t pa = row_rsrc(fb + oh);
+            const __amdgpu_buffer_rsrc_t pc = row_rsrc(fb + (oX - oh - oL));
+            uint32_t a[8], c[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                a[j] = ld_pair(pa, va[j]);
+                c[j] = ld_pair(pc, vc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t sm = add_u16x2(a[j], c[j]);
+                const uint32_t Y = Yh | yl[j];
+                klo = min(klo, (sm << 16) | Y);
+                khi = min(khi, (sm & hmask) | Y);
+            }
+            Yh = (Yh - 1u) & H;
+        }
+        if (o1 == nH) {  // Yh == 0: the seven nonempty subsets of L
+            const __amdgpu_buffer_rsrc_t pa = row_rsrc(fb);
+            const __amdgpu_buffer_rsrc_t pc = row_rsrc(fb + (oX - oL));
+            uint32_t a[8], c[8];
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                a[j] = ld_pair(pa, va[j]);
+                c[j] = ld_pair(pc, vc[j]);
+            }
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                const uint32_t sm = add_u16x2(a[j], c[j]);
+                klo = min(klo, (sm << 16) | yl[j]);
+                khi = min(khi, (sm & hmask) | yl[j]);
+            }
+        }
+        return;
+    }
+    for (uint32_t Y = L; Y != 0; Y = (Y - 1u) & L) {
+        const uint32_t sm = add_u16x2(ld_pair(row_rsrc(fb + (int64_t)Y * rb), lb),
+                                      ld_pair(row_rsrc(fb + (int64_t)(X ^ Y) * rb), lb));
+        klo = min(klo, (sm << 16) | Y);
+        khi = min(khi, (sm & hmask) | Y);
+    }
+}
+
+// (sum << 16) | Y  ->  (sum << k) | Y; all ones (no pair) stays all ones.
+template <typename P>
+__device__ __forceinline__ P key_packed(uint32_t key, int k) {
+    return key == 0xFFFFFFFFu ? ~(P)0 : (((P)(key >> 16) << k) | (P)(key & 0xFFFFu));
+}
+
+// k_merge_v for 16-bit cells: one wave = 128 consecutive vertices (two per
+// lane) x XG rows.  np is a multiple of 64, so in the last tile of an np
+// that is not a multiple of 128 the upper half of the lanes has no vertices:
+// they read the last pair and store nothing.
+template <typename P, typename DT, int XG>
+__global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict__ f, int64_t np,
+                                                     const uint32_t *__restrict__ xs, int nx,
+                                                     int nsub, int64_t ntiles, int k,
+                                                     DT *__restrict__ D, P *__restrict__ CODE,
+                                                     int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t tile = item / nsub;
+    const int sub = (int)(item - tile * nsub);
+    if (tile >= ntiles) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    const bool live = v < np;
+    const char *fb = reinterpret_cast<const char *>(f + v0);
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+    const DT INF = (DT)~(DT)0;
+    uint32_t klo[XG], khi[XG];
+#pragma unroll
+    for (int jx = 0; jx < XG; jx++) {
+        const int xi = sub * XG + jx;
+        klo[jx] = khi[jx] = 0xFFFFFFFFu;
+        if (xi < nx) {
+            const uint32_t X = xs[xi];
+            merge_outer_h2(fb, lb, np, X, 0u, merge_steps(X), klo[jx], khi[jx]);
+        }
+    }
+    if (!live) return;
+    const int chunk = sub / (kWave / XG);
+    const int loff = (sub % (kWave / XG)) * XG;
+    DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
+    P *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
+#pragma unroll
+    for (int jx = 0; jx < XG; jx++) {
+        const P plo = key_packed<P>(klo[jx], k), phi = key_packed<P>(khi[jx], k);
+        dd[jx] = klo[jx] == 0xFFFFFFFFu ? INF : (DT)(klo[jx] >> 16);
+        cd[jx] = plo;
+        dd[kWave + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
+        cd[kWave + jx] = phi;
+    }
+}
+
+// k_merge_split for 16-bit cells, two vertices per lane (128-vertex tiles).
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__restrict__ f,
+                                                           int64_t np,
+                                                           const uint32_t *__restrict__ xs,
+                                                           int nx, int S, int64_t ntiles, int k,
+                                                           P *__restrict__ M, int nxcd,
+                                                           int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)nx * S;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int xi = rem / S;
+    const int part = rem - xi * S;
+    const uint32_t X = xs[xi];
+    const uint32_t steps = merge_steps(X);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / S);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / S);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    const bool live = v < np;
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+    uint32_t klo = 0xFFFFFFFFu, khi = 0xFFFFFFFFu;
+    merge_outer_h2(reinterpret_cast<const char *>(f + v0), lb, np, X, o0, o1, klo, khi);
+    if (!live) return;
+    atomicMin(&M[(int64_t)xi * np + v], key_packed<P>(klo, k));
+    atomicMin(&M[(int64_t)xi * np + v + 1], key_packed<P>(khi, k));
+}
+
 // Packed accumulators M[xi][v] -> vertex-major staging (label, packed word)
 // through an LDS transpose of 64 rows x 64 vertices per workgroup; rows past
 // nx are INF.
