@@ -1137,9 +1137,42 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
             vc[j] = lb + (uint32_t)((int64_t)(L - yl[j]) * rb);
         }
         const int64_t oL = (int64_t)L * rb;
+        // per-j keys (sum << 16) | Yh: for a fixed j, Y = Yh | yl[j] orders
+        // like Yh, so yl[j] is OR-ed in once at the end
+        uint32_t alo[8], ahi[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) alo[j] = ahi[j] = 0xFFFFFFFFu;
         uint32_t Yh = deposit_bits(nH - 1u - o0, H);
         const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Yh == 0 last, apart
-        for (uint32_t o = o0; o < oend; o++) {
+        uint32_t o = o0;
+        for (; o + 2u <= oend; o += 2u) {  // two outer steps per iteration
+            const uint32_t Y2 = (Yh - 1u) & H;
+            const int64_t oh = (int64_t)Yh * rb, oh2 = (int64_t)Y2 * rb;
+            const __amdgpu_buffer_rsrc_t pa = row_rsrc(fb + oh);
+            const __amdgpu_buffer_rsrc_t pc = row_rsrc(fb + (oX - oh - oL));
+            const __amdgpu_buffer_rsrc_t pa2 = row_rsrc(fb + oh2);
+            const __amdgpu_buffer_rsrc_t pc2 = row_rsrc(fb + (oX - oh2 - oL));
+            uint32_t a[8], c[8], a2[8], c2[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                a[j] = ld_pair(pa, va[j]);
+                c[j] = ld_pair(pc, vc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                a2[j] = ld_pair(pa2, va[j]);
+                c2[j] = ld_pair(pc2, vc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const uint32_t sm = add_u16x2(a[j], c[j]);
+                const uint32_t sm2 = add_u16x2(a2[j], c2[j]);
+                alo[j] = min(alo[j], min((sm << 16) | Yh, (sm2 << 16) | Y2));
+                ahi[j] = min(ahi[j], min((sm & hmask) | Yh, (sm2 & hmask) | Y2));
+            }
+            Yh = (Y2 - 1u) & H;
+        }
+        if (o < oend) {
             const int64_t oh = (int64_t)Yh * rb;
             const __amdgpu_buffer_rsrc_t pa = row_rsrc(fb + oh);
             const __amdgpu_buffer_rsrc_t pc = row_rsrc(fb + (oX - oh - oL));
@@ -1152,11 +1185,9 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const uint32_t sm = add_u16x2(a[j], c[j]);
-                const uint32_t Y = Yh | yl[j];
-                klo = min(klo, (sm << 16) | Y);
-                khi = min(khi, (sm & hmask) | Y);
+                alo[j] = min(alo[j], (sm << 16) | Yh);
+                ahi[j] = min(ahi[j], (sm & hmask) | Yh);
             }
-            Yh = (Yh - 1u) & H;
         }
         if (o1 == nH) {  // Yh == 0: the seven nonempty subsets of L
             const __amdgpu_buffer_rsrc_t pa = row_rsrc(fb);
@@ -1170,9 +1201,14 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
 #pragma unroll
             for (int j = 1; j < 8; j++) {
                 const uint32_t sm = add_u16x2(a[j], c[j]);
-                klo = min(klo, (sm << 16) | yl[j]);
-                khi = min(khi, (sm & hmask) | yl[j]);
+                alo[j] = min(alo[j], sm << 16);
+                ahi[j] = min(ahi[j], sm & hmask);
             }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            klo = min(klo, alo[j] | yl[j]);
+            khi = min(khi, ahi[j] | yl[j]);
         }
         return;
     }
