@@ -59,6 +59,8 @@ struct GMerge {
     int32_t S;       // split merge: parts per row (0: rows merged whole by k_gmerge)
     int32_t tbits;   // table cells: 16 or 32 bits (at the same tbase either way)
     int64_t mbase;   // split merge: accumulator of row xi at Macc[mbase + xi * np + v]
+    int32_t h2;      // 16-bit cells merged two vertices per lane (128-vertex merge tiles)
+    int32_t mtiles;  // merge tiles: np / 64, or ceil(np / 128) when h2
 };
 
 template <typename DT>
@@ -120,6 +122,24 @@ __global__ __launch_bounds__(kBlock) void k_gseed(const GChunk *__restrict__ ch,
 // ------------------------------------------------------------------ merge
 // Same recurrence and packing as merge_one (reader-el.c:2068-2088), per
 // instance: f rows of the instance's table, shift by the instance's k.
+// A wave-uniform 64-bit value (computed from LDS or vector loads) moved to
+// SGPRs, so that what derives from it is scalar too.
+__device__ __forceinline__ int64_t wave_uniform(int64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// merge_outer_h2 on an h2 instance: the keys of vertices v0 + 2 * lane and
+// v0 + 2 * lane + 1 (lanes past np read the last pair).
+__device__ __forceinline__ void merge_cells_h2(const int32_t *__restrict__ f, const GMerge &M,
+                                               int64_t v0, uint32_t X, uint32_t o0, uint32_t o1,
+                                               uint32_t &klo, uint32_t &khi) {
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)__lane_id(), ((int64_t)M.np - v0) / 2 - 1);
+    merge_outer_h2(reinterpret_cast<const char *>(f + M.tbase) + 2 * v0, lb, (int64_t)M.np, X,
+                   o0, o1, klo, khi);
+}
+
 template <typename P, int XG, typename DT>
 __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f,
                                                    const GMerge *__restrict__ mg,
@@ -134,16 +154,45 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     __shared__ int64_t sh_pre[kSegCache];
     const int64_t *pre = cache_pre(item_pre, nmg, sh_pre);
-    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) +
+                         __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if (item >= pre[nmg]) return;
-    const int s = find_seg(pre, nmg, item);
+    const int s = __builtin_amdgcn_readfirstlane(find_seg(pre, nmg, item));
     const GMerge M = mg[s];
-    const int64_t local = item - pre[s];
+    const int64_t local = wave_uniform(item - pre[s]);
     const int64_t tile = local / M.nsub;
     const int sub = (int)(local - tile * M.nsub);
-    const int64_t v = tile * kWave + lane;
     const int k = M.k;
     const int nrows = M.nrows;
+    if (M.h2) {
+        const int64_t v = tile * (2 * kWave) + 2 * lane;
+        uint32_t klo[XG], khi[XG];
+#pragma unroll
+        for (int jx = 0; jx < XG; jx++) {
+            const int xi = sub * XG + jx;
+            klo[jx] = khi[jx] = 0xFFFFFFFFu;
+            if (xi < nrows) {
+                const uint32_t X = xs[M.x0 + xi];
+                merge_cells_h2(f, M, tile * (2 * kWave), X, 0u, merge_steps(X), klo[jx], khi[jx]);
+            }
+        }
+        if (v >= M.np) return;
+        const int chunk = sub / (kWave / XG);
+        const int loff = (sub % (kWave / XG)) * XG;
+        const int64_t dbase = M.dbase0 + (int64_t)chunk * M.np * kWave;
+        DT *dd = chunk_labels<DT>(D, dbase) + v * kWave + loff;
+        P *cd = CODE + dbase + v * kWave + loff;
+        const DT INF = (DT)~(DT)0;
+#pragma unroll
+        for (int jx = 0; jx < XG; jx++) {
+            dd[jx] = klo[jx] == 0xFFFFFFFFu ? INF : (DT)(klo[jx] >> 16);
+            cd[jx] = key_packed<P>(klo[jx], k);
+            dd[kWave + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
+            cd[kWave + jx] = key_packed<P>(khi[jx], k);
+        }
+        return;
+    }
+    const int64_t v = tile * kWave + lane;
     DT dv[XG];
     P pv[XG];
 #pragma unroll
@@ -184,11 +233,12 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_split(
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     __shared__ int64_t sh_pre[kSegCache];
     const int64_t *pre = cache_pre(item_pre, nmg, sh_pre);
-    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + threadIdx.x / kWave;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) +
+                         __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if (item >= pre[nmg]) return;
-    const int s = find_seg(pre, nmg, item);
+    const int s = __builtin_amdgcn_readfirstlane(find_seg(pre, nmg, item));
     const GMerge M = mg[s];
-    const int64_t local = item - pre[s];
+    const int64_t local = wave_uniform(item - pre[s]);
     const int64_t per_tile = (int64_t)M.nrows * M.S;
     const int64_t tile = local / per_tile;
     const int rem = (int)(local - tile * per_tile);
@@ -199,6 +249,16 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_split(
     const uint32_t o0 = (uint32_t)((uint64_t)steps * part / M.S);
     const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / M.S);
     if (o0 >= o1) return;
+    if (M.h2) {
+        const int64_t v = tile * (2 * kWave) + 2 * lane;
+        uint32_t klo = 0xFFFFFFFFu, khi = 0xFFFFFFFFu;
+        merge_cells_h2(f, M, tile * (2 * kWave), X, o0, o1, klo, khi);
+        if (v >= M.np) return;
+        P *acc = &Macc[M.mbase + (int64_t)xi * M.np + v];
+        atomicMin(acc, key_packed<P>(klo, M.k));
+        atomicMin(acc + 1, key_packed<P>(khi, M.k));
+        return;
+    }
     const int64_t v = tile * kWave + lane;
     const P acc = merge_cells<P>(f, M, v, X, o0, o1);
     atomicMin(&Macc[M.mbase + (int64_t)xi * M.np + v], acc);

@@ -583,6 +583,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.S = 0;
             M.tbits = I.tbits;
             M.mbase = 0;
+            M.h2 = merge_h2_ok(I.tbits, I.k, G.np) ? 1 : 0;
+            M.mtiles = M.h2 ? (int32_t)((G.np + 2 * kWave - 1) / (2 * kWave)) : M.ntiles;
             if (splits) {
                 // parts so that a tile has about split_items items (a step
                 // holds many instances' tiles, so few parts pay: fewer
@@ -706,14 +708,14 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         for (int i = 0; i < SG.nmg; i++) {
             GMerge &M = P.mg[SG.mg0 + i];
             M.nsub = M.nchunks * (kWave / P.xg);
-            SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.ntiles * M.nsub);
+            SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.mtiles * M.nsub);
         }
         SG.split_pre.assign(1, 0);
         SG.pack_pre.assign(1, 0);
         for (int i = 0; i < SG.nms; i++) {
             GMerge &M = P.mg[SG.ms0 + i];
             M.nsub = M.nchunks * kWave;
-            SG.split_pre.push_back(SG.split_pre.back() + (int64_t)M.ntiles * M.nrows * M.S);
+            SG.split_pre.push_back(SG.split_pre.back() + (int64_t)M.mtiles * M.nrows * M.S);
             SG.pack_pre.push_back(SG.pack_pre.back() + (int64_t)M.ntiles * M.nchunks);
         }
         const int64_t nitems = group_blocks(gsel) ? 0 : SG.relax_pre.back();

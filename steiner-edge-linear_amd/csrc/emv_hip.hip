@@ -397,25 +397,9 @@ int merge_rows_per_wave(int nx) {
     return 1;
 }
 
-// 16-bit cells take the two-vertices-per-lane merges (keys need k <= 16);
-// EMV_MERGE_H2=0 keeps one vertex per lane.
-bool merge_h2_enabled() {
-    static const int v = [] {
-        const char *e = getenv("EMV_MERGE_H2");
-        return e ? atoi(e) : 1;
-    }();
-    return v != 0;
-}
-
 template <typename T>
 bool use_merge_h2(int k, int64_t np) {
-    // keys hold Y in 16 bits; lane offsets of the low rows fit 32 bits.  The
-    // low rows are the three lowest bits of X without its top bit, at most
-    // bits k-4..k-2 (or below 2^(k-1) for k < 4).
-    const uint64_t lmax = k >= 4 ? 7ull << (k - 4) : (1ull << (k - 1)) - 1ull;
-    const uint64_t span = 2ull * lmax * (uint64_t)np + 256ull;
-    return std::is_same<T, uint16_t>::value && k <= 16 && span < (1ull << 32) &&
-           merge_h2_enabled();
+    return std::is_same<T, uint16_t>::value && merge_h2_ok(16, k, np);
 }
 
 template <typename T, typename P, typename DT, int XG>

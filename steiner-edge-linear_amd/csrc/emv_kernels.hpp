@@ -1220,6 +1220,21 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
     }
 }
 
+// Host: whether a solve with tbits-bit cells, k terminals and row stride np
+// takes the two-vertices-per-lane merge: 16-bit cells, keys that hold Y in 16
+// bits, and lane offsets of the low rows within 32 bits (the low rows are
+// the three lowest bits of X without its top bit, at most bits k-4..k-2).
+// EMV_MERGE_H2=0 keeps one vertex per lane.
+inline bool merge_h2_ok(int tbits, int k, int64_t np) {
+    static const bool on = [] {
+        const char *e = getenv("EMV_MERGE_H2");
+        return !(e && atoi(e) == 0);
+    }();
+    if (!on || tbits != 16 || k > 16 || k < 2) return false;
+    const uint64_t lmax = k >= 4 ? 7ull << (k - 4) : (1ull << (k - 1)) - 1ull;
+    return 2ull * lmax * (uint64_t)np + 256ull < (1ull << 32);
+}
+
 // (sum << 16) | Y  ->  (sum << k) | Y; all ones (no pair) stays all ones.
 template <typename P>
 __device__ __forceinline__ P key_packed(uint32_t key, int k) {
