@@ -46,6 +46,7 @@ extern "C" {
 #define EMV_FORCE_P64   0x8u  /* 64-bit merge packing and relaxation labels */
 #define EMV_WIDE_LABELS 0x10u /* no 16-bit relaxation labels even when they fit */
 #define EMV_TABLE32     0x20u /* int32 table even when the bound allows 16-bit cells */
+#define EMV_MERGE_1V    0x40u /* merges of 16-bit cells one vertex per lane, as wider cells */
 
 typedef struct emv_ctx emv_ctx;
 

@@ -583,7 +583,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.S = 0;
             M.tbits = I.tbits;
             M.mbase = 0;
-            M.h2 = merge_h2_ok(I.tbits, I.k, G.np) ? 1 : 0;
+            M.h2 = (!(I.flags & EMV_MERGE_1V) && merge_h2_ok(I.tbits, I.k, G.np)) ? 1 : 0;
             M.mtiles = M.h2 ? (int32_t)((G.np + 2 * kWave - 1) / (2 * kWave)) : M.ntiles;
             if (splits) {
                 // parts so that a tile has about split_items items (a step
@@ -1498,7 +1498,8 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
     for (const Inst &I : e->inst)
         sig.insert(sig.end(), {(int64_t)I.state, (int64_t)I.stage, (int64_t)I.k,
                                (int64_t)I.graph, (int64_t)I.dbits, (int64_t)I.pbits,
-                               (int64_t)I.tbits, I.abase, (int64_t)I.slot});
+                               (int64_t)I.tbits, I.abase, (int64_t)I.slot,
+                               (int64_t)(I.flags & EMV_MERGE_1V)});
     std::vector<StepPlan> &plans = e->gplans;
     std::vector<char *> dbs(runs.size());
     if (!(sig == e->gplan_sig && plans.size() == runs.size())) {

@@ -73,6 +73,7 @@ struct emv_ctx {
     // relaxation workspace, sized in chunks of 64 rows
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
+    bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
     uint32_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
     uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
     uint32_t qcap = 0;
@@ -398,15 +399,15 @@ int merge_rows_per_wave(int nx) {
 }
 
 template <typename T>
-bool use_merge_h2(int k, int64_t np) {
-    return std::is_same<T, uint16_t>::value && merge_h2_ok(16, k, np);
+bool use_merge_h2(const emv_ctx *c) {
+    return std::is_same<T, uint16_t>::value && c->merge_h2;
 }
 
 template <typename T, typename P, typename DT, int XG>
 int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     const int nsub = (int)(nchunks * (kWave / XG));
     if constexpr (std::is_same<T, uint16_t>::value) {
-        if (use_merge_h2<T>(k, c->np)) {
+        if (use_merge_h2<T>(c)) {
             const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
             int64_t per;
             const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
@@ -447,7 +448,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
     const int64_t S = std::max<int64_t>(1, std::min<int64_t>((512 + nx - 1) / nx, steps / 4));
     bool h2 = false;
     if constexpr (std::is_same<T, uint16_t>::value) {
-        if (use_merge_h2<T>(k, c->np)) {
+        if (use_merge_h2<T>(c)) {
             const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
             int64_t per;
             const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
@@ -968,6 +969,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
     if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
+    c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
     if (flags & EMV_FORCE_P64)
         st.dbits = 64;
     else if (top < 0xFFFFull && !(flags & EMV_WIDE_LABELS))

@@ -22,6 +22,7 @@ KEEP_TABLE = 0x4
 FORCE_P64 = 0x8
 WIDE_LABELS = 0x10
 TABLE32 = 0x20
+MERGE_1V = 0x40
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",
           -5: "ERR_DISCONNECTED", -6: "ERR_RANGE", -7: "ERR_NODEV", -8: "ERR_CAPACITY",

@@ -140,12 +140,14 @@ def test_server_admission_by_footprint():
         assert s.poll(t2).queue_ms > 0.0
 
 
-@pytest.mark.parametrize("wide", [emv.WIDE_LABELS, emv.TABLE32, emv.WIDE_LABELS | emv.TABLE32])
+@pytest.mark.parametrize("wide", [emv.WIDE_LABELS, emv.TABLE32, emv.WIDE_LABELS | emv.TABLE32,
+                                  emv.MERGE_1V])
 def test_batch_label_widths_agree(wide):
     """Instances whose bound allows it relax on 16-bit labels (and keep
     16-bit table cells), the others on 32-bit ones, in the same step (two
-    launches per round).  Forcing 32-bit labels (WIDE_LABELS) or int32 cells
-    (TABLE32) everywhere gives the same tables and records."""
+    launches per round).  Forcing 32-bit labels (WIDE_LABELS), int32 cells
+    (TABLE32) or one vertex per lane in the merges of 16-bit cells (MERGE_1V)
+    everywhere gives the same tables and records."""
     insts = mixed_instances(6, 3)
     # one instance with large weights, whose bound needs 32-bit labels
     n = 700

@@ -62,11 +62,13 @@ def test_regular_1000_full_table(gpu_ctx):
 
 @pytest.mark.parametrize("flags", [emv.WIDE_LABELS, emv.FORCE_P64, emv.FORCE_I64,
                                    emv.FORCE_I64 | emv.FORCE_P64, emv.TABLE32,
-                                   emv.TABLE32 | emv.WIDE_LABELS])
+                                   emv.TABLE32 | emv.WIDE_LABELS, emv.MERGE_1V,
+                                   emv.MERGE_1V | emv.WIDE_LABELS])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
 def test_wide_paths_bit_exact(gpu_ctx, name, flags):
     """Every width combination (16/32/64-bit labels, 32/64-bit merge
-    accumulators, 16-bit/int32/int64 table) gives the reference's table."""
+    accumulators, 16-bit/int32/int64 table) and both merges of 16-bit cells
+    (two vertices per lane, or one: MERGE_1V) give the reference's table."""
     inst = load_instance(name)
     print(f"case {name} flags={flags}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst, flags)
