@@ -80,6 +80,9 @@ typedef struct {
     int64_t block_dist_bits; /* 16 / 32: block visits after the first apply in-block distances
                                 from the lowered border slots; 0: they sweep */
     double merge_split_ms;   /* part of merge_ms in the layers merged in parts (rows of >= 128 pairs) */
+    int64_t relative_label_layers; /* layers relaxed on 32-bit labels relative to a per-row base:
+                                      64-bit labels were needed by the bound, but every row's merged
+                                      values span less than 2^32 - 1 - max w */
 } emv_result;
 
 /* Context on HIP device `device`. */

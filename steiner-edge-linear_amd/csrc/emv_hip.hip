@@ -74,6 +74,8 @@ struct emv_ctx {
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
+    void *d_rspan = nullptr;  // per staged row: min and max merged value
+    size_t rspan_cap = 0;
     uint32_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
     uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
     uint32_t qcap = 0;
@@ -91,6 +93,12 @@ struct emv_ctx {
 };
 
 namespace {
+
+// A switch in the environment: set and not "0".
+bool env_flag(const char *name) {
+    const char *e = getenv(name);
+    return e && atoi(e) != 0;
+}
 
 void set_err(emv_ctx *c, const char *fmt, ...) {
     if (!c) return;
@@ -190,6 +198,8 @@ int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
 void free_ws(emv_ctx *c) {
     dfree(c->d_M);
     c->m_cap = 0;
+    dfree(c->d_rspan);
+    c->rspan_cap = 0;
     dfree(c->d_q[0]);
     dfree(c->d_q[1]);
     c->qcap = 0;
@@ -387,6 +397,7 @@ struct SolveState {
     int tbytes;
     int pbits;
     int dbits;
+    bool rel32;  // 64-bit labels: try 32-bit labels relative to each row's base, per batch
 };
 
 // Rows per merge wave: few rows per wave when the batch has few rows, so
@@ -485,6 +496,47 @@ int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) 
     }
 }
 
+// 64-bit labels: whether every row of the staged batch has merged values
+// spanning less than 2^32 - 1 - max w.  Relaxation keeps each label between
+// its row's minimum and maximum merged value, so the rows then relax exactly
+// on 32-bit labels relative to the row minimum (INF = all ones stays out of
+// reach of label + w).  If so, the relative labels replace the staging.
+template <typename P>
+int relative_labels_fit(emv_ctx *c, int nx, int64_t nchunks, int k, bool *fit) {
+    *fit = false;
+    const size_t need = 2 * sizeof(unsigned long long) * (size_t)nchunks * kWave;
+    if (c->rspan_cap < need) {
+        dfree(c->d_rspan);
+        c->rspan_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_rspan, need));
+        c->rspan_cap = need;
+    }
+    unsigned long long *rs = (unsigned long long *)c->d_rspan;
+    const int64_t rows = nchunks * kWave;
+    hipLaunchKernelGGL(k_fill_span<unsigned long long>, dim3((unsigned)((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       c->stream, rs, rows);
+    const int64_t vper = 64;  // vertices per wave
+    const int64_t vblocks = (c->n + vper * (kBlock / kWave) - 1) / (vper * (kBlock / kWave));
+    hipLaunchKernelGGL((k_row_span<P>), dim3((unsigned)vblocks, (unsigned)nchunks), dim3(kBlock), 0,
+                       c->stream, (const P *)c->d_code, c->np, (int)c->n, nx, k, rs, vper);
+    HIPCHK(c, hipGetLastError());
+    std::vector<unsigned long long> h((size_t)rows * 2);
+    HIPCHK(c, hipMemcpyAsync(h.data(), rs, sizeof(unsigned long long) * h.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const unsigned long long lim = 0xFFFFFFFFull - 1ull - (unsigned long long)c->maxw;
+    for (int xi = 0; xi < nx; xi++)
+        if (h[2 * xi + 1] < h[2 * xi] || h[2 * xi + 1] - h[2 * xi] >= lim) return EMV_OK;
+    const int64_t cells = nchunks * c->np * kWave;
+    const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL((k_rel_labels<P>), dim3(fb), dim3(kBlock), 0, c->stream,
+                       (const P *)c->d_code, c->np, (int)c->n, nx, k,
+                       (const unsigned long long *)rs, (uint32_t *)c->d_D, cells);
+    HIPCHK(c, hipGetLastError());
+    *fit = true;
+    return EMV_OK;
+}
+
 template <typename T, typename P, typename DT>
 int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *res,
                std::vector<hipEvent_t> &ev) {
@@ -518,16 +570,30 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
-            // relax
-            CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
-            HIPCHK(c, hipEventRecord(e2, c->stream));
-            // records + unpack
             const int64_t vtiles = (n + kWave - 1) / kWave;
             int64_t per;
             const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
-            hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0, c->stream,
-                               (const DT *)c->d_D, (const P *)c->d_code, np, c->d_xs, nx, (int)n,
-                               k, c->d_rp, c->d_adj, f, b, c->nxcd, per, vtiles);
+            bool rel = false;
+            if constexpr (sizeof(DT) == 8) {
+                if (st.rel32) CHK(relative_labels_fit<P>(c, nx, nchunks, k, &rel));
+            }
+            if (rel) {
+                // relax + records on 32-bit labels relative to each row's base
+                CHK(relax_chunks<uint32_t>(c, nchunks, &res->relax_rounds));
+                HIPCHK(c, hipEventRecord(e2, c->stream));
+                hipLaunchKernelGGL((k_unpack_v<T, uint32_t, P>), dim3(grid), dim3(kBlock), 0,
+                                   c->stream, (const uint32_t *)c->d_D, (const P *)c->d_code, np,
+                                   c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
+                                   vtiles, (const unsigned long long *)c->d_rspan);
+                res->relative_label_layers++;
+            } else {
+                CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
+                HIPCHK(c, hipEventRecord(e2, c->stream));
+                hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0,
+                                   c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
+                                   c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
+                                   vtiles, (const unsigned long long *)nullptr);
+            }
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e3, c->stream));
         }
@@ -970,6 +1036,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
     c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
+    st.rel32 = !(flags & EMV_FORCE_P64) && !env_flag("EMV_NO_REL32");
     if (flags & EMV_FORCE_P64)
         st.dbits = 64;
     else if (top < 0xFFFFull && !(flags & EMV_WIDE_LABELS))

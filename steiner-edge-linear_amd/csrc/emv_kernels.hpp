@@ -1396,6 +1396,7 @@ __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, 
 // Vertex-major staging -> table rows f[X], b[X] through an LDS transpose of
 // 64 vertices x 64 rows per workgroup; each wave decides the records of 16
 // vertices (cell_record) on the way into LDS.
+// rspan (relative labels, else null): row xi's labels are f - rspan[2 xi].
 template <typename T, typename DT, typename P>
 __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      const P *__restrict__ CODE, int64_t np,
@@ -1403,7 +1404,8 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      int n, int k, const int32_t *__restrict__ rp,
                                                      const Edge *__restrict__ adj,
                                                      T *__restrict__ f, int32_t *__restrict__ b,
-                                                     int nxcd, int64_t per_xcd, int64_t vtiles) {
+                                                     int nxcd, int64_t per_xcd, int64_t vtiles,
+                                                     const unsigned long long *__restrict__ rspan) {
     __shared__ DT sd[kWave][kWave + 1];
     __shared__ int32_t sc[kWave][kWave + 1];
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
@@ -1417,6 +1419,8 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
     const int64_t v0 = vt * kWave;
     const DT *Dc = D + chunk * np * kWave;
     const bool real = chunk * kWave + lane < nx;
+    // relative labels: compare with the merged value less the row's base
+    const P pbase = (rspan && real) ? ((P)rspan[2 * (chunk * kWave + lane)] << k) : (P)0;
     // the labels and packed words of kPre vertices are loaded together, then
     // their records decided one vertex at a time
     constexpr int kPre = 4;
@@ -1436,7 +1440,8 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
             const int vv = grp + (r0 + u) * (kBlock / kWave);
             const int64_t v = v0 + vv;
             int32_t code = 0;
-            if (v < n) code = cell_record<DT, P>(Dc, Ls[u], Ps[u], k, real, (int)v, rp, adj);
+            if (v < n)
+                code = cell_record<DT, P>(Dc, Ls[u], Ps[u] - pbase, k, real, (int)v, rp, adj);
             sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }
@@ -1449,9 +1454,67 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
         const int64_t v = v0 + lane;
         if (xi < nx && v < n) {
             const int64_t X = xs[xi];
-            f[X * np + v] = (T)sd[lane][xl];
+            const unsigned long long base = rspan ? rspan[2 * xi] : 0ull;
+            f[X * np + v] = (T)((unsigned long long)sd[lane][xl] + base);
             b[X * np + v] = sc[lane][xl];
         }
+    }
+}
+
+// Per row xi of the staged batch: the minimum and maximum merged value over
+// the real vertices, rspan[2 xi] and rspan[2 xi + 1] (start: all ones / 0).
+// Relaxation only lowers labels and a relaxed label is at least the row's
+// minimum merged value (f[X][v] = min over u of merged[u] + dist(u, v)), so
+// every label of the row stays in [min, max].
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_row_span(const P *__restrict__ CODE, int64_t np,
+                                                     int n, int nx, int k,
+                                                     unsigned long long *__restrict__ rspan,
+                                                     int64_t vper) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t chunk = blockIdx.y;
+    const int64_t xi = chunk * kWave + lane;
+    if (xi >= nx) return;
+    const int64_t v0 = ((int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * vper;
+    const int64_t v1 = min(v0 + vper, (int64_t)n);
+    unsigned long long lo = ~0ull, hi = 0ull;
+    for (int64_t v = v0; v < v1; v++) {
+        const unsigned long long x = (unsigned long long)(CODE[(chunk * np + v) * kWave + lane] >> k);
+        lo = x < lo ? x : lo;
+        hi = x > hi ? x : hi;
+    }
+    if (v0 < v1) {
+        atomicMin(&rspan[2 * xi], lo);
+        atomicMax(&rspan[2 * xi + 1], hi);
+    }
+}
+
+template <typename U>
+__global__ void k_fill_span(U *__restrict__ rspan, int64_t rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows) {
+        rspan[2 * i] = ~(U)0;
+        rspan[2 * i + 1] = (U)0;
+    }
+}
+
+// Relative 32-bit labels from the merged values: merged - rspan[2 xi] for
+// the real vertices of real rows, else INF.
+template <typename P>
+__global__ void k_rel_labels(const P *__restrict__ CODE, int64_t np, int n, int nx, int k,
+                             const unsigned long long *__restrict__ rspan,
+                             uint32_t *__restrict__ D, int64_t cells) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < cells; i += stride) {
+        const int lane = (int)(i & (kWave - 1));
+        const int64_t cv = i / kWave;  // chunk * np + v
+        const int64_t chunk = cv / np;
+        const int64_t v = cv - chunk * np;
+        const int64_t xi = chunk * kWave + lane;
+        uint32_t L = 0xFFFFFFFFu;
+        if (xi < nx && v < n) L = (uint32_t)((unsigned long long)(CODE[i] >> k) - rspan[2 * xi]);
+        D[i] = L;
     }
 }
 

@@ -44,7 +44,7 @@ class EmvResult(ctypes.Structure):
         ("device_bytes", ctypes.c_int64), ("npairs", ctypes.c_int64), ("pairs", _i64p),
         ("relax_blocks", ctypes.c_int64), ("unpack_ms", ctypes.c_double),
         ("singleton_rounds", ctypes.c_int64), ("block_dist_bits", ctypes.c_int64),
-        ("merge_split_ms", ctypes.c_double),
+        ("merge_split_ms", ctypes.c_double), ("relative_label_layers", ctypes.c_int64),
     ]
 
 
@@ -114,6 +114,7 @@ class Solution:
         self.singleton_rounds = r.singleton_rounds
         self.block_dist_bits = r.block_dist_bits
         self.merge_split_ms = r.merge_split_ms
+        self.relative_label_layers = r.relative_label_layers
         self.pairs = pairs
 
 
