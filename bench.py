@@ -3,8 +3,9 @@
 A step is one full EMV solve (singleton shortest paths, every merge and
 relaxation layer; no traceback) of the plain single-instance workload:
 random connected 8-regular graph, n = 262,144, m = 1,048,576, weights
-uniform in [1, 1000], k = 14 uniform terminals (generator seed 2), int32
-table of 2^14 x 2^18 cells.  The graph is resident in HBM before timing.
+uniform in [1, 1000], k = 14 uniform terminals (generator seed 2), a table
+of 2^14 x 2^18 cells (16-bit: the distance bound is below 0xFFFF).  The
+graph is resident in HBM before timing.
 
 Prints one JSON line.  `value` = DP cells (2^k * n) per second of wall time
 over the timed steps.  `roofline` covers the dominant kernel (the kernel
@@ -30,9 +31,13 @@ the plain one-instance-at-a-time GPU path (Context.solve):
              latency from arrival to completion; the plain serial figures
              replay the same arrival times through a FIFO single server
              whose service times are the measured plain solve times.
+A third leg, "group", solves the group Steiner workload through the
+reduction (1024 x 1024 grid, w ~ U[1,1000], 13 groups of 32 vertices,
+seed 5; virtual terminals joined by edges of weight M = 1 + sum of w) and
+reports its time and OPT_group = OPT(reduced) - 13 M.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-traffic]
-                    [--legs plain,batch,serve] [--queries 24]
+                    [--legs plain,batch,serve,group] [--queries 24]
 """
 import argparse
 import json
@@ -311,6 +316,39 @@ def batch_leg(args):
                       seconds=round(tune_s, 2), table=table))
 
 
+def group_leg(args):
+    """Group Steiner through the reduction: 1024 x 1024 grid, 13 groups of 32."""
+    import emv
+    import workloads
+
+    R = C = 1024
+    g, size = 13, 32
+    n, e, grp, nn, red, kk, M = workloads.group_instance(R, C, g, size, 1000, 5)
+    ctx = emv.Context(0)
+    t0 = time.perf_counter()
+    ctx.load_edges(nn, red)
+    t_load = time.perf_counter() - t0
+    times, sol = [], None
+    for i in range(args.group_steps):
+        t0 = time.perf_counter()
+        sol = ctx.solve(kk)
+        times.append(time.perf_counter() - t0)
+        log(f"group leg: solve {i} {times[-1]:.3f}s kernel={sol.kernel_ms:.1f} ms "
+            f"merge={sol.merge_ms:.1f} relax={sol.relax_ms:.1f} records={sol.unpack_ms:.1f} "
+            f"relative_label_layers={sol.relative_label_layers} cost={sol.cost}")
+    ctx.close()
+    return dict(
+        workload=f"group Steiner via reduction: {R}x{C} grid, 4-neighbour, w~U[1,1000], "
+                 f"{g} groups of {size} distinct vertices, seed 5; virtual terminals joined "
+                 "by weight-M edges, M = 1 + sum of w",
+        n=n, m=len(e), reduced_n=nn, reduced_m=len(red), k=len(kk), M=M,
+        dp_cells=(1 << len(kk)) * nn, solve_s=sum(times) / len(times), graph_load_s=t_load,
+        kernel_ms=sol.kernel_ms, merge_ms=sol.merge_ms, relax_ms=sol.relax_ms,
+        records_ms=sol.unpack_ms, relax_rounds=sol.relax_rounds, table_bytes=sol.table_bytes,
+        label_bits=sol.label_bits, relative_label_layers=sol.relative_label_layers,
+        relax_blocks=sol.relax_blocks, cost=sol.cost, group_cost=sol.cost - g * M)
+
+
 def _percentile(xs, p):
     import numpy as np
 
@@ -425,7 +463,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=WORKLOAD["n"])
     ap.add_argument("--k", type=int, default=WORKLOAD["k"])
-    ap.add_argument("--legs", default="plain,batch,serve")
+    ap.add_argument("--legs", default="plain,batch,serve,group")
+    ap.add_argument("--group-steps", type=int, default=2)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--queries", type=int, default=24)
@@ -455,6 +494,8 @@ def main():
         out["batched"] = batch_leg(args)
     if "serve" in legs:
         out["served"] = serve_leg(args)
+    if "group" in legs:
+        out["group"] = group_leg(args)
     if not args.no_cpu and "plain" in legs:
         log("cpu baseline (reference reader-el)...")
         out["cpu_baseline"] = cpu_baseline()

@@ -408,6 +408,24 @@ def test_large_weights_64bit_labels(gpu_ctx, shape):
 
 
 
+@pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16)])
+@pytest.mark.parametrize("flags", [0, emv.MERGE_1V])
+def test_many_terminals_16bit_cells(gpu_ctx, n, d, k, flags):
+    """k = 15 and 16 on 16-bit cells: the two-vertex merge's keys hold Y in
+    16 bits and its low rows in 32-bit lane offsets at the largest k it
+    takes; cells, records and cost equal the oracle's (and the one-vertex
+    merge's)."""
+    e = emv.regular_graph(n, d, 1000, 7)
+    kk = emv.terminals(n, k, 7)
+    print(f"case n={n} d={d} k={k} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    assert sol.table_bytes == 2
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
 @pytest.mark.parametrize("blocks", ["1", "0"])
 def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks):
     """Group-reduced grid with weights up to 2^20: the bound needs 64-bit
