@@ -113,6 +113,21 @@ int emv_solve(emv_ctx *ctx, int64_t k, const int64_t *kk, uint32_t flags,
 int emv_read_rows(emv_ctx *ctx, int64_t X0, int64_t nrows, int64_t *f_out,
                   int64_t *code_out);
 
+/* After a solve with EMV_KEEP_TABLE: the table as a canonical file (the
+ * reference keeps its table in memory only; this is the dump format of the
+ * path's table, for comparing solves byte for byte).  Little-endian:
+ *   64-byte header: char magic[8] = "EMVTABLE"; uint32 version = 1;
+ *   uint32 cell_bytes = 8; int64 k; int64 n; int64 inf = INT64_MAX
+ *   (reader-el.c:69); uint32 records (0/1); uint32 reserved = 0;
+ *   int64 cost (the answer cell, reader-el.c:2137); 8 bytes of zeros;
+ *   then f[X][v] as int64 for X = 0 .. 2^k-1, v = 0 .. n-1 (row 0 is INF);
+ *   then, with EMV_DUMP_RECORDS, the record codes of every cell as int32 in
+ *   the same order (as emv_read_rows).
+ * EMV_ERR_STATE without a kept table; EMV_ERR_ARG if the file cannot be
+ * written. */
+#define EMV_DUMP_RECORDS 0x1u
+int emv_write_table(emv_ctx *ctx, const char *path, uint32_t flags);
+
 /* Releases the table kept by EMV_KEEP_TABLE. */
 int emv_release_table(emv_ctx *ctx);
 

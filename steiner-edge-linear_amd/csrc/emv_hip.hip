@@ -69,6 +69,7 @@ struct emv_ctx {
     size_t f_cap = 0, b_cap = 0;
     int64_t np = 0;
     int tbytes = 0;  // nonzero while a kept table is readable
+    int64_t kept_cost = 0;  // answer cell of the last solve
     int64_t tk = 0;
     // relaxation workspace, sized in chunks of 64 rows
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
@@ -1186,7 +1187,69 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
                         c->ws_chunks * np * (int64_t)(kWave * 16 + 8);
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    c->kept_cost = cost;
     if (!(flags & EMV_KEEP_TABLE)) invalidate_table(c);
+    return EMV_OK;
+}
+
+extern "C" int emv_write_table(emv_ctx *c, const char *path, uint32_t flags) {
+    if (!c || !path) return EMV_ERR_ARG;
+    if (!c->d_f || c->tk == 0) {
+        set_err(c, "no table kept (solve with EMV_KEEP_TABLE)");
+        return EMV_ERR_STATE;
+    }
+    FILE *out = fopen(path, "wb");
+    if (!out) {
+        set_err(c, "cannot open '%s' for writing", path);
+        return EMV_ERR_ARG;
+    }
+    const int64_t k = c->tk, n = c->n, S2 = (int64_t)1 << k;
+    const int records = (flags & EMV_DUMP_RECORDS) ? 1 : 0;
+    unsigned char hdr[64];
+    memset(hdr, 0, sizeof(hdr));
+    memcpy(hdr, "EMVTABLE", 8);
+    const uint32_t version = 1, cell_bytes = 8, rec = (uint32_t)records;
+    const int64_t inf = INT64_MAX, cost = c->kept_cost;
+    memcpy(hdr + 8, &version, 4);
+    memcpy(hdr + 12, &cell_bytes, 4);
+    memcpy(hdr + 16, &k, 8);
+    memcpy(hdr + 24, &n, 8);
+    memcpy(hdr + 32, &inf, 8);
+    memcpy(hdr + 40, &rec, 4);
+    memcpy(hdr + 48, &cost, 8);
+    bool ok = fwrite(hdr, 1, sizeof(hdr), out) == sizeof(hdr);
+    // rows in batches through emv_read_rows (device cells -> int64, INF = INT64_MAX)
+    const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(S2, (64ll << 20) / (8 * std::max<int64_t>(n, 1))));
+    std::vector<int64_t> fb((size_t)(batch * n)), cb;
+    std::vector<int32_t> c32;
+    for (int64_t X = 0; ok && X < S2; X += batch) {
+        const int64_t nr = std::min(batch, S2 - X);
+        const int st = emv_read_rows(c, X, nr, fb.data(), nullptr);
+        if (st != EMV_OK) {
+            fclose(out);
+            return st;
+        }
+        ok = fwrite(fb.data(), 8, (size_t)(nr * n), out) == (size_t)(nr * n);
+    }
+    if (ok && records) {
+        cb.resize((size_t)(batch * n));
+        c32.resize((size_t)(batch * n));
+        for (int64_t X = 0; ok && X < S2; X += batch) {
+            const int64_t nr = std::min(batch, S2 - X);
+            const int st = emv_read_rows(c, X, nr, nullptr, cb.data());
+            if (st != EMV_OK) {
+                fclose(out);
+                return st;
+            }
+            for (int64_t i = 0; i < nr * n; i++) c32[(size_t)i] = (int32_t)cb[(size_t)i];
+            ok = fwrite(c32.data(), 4, (size_t)(nr * n), out) == (size_t)(nr * n);
+        }
+    }
+    if (fclose(out) != 0) ok = false;
+    if (!ok) {
+        set_err(c, "short write to '%s'", path);
+        return EMV_ERR_ARG;
+    }
     return EMV_OK;
 }
 

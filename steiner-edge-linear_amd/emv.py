@@ -23,6 +23,7 @@ FORCE_P64 = 0x8
 WIDE_LABELS = 0x10
 TABLE32 = 0x20
 MERGE_1V = 0x40
+DUMP_RECORDS = 0x1  # emv_write_table flag
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",
           -5: "ERR_DISCONNECTED", -6: "ERR_RANGE", -7: "ERR_NODEV", -8: "ERR_CAPACITY",
@@ -76,6 +77,7 @@ def lib():
                                 ctypes.POINTER(EmvResult)]
         L.emv_read_rows.argtypes = [vp, ctypes.c_int64, ctypes.c_int64, _i64p, _i64p]
         L.emv_release_table.argtypes = [vp]
+        L.emv_write_table.argtypes = [vp, ctypes.c_char_p, ctypes.c_uint32]
         L.emv_sssp.argtypes = [vp, ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(ctypes.c_double)]
         L.emv_result_free.argtypes = [ctypes.POINTER(EmvResult)]
         L.emv_result_free.restype = None
@@ -172,6 +174,11 @@ class Context:
 
     def release_table(self):
         self._check(lib().emv_release_table(self._h))
+
+    def write_table(self, path, records=False):
+        """The kept table as a canonical table file (read_table_file)."""
+        self._check(lib().emv_write_table(self._h, os.fsencode(path),
+                                          DUMP_RECORDS if records else 0))
 
     def sssp(self, s):
         d = np.empty(self.n, dtype=np.int64)
@@ -438,3 +445,31 @@ class Server(_Engine):
 
     def footprint(self, k):
         return _bind_engine().emv_request_footprint(self._h, k)
+
+
+TABLE_MAGIC = b"EMVTABLE"
+TABLE_HEADER = 64
+
+
+def read_table_file(path):
+    """A canonical table file (emv_write_table): dict with k, n, inf, cost,
+    f (2^k x n int64, INF = INT64_MAX) and code (2^k x n int32, or None)."""
+    with open(path, "rb") as fh:
+        hdr = fh.read(TABLE_HEADER)
+    if len(hdr) != TABLE_HEADER or hdr[:8] != TABLE_MAGIC:
+        raise ValueError(f"{path}: not a table file")
+    version, cell_bytes = np.frombuffer(hdr[8:16], dtype="<u4")
+    k, n, inf = np.frombuffer(hdr[16:40], dtype="<i8")
+    records = int(np.frombuffer(hdr[40:44], dtype="<u4")[0])
+    cost = int(np.frombuffer(hdr[48:56], dtype="<i8")[0])
+    if version != 1 or cell_bytes != 8:
+        raise ValueError(f"{path}: version {version}, cell bytes {cell_bytes} not supported")
+    rows, n = 1 << int(k), int(n)
+    f = np.fromfile(path, dtype="<i8", count=rows * n, offset=TABLE_HEADER).reshape(rows, n)
+    code = None
+    if records:
+        code = np.fromfile(path, dtype="<i4", count=rows * n,
+                           offset=TABLE_HEADER + 8 * rows * n).reshape(rows, n)
+    if f.size != rows * n or (records and code.size != rows * n):
+        raise ValueError(f"{path}: truncated")
+    return dict(k=int(k), n=n, inf=int(inf), cost=cost, f=f, code=code)

@@ -8,7 +8,7 @@
  * prefix ("gpu:", "group:").
  *
  *   emv-reader -in <file.stp> -el [-list] [-seed s] [-dijkstra]
- *              [-groups <file>] [-device d]
+ *              [-groups <file>] [-device d] [-dump <table file>]
  */
 #define _GNU_SOURCE
 #include <stdio.h>
@@ -27,6 +27,7 @@ static const char *cmd_legend[] = {"no operation", "Dijkstra Single-Source-Short
                                    "Erickson-Monma-Veinott"};
 
 static emv_ctx *ctx = NULL;
+static const char *dumpfile = NULL; /* -dump: canonical table file after the solve */
 static double ctx_init_ms = 0.0;
 static int device_id = 0;
 
@@ -114,7 +115,9 @@ static el_out erickson_gpu(const stp_query *q, int list)
     }
 
     emv_result res;
-    gpu_check(emv_solve(ctx, q->k, q->kk, list ? EMV_LIST : 0u, &res), "emv_solve");
+    gpu_check(emv_solve(ctx, q->k, q->kk, (list ? EMV_LIST : 0u) | (dumpfile ? EMV_KEEP_TABLE : 0u),
+                        &res),
+              "emv_solve");
     double bytes = res.merge_alg_bytes + res.relax_alg_bytes;
     printf("erickson: [zero: %.2lf ms] [kernel: %.2lf ms %.2lfGiB/s] ", res.zero_ms,
            res.kernel_ms, bytes / (res.kernel_ms / 1e3) / (double)(1 << 30));
@@ -138,6 +141,12 @@ static el_out erickson_gpu(const stp_query *q, int list)
            res.relax_ms, res.relax_ms > 0 ? res.relax_alg_bytes / (res.relax_ms * 1e6) : 0.0,
            (long)res.relax_rounds, res.unpack_ms, cells, cells / (res.kernel_ms / 1e3),
            (double)res.device_bytes / (double)(1 << 30));
+    if (dumpfile) {
+        gpu_check(emv_write_table(ctx, dumpfile, EMV_DUMP_RECORDS), "emv_write_table");
+        gpu_check(emv_release_table(ctx), "emv_release_table");
+        printf("dump: [file: %s] [k: %ld] [n: %ld] [rows: %ld] [records: int32]\n", dumpfile,
+               (long)q->k, (long)q->n, (long)(1l << q->k));
+    }
     fflush(stdout);
     emv_result_free(&res);
     return out;
@@ -224,6 +233,10 @@ int main(int argc, char **argv)
             if (f == argc - 1) FAIL("groups file missing from command line");
             groupfile = argv[++f];
         }
+        if (!strcmp(argv[f], "-dump")) {
+            if (f == argc - 1) FAIL("table file missing from command line");
+            dumpfile = argv[++f];
+        }
         if (!strcmp(argv[f], "-device")) {
             if (f == argc - 1) FAIL("device id missing from command line");
             device_id = atoi(argv[++f]);
@@ -238,6 +251,7 @@ int main(int argc, char **argv)
                    "\t-list : Output Steiner tree\n"
                    "\t-groups <file> : group Steiner tree by reduction (lines \"g v1 v2 ...\")\n"
                    "\t-device <id> : HIP device\n"
+                   "\t-dump <file> : write the DP table and records as a table file (with -el)\n"
                    "\n",
                    argv[0]);
             return 0;

@@ -164,3 +164,46 @@ def test_python_reduction_matches_host_reduction_rule():
     n2, red, kk, M = emv.group_reduce(3, e, [[0], [2]])
     assert M == 13 and n2 == 5 and list(kk) == [3, 4]
     assert red.tolist()[-2:] == [[3, 0, 13], [4, 2, 13]]
+
+
+def _table_bytes(k, n, cost, f, code):
+    hdr = bytearray(64)
+    hdr[0:8] = b"EMVTABLE"
+    hdr[8:16] = np.array([1, 8], dtype="<u4").tobytes()
+    hdr[16:40] = np.array([k, n, 2 ** 63 - 1], dtype="<i8").tobytes()
+    hdr[40:44] = np.array([0 if code is None else 1], dtype="<u4").tobytes()
+    hdr[48:56] = np.array([cost], dtype="<i8").tobytes()
+    body = np.asarray(f, dtype="<i8").tobytes()
+    if code is not None:
+        body += np.asarray(code, dtype="<i4").tobytes()
+    return bytes(hdr) + body
+
+
+def test_table_file_reader_follows_the_header(tmp_path):
+    """read_table_file parses the documented layout (64-byte little-endian
+    header, int64 rows, optional int32 records) of the oracle's table, and
+    rejects a wrong magic and a truncated file."""
+    import oracle
+
+    n, k = 30, 4
+    e = emv.regular_graph(n, 3, 50, 5)
+    kk = emv.terminals(n, k, 5)
+    cost, f, code = oracle.emv(n, e, kk)
+    for rec in (None, code):
+        p = tmp_path / "t.emvt"
+        p.write_bytes(_table_bytes(k, n, cost, f, rec))
+        t = emv.read_table_file(p)
+        assert (t["k"], t["n"], t["inf"], t["cost"]) == (k, n, 2 ** 63 - 1, cost)
+        np.testing.assert_array_equal(t["f"], np.asarray(f).reshape(1 << k, n))
+        if rec is None:
+            assert t["code"] is None
+        else:
+            np.testing.assert_array_equal(t["code"], np.asarray(code).reshape(1 << k, n))
+    bad = tmp_path / "bad.emvt"
+    bad.write_bytes(b"EMVTABLX" + _table_bytes(k, n, cost, f, None)[8:])
+    with pytest.raises(ValueError):
+        emv.read_table_file(bad)
+    short = tmp_path / "short.emvt"
+    short.write_bytes(_table_bytes(k, n, cost, f, None)[:-8])
+    with pytest.raises(ValueError):
+        emv.read_table_file(short)

@@ -260,6 +260,41 @@ def test_cli_k2_path_and_dijkstra():
         assert rc == 0, err
 
 
+@pytest.mark.parametrize("name", ["small-instances/B/b01", CFG1])
+def test_table_file_round_trip(gpu_ctx, tmp_path, name):
+    """emv_write_table writes the canonical table file: header fields, every
+    cell (INF = INT64_MAX) and record code equal the oracle's, and the CLI's
+    -dump writes the same bytes."""
+    inst = load_instance(name)
+    print(f"case table file {name}", flush=True)
+    gpu_ctx.load_edges(inst.n, inst.edges)
+    sol = gpu_ctx.solve(inst.terminals, emv.KEEP_TABLE)
+    path = tmp_path / "t.emvt"
+    gpu_ctx.write_table(path, records=True)
+    gpu_ctx.release_table()
+    t = emv.read_table_file(path)
+    assert (t["k"], t["n"], t["inf"], t["cost"]) == (inst.k, inst.n, 2 ** 63 - 1, sol.cost)
+    assert os.path.getsize(path) == 64 + (1 << inst.k) * inst.n * 12
+    ocost, of, ocode = oracle.emv(inst.n, inst.edges, inst.terminals)
+    np.testing.assert_array_equal(t["f"], of)
+    np.testing.assert_array_equal(t["code"], ocode)
+    stp = tmp_path / "x.stp"
+    write_stp(str(stp), inst)
+    cli = tmp_path / "cli.emvt"
+    rc, out, err = run_reader(["-in", str(stp), "-el", "-dump", str(cli)])
+    assert rc == 0, err
+    assert any(x.startswith("dump: [file: ") for x in out.splitlines())
+    assert cli.read_bytes() == path.read_bytes()
+
+
+def test_write_table_needs_a_kept_table(gpu_ctx, tmp_path):
+    inst = load_instance("small-instances/B/b01")
+    gpu_ctx.load_edges(inst.n, inst.edges)
+    gpu_ctx.solve(inst.terminals)
+    with pytest.raises(emv.EmvError, match="ERR_STATE"):
+        gpu_ctx.write_table(tmp_path / "t.emvt")
+
+
 def test_cli_group_reduction(tmp_path):
     """Group Steiner by reduction: cost = OPT(reduced) - g*M and the printed
     group tree is a real tree of that cost touching every group."""
