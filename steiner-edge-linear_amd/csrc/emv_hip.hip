@@ -403,11 +403,29 @@ struct SolveState {
 
 // Rows per merge wave: few rows per wave when the batch has few rows, so
 // that every tile still spreads over many waves.
+// Rows with at least this many pairs take the split merge (EMV_SPLIT_PAIRS
+// overrides kSplitPairs; measurements).
+uint32_t split_pairs() {
+    static const uint32_t v = [] {
+        const char *e = getenv("EMV_SPLIT_PAIRS");
+        const long x = e ? atol(e) : 0;
+        return x > 0 ? (uint32_t)x : kSplitPairs;
+    }();
+    return v;
+}
+
+// EMV_MERGE_ROWS=<1|2|4|8> caps it (measurements).
 int merge_rows_per_wave(int nx) {
-    if (nx >= 1024) return 8;
-    if (nx >= 256) return 4;
-    if (nx >= 64) return 2;
-    return 1;
+    static const int cap = [] {
+        const char *e = getenv("EMV_MERGE_ROWS");
+        const int v = e ? atoi(e) : 8;
+        return (v == 1 || v == 2 || v == 4) ? v : 8;
+    }();
+    int r = 1;
+    if (nx >= 1024) r = 8;
+    else if (nx >= 256) r = 4;
+    else if (nx >= 64) r = 2;
+    return std::min(r, cap);
 }
 
 template <typename T>
@@ -487,7 +505,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
 
 template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
-    if (((1u << (m - 1)) - 1u) >= kSplitPairs)
+    if (((1u << (m - 1)) - 1u) >= split_pairs())
         return launch_merge_split<T, P, DT>(c, f, nx, nchunks, k, m);
     switch (merge_rows_per_wave(nx)) {
     case 8: return launch_merge_xg<T, P, DT, 8>(c, f, nx, nchunks, k);
@@ -565,7 +583,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 HIPCHK(c, hipEventCreate(pe));
                 ev.push_back(*pe);
             }
-            c->batch_split.push_back(((1u << (m - 1)) - 1u) >= kSplitPairs ? 1 : 0);
+            c->batch_split.push_back(((1u << (m - 1)) - 1u) >= split_pairs() ? 1 : 0);
             HIPCHK(c, hipEventRecord(e0, c->stream));
             // merge -> staging
             CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));

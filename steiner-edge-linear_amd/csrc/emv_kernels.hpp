@@ -906,6 +906,18 @@ __device__ __forceinline__ uint32_t deposit_bits(uint32_t x, uint32_t mask) {
     return r;
 }
 
+// The eight subsets of a three-bit mask L, yl[j] = the bits of L that j
+// selects (bit i of j picks the i-th lowest bit of L): deposit_bits(j, L)
+// without its loop.
+__device__ __forceinline__ void low_subsets(uint32_t L, uint32_t yl[8]) {
+    const uint32_t b0 = L & (~L + 1u);
+    const uint32_t r = L ^ b0;
+    const uint32_t b1 = r & (~r + 1u);
+    const uint32_t b2 = r ^ b1;
+#pragma unroll
+    for (int j = 0; j < 8; j++) yl[j] = ((j & 1) ? b0 : 0u) | ((j & 2) ? b1 : 0u) | ((j & 4) ? b2 : 0u);
+}
+
 // Pairs of X, enumerated for cheap addressing.  Y runs over the nonempty
 // subsets of Xp = X without its top bit (each unordered pair once, Y the side
 // without the top bit, which is the smaller one).  Row offsets of disjoint
@@ -934,11 +946,9 @@ __device__ __forceinline__ P merge_outer(const T *__restrict__ fv, int64_t np, u
     if (nl == 3) {
         uint32_t yl[8];
         int64_t ol[8];
+        low_subsets(L, yl);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            yl[j] = deposit_bits((uint32_t)j, L);
-            ol[j] = (int64_t)yl[j] * np;
-        }
+        for (int j = 0; j < 8; j++) ol[j] = (int64_t)yl[j] * np;
         const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Yh == 0 last, apart
         for (uint32_t o = o0; o < oend; o++) {
             const int64_t oh = (int64_t)Yh * np;
@@ -1127,8 +1137,7 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
     const int64_t oX = (int64_t)X * rb;
     if (nl == 3) {
         uint32_t yl[8], va[8], vc[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) yl[j] = deposit_bits((uint32_t)j, L);
+        low_subsets(L, yl);
         // a row Yh|yl[j] at (fb + Yh*rb) + va[j]; its partner X\Y at
         // (fb + oX - Yh*rb - L*rb) + vc[j]
 #pragma unroll
