@@ -346,6 +346,7 @@ def group_leg(args):
         kernel_ms=sol.kernel_ms, merge_ms=sol.merge_ms, relax_ms=sol.relax_ms,
         records_ms=sol.unpack_ms, relax_rounds=sol.relax_rounds, table_bytes=sol.table_bytes,
         label_bits=sol.label_bits, relative_label_layers=sol.relative_label_layers,
+        compact_merge_layers=sol.compact_merge_layers,
         relax_blocks=sol.relax_blocks, cost=sol.cost, group_cost=sol.cost - g * M)
 
 

@@ -83,6 +83,8 @@ typedef struct {
     int64_t relative_label_layers; /* layers relaxed on 32-bit labels relative to a per-row base:
                                       64-bit labels were needed by the bound, but every row's merged
                                       values span less than 2^32 - 1 - max w */
+    int64_t compact_merge_layers;  /* int64 tables: layer batches merged from the int32 copy of
+                                      the rows (cell - row base) instead of the int64 cells */
 } emv_result;
 
 /* Context on HIP device `device`. */

@@ -77,6 +77,12 @@ struct emv_ctx {
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
     void *d_rspan = nullptr;  // per staged row: min and max merged value
     size_t rspan_cap = 0;
+    // int64 tables: compact int32 copy of the rows (cell - row base) that the
+    // merges read while every row fits
+    int32_t *d_fcopy = nullptr;
+    int64_t *d_fbase = nullptr;
+    size_t fcopy_cap = 0, fbase_cap = 0;
+    bool fcopy_on = false;
     uint32_t *d_flag[2] = {nullptr, nullptr};  // per (chunk, vertex) activity, alternating rounds
     uint32_t *d_q[2] = {nullptr, nullptr};    // worklists of flagged items, alternating rounds
     uint32_t qcap = 0;
@@ -168,6 +174,10 @@ int free_table(emv_ctx *c) {
     dfree(c->d_f);
     dfree(c->d_b);
     c->f_cap = c->b_cap = 0;
+    dfree(c->d_fcopy);
+    dfree(c->d_fbase);
+    c->fcopy_cap = c->fbase_cap = 0;
+    c->fcopy_on = false;
     c->tbytes = 0;
     c->tk = 0;
     return EMV_OK;
@@ -195,6 +205,22 @@ int ensure_table(emv_ctx *c, size_t fbytes, size_t bbytes) {
 }
 
 int64_t chunk_cells(const emv_ctx *c) { return c->np * kWave; }
+
+int ensure_fcopy(emv_ctx *c, int64_t cells, int64_t rows) {
+    if (c->fcopy_cap < (size_t)cells * 4) {
+        dfree(c->d_fcopy);
+        c->fcopy_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_fcopy, (size_t)cells * 4));
+        c->fcopy_cap = (size_t)cells * 4;
+    }
+    if (c->fbase_cap < (size_t)rows * 8) {
+        dfree(c->d_fbase);
+        c->fbase_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_fbase, (size_t)rows * 8));
+        c->fbase_cap = (size_t)rows * 8;
+    }
+    return EMV_OK;
+}
 
 void free_ws(emv_ctx *c) {
     dfree(c->d_M);
@@ -428,6 +454,12 @@ int merge_rows_per_wave(int nx) {
     return std::min(r, cap);
 }
 
+// Row bases of the compact copy when the merges read it (int32 view).
+template <typename T>
+const int64_t *merge_fbase(const emv_ctx *c) {
+    return (std::is_same<T, int32_t>::value && c->fcopy_on) ? c->d_fbase : nullptr;
+}
+
 template <typename T>
 bool use_merge_h2(const emv_ctx *c) {
     return std::is_same<T, uint16_t>::value && c->merge_h2;
@@ -453,7 +485,7 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
     hipLaunchKernelGGL((k_merge_v<T, P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream, f,
                        c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
-                       per);
+                       per, merge_fbase<T>(c));
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
 }
@@ -493,7 +525,8 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
         int64_t per;
         const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
         hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                           c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per);
+                           c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
+                           merge_fbase<T>(c));
     }
     const int64_t vtiles = c->np / kWave;
     hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
@@ -521,8 +554,9 @@ int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) 
 // on 32-bit labels relative to the row minimum (INF = all ones stays out of
 // reach of label + w).  If so, the relative labels replace the staging.
 template <typename P>
-int relative_labels_fit(emv_ctx *c, int nx, int64_t nchunks, int k, bool *fit) {
+int relative_labels_fit(emv_ctx *c, int nx, int64_t nchunks, int k, bool *fit, bool *copy_fit) {
     *fit = false;
+    *copy_fit = false;
     const size_t need = 2 * sizeof(unsigned long long) * (size_t)nchunks * kWave;
     if (c->rspan_cap < need) {
         dfree(c->d_rspan);
@@ -544,8 +578,12 @@ int relative_labels_fit(emv_ctx *c, int nx, int64_t nchunks, int k, bool *fit) {
                              hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const unsigned long long lim = 0xFFFFFFFFull - 1ull - (unsigned long long)c->maxw;
-    for (int xi = 0; xi < nx; xi++)
+    bool cfit = true;  // every final cell of the rows also fits the int32 compact copy
+    for (int xi = 0; xi < nx; xi++) {
         if (h[2 * xi + 1] < h[2 * xi] || h[2 * xi + 1] - h[2 * xi] >= lim) return EMV_OK;
+        if (h[2 * xi + 1] - h[2 * xi] >= (unsigned long long)INT32_MAX) cfit = false;
+    }
+    *copy_fit = cfit;
     const int64_t cells = nchunks * c->np * kWave;
     const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
     hipLaunchKernelGGL((k_rel_labels<P>), dim3(fb), dim3(kBlock), 0, c->stream,
@@ -585,17 +623,22 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             }
             c->batch_split.push_back(((1u << (m - 1)) - 1u) >= split_pairs() ? 1 : 0);
             HIPCHK(c, hipEventRecord(e0, c->stream));
-            // merge -> staging
-            CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
+            // merge -> staging (int64 tables: from the compact copy while it holds)
+            if (sizeof(T) == 8 && c->fcopy_on) {
+                CHK((launch_merge<int32_t, P, DT>(c, c->d_fcopy, nx, nchunks, k, m)));
+                res->compact_merge_layers++;
+            } else
+                CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             const int64_t vtiles = (n + kWave - 1) / kWave;
             int64_t per;
             const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
-            bool rel = false;
+            bool rel = false, copy_fit = false;
             if constexpr (sizeof(DT) == 8) {
-                if (st.rel32) CHK(relative_labels_fit<P>(c, nx, nchunks, k, &rel));
+                if (st.rel32) CHK(relative_labels_fit<P>(c, nx, nchunks, k, &rel, &copy_fit));
             }
+            if (!rel || !copy_fit) c->fcopy_on = false;
             if (rel) {
                 // relax + records on 32-bit labels relative to each row's base
                 CHK(relax_chunks<uint32_t>(c, nchunks, &res->relax_rounds));
@@ -603,7 +646,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 hipLaunchKernelGGL((k_unpack_v<T, uint32_t, P>), dim3(grid), dim3(kBlock), 0,
                                    c->stream, (const uint32_t *)c->d_D, (const P *)c->d_code, np,
                                    c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
-                                   vtiles, (const unsigned long long *)c->d_rspan);
+                                   vtiles, (const unsigned long long *)c->d_rspan,
+                                   c->fcopy_on ? c->d_fcopy : nullptr, c->d_fbase);
                 res->relative_label_layers++;
             } else {
                 CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
@@ -611,7 +655,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0,
                                    c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
                                    c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
-                                   vtiles, (const unsigned long long *)nullptr);
+                                   vtiles, (const unsigned long long *)nullptr,
+                                   (int32_t *)nullptr, (int64_t *)nullptr);
             }
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e3, c->stream));
@@ -1092,6 +1137,23 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     else
         unpack_singletons<int64_t>(c, k, (int64_t *)c->d_f, c->d_b);
     HIPCHK(c, hipGetLastError());
+    c->fcopy_on = false;
+    if (st.tbytes == 8 && st.dbits == 64 && st.rel32 && !env_flag("EMV_NO_COPY32")) {
+        // compact int32 copy for the merges: singleton rows as they are
+        // (base 0) while every distance fits
+        bool ok = true;
+        for (int64_t t = 0; t < k; t++) ok = ok && rowmax[t] < (unsigned long long)INT32_MAX;
+        if (ok) {
+            CHK(ensure_fcopy(c, cells, st.S2));
+            hipLaunchKernelGGL(k_fill<int64_t>, dim3((unsigned)std::min<int64_t>((st.S2 + kBlock - 1) / kBlock, 4096)),
+                               dim3(kBlock), 0, c->stream, c->d_fbase, st.S2, (int64_t)0);
+            hipLaunchKernelGGL(k_copy_singletons<int32_t>, dim3((unsigned)((n + kBlock - 1) / kBlock), (unsigned)k),
+                               dim3(kBlock), 0, c->stream, (const int64_t *)c->d_f, np, (int)n, (int)k,
+                               c->d_fcopy);
+            HIPCHK(c, hipGetLastError());
+            c->fcopy_on = true;
+        }
+    }
     HIPCHK(c, hipEventRecord(e_single1, c->stream));
     c->tbytes = st.tbytes;
     c->tk = k;
@@ -1201,7 +1263,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
                            (cb + ab) * dn * ((double)st.S2 - (double)k - 1.0);
     res->relax_alg_bytes = ((double)st.S2 - 1.0) * (cb * dn + cb * 2 * dm + (cb + ab) * dn) +
                            (double)k * (4.0 * (dn + 1) + 16.0 * dm);
-    res->device_bytes = (int64_t)(c->f_cap + c->b_cap) +
+    res->device_bytes = (int64_t)(c->f_cap + c->b_cap + c->fcopy_cap + c->fbase_cap) +
                         c->ws_chunks * np * (int64_t)(kWave * 16 + 8);
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();

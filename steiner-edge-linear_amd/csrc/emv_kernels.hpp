@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace emv {
 
 struct Edge {
@@ -1007,6 +1009,85 @@ __device__ __forceinline__ P merge_one(const T *__restrict__ fv, int64_t np, uin
     return merge_outer<T, P>(fv, np, X, k, 0u, merge_steps(X));
 }
 
+// merge_outer over a compact copy of an int64 table: row Y holds int32
+// cells f[Y][v] - fbase[Y] (the row's base, int64).  A pair's sum adds the
+// two rows' bases back, so the packed minimum is the one over the int64
+// cells, at half their bytes.
+template <typename P>
+__device__ __forceinline__ P merge_outer_rel(const int32_t *__restrict__ fv,
+                                             const int64_t *__restrict__ fbase, int64_t np,
+                                             uint32_t X, int k, uint32_t o0, uint32_t o1) {
+    const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
+    uint32_t L = 0, H = Xp;
+    for (int i = 0; i < 3 && H; i++) {
+        const uint32_t lo = H & (~H + 1u);
+        L |= lo;
+        H ^= lo;
+    }
+    const int nl = __popc(L);
+    const uint32_t nH = 1u << __popc(H);
+    const int64_t oX = (int64_t)X * np;
+    P acc = ~(P)0;
+    if (nl == 3) {
+        uint32_t yl[8];
+        int64_t ol[8];
+        low_subsets(L, yl);
+#pragma unroll
+        for (int j = 0; j < 8; j++) ol[j] = (int64_t)yl[j] * np;
+        uint32_t Yh = deposit_bits(nH - 1u - o0, H);
+        const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Yh == 0 last, apart
+        for (uint32_t o = o0; o < oend; o++) {
+            const int64_t oh = (int64_t)Yh * np;
+            const int32_t *pa = fv + oh;
+            const int32_t *pc = fv + (oX - oh);
+            int32_t a[8], c[8];
+            int64_t bs[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                a[j] = pa[ol[j]];
+                c[j] = pc[-ol[j]];
+                const uint32_t Y = Yh | yl[j];
+                bs[j] = fbase[Y] + fbase[X ^ Y];
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const P sum = (P)(uint32_t)a[j] + (P)(uint32_t)c[j] + (P)bs[j];
+                const P s = (sum << k) | (P)(Yh | yl[j]);
+                acc = s < acc ? s : acc;
+            }
+            Yh = (Yh - 1u) & H;
+        }
+        if (o1 == nH) {  // Yh == 0: the seven nonempty subsets of L
+            const int32_t *pc = fv + oX;
+#pragma unroll
+            for (int j = 1; j < 8; j++) {
+                const P sum = (P)(uint32_t)fv[ol[j]] + (P)(uint32_t)pc[-ol[j]] +
+                              (P)(fbase[yl[j]] + fbase[X ^ yl[j]]);
+                const P s = (sum << k) | (P)yl[j];
+                acc = s < acc ? s : acc;
+            }
+        }
+        return acc;
+    }
+    for (uint32_t Y = L; Y != 0; Y = (Y - 1u) & L) {
+        const P sum = (P)(uint32_t)fv[(int64_t)Y * np] + (P)(uint32_t)fv[(int64_t)(X ^ Y) * np] +
+                      (P)(fbase[Y] + fbase[X ^ Y]);
+        const P s = (sum << k) | (P)Y;
+        acc = s < acc ? s : acc;
+    }
+    return acc;
+}
+
+// All pairs of X from the table (fbase null) or from its compact copy.
+template <typename T, typename P>
+__device__ __forceinline__ P merge_any(const T *__restrict__ fv, const int64_t *__restrict__ fbase,
+                                       int64_t np, uint32_t X, int k, uint32_t o0, uint32_t o1) {
+    if constexpr (std::is_same<T, int32_t>::value) {
+        if (fbase) return merge_outer_rel<P>(fv, fbase, np, X, k, o0, o1);
+    }
+    return merge_outer<T, P>(fv, np, X, k, o0, o1);
+}
+
 // Work item = one wave: 64 consecutive vertices (one per lane, 256 B
 // coalesced row loads) x XG consecutive rows of the layer batch.  Items are
 // tile-major and dealt to XCDs in contiguous ranges, so the waves of an XCD
@@ -1019,9 +1100,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
                                                     const uint32_t *__restrict__ xs, int nx,
                                                     int nsub, int64_t ntiles, int k,
                                                     DT *__restrict__ D, P *__restrict__ CODE,
-                                                    int nxcd, int64_t per_xcd) {
+                                                    int nxcd, int64_t per_xcd,
+                                                    const int64_t *__restrict__ fbase) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
     const int64_t tile = item / nsub;
@@ -1038,7 +1120,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
         dv[jx] = INF;
         pv[jx] = ~(P)0;
         if (xi < nx) {
-            pv[jx] = merge_one<T, P>(fv, np, xs[xi], k);
+            const uint32_t X = xs[xi];
+            pv[jx] = merge_any<T, P>(fv, fbase, np, X, k, 0u, merge_steps(X));
             dv[jx] = (DT)(pv[jx] >> k);
         }
     }
@@ -1066,9 +1149,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f,
                                                         const uint32_t *__restrict__ xs, int nx,
                                                         int S, int64_t ntiles, int k,
                                                         P *__restrict__ M, int nxcd,
-                                                        int64_t per_xcd) {
+                                                        int64_t per_xcd,
+                                                        const int64_t *__restrict__ fbase) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
     const int64_t per_tile = (int64_t)nx * S;
@@ -1083,7 +1167,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f,
     const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / S);
     if (o0 >= o1) return;
     const int64_t v = tile * kWave + lane;
-    const P acc = merge_outer<T, P>(f + v, np, X, k, o0, o1);
+    const P acc = merge_any<T, P>(f + v, fbase, np, X, k, o0, o1);
     atomicMin(&M[(int64_t)xi * np + v], acc);
 }
 
@@ -1414,7 +1498,9 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      const Edge *__restrict__ adj,
                                                      T *__restrict__ f, int32_t *__restrict__ b,
                                                      int nxcd, int64_t per_xcd, int64_t vtiles,
-                                                     const unsigned long long *__restrict__ rspan) {
+                                                     const unsigned long long *__restrict__ rspan,
+                                                     int32_t *__restrict__ fcopy,
+                                                     int64_t *__restrict__ fbase) {
     __shared__ DT sd[kWave][kWave + 1];
     __shared__ int32_t sc[kWave][kWave + 1];
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
@@ -1466,7 +1552,25 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
             const unsigned long long base = rspan ? rspan[2 * xi] : 0ull;
             f[X * np + v] = (T)((unsigned long long)sd[lane][xl] + base);
             b[X * np + v] = sc[lane][xl];
+            // compact copy (relative labels that fit int32): the relative
+            // label itself, with the row's base
+            if (fcopy) {
+                fcopy[X * np + v] = (int32_t)sd[lane][xl];
+                if (v == 0) fbase[X] = (int64_t)base;
+            }
         }
+    }
+}
+
+// Compact copy of the singleton rows 2^t (base 0): int32 distances.
+template <typename CT>
+__global__ void k_copy_singletons(const int64_t *__restrict__ f, int64_t np, int n, int k,
+                                  CT *__restrict__ fcopy) {
+    const int t = blockIdx.y;
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < k && v < n) {
+        const int64_t X = 1ll << t;
+        fcopy[X * np + v] = (CT)f[X * np + v];
     }
 }
 

@@ -46,6 +46,7 @@ class EmvResult(ctypes.Structure):
         ("relax_blocks", ctypes.c_int64), ("unpack_ms", ctypes.c_double),
         ("singleton_rounds", ctypes.c_int64), ("block_dist_bits", ctypes.c_int64),
         ("merge_split_ms", ctypes.c_double), ("relative_label_layers", ctypes.c_int64),
+        ("compact_merge_layers", ctypes.c_int64),
     ]
 
 
@@ -117,6 +118,7 @@ class Solution:
         self.block_dist_bits = r.block_dist_bits
         self.merge_split_ms = r.merge_split_ms
         self.relative_label_layers = r.relative_label_layers
+        self.compact_merge_layers = r.compact_merge_layers
         self.pairs = pairs
 
 

@@ -466,8 +466,9 @@ def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks):
     """Group-reduced grid with weights up to 2^20: the bound needs 64-bit
     labels (M is about 9e8), but every row's merged values span less than
     2^32, so the layers relax on 32-bit labels relative to each row's
-    minimum.  Cells, records and cost equal the oracle's, with blocks and
-    with vertex-wise rounds, and with 64-bit labels forced (FORCE_P64)."""
+    minimum, and merge from the int32 copy of the rows (cell - row base).
+    Cells, records and cost equal the oracle's, with blocks and with
+    vertex-wise rounds, and with 64-bit labels forced (FORCE_P64)."""
     import workloads
 
     n, e, grp, nn, red, kk, M = workloads.group_instance(30, 30, 5, 6, 1 << 20, 35)
@@ -477,13 +478,15 @@ def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks):
     sol, f, code = solve_full(gpu_ctx, inst)
     assert sol.label_bits == 64 and sol.table_bytes == 8
     assert sol.relative_label_layers == len(kk) - 1
+    # every layer merges from the int32 copy of the int64 rows
+    assert sol.compact_merge_layers == len(kk) - 1
     assert (sol.relax_blocks > 0) == (blocks == "1")
     ocost, of, ocode = oracle.emv(nn, red, kk)
     assert sol.cost == ocost
     np.testing.assert_array_equal(f, of)
     np.testing.assert_array_equal(code, ocode)
     s64, f64, c64 = solve_full(gpu_ctx, inst, emv.FORCE_P64)
-    assert s64.relative_label_layers == 0
+    assert s64.relative_label_layers == 0 and s64.compact_merge_layers == 0
     np.testing.assert_array_equal(f64, of)
     np.testing.assert_array_equal(c64, ocode)
 
