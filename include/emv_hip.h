@@ -47,6 +47,7 @@ extern "C" {
 #define EMV_WIDE_LABELS 0x10u /* no 16-bit relaxation labels even when they fit */
 #define EMV_TABLE32     0x20u /* int32 table even when the bound allows 16-bit cells */
 #define EMV_MERGE_1V    0x40u /* merges of 16-bit cells one vertex per lane, as wider cells */
+#define EMV_MERGE_ROWS  0x80u /* merges of 16-bit cells row by row, not in stars of rows sharing a base set */
 
 typedef struct emv_ctx emv_ctx;
 

@@ -75,6 +75,11 @@ struct emv_ctx {
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
+    bool merge_star = false;  // ... in stars of rows sharing a base set
+    const uint32_t *h_xs = nullptr;         // host copy of the batch's rows (star building)
+    std::vector<MergeStar> h_stars;         // stars of the last batch (kept for the async copy)
+    MergeStar *d_stars = nullptr;
+    size_t stars_cap = 0;
     void *d_rspan = nullptr;  // per staged row: min and max merged value
     size_t rspan_cap = 0;
     // int64 tables: compact int32 copy of the rows (cell - row base) that the
@@ -227,6 +232,8 @@ void free_ws(emv_ctx *c) {
     c->m_cap = 0;
     dfree(c->d_rspan);
     c->rspan_cap = 0;
+    dfree(c->d_stars);
+    c->stars_cap = 0;
     dfree(c->d_q[0]);
     dfree(c->d_q[1]);
     c->qcap = 0;
@@ -536,8 +543,108 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
     return EMV_OK;
 }
 
+// EMV_MERGE_STAR=0: 16-bit merges one row per item (k_merge_h2 /
+// k_merge_split_h2) instead of stars of rows sharing a base set.
+bool merge_star_enabled() {
+    static const bool on = [] {
+        const char *e = getenv("EMV_MERGE_STAR");
+        return !(e && atoi(e) == 0);
+    }();
+    return on;
+}
+
+// Stars covering the batch's rows xs[0..nx) (all of m elements): each
+// uncovered row, in order, takes the base set S = X without the element that
+// leaves the most uncovered rows S | a in the batch, and up to kStarMax of
+// them (X first, then by increasing a) form its star.
+void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out) {
+    out.clear();
+    std::vector<int32_t> idx((size_t)1 << k, -1);
+    for (int i = 0; i < nx; i++) idx[xs[i]] = i;
+    std::vector<char> covered((size_t)nx, 0);
+    auto free_row = [&](uint32_t Y) { return idx[Y] >= 0 && !covered[(size_t)idx[Y]]; };
+    for (int i = 0; i < nx; i++) {
+        if (covered[(size_t)i]) continue;
+        const uint32_t X = xs[i];
+        uint32_t bestS = 0;
+        int best = -1;
+        for (uint32_t rest = X; rest; rest &= rest - 1) {
+            const uint32_t S = X ^ (rest & (~rest + 1u));
+            int cnt = 0;
+            for (int b = 0; b < k; b++)
+                if (!(S >> b & 1u) && free_row(S | (1u << b))) cnt++;
+            if (cnt > best) {
+                best = cnt;
+                bestS = S;
+            }
+        }
+        MergeStar st;
+        memset(&st, 0, sizeof(st));
+        st.S = bestS;
+        st.a[0] = X ^ bestS;
+        st.xi[0] = i;
+        covered[(size_t)i] = 1;
+        st.r = 1;
+        for (int b = 0; b < k && st.r < (uint32_t)kStarMax; b++) {
+            const uint32_t Y = bestS | (1u << b);
+            if ((bestS >> b & 1u) || !free_row(Y)) continue;
+            st.a[st.r] = 1u << b;
+            st.xi[st.r] = idx[Y];
+            covered[(size_t)idx[Y]] = 1;
+            st.r++;
+        }
+        out.push_back(st);
+    }
+}
+
+template <typename P, typename DT>
+int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, int k, int m) {
+    build_stars(c->h_xs, nx, k, c->h_stars);
+    const int nstars = (int)c->h_stars.size();
+    if (c->stars_cap < (size_t)nstars) {
+        dfree(c->d_stars);
+        c->stars_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_stars, sizeof(MergeStar) * (size_t)nstars));
+        c->stars_cap = (size_t)nstars;
+    }
+    // h_stars is rebuilt only by the next batch's merge, after this batch's
+    // relaxation has synchronised the stream
+    HIPCHK(c, hipMemcpyAsync(c->d_stars, c->h_stars.data(), sizeof(MergeStar) * (size_t)nstars,
+                             hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = sizeof(P) * (size_t)nx * (size_t)c->np;
+    if (c->m_cap < bytes) {
+        dfree(c->d_M);
+        c->m_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_M, bytes));
+        c->m_cap = bytes;
+    }
+    const int64_t cells = (int64_t)nx * c->np;
+    const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL(k_fill<P>, dim3(fb), dim3(kBlock), 0, c->stream, (P *)c->d_M, cells, ~(P)0);
+    // outer steps of a star: subsets of S's high part (S has m - 1 elements)
+    const int64_t steps = (m - 1 <= 3) ? 1 : (1ll << (m - 1 - 3));
+    const int64_t parts =
+        std::max<int64_t>(1, std::min<int64_t>((512 + nstars - 1) / nstars, steps / 4));
+    const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+    int64_t per;
+    const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
+    hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
+                       (const MergeStar *)c->d_stars, nstars, (int)parts, ntiles, k,
+                       (P *)c->d_M, c->nxcd, per);
+    const int64_t vtiles = c->np / kWave;
+    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
+                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
+                       vtiles);
+    HIPCHK(c, hipGetLastError());
+    return EMV_OK;
+}
+
 template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
+    if constexpr (std::is_same<T, uint16_t>::value) {
+        if (use_merge_h2<T>(c) && c->merge_star && c->h_xs)
+            return launch_merge_star<P, DT>(c, f, nx, nchunks, k, m);
+    }
     if (((1u << (m - 1)) - 1u) >= split_pairs())
         return launch_merge_split<T, P, DT>(c, f, nx, nchunks, k, m);
     switch (merge_rows_per_wave(nx)) {
@@ -623,12 +730,15 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             }
             c->batch_split.push_back(((1u << (m - 1)) - 1u) >= split_pairs() ? 1 : 0);
             HIPCHK(c, hipEventRecord(e0, c->stream));
+            c->h_xs = xs.data() + off;
             // merge -> staging (int64 tables: from the compact copy while it holds)
             if (sizeof(T) == 8 && c->fcopy_on) {
                 CHK((launch_merge<int32_t, P, DT>(c, c->d_fcopy, nx, nchunks, k, m)));
                 res->compact_merge_layers++;
-            } else
+            } else {
                 CHK((launch_merge<T, P, DT>(c, f, nx, nchunks, k, m)));
+            }
+            c->h_xs = nullptr;
             res->merge_launches++;
             HIPCHK(c, hipEventRecord(e1, c->stream));
             const int64_t vtiles = (n + kWave - 1) / kWave;
@@ -1100,6 +1210,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
     c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
+    c->merge_star = c->merge_h2 && !(flags & EMV_MERGE_ROWS) && merge_star_enabled();
     st.rel32 = !(flags & EMV_FORCE_P64) && !env_flag("EMV_NO_REL32");
     if (flags & EMV_FORCE_P64)
         st.dbits = 64;

@@ -1416,6 +1416,164 @@ __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__res
     atomicMin(&M[(int64_t)xi * np + v + 1], key_packed<P>(khi, k));
 }
 
+// ------------------------------------------- merge by stars (16-bit cells)
+// Every pair {Y, X\Y} of X has exactly one side without a chosen element a of
+// X: with S = X \ {a}, the pairs of X are {T, (S\T) | a} over the nonempty
+// T subset of S.  Rows X_i = S | a_i that share S therefore share every row T:
+// a star of r rows loads each T once and its r partners, (1 + r) loads per r
+// pairs instead of 2 r.  The packed key takes the side without X's top bit
+// (the smaller set), as merge_outer does: T, unless top(X) is in T (a_i below
+// top(S) and top(S) in T), then X ^ T.
+constexpr int kStarMax = 4;  // rows per star
+
+struct MergeStar {
+    uint32_t S;                 // the shared set, m - 1 elements
+    uint32_t r;                 // rows in the star, 1 .. kStarMax
+    uint32_t a[kStarMax];       // the extra element of each row (one bit)
+    int32_t xi[kStarMax];       // the row's index in the layer batch
+};
+
+// Outer steps of a star: subsets Th of S's high part H (S without its three
+// lowest bits L), as merge_steps.
+__device__ __forceinline__ uint32_t star_steps(uint32_t S) {
+    const int r = __popc(S);
+    return r <= 3 ? 1u : (1u << (r - 3));
+}
+
+// Item = (tile, star, part), tile-major per XCD; parts fold into M with
+// unsigned atomicMin like k_merge_split_h2.  Two vertices per lane.
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_star_h2(
+    const uint16_t *__restrict__ f, int64_t np, const MergeStar *__restrict__ stars, int nstars,
+    int parts, int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)nstars * parts;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int g = rem / parts;
+    const int part = rem - g * parts;
+    const uint32_t S = stars[g].S;
+    const int r = (int)stars[g].r;
+    uint32_t a[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) a[i] = stars[g].a[i];
+    const uint32_t steps = star_steps(S);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / parts);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    const bool live = v < np;
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+    const char *fb = reinterpret_cast<const char *>(f + v0);
+    uint32_t hmask = 0xFFFF0000u;
+    asm volatile("" : "+v"(hmask));
+    const int64_t rb = 2 * np;
+    const uint32_t topS = 1u << (31 - __clz(S));
+    uint32_t L = 0, H = S;
+    for (int q = 0; q < 3 && H; q++) {
+        const uint32_t lo = H & (~H + 1u);
+        L |= lo;
+        H ^= lo;
+    }
+    const uint32_t nH = 1u << __popc(H);
+    uint32_t klo[kStarMax], khi[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) klo[i] = khi[i] = 0xFFFFFFFFu;
+    // the key's set for row i: T, or X_i ^ T when top(X_i) = top(S) lies in T
+    auto yset = [&](int i, uint32_t T) -> uint32_t {
+        return (a[i] < topS && (T & topS)) ? ((S | a[i]) ^ T) : T;
+    };
+    // |S| >= 4: L holds S's three lowest bits with a fourth above them, so
+    // L <= 7 * 2^(k-4) and its row offsets fit the 32-bit lane offsets (the
+    // host's merge_h2_ok bound); smaller S take the loop below
+    if (__popc(S) >= 4) {
+        uint32_t yl[8], va[8], vc[8];
+        low_subsets(L, yl);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            va[j] = lb + (uint32_t)((int64_t)yl[j] * rb);
+            vc[j] = lb + (uint32_t)((int64_t)(L - yl[j]) * rb);
+        }
+        uint32_t Th = deposit_bits(nH - 1u - o0, H);
+        const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Th == 0 last, apart
+        for (uint32_t o = o0; o < oend; o++) {
+            const __amdgpu_buffer_rsrc_t pt = row_rsrc(fb + (int64_t)Th * rb);
+            uint32_t t[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) t[j] = ld_pair(pt, va[j]);
+#pragma unroll
+            for (int i = 0; i < kStarMax; i++) {
+                if (i < r) {
+                    const __amdgpu_buffer_rsrc_t pp =
+                        row_rsrc(fb + (int64_t)((H ^ Th) | a[i]) * rb);
+                    uint32_t pv[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) pv[j] = ld_pair(pp, vc[j]);
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const uint32_t sm = add_u16x2(t[j], pv[j]);
+                        const uint32_t Y = yset(i, Th | yl[j]);
+                        klo[i] = min(klo[i], (sm << 16) | Y);
+                        khi[i] = min(khi[i], (sm & hmask) | Y);
+                    }
+                }
+            }
+            Th = (Th - 1u) & H;
+        }
+        if (o1 == nH) {  // Th == 0: the seven nonempty subsets of L
+            const __amdgpu_buffer_rsrc_t pt = row_rsrc(fb);
+            uint32_t t[8];
+#pragma unroll
+            for (int j = 1; j < 8; j++) t[j] = ld_pair(pt, va[j]);
+#pragma unroll
+            for (int i = 0; i < kStarMax; i++) {
+                if (i < r) {
+                    const __amdgpu_buffer_rsrc_t pp = row_rsrc(fb + (int64_t)(H | a[i]) * rb);
+                    uint32_t pv[8];
+#pragma unroll
+                    for (int j = 1; j < 8; j++) pv[j] = ld_pair(pp, vc[j]);
+#pragma unroll
+                    for (int j = 1; j < 8; j++) {
+                        const uint32_t sm = add_u16x2(t[j], pv[j]);
+                        const uint32_t Y = yset(i, yl[j]);
+                        klo[i] = min(klo[i], (sm << 16) | Y);
+                        khi[i] = min(khi[i], (sm & hmask) | Y);
+                    }
+                }
+            }
+        }
+    } else {
+        // |S| <= 3: one outer step over the nonempty subsets of S
+        for (uint32_t T = S; T != 0; T = (T - 1u) & S) {
+            const uint32_t t = ld_pair(row_rsrc(fb + (int64_t)T * rb), lb);
+#pragma unroll
+            for (int i = 0; i < kStarMax; i++) {
+                if (i < r) {
+                    const uint32_t sm =
+                        add_u16x2(t, ld_pair(row_rsrc(fb + (int64_t)((S ^ T) | a[i]) * rb), lb));
+                    const uint32_t Y = yset(i, T);
+                    klo[i] = min(klo[i], (sm << 16) | Y);
+                    khi[i] = min(khi[i], (sm & hmask) | Y);
+                }
+            }
+        }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) {
+        if (i < r) {
+            P *acc = &M[(int64_t)stars[g].xi[i] * np + v];
+            atomicMin(acc, key_packed<P>(klo[i], k));
+            atomicMin(acc + 1, key_packed<P>(khi[i], k));
+        }
+    }
+}
+
 // Packed accumulators M[xi][v] -> vertex-major staging (label, packed word)
 // through an LDS transpose of 64 rows x 64 vertices per workgroup; rows past
 // nx are INF.

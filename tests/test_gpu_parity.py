@@ -63,12 +63,13 @@ def test_regular_1000_full_table(gpu_ctx):
 @pytest.mark.parametrize("flags", [emv.WIDE_LABELS, emv.FORCE_P64, emv.FORCE_I64,
                                    emv.FORCE_I64 | emv.FORCE_P64, emv.TABLE32,
                                    emv.TABLE32 | emv.WIDE_LABELS, emv.MERGE_1V,
-                                   emv.MERGE_1V | emv.WIDE_LABELS])
+                                   emv.MERGE_1V | emv.WIDE_LABELS, emv.MERGE_ROWS])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
 def test_wide_paths_bit_exact(gpu_ctx, name, flags):
     """Every width combination (16/32/64-bit labels, 32/64-bit merge
-    accumulators, 16-bit/int32/int64 table) and both merges of 16-bit cells
-    (two vertices per lane, or one: MERGE_1V) give the reference's table."""
+    accumulators, 16-bit/int32/int64 table) and every merge of 16-bit cells
+    (stars of rows sharing a base set, row by row: MERGE_ROWS, one vertex
+    per lane: MERGE_1V) give the reference's table."""
     inst = load_instance(name)
     print(f"case {name} flags={flags}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst, flags)
@@ -444,7 +445,7 @@ def test_large_weights_64bit_labels(gpu_ctx, shape):
 
 
 @pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16)])
-@pytest.mark.parametrize("flags", [0, emv.MERGE_1V])
+@pytest.mark.parametrize("flags", [0, emv.MERGE_ROWS, emv.MERGE_1V])
 def test_many_terminals_16bit_cells(gpu_ctx, n, d, k, flags):
     """k = 15 and 16 on 16-bit cells: the two-vertex merge's keys hold Y in
     16 bits and its low rows in 32-bit lane offsets at the largest k it
