@@ -328,7 +328,10 @@ def group_leg(args):
     t0 = time.perf_counter()
     ctx.load_edges(nn, red)
     t_load = time.perf_counter() - t0
-    times, sol = [], None
+    t0 = time.perf_counter()
+    sol = ctx.solve(kk)  # warm: table and workspace allocations, code objects
+    log(f"group leg: warm-up solve {time.perf_counter() - t0:.3f}s")
+    times = []
     for i in range(args.group_steps):
         t0 = time.perf_counter()
         sol = ctx.solve(kk)

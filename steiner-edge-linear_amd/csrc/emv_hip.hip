@@ -642,7 +642,10 @@ int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
 template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
     if constexpr (std::is_same<T, uint16_t>::value) {
-        if (use_merge_h2<T>(c) && c->merge_star && c->h_xs)
+        // stars cost two more launches (fill, pack) than whole rows: only
+        // layer batches of >= 2^30 pair cells take them
+        const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
+        if (use_merge_h2<T>(c) && c->merge_star && c->h_xs && pair_cells >= 1073741824.0)
             return launch_merge_star<P, DT>(c, f, nx, nchunks, k, m);
     }
     if (((1u << (m - 1)) - 1u) >= split_pairs())
