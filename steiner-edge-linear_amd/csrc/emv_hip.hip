@@ -76,6 +76,7 @@ struct emv_ctx {
     void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
     bool merge_star = false;  // ... in stars of rows sharing a base set
+    double star_min_cells = 1073741824.0;  // from layer batches of this many pair cells on
     const uint32_t *h_xs = nullptr;         // host copy of the batch's rows (star building)
     std::vector<MergeStar> h_stars;         // stars of the last batch (kept for the async copy)
     MergeStar *d_stars = nullptr;
@@ -643,9 +644,9 @@ template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
     if constexpr (std::is_same<T, uint16_t>::value) {
         // stars cost two more launches (fill, pack) than whole rows: only
-        // layer batches of >= 2^30 pair cells take them
+        // layer batches of >= star_min_cells pair cells take them
         const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
-        if (use_merge_h2<T>(c) && c->merge_star && c->h_xs && pair_cells >= 1073741824.0)
+        if (use_merge_h2<T>(c) && c->merge_star && c->h_xs && pair_cells >= c->star_min_cells)
             return launch_merge_star<P, DT>(c, f, nx, nchunks, k, m);
     }
     if (((1u << (m - 1)) - 1u) >= split_pairs())
@@ -1214,6 +1215,11 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
     c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
     c->merge_star = c->merge_h2 && !(flags & EMV_MERGE_ROWS) && merge_star_enabled();
+    {
+        // EMV_STAR_MIN_CELLS: pair cells from which a layer batch merges in stars
+        const char *e = getenv("EMV_STAR_MIN_CELLS");
+        c->star_min_cells = e ? atof(e) : 1073741824.0;
+    }
     st.rel32 = !(flags & EMV_FORCE_P64) && !env_flag("EMV_NO_REL32");
     if (flags & EMV_FORCE_P64)
         st.dbits = 64;

@@ -444,13 +444,31 @@ def test_large_weights_64bit_labels(gpu_ctx, shape):
 
 
 
+@pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
+def test_star_merges_bit_exact(gpu_ctx, monkeypatch, name):
+    """Every layer merged in stars of rows sharing a base set (forced for
+    small layers, which otherwise merge row by row): the reference's table
+    and record kinds, as row by row."""
+    monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    inst = load_instance(name)
+    print(f"case stars {name}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst)
+    assert sol.table_bytes == 2
+    assert digest(f) == META[name]["f_sha256"]
+    assert digest(kinds(code, inst.k)) == META[name]["kind_sha256"]
+    s2, f2, c2 = solve_full(gpu_ctx, inst, emv.MERGE_ROWS)
+    np.testing.assert_array_equal(f, f2)
+    np.testing.assert_array_equal(code, c2)
+
+
 @pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16)])
 @pytest.mark.parametrize("flags", [0, emv.MERGE_ROWS, emv.MERGE_1V])
-def test_many_terminals_16bit_cells(gpu_ctx, n, d, k, flags):
-    """k = 15 and 16 on 16-bit cells: the two-vertex merge's keys hold Y in
-    16 bits and its low rows in 32-bit lane offsets at the largest k it
-    takes; cells, records and cost equal the oracle's (and the one-vertex
-    merge's)."""
+def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags):
+    """k = 15 and 16 on 16-bit cells: the two-vertex merges' keys hold Y in
+    16 bits and their low rows in 32-bit lane offsets at the largest k they
+    take; cells, records and cost equal the oracle's, in stars (forced for
+    these small layers), row by row and one vertex per lane."""
+    monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
     e = emv.regular_graph(n, d, 1000, 7)
     kk = emv.terminals(n, k, 7)
     print(f"case n={n} d={d} k={k} flags={flags}", flush=True)
