@@ -3,14 +3,18 @@
 // Table in HBM (subset-major as reader-el.c:690-692, rows padded to np, an
 // odd number of 64-vertex tiles; the padding columns v >= n hold whatever
 // the allocation held and only feed lanes whose results are dropped):
-//   f[X * np + v]   cost, int32 or int64
+//   f[X * np + v]   cost: 16-bit (0xFFFF = INF), int32 or int64, the
+//                   narrowest that holds the distance bound
 //   b[X * np + v]   int32 record code: Y for a merge split (Y == X stops the
 //                   traceback), 2^k + u for an edge from predecessor u
+// An int64 table may carry a compact int32 copy for the merges,
+// fcopy[X * np + v] = f[X][v] - fbase[X].
 //
 // Layer staging (vertex-major): the rows of one layer batch are grouped in
 // chunks of 64; chunk c holds, for every vertex v, the 64 rows' labels
 // side by side:
 //   D[(c * np + v) * 64 + lane]     distance label of row c*64+lane at v
+//                                   (or its offset from the row's base)
 //   CODE[(c * np + v) * 64 + lane]  the merge's packed word (sum << k) | Y
 // One wave owns one (chunk, vertex) item: lane = row.  A neighbour's labels
 // for the whole chunk are one coalesced 128 B (u16), 256 B (u32) or 512 B
