@@ -521,26 +521,32 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     db[q] = (two && ub[q] >= 0) ? Db[(int64_t)ub[q] * kWave + lane] : INF;
                 }
                 DT besta = cura, bestb = curb;
-                int na = -1, nb = -1;  // lane q < DMAX holds neighbour q (for the flags)
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
                     besta = relax_min(da[q], ua[q], wa[q], INF, besta);
                     bestb = relax_min(db[q], ub[q], wb[q], INF, bestb);
-                    if (lane == q) {
-                        na = ua[q];
-                        nb = ub[q];
-                    }
                 }
                 const bool chga = besta < cura, chgb = two && bestb < curb;
                 if (chga) Da[ia] = besta;
                 if (chgb) Db[ib] = bestb;
-                if (__ballot(chga) != 0) {
-                    gflag(c1, item_pre[ga], na, Fnext, Qnext, cnext, qcap, list_open);
-                    raised = true;
-                }
-                if (__ballot(chgb) != 0) {
-                    gflag(c2, item_pre[gb], nb, Fnext, Qnext, cnext, qcap, list_open);
-                    raised = true;
+                if (__ballot(chga | chgb) != 0) {
+                    // flag only the neighbours that a dropped row now
+                    // improves (as k_relax_v); lane q < DMAX holds neighbour q
+                    int na = -1, nb = -1;
+#pragma unroll
+                    for (int q = 0; q < DMAX; q++) {
+                        const bool ia2 = chga && (DT)(besta + (DT)wa[q]) < da[q];
+                        const bool ib2 = chgb && (DT)(bestb + (DT)wb[q]) < db[q];
+                        const bool fa = __ballot(ia2) != 0, fb = __ballot(ib2) != 0;
+                        if (lane == q) {
+                            na = fa ? ua[q] : -1;
+                            nb = fb ? ub[q] : -1;
+                        }
+                    }
+                    const bool anya = __ballot(na >= 0) != 0, anyb = __ballot(nb >= 0) != 0;
+                    if (anya) gflag(c1, item_pre[ga], na, Fnext, Qnext, cnext, qcap, list_open);
+                    if (anyb) gflag(c2, item_pre[gb], nb, Fnext, Qnext, cnext, qcap, list_open);
+                    if (anya || anyb) raised = true;
                 }
             }
         } else {

@@ -246,18 +246,32 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 if (chgb) Db[ib] = bestb;
                 const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
                 if (anya || anyb) {
+                    // Only neighbours that a dropped row now improves are
+                    // flagged: u is left alone when label + w >= u's label
+                    // (as read above, and labels only drop) in every dropped
+                    // row.  A row that did not drop offers u nothing new: its
+                    // last drop was checked the same way.  The dropped labels
+                    // are finite and label + w stays below INF.
+                    uint32_t fa = 0, fb = 0;
+#pragma unroll
+                    for (int q = 0; q < DMAX; q++) {
+                        const bool ia = chga && (DT)(besta + (DT)wa[q]) < da[q];
+                        const bool ib = chgb && (DT)(bestb + (DT)wb[q]) < db[q];
+                        if (__ballot(ia) != 0) fa |= 1u << q;
+                        if (__ballot(ib) != 0) fb |= 1u << q;
+                    }
                     // lane q < DMAX takes neighbour q
                     int na = -1, nb = -1;
 #pragma unroll
                     for (int q = 0; q < DMAX; q++) {
                         if (lane == q) {
-                            na = ua[q];
-                            nb = ub[q];
+                            na = (fa >> q) & 1u ? ua[q] : -1;
+                            nb = (fb >> q) & 1u ? ub[q] : -1;
                         }
                     }
-                    if (anya) flag_neighbours(ca, np, n, na, Fnext, Qnext, cnext, qcap, list_open);
-                    if (anyb) flag_neighbours(cb, np, n, nb, Fnext, Qnext, cnext, qcap, list_open);
-                    raised = true;
+                    if (fa) flag_neighbours(ca, np, n, na, Fnext, Qnext, cnext, qcap, list_open);
+                    if (fb) flag_neighbours(cb, np, n, nb, Fnext, Qnext, cnext, qcap, list_open);
+                    if (fa | fb) raised = true;
                 }
             }
         } else {
