@@ -398,13 +398,19 @@ __global__ __launch_bounds__(kBlock) void k_glanes(void *__restrict__ D,
 __device__ __forceinline__ void gflag(const GChunk &c, int64_t base_item, int nb,
                                       uint32_t *__restrict__ Fnext, uint32_t *__restrict__ Qnext,
                                       uint32_t *__restrict__ cnext, uint32_t qcap, bool list_open) {
+    if (!list_open) {
+        // flag scan only: a plain store, no read of the flag (list_open is
+        // wave-uniform)
+        if (nb >= 0) Fnext[c.fbase + nb] = 1;
+        return;
+    }
     bool want = false;
     uint32_t ent = 0;
     if (nb >= 0) {
         uint32_t *fp = Fnext + c.fbase + nb;
         if (*fp == 0) {
             *fp = 1;
-            want = list_open;
+            want = true;
             ent = (uint32_t)(base_item + nb);
         }
     }

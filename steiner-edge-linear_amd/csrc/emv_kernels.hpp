@@ -135,13 +135,19 @@ __device__ __forceinline__ void flag_neighbours(int64_t c, int64_t np, int n, in
                                                 uint32_t *__restrict__ Qnext,
                                                 uint32_t *__restrict__ cnext, uint32_t qcap,
                                                 bool list_open) {
+    if (!list_open) {
+        // flag scan only: a plain store, no read of the flag (list_open is
+        // wave-uniform)
+        if (nb >= 0) Fnext[c * np + nb] = 1;
+        return;
+    }
     bool want = false;
     uint32_t ent = 0;
     if (nb >= 0) {
         uint32_t *fp = Fnext + c * np + nb;
         if (*fp == 0) {
             *fp = 1;
-            want = list_open;
+            want = true;
             ent = (uint32_t)(c * n + nb);
         }
     }
