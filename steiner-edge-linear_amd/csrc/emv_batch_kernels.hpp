@@ -427,14 +427,18 @@ template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_grelax(
     void *__restrict__ D, const GChunk *__restrict__ ch,
     const int64_t *__restrict__ item_pre, const int32_t *__restrict__ blk_chunk, int nch, const int32_t *__restrict__ rp,
-    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int dense,
+    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int dense0,
     uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
     uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
-    uint32_t *__restrict__ cnt, int r) {
+    uint32_t *__restrict__ cnt, int r, int dense_rounds) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         any[(r + 2) & 3] = 0;
         cnt[(r + 2) & 3] = 0;
     }
+    // dense stretch, as k_relax_v: rounds 1 .. dense_rounds take every item,
+    // rounds 0 .. dense_rounds-1 write no flags
+    const bool dense = dense0 || (r >= 1 && r <= dense_rounds);
+    const bool noflags = r < dense_rounds;
     if (!dense && any[r & 3] == 0) return;
     const uint32_t qn = dense ? 0u : cnt[r & 3];
     const bool use_list = !dense && qn < qcap;
@@ -535,7 +539,9 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const bool chga = besta < cura, chgb = two && bestb < curb;
                 if (chga) Da[ia] = besta;
                 if (chgb) Db[ib] = bestb;
-                if (__ballot(chga | chgb) != 0) {
+                if (noflags) {
+                    if (__ballot(chga | chgb) != 0) raised = true;
+                } else if (__ballot(chga | chgb) != 0) {
                     // flag only the neighbours that a dropped row now
                     // improves (as k_relax_v); lane q < DMAX holds neighbour q
                     int na = -1, nb = -1;
@@ -581,11 +587,13 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const bool dchg = best < cur;
                 if (dchg) Dc[idx] = best;
                 if (__ballot(dchg) != 0) {
-                    const int64_t base_item = item_pre[g];
-                    for (int eb = e0; eb < e1; eb += kWave) {
-                        const int e2 = eb + lane;
-                        gflag(c, base_item, e2 < e1 ? adj[e2].v : -1, Fnext, Qnext, cnext, qcap,
-                              list_open);
+                    if (!noflags) {
+                        const int64_t base_item = item_pre[g];
+                        for (int eb = e0; eb < e1; eb += kWave) {
+                            const int e2 = eb + lane;
+                            gflag(c, base_item, e2 < e1 ? adj[e2].v : -1, Fnext, Qnext, cnext,
+                                  qcap, list_open);
+                        }
                     }
                     raised = true;
                 }

@@ -491,6 +491,7 @@ struct StepPlan {
     std::vector<int64_t> insts;    // tickets stepped
     int xg = 1;
     int pbits = 32;
+    int dense_rounds = 0;  // vertex-wise rounds taking every item (0: a singleton-only step)
     int64_t cells = 0, flagbytes = 0;
     int64_t macc_cells = 0;  // split-merge accumulators of the step
     // device layout (byte offsets into the descriptor buffer)
@@ -607,6 +608,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 dcur += G.np * kWave;
                 fcur += nflags;
             }
+            // a layer step starts near-dense (a singleton-only step does not)
+            P.dense_rounds = layer_dense_rounds();
             if (splits)
                 split.push_back(M);
             else
@@ -818,7 +821,7 @@ void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
                        (const int64_t *)(db + SG.off_rp), (const int32_t *)(db + SG.off_rb),
                        SG.nrch, e->d_rp, e->d_adj, e->d_ell, dense, e->d_flag[r & 1],
                        e->d_flag[(r + 1) & 1], (const uint32_t *)e->d_q[r & 1],
-                       e->d_q[(r + 1) & 1], qcap, e->d_any, e->d_cnt, r);
+                       e->d_q[(r + 1) & 1], qcap, e->d_any, e->d_cnt, r, P.dense_rounds);
 }
 
 template <typename DT>

@@ -75,6 +75,7 @@ struct emv_ctx {
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
     void *d_code = nullptr;       // merge's packed words, 8 bytes per cell reserved
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
+    int dense_rounds = 0;   // vertex-wise rounds of a relaxation that take every item
     bool merge_star = false;  // ... in stars of rows sharing a base set
     double star_min_cells = 1073741824.0;  // from layer batches of this many pair cells on
     const uint32_t *h_xs = nullptr;         // host copy of the batch's rows (star building)
@@ -291,6 +292,7 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // (chunk, vertex), then flagged rounds, launched in groups between polls of
 // the "round has work" word, until a round flags nothing.  Rounds launched
 // after convergence return at once.
+
 template <typename DT, int DMAX>
 void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
     // no more workgroups than 64-item blocks of waves (small instances)
@@ -301,7 +303,7 @@ void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
                        (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
                        (const uint32_t *)c->d_q[r & 1], c->d_q[(r + 1) & 1], c->qcap, c->d_any,
-                       c->d_cnt, r, c->relax_nxcd);
+                       c->d_cnt, r, c->relax_nxcd, c->dense_rounds);
 }
 
 template <typename DT>
@@ -753,6 +755,9 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 if (st.rel32) CHK(relative_labels_fit<P>(c, nx, nchunks, k, &rel, &copy_fit));
             }
             if (!rel || !copy_fit) c->fcopy_on = false;
+            // a layer's relaxation starts near-dense: its first rounds take
+            // every item (the single-source phase keeps flags from round 0)
+            c->dense_rounds = layer_dense_rounds();
             if (rel) {
                 // relax + records on 32-bit labels relative to each row's base
                 CHK(relax_chunks<uint32_t>(c, nchunks, &res->relax_rounds));
@@ -772,6 +777,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                                    vtiles, (const unsigned long long *)nullptr,
                                    (int32_t *)nullptr, (int64_t *)nullptr);
             }
+            c->dense_rounds = 0;
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e3, c->stream));
         }

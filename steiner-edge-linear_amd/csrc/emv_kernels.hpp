@@ -154,17 +154,25 @@ __device__ __forceinline__ void flag_neighbours(int64_t c, int64_t np, int n, in
     wave_append(want, ent, Qnext, cnext, qcap);
 }
 
+// Dense stretch (dense_rounds = R > 0): rounds 0 .. R-1 write no flags and
+// rounds 1 .. R take every item.  Early in a layer almost every item is
+// flagged anyway, so the few extra visits cost less than the random flag
+// stores they replace.  Round R writes flags again for round R + 1; the
+// rounds without flags leave F_cur / F_next clean, as a dense round reads
+// no flags.
 template <typename DT, int DMAX>
 __global__ __launch_bounds__(kBlock) void k_relax_v(
     DT *__restrict__ D, int64_t np, int n, const int32_t *__restrict__ rp,
-    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int64_t nitems, int dense,
+    const Edge *__restrict__ adj, const Edge *__restrict__ ell, int64_t nitems, int dense0,
     uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
     uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
-    uint32_t *__restrict__ cnt, int r, int nxcd) {
+    uint32_t *__restrict__ cnt, int r, int nxcd, int dense_rounds) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         any[(r + 2) & 3] = 0;
         cnt[(r + 2) & 3] = 0;
     }
+    const bool dense = dense0 || (r >= 1 && r <= dense_rounds);
+    const bool noflags = r < dense_rounds;  // the next round takes every item
     if (!dense && any[r & 3] == 0) return;  // nothing flagged for this round
     const uint32_t qn = dense ? 0u : cnt[r & 3];
     const bool use_list = !dense && qn < qcap;
@@ -251,7 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 if (chga) Da[ia] = besta;
                 if (chgb) Db[ib] = bestb;
                 const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
-                if (anya || anyb) {
+                if (noflags) {
+                    if (anya || anyb) raised = true;
+                } else if (anya || anyb) {
                     // Only neighbours that a dropped row now improves are
                     // flagged: u is left alone when label + w >= u's label
                     // (as read above, and labels only drop) in every dropped
@@ -320,10 +330,12 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 const bool dchg = best < cur;
                 if (dchg) Dc[idx] = best;
                 if (__ballot(dchg) != 0) {
-                    for (int eb = e0; eb < e1; eb += kWave) {
-                        const int e = eb + lane;
-                        flag_neighbours(c, np, n, e < e1 ? adj[e].v : -1, Fnext, Qnext, cnext, qcap,
-                                        list_open);
+                    if (!noflags) {
+                        for (int eb = e0; eb < e1; eb += kWave) {
+                            const int e = eb + lane;
+                            flag_neighbours(c, np, n, e < e1 ? adj[e].v : -1, Fnext, Qnext, cnext,
+                                            qcap, list_open);
+                        }
                     }
                     raised = true;
                 }
@@ -1350,6 +1362,16 @@ inline bool merge_h2_ok(int tbits, int k, int64_t np) {
     if (!on || tbits != 16 || k > 16 || k < 2) return false;
     const uint64_t lmax = k >= 4 ? 7ull << (k - 4) : (1ull << (k - 1)) - 1ull;
     return 2ull * lmax * (uint64_t)np + 256ull < (1ull << 32);
+}
+
+// Host: vertex-wise rounds 1 .. R of a layer's relaxation that take every
+// item (rounds 0 .. R-1 write no flags); EMV_DENSE_ROUNDS overrides R = 6.
+inline int layer_dense_rounds() {
+    static const int v = [] {
+        const char *e = getenv("EMV_DENSE_ROUNDS");
+        return e ? (atoi(e) > 0 ? atoi(e) : 0) : 6;
+    }();
+    return v;
 }
 
 // (sum << 16) | Y  ->  (sum << k) | Y; all ones (no pair) stays all ones.
