@@ -11,14 +11,20 @@
  *
  * Conventions as in emv_hip.h: EMV_OK or a negative EMV_ERR_* status,
  * emv_engine_last_error() for the text, 0-based vertex ids, inputs owned by
- * the caller.  Engine instances use int32 tables and 32-bit labels; an
- * instance whose worst-case cost k*(n-1)*max w reaches 2^31 is refused with
- * EMV_ERR_RANGE (emv_solve handles it).  k must be in [2, 30].
+ * the caller.  k must be in [2, 30].  Engine instances keep int32 tables, or
+ * 16-bit cells when an instance's distance bound B (the sum over its
+ * singleton rows of the row maximum) is below 0xFFFF.  A graph whose
+ * n * max w reaches 2^32 - 1 is refused with EMV_ERR_RANGE at submission (the
+ * singleton stage relaxes on 32-bit labels); an instance whose bound B reaches
+ * 2^31 - 1 ends with status EMV_ERR_RANGE after its singleton stage
+ * (emv_solve, with its int64 table, handles both).
  */
 #ifndef EMV_ENGINE_H
 #define EMV_ENGINE_H
 
 #include <stdint.h>
+
+#include "emv_hip.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -63,6 +69,13 @@ typedef struct {
     double queue_ms;    /* submit -> admission */
     double latency_ms;  /* submit -> done */
     int64_t npairs;     /* traceback pairs held (EMV_LIST) */
+    int64_t reduced_cost; /* group query: cost + groups * M, the cost of the reduced instance
+                             (virtual terminals joined to their members by weight M =
+                             emv_server_group_weight()); terminal query: cost */
+    int32_t groups;     /* group query: number of groups (= k); 0 for a terminal query */
+    int32_t reserved;
+    int64_t root;       /* vertex the traceback starts at: kk[k-1], or for a group query the
+                           smallest member of the last group at least cost */
 } emv_request_info;
 
 int emv_engine_default_params(emv_engine_params *p);
@@ -87,6 +100,21 @@ int emv_server_create(int device, int64_t n, int64_t m, const int64_t *edges,
  * 2 * 2^k * np * 4 bytes can never fit the arena. */
 int emv_server_submit(emv_engine *e, int64_t k, const int64_t *kk, uint32_t flags,
                       int64_t *ticket);
+/* Queues a group Steiner query on the resident graph: g groups (2..30),
+ * group j = members[offsets[j] .. offsets[j+1]).  The reference solves such
+ * a query through a reduction (README.txt:4-5): one virtual terminal per
+ * group, joined to each member by an edge of weight M = 1 + the sum of all
+ * edge weights, then an ordinary solve on the reduced STP.  Here the DP runs
+ * on the resident graph itself, with singleton row {j} = the distance to the
+ * nearest member of group j: M exceeds every tree of the graph, so an optimal
+ * reduced tree uses each virtual terminal as a leaf and the reduced table at
+ * a graph vertex is |X| * M + this table.  info.cost is the group Steiner
+ * cost, info.reduced_cost = cost + g * M the reduced instance's cost; the
+ * traceback lists graph edges only (the tree without its virtual leaves). */
+int emv_server_submit_groups(emv_engine *e, int64_t g, const int64_t *offsets,
+                             const int64_t *members, uint32_t flags, int64_t *ticket);
+/* M = 1 + the sum of the resident graph's edge weights. */
+int64_t emv_server_group_weight(const emv_engine *e);
 /* Admits queued requests that fit (FIFO), then advances every running
  * request by one DP stage in shared launches. */
 int emv_server_step(emv_engine *e, int64_t *running, int64_t *queued);
@@ -101,6 +129,11 @@ int emv_engine_pairs(emv_engine *e, int64_t ticket, int64_t *pairs, int64_t cap,
                      int64_t *npairs);
 /* Frees a finished request's table kept with EMV_KEEP_TABLE. */
 int emv_engine_release(emv_engine *e, int64_t ticket);
+/* Certificate of a finished instance's kept table (EMV_KEEP_TABLE), as
+ * emv_certify (emv_hip.h); a group query's singleton rows carry the distance
+ * to the nearest member and edge records (2^k + the smallest neighbour on a
+ * shortest path), {j} at the members. */
+int emv_engine_certify(emv_engine *e, int64_t ticket, emv_certificate *out);
 
 #ifdef __cplusplus
 }

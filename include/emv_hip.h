@@ -54,7 +54,8 @@ typedef struct emv_ctx emv_ctx;
 typedef struct {
     int64_t cost;            /* f[C][q], C = 2^(k-1)-1, q = kk[k-1] (reader-el.c:2137) */
     int32_t k;
-    int32_t table_bytes;     /* 4 (int32 table) or 8 (int64 table) */
+    int32_t table_bytes;     /* 2 (16-bit cells, when the distance bound is below 0xFFFF),
+                                4 (int32 table) or 8 (int64 table) */
     int32_t merge_bits;      /* 32 or 64: width of the packed merge accumulator */
     int32_t label_bits;      /* width of the relaxation labels: 16, 32 or 64 */
     int64_t n;
@@ -142,6 +143,41 @@ int emv_sssp(emv_ctx *ctx, int64_t s, int64_t *d_out, int64_t *pred_out,
              double *ms);
 
 void emv_result_free(emv_result *res);
+
+/* Certificate of a table: counts of the cells that break the EMV fixed point
+ * or the record rule.  A connected graph with positive weights has exactly
+ * one table that satisfies, for every row X != 0 and vertex v,
+ *   f[X][v] = least of: 0 (X = {t}, v = kk[t]); f[X][u] + w(u,v) over the
+ *             edges; f[Y][v] + f[X\Y][v] over the splits of X,
+ * so a table with no failing cell equals the reference's (reader-el.c:1964-
+ * 2139) in every cell.  Records: row 0 holds -1; a singleton row holds {t}
+ * at kk[t] and 2^k + kk[t] elsewhere (reader-el.c:2015-2027); other rows
+ * the smallest least split Y (taken without X's top element) when the cell
+ * equals the least split sum (reader-el.c:2070-2085), else 2^k + the
+ * smallest neighbour u with f[u] + w == f[v] (2117-2126).  The reference has
+ * no such check; its only self-check compares the cost with the STP file's
+ * cost line (reader-el.c:2489-2491).  The check runs on the device, one
+ * thread per cell, independently of the solver's kernels. */
+typedef struct {
+    int64_t cells;       /* cells checked: 2^k * n, row 0 included */
+    int64_t bad_value;   /* cells that are not the least of their candidates */
+    int64_t not_finite;  /* cells of rows X != 0 holding INF */
+    int64_t bad_record;  /* record codes other than the rule's */
+    int64_t first_X;     /* smallest failing cell (X, then v); -1 when none fails */
+    int64_t first_v;
+    double ms;           /* device time of the check */
+} emv_certificate;
+
+/* Certificate of the table kept by the last emv_solve (EMV_KEEP_TABLE).
+ * EMV_ERR_STATE without a kept table. */
+int emv_certify(emv_ctx *ctx, emv_certificate *out);
+
+/* Certificate of a table held by the caller, for terminals kk[0..k-1] on the
+ * loaded graph: f as int64 (INF = INT64_MAX), code as int32, each 2^k x n
+ * row-major -- a table file (emv_write_table), the output of another solver,
+ * or a table with a planted change.  The device copy is freed on return. */
+int emv_certify_table(emv_ctx *ctx, int64_t k, const int64_t *kk, const int64_t *f,
+                      const int32_t *code, emv_certificate *out);
 
 /* Device memory the solve of a k-terminal query on the loaded graph needs,
  * in bytes (tables + relaxation staging). */

@@ -38,7 +38,8 @@ struct GChunk {
     int32_t k;      // instance terminals; codes of edge records start at 2^k
     int32_t x0;     // first row of the chunk in the step's row list
     int32_t nx;     // rows in the chunk (lane j < nx is row x0 + j)
-    int32_t kind;   // 0: layer rows, 1: singleton rows (row j = terminal j)
+    int32_t kind;   // 0: layer rows, 1: singleton rows (row j = terminal j),
+                    // 2: singleton rows of a group query (row j = group j)
     int32_t inst;   // engine slot of the instance
     int32_t b0;     // block relaxation: first block of the graph in the shared block arrays
     int32_t nblk;   //   and its blocks (0: vertex-wise relaxation)
@@ -117,6 +118,15 @@ __global__ __launch_bounds__(kBlock) void k_gseed(const GChunk *__restrict__ ch,
         const int64_t off = i - pre[g];  // v * 64 + lane
         chunk_labels<uint32_t>(D, ch[g].dbase)[off] = ((int)(off / kWave) == src) ? 0u : ~0u;
     }
+}
+
+// Group queries: after k_gseed, label 0 at each (member, lane of its group),
+// given as staging cells (the singleton chunks hold 32-bit labels).
+__global__ __launch_bounds__(kBlock) void k_gseed_cells(const int64_t *__restrict__ cells,
+                                                        int64_t count, void *__restrict__ D) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride)
+        static_cast<uint32_t *>(D)[cells[i]] = 0u;
 }
 
 // ------------------------------------------------------------------ merge
@@ -712,6 +722,13 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
             int32_t code = 0;
             if (c.kind == 0 && v < c.n)
                 code = cell_record<DT, P>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v, rp, adj);
+            else if (c.kind == 2 && v < c.n)
+                // group rows: {j} at the members (label 0: weights are >= 1),
+                // elsewhere the edge from the smallest neighbour on a
+                // shortest path (the merge word's sum is out of reach)
+                code = cell_record<DT, uint64_t>(
+                    Dc, Ls[u], Ls[u] == (DT)0 ? (uint64_t)(1ull << lane) : ~0ull, c.k, real,
+                    c.voff + (int)v, rp, adj);
             sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }
@@ -740,11 +757,14 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
                 const int32_t src = kk_all[kk_off[c.inst] + xl];
                 const int64_t row = c.k >= 3 ? S2 - c.k + xl : X;  // staged copy, or in place
                 f[c.tbase + row * c.np + v] = (d == ~0u) ? INT32_MAX : (int32_t)d;
-                b[bbase + X * c.np + v] = (v == src) ? (int32_t)X : (int32_t)(S2 + src);
+                if (c.kind == 1)
+                    b[bbase + X * c.np + v] = (v == src) ? (int32_t)X : (int32_t)(S2 + src);
+                else
+                    b[bbase + X * c.np + v] = sc[lane][xl];
                 dmax = d;
             }
         }
-        if (c.kind == 1) {
+        if (c.kind != 0) {
             for (int off = 32; off > 0; off >>= 1) {
                 const unsigned long long o = __shfl_xor(dmax, off);
                 dmax = o > dmax ? o : dmax;

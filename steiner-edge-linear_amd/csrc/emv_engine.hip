@@ -31,6 +31,7 @@
 
 #include "emv_batch_kernels.hpp"
 #include "emv_blocks.hpp"
+#include "emv_certify.hpp"
 #include "emv_engine.h"
 #include "emv_hip.h"
 
@@ -108,6 +109,7 @@ struct Arena {
 
 struct GraphRec {
     int64_t voff, n, np, maxw;
+    int64_t wsum;  // sum of the edge weights (a group query's M = 1 + wsum)
     int maxdeg;
     int32_t b0 = 0, nblk = 0;  // blocks of block-local relaxation (nblk 0: vertex-wise)
 };
@@ -119,6 +121,11 @@ struct Inst {
     int graph = 0;
     int k = 0;
     std::vector<int32_t> kk;
+    // group query: groups[j] = the members of group j (k = groups); the
+    // singleton row {j} holds the distance to the nearest member
+    std::vector<std::vector<int32_t>> groups;
+    int64_t root = -1;          // traceback start: kk[k-1], or the nearest member of the last group
+    int64_t reduced_cost = 0;   // group query: cost + k * M (the reduced instance's cost)
     uint32_t flags = 0;
     int state = ST_QUEUED;
     int stage = 1;  // next stage: 1 = singletons, m >= 2 = layer m
@@ -326,7 +333,7 @@ int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
         const std::vector<int64_t> &E = edges[g];
         const int64_t m = (int64_t)E.size() / 3;
         std::vector<int32_t> deg(n, 0);
-        int64_t maxw = 0;
+        int64_t maxw = 0, wsum = 0;
         for (int64_t j = 0; j < m; j++) {
             const int64_t u = E[3 * j], v = E[3 * j + 1], w = E[3 * j + 2];
             if (u < 0 || v < 0 || u >= n || v >= n) {
@@ -338,6 +345,7 @@ int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
                 return EMV_ERR_RANGE;
             }
             maxw = std::max(maxw, w);
+            wsum += w;
             deg[u]++;
             deg[v]++;
         }
@@ -367,6 +375,7 @@ int upload_graphs(emv_engine *e, const std::vector<std::vector<int64_t>> &edges,
         gr.n = n;
         gr.np = row_stride(n);
         gr.maxw = maxw;
+        gr.wsum = wsum;
         gr.maxdeg = maxdeg;
         e->graphs.push_back(gr);
         voff += n;
@@ -488,6 +497,7 @@ struct StepPlan {
     std::vector<int64_t> seed_pre;
     std::vector<uint32_t> xs;
     std::vector<int> seed_chunks;  // indices into ch
+    std::vector<int64_t> member_cells;  // group queries: staging cells seeded with 0 (a member, its lane)
     std::vector<int64_t> insts;    // tickets stepped
     int xg = 1;
     int pbits = 32;
@@ -496,7 +506,7 @@ struct StepPlan {
     int64_t macc_cells = 0;  // split-merge accumulators of the step
     // device layout (byte offsets into the descriptor buffer)
     size_t off_ch = 0, off_mg = 0, off_sp = 0, off_xs = 0, off_seedch = 0, bytes = 0;
-    size_t off_rch = 0, off_bins = 0, off_lanes = 0, off_pl = 0, off_plp = 0;
+    size_t off_rch = 0, off_bins = 0, off_lanes = 0, off_pl = 0, off_plp = 0, off_mc = 0;
     std::vector<char> blob;
     std::vector<GChunk> seed_ch;
 };
@@ -543,7 +553,11 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 c.fbase = fcur;
                 c.x0 = 0;
                 c.nx = I.k;
-                c.kind = 1;
+                c.kind = I.groups.empty() ? 1 : 2;
+                // group query: lane j starts at 0 on every member of group j
+                for (int j = 0; j < (int)I.groups.size(); j++)
+                    for (int32_t v : I.groups[(size_t)j])
+                        P.member_cells.push_back(dcur + (int64_t)v * kWave + j);
                 P.seed_chunks.push_back((int)P.ch.size());
                 P.ch.push_back(c);
                 dcur += G.np * kWave;
@@ -755,6 +769,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     P.off_lanes = place(sizeof(GLane) * P.lanes.size());
     P.off_pl = place(sizeof(GPlace) * P.places.size());
     P.off_plp = place(sizeof(int64_t) * P.place_pre.size());
+    P.off_mc = place(sizeof(int64_t) * P.member_cells.size());
     for (StepGroup &SG : P.grp) {
         SG.off_rp = place(sizeof(int64_t) * SG.relax_pre.size());
         SG.off_mp = place(sizeof(int64_t) * SG.merge_pre.size());
@@ -778,6 +793,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     put(P.off_lanes, P.lanes.data(), sizeof(GLane) * P.lanes.size());
     put(P.off_pl, P.places.data(), sizeof(GPlace) * P.places.size());
     put(P.off_plp, P.place_pre.data(), sizeof(int64_t) * P.place_pre.size());
+    put(P.off_mc, P.member_cells.data(), sizeof(int64_t) * P.member_cells.size());
     for (StepGroup &SG : P.grp) {
         put(SG.off_rp, SG.relax_pre.data(), sizeof(int64_t) * SG.relax_pre.size());
         put(SG.off_mp, SG.merge_pre.data(), sizeof(int64_t) * SG.merge_pre.size());
@@ -955,6 +971,13 @@ int enqueue_front(emv_engine *e, char *db, const StepPlan &P) {
         hipLaunchKernelGGL(k_gseed, dim3(blocks), dim3(kBlock), 0, e->stream,
                            (const GChunk *)(db + P.off_seedch), (const int64_t *)(db + P.off_sp),
                            (int)P.seed_ch.size(), e->d_kk, e->d_kkoff, e->d_D);
+        if (!P.member_cells.empty()) {
+            const int64_t cnt = (int64_t)P.member_cells.size();
+            hipLaunchKernelGGL(k_gseed_cells,
+                               dim3((unsigned)std::min<int64_t>((cnt + kBlock - 1) / kBlock, 65536)),
+                               dim3(kBlock), 0, e->stream, (const int64_t *)(db + P.off_mc), cnt,
+                               e->d_D);
+        }
     }
     if (!P.places.empty())
         hipLaunchKernelGGL(k_gplace, dim3((unsigned)P.place_pre.back()), dim3(kBlock), 0,
@@ -1096,12 +1119,56 @@ int advance(emv_engine *e, const StepPlan &P) {
 
 int traceback_inst(emv_engine *e, Inst &I);
 
+// Group query (k groups): the reduced instance joins a virtual terminal to
+// every member of its group by an edge of weight M = 1 + sum of weights.
+// M exceeds every tree of the graph, so an optimal reduced tree uses each
+// virtual terminal as a leaf: its cells at graph vertices are
+// |X| * M + f[X][v], f being this table (singleton rows = distance to the
+// nearest member), and its answer cell f[C][virtual terminal k-1] =
+// k * M + min over the last group's members s of f[C][s].  The traceback
+// starts at the smallest such s (the reduced instance's record at the
+// virtual terminal is its smallest neighbour on a shortest path).
+int group_answer(emv_engine *e, Inst &I) {
+    const GraphRec &G = e->graphs[I.graph];
+    const int64_t C = ((int64_t)1 << (I.k - 1)) - 1;
+    std::vector<int32_t> row((size_t)G.np);
+    if (I.tbits == 16) {
+        std::vector<uint16_t> r16((size_t)G.np);
+        EHIP(e, hipMemcpyAsync(r16.data(), (const uint16_t *)(e->d_arena + I.abase) + C * G.np,
+                               sizeof(uint16_t) * G.n, hipMemcpyDeviceToHost, e->stream));
+        EHIP(e, hipStreamSynchronize(e->stream));
+        for (int64_t v = 0; v < G.n; v++)
+            row[(size_t)v] = r16[(size_t)v] == 0xFFFF ? INT32_MAX : (int32_t)r16[(size_t)v];
+    } else {
+        EHIP(e, hipMemcpyAsync(row.data(), e->d_arena + I.abase + C * G.np, sizeof(int32_t) * G.n,
+                               hipMemcpyDeviceToHost, e->stream));
+        EHIP(e, hipStreamSynchronize(e->stream));
+    }
+    int64_t best = INT64_MAX, root = -1;
+    for (int32_t s : I.groups.back()) {
+        const int64_t x = row[(size_t)s] == INT32_MAX ? INT64_MAX : (int64_t)row[(size_t)s];
+        if (x < best || (x == best && x != INT64_MAX && s < root)) {
+            best = x;
+            root = s;
+        }
+    }
+    I.cost = best;
+    I.root = root;
+    const int64_t M = 1 + G.wsum;
+    I.reduced_cost = (best == INT64_MAX) ? INT64_MAX : best + (int64_t)I.k * M;
+    return EMV_OK;
+}
+
 int read_costs_and_finish(emv_engine *e, const std::vector<int64_t> &done) {
     for (int64_t t : done) {
         Inst &I = e->inst[t];
         const GraphRec &G = e->graphs[I.graph];
-        if (I.status == EMV_OK) {
+        if (I.status == EMV_OK && !I.groups.empty()) {
+            ECHK(group_answer(e, I));
+            if (I.cost != INT64_MAX && (I.flags & EMV_LIST)) ECHK(traceback_inst(e, I));
+        } else if (I.status == EMV_OK) {
             const int64_t C = (I.k == 1) ? 0 : (((int64_t)1 << (I.k - 1)) - 1);
+            I.root = I.kk[I.k - 1];
             if (C == 0) {
                 I.cost = INT64_MAX;
             } else {
@@ -1121,6 +1188,7 @@ int read_costs_and_finish(emv_engine *e, const std::vector<int64_t> &done) {
                 }
                 if (I.flags & EMV_LIST) ECHK(traceback_inst(e, I));
             }
+            I.reduced_cost = I.cost;
         }
         I.state = ST_DONE;
         I.t_done = now_ms();
@@ -1151,8 +1219,8 @@ int traceback_inst(emv_engine *e, Inst &I) {
         return EMV_ERR_NOMEM;
     }
     hipLaunchKernelGGL(k_traceback, dim3(1), dim3(64), 0, e->stream,
-                       e->d_arena + I.abase + S2 * G.np, (int64_t)G.np, I.k,
-                       (int64_t)I.kk[I.k - 1], C, d_pairs, cap, d_stack, stcap, d_res);
+                       e->d_arena + I.abase + S2 * G.np, (int64_t)G.np, I.k, I.root, C,
+                       d_pairs, cap, d_stack, stcap, d_res);
     int64_t cnt = 0;
     hipError_t x1 = hipMemcpyAsync(&cnt, d_res, sizeof(int64_t), hipMemcpyDeviceToHost, e->stream);
     hipError_t x2 = hipStreamSynchronize(e->stream);
@@ -1190,7 +1258,8 @@ int admit(emv_engine *e) {
         I.stage = 1;
         I.t_admit = now_ms();
         std::vector<int32_t> k64(kWave, -1);
-        for (int j = 0; j < I.k; j++) k64[j] = I.kk[j];
+        // group queries seed their members separately (k_gseed_cells)
+        for (int j = 0; j < I.k; j++) k64[j] = I.groups.empty() ? I.kk[j] : -1;
         EHIP(e, hipMemcpyAsync(e->d_kk + (int64_t)I.slot * kWave, k64.data(),
                                sizeof(int32_t) * kWave, hipMemcpyHostToDevice, e->stream));
         // table: row 0 INF with record -1 (every other real cell is written by
@@ -1363,17 +1432,21 @@ static int check_instance(emv_engine *e, const GraphRec &G, int64_t k, const int
             eset(e, "terminal %lld out of range", (long long)kk[j]);
             return EMV_ERR_ARG;
         }
-    // int32 table and 32-bit labels: every cell is at most k*(n-1)*max w
-    if ((double)k * (double)(G.n - 1) * (double)G.maxw >= (double)INT32_MAX ||
-        (1ll << k) + G.n >= INT32_MAX) {
-        eset(e, "instance exceeds the engine's 32-bit range (use emv_solve)");
+    // The singleton stage relaxes on 32-bit labels: a distance (at most
+    // (n-1) * max w) plus an edge must stay below the all-ones INF.  The
+    // int32 table is decided by the bound B = sum of the singleton rows'
+    // maxima once they are known (advance: B >= 2^31 - 1 ends the request
+    // with EMV_ERR_RANGE); group queries keep B small, since their rows hold
+    // distances to the nearest member, not the reduction's M.
+    if ((double)G.n * (double)G.maxw >= 4294967295.0 || (1ll << k) + G.n >= INT32_MAX) {
+        eset(e, "graph distances exceed the engine's 32-bit labels (use emv_solve)");
         return EMV_ERR_RANGE;
     }
     return EMV_OK;
 }
 
 static int submit(emv_engine *e, int graph, int64_t k, const int64_t *kk, uint32_t flags,
-                  int64_t *ticket) {
+                  int64_t *ticket, std::vector<std::vector<int32_t>> *groups = nullptr) {
     const GraphRec &G = e->graphs[graph];
     ECHK(check_instance(e, G, k, kk));
     const int64_t cells = 2 * ((int64_t)1 << k) * G.np;
@@ -1387,6 +1460,7 @@ static int submit(emv_engine *e, int graph, int64_t k, const int64_t *kk, uint32
     I.graph = graph;
     I.k = (int)k;
     I.kk.assign(kk, kk + k);
+    if (groups) I.groups.swap(*groups);
     I.flags = flags;
     I.t_submit = now_ms();
     e->inst.push_back(I);
@@ -1756,6 +1830,33 @@ extern "C" int emv_server_submit(emv_engine *e, int64_t k, const int64_t *kk, ui
     return submit(e, 0, k, kk, flags, ticket);
 }
 
+extern "C" int emv_server_submit_groups(emv_engine *e, int64_t g, const int64_t *offsets,
+                                        const int64_t *members, uint32_t flags, int64_t *ticket) {
+    if (!e || !ticket || !offsets || !members) return EMV_ERR_ARG;
+    if (g < 2 || g > 30) {
+        eset(e, "groups=%lld outside [2, 30]", (long long)g);
+        return EMV_ERR_ARG;
+    }
+    const GraphRec &G = e->graphs[0];
+    std::vector<std::vector<int32_t>> groups((size_t)g);
+    std::vector<int64_t> first((size_t)g);
+    for (int64_t j = 0; j < g; j++) {
+        if (offsets[j + 1] <= offsets[j] || offsets[j] < 0) {
+            eset(e, "group %lld is empty", (long long)j);
+            return EMV_ERR_ARG;
+        }
+        for (int64_t i = offsets[j]; i < offsets[j + 1]; i++) {
+            if (members[i] < 0 || members[i] >= G.n) {
+                eset(e, "member %lld of group %lld out of range", (long long)members[i], (long long)j);
+                return EMV_ERR_ARG;
+            }
+            groups[(size_t)j].push_back((int32_t)members[i]);
+        }
+        first[(size_t)j] = members[offsets[j]];
+    }
+    return submit(e, 0, g, first.data(), flags, ticket, &groups);
+}
+
 extern "C" int emv_server_step(emv_engine *e, int64_t *running, int64_t *queued) {
     if (!e) return EMV_ERR_ARG;
     EHIP(e, hipSetDevice(e->device));
@@ -1782,6 +1883,45 @@ extern "C" int emv_server_poll(emv_engine *e, int64_t ticket, emv_request_info *
     info->queue_ms = (I.state == ST_QUEUED) ? 0.0 : I.t_admit - I.t_submit;
     info->latency_ms = (I.state == ST_DONE) ? I.t_done - I.t_submit : 0.0;
     info->npairs = (int64_t)I.pairs.size() / 2;
+    info->reduced_cost = I.reduced_cost;
+    info->groups = I.groups.empty() ? 0 : I.k;
+    info->root = I.root;
+    return EMV_OK;
+}
+
+extern "C" int64_t emv_server_group_weight(const emv_engine *e) {
+    return (e && !e->graphs.empty()) ? 1 + e->graphs[0].wsum : -1;
+}
+
+extern "C" int emv_engine_certify(emv_engine *e, int64_t ticket, emv_certificate *out) {
+    if (!e || !out || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    out->first_X = out->first_v = -1;
+    const Inst &I = e->inst[ticket];
+    if (I.abase < 0 || I.state != ST_DONE || I.status != EMV_OK) {
+        eset(e, "no table kept for ticket %lld", (long long)ticket);
+        return EMV_ERR_STATE;
+    }
+    EHIP(e, hipSetDevice(e->device));
+    const GraphRec &G = e->graphs[I.graph];
+    const int64_t S2 = 1ll << I.k;
+    std::vector<std::vector<int32_t>> srcs = I.groups;
+    if (srcs.empty())
+        for (int j = 0; j < I.k; j++) srcs.push_back(std::vector<int32_t>(1, I.kk[(size_t)j]));
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned long long cnt[4];
+    EHIP(e, certify_table(e->stream, e->d_arena + I.abase, I.tbits / 8,
+                          e->d_arena + I.abase + S2 * G.np, G.np, (int)G.n, I.k,
+                          e->d_rp + G.voff, e->d_adj, srcs, I.kk, I.groups.empty() ? 0 : 1, cnt));
+    out->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    out->cells = S2 * G.n;
+    out->bad_value = (int64_t)cnt[0];
+    out->not_finite = (int64_t)cnt[1];
+    out->bad_record = (int64_t)cnt[2];
+    if (cnt[3] != ~0ull) {
+        out->first_X = (int64_t)(cnt[3] >> 32);
+        out->first_v = (int64_t)(cnt[3] & 0xFFFFFFFFull);
+    }
     return EMV_OK;
 }
 

@@ -27,6 +27,7 @@
 
 #include "emv_hip.h"
 #include "emv_blocks.hpp"
+#include "emv_certify.hpp"
 #include "emv_kernels.hpp"
 
 using namespace emv;
@@ -70,6 +71,7 @@ struct emv_ctx {
     int64_t np = 0;
     int tbytes = 0;  // nonzero while a kept table is readable
     int64_t kept_cost = 0;  // answer cell of the last solve
+    std::vector<int32_t> kept_kk;  // terminals of the last solve
     int64_t tk = 0;
     // relaxation workspace, sized in chunks of 64 rows
     void *d_D = nullptr;          // labels, 8 bytes per cell reserved
@@ -1394,6 +1396,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     auto t_end = std::chrono::steady_clock::now();
     res->total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
     c->kept_cost = cost;
+    c->kept_kk.assign(src.begin(), src.end());
     if (!(flags & EMV_KEEP_TABLE)) invalidate_table(c);
     return EMV_OK;
 }
@@ -1552,6 +1555,92 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
     HIPCHK(c, e2);
     HIPCHK(c, e3);
     return EMV_OK;
+}
+
+namespace {
+
+// Runs the certificate over a table on the device and fills *out.
+int run_certificate(emv_ctx *c, const void *f, int tb, const int32_t *b, int64_t np, int k,
+                    const std::vector<int32_t> &kk, emv_certificate *out) {
+    std::vector<std::vector<int32_t>> srcs((size_t)k);
+    for (int t = 0; t < k; t++) srcs[(size_t)t].push_back(kk[(size_t)t]);
+    hipEvent_t a, e;
+    EventList evl;
+    HIPCHK(c, hipEventCreate(&a));
+    evl.v.push_back(a);
+    HIPCHK(c, hipEventCreate(&e));
+    evl.v.push_back(e);
+    HIPCHK(c, hipEventRecord(a, c->stream));
+    unsigned long long cnt[4];
+    HIPCHK(c, certify_table(c->stream, f, tb, b, np, (int)c->n, k, c->d_rp, c->d_adj, srcs, kk, 0,
+                            cnt));
+    HIPCHK(c, hipEventRecord(e, c->stream));
+    HIPCHK(c, hipEventSynchronize(e));
+    out->cells = ((int64_t)1 << k) * c->n;
+    out->bad_value = (int64_t)cnt[0];
+    out->not_finite = (int64_t)cnt[1];
+    out->bad_record = (int64_t)cnt[2];
+    if (cnt[3] != ~0ull) {
+        out->first_X = (int64_t)(cnt[3] >> 32);
+        out->first_v = (int64_t)(cnt[3] & 0xFFFFFFFFull);
+    }
+    out->ms = ev_ms(a, e);
+    return EMV_OK;
+}
+
+}  // namespace
+
+extern "C" int emv_certify(emv_ctx *c, emv_certificate *out) {
+    if (!c || !out) return EMV_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    out->first_X = out->first_v = -1;
+    if (!c->d_f || c->tk == 0 || (int64_t)c->kept_kk.size() != c->tk) {
+        set_err(c, "no table kept (solve with EMV_KEEP_TABLE)");
+        return EMV_ERR_STATE;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    return run_certificate(c, c->d_f, c->tbytes, c->d_b, c->np, (int)c->tk, c->kept_kk, out);
+}
+
+extern "C" int emv_certify_table(emv_ctx *c, int64_t k, const int64_t *kk, const int64_t *f,
+                                 const int32_t *code, emv_certificate *out) {
+    if (!c || !out || !kk || !f || !code) return EMV_ERR_ARG;
+    memset(out, 0, sizeof(*out));
+    out->first_X = out->first_v = -1;
+    if (c->n <= 0 || !c->d_rp) {
+        set_err(c, "no graph loaded");
+        return EMV_ERR_STATE;
+    }
+    if (k < 1 || k > 30) {
+        set_err(c, "k=%lld outside [1,30]", (long long)k);
+        return EMV_ERR_ARG;
+    }
+    std::vector<int32_t> kk32((size_t)k);
+    for (int64_t t = 0; t < k; t++) {
+        if (kk[t] < 0 || kk[t] >= c->n) {
+            set_err(c, "terminal %lld out of range", (long long)kk[t]);
+            return EMV_ERR_ARG;
+        }
+        kk32[(size_t)t] = (int32_t)kk[t];
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t cells = ((size_t)1 << k) * (size_t)c->n;
+    int64_t *d_f = nullptr;
+    int32_t *d_b = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d_f, sizeof(int64_t) * cells));
+    int rc = EMV_OK;
+    hipError_t err = hipMalloc((void **)&d_b, sizeof(int32_t) * cells);
+    if (err == hipSuccess) err = hipMemcpy(d_f, f, sizeof(int64_t) * cells, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(d_b, code, sizeof(int32_t) * cells, hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        set_err(c, "table upload failed: %s", hipGetErrorString(err));
+        rc = (err == hipErrorOutOfMemory) ? EMV_ERR_NOMEM : EMV_ERR_HIP;
+    } else {
+        rc = run_certificate(c, d_f, 8, d_b, c->n, (int)k, kk32, out);
+    }
+    (void)hipFree(d_f);
+    if (d_b) (void)hipFree(d_b);
+    return rc;
 }
 
 extern "C" void emv_result_free(emv_result *res) {

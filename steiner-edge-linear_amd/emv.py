@@ -51,6 +51,23 @@ class EmvResult(ctypes.Structure):
     ]
 
 
+class Certificate(ctypes.Structure):
+    """emv_certificate: cells of a table that break the EMV fixed point or
+    the record rule (all zero: the table is the reference's, cell for cell)."""
+    _fields_ = [("cells", ctypes.c_int64), ("bad_value", ctypes.c_int64),
+                ("not_finite", ctypes.c_int64), ("bad_record", ctypes.c_int64),
+                ("first_X", ctypes.c_int64), ("first_v", ctypes.c_int64), ("ms", ctypes.c_double)]
+
+    @property
+    def ok(self):
+        return self.bad_value == 0 and self.not_finite == 0 and self.bad_record == 0
+
+    def __repr__(self):
+        return (f"Certificate(cells={self.cells}, bad_value={self.bad_value}, "
+                f"not_finite={self.not_finite}, bad_record={self.bad_record}, "
+                f"first=({self.first_X}, {self.first_v}), ms={self.ms:.1f})")
+
+
 class EmvError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{ERRORS.get(status, status)}: {msg}")
@@ -85,6 +102,9 @@ def lib():
         L.emv_result_free.restype = None
         L.emv_solve_footprint.argtypes = [vp, ctypes.c_int64, ctypes.c_uint32]
         L.emv_solve_footprint.restype = ctypes.c_int64
+        L.emv_certify.argtypes = [vp, ctypes.POINTER(Certificate)]
+        L.emv_certify_table.argtypes = [vp, ctypes.c_int64, _i64p, _i64p,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Certificate)]
         _LIB = L
     return _LIB
 
@@ -193,6 +213,26 @@ class Context:
     def footprint(self, k, flags=0):
         return lib().emv_solve_footprint(self._h, k, flags)
 
+    def certify(self):
+        """Certificate of the kept table (solve with KEEP_TABLE)."""
+        c = Certificate()
+        self._check(lib().emv_certify(self._h, ctypes.byref(c)))
+        return c
+
+    def certify_table(self, kk, f, code):
+        """Certificate of a caller's table (f int64 with INF = INT64_MAX,
+        code int32, both 2^k x n) for terminals kk on the loaded graph."""
+        kk = np.ascontiguousarray(kk, dtype=np.int64)
+        f = np.ascontiguousarray(f, dtype=np.int64)
+        code = np.ascontiguousarray(code, dtype=np.int32)
+        if f.size != (1 << len(kk)) * self.n or code.size != f.size:
+            raise ValueError("table shape does not match 2^k x n")
+        c = Certificate()
+        self._check(lib().emv_certify_table(
+            self._h, len(kk), _p(kk), _p(f),
+            code.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(c)))
+        return c
+
 
 # ------------------------------------------------------------- generator
 
@@ -297,7 +337,9 @@ class BatchStats(ctypes.Structure):
 class RequestInfo(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("cost", ctypes.c_int64),
                 ("k", ctypes.c_int32), ("stage", ctypes.c_int32), ("queue_ms", ctypes.c_double),
-                ("latency_ms", ctypes.c_double), ("npairs", ctypes.c_int64)]
+                ("latency_ms", ctypes.c_double), ("npairs", ctypes.c_int64),
+                ("reduced_cost", ctypes.c_int64), ("groups", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("root", ctypes.c_int64)]
 
 
 _ENG_BOUND = False
@@ -334,6 +376,11 @@ def _bind_engine():
     L.emv_engine_pairs.argtypes = [vp, ctypes.c_int64, _i64p, ctypes.c_int64,
                                    ctypes.POINTER(ctypes.c_int64)]
     L.emv_engine_release.argtypes = [vp, ctypes.c_int64]
+    L.emv_engine_certify.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(Certificate)]
+    L.emv_server_submit_groups.argtypes = [vp, ctypes.c_int64, _i64p, _i64p, ctypes.c_uint32,
+                                           ctypes.POINTER(ctypes.c_int64)]
+    L.emv_server_group_weight.argtypes = [vp]
+    L.emv_server_group_weight.restype = ctypes.c_int64
     _ENG_BOUND = True
     return L
 
@@ -381,6 +428,12 @@ class _Engine:
 
     def release(self, ticket):
         self._check(_bind_engine().emv_engine_release(self._h, ticket))
+
+    def certify(self, ticket):
+        """Certificate of a finished instance's kept table (KEEP_TABLE)."""
+        c = Certificate()
+        self._check(_bind_engine().emv_engine_certify(self._h, ticket, ctypes.byref(c)))
+        return c
 
 
 class Batch(_Engine):
@@ -432,6 +485,22 @@ class Server(_Engine):
         self._check(_bind_engine().emv_server_submit(self._h, len(t), _p(t), flags,
                                                      ctypes.byref(ticket)))
         return ticket.value
+
+    def submit_groups(self, groups, flags=0):
+        """Group Steiner query: groups = list of member lists (2..30 groups)."""
+        off = np.zeros(len(groups) + 1, dtype=np.int64)
+        off[1:] = np.cumsum([len(g) for g in groups])
+        mem = np.ascontiguousarray(np.concatenate([np.asarray(g, dtype=np.int64).reshape(-1)
+                                                   for g in groups]), dtype=np.int64)
+        ticket = ctypes.c_int64(-1)
+        self._check(_bind_engine().emv_server_submit_groups(self._h, len(groups), _p(off),
+                                                            _p(mem), flags,
+                                                            ctypes.byref(ticket)))
+        return ticket.value
+
+    def group_weight(self):
+        """M = 1 + sum of the resident graph's weights (the reduction's edge weight)."""
+        return _bind_engine().emv_server_group_weight(self._h)
 
     def step(self):
         r, q = ctypes.c_int64(0), ctypes.c_int64(0)

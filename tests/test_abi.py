@@ -65,6 +65,31 @@ int main(void){ printf("%zu %zu %zu\n", sizeof(emv_result), offsetof(emv_result,
     assert int(out[2]) == emv.EmvResult.merge_alg_bytes.offset
 
 
+@pytest.mark.parametrize("struct,header,fields", [
+    ("emv_certificate", "emv_hip.h", ["cells", "bad_record", "first_v", "ms"]),
+    ("emv_request_info", "emv_engine.h", ["cost", "npairs", "reduced_cost", "groups", "root"]),
+    ("emv_batch_stats", "emv_engine.h", ["relax_rounds", "graph_fallback"]),
+    ("emv_engine_params", "emv_engine.h", ["merge_rows_per_wave", "graph_round_margin"]),
+])
+def test_struct_layouts_match_headers(struct, header, fields):
+    """Every ctypes mirror agrees with its C struct in size and field offsets."""
+    mirror = {"emv_certificate": emv.Certificate, "emv_request_info": emv.RequestInfo,
+              "emv_batch_stats": emv.BatchStats, "emv_engine_params": emv.EngineParams}[struct]
+    prints = " ".join(f'printf("%zu ", offsetof({struct}, {f}));' for f in fields)
+    src = (f'#include <stdio.h>\n#include <stddef.h>\n#include "{header}"\n'
+           f'int main(void){{ printf("%zu ", sizeof({struct})); {prints} return 0; }}\n')
+    tmp = os.path.join("/tmp", f"emv_layout_{struct}_{os.getpid()}")
+    with open(tmp + ".c", "w") as fh:
+        fh.write(src)
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", tmp, tmp + ".c"], check=True)
+    out = [int(x) for x in subprocess.run([tmp], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    os.remove(tmp)
+    os.remove(tmp + ".c")
+    assert out[0] == ctypes.sizeof(mirror)
+    assert out[1:] == [getattr(mirror, f).offset for f in fields]
+
+
 def test_status_codes_without_device():
     lib = emv.lib()
     h = ctypes.c_void_p()
@@ -76,6 +101,8 @@ def test_status_codes_without_device():
     assert lib.emv_load_edges(None, 4, 0, None) == -1
     assert lib.emv_solve(None, 2, None, 0, None) == -1
     assert lib.emv_solve_footprint(None, 3, 0) == -1
+    assert lib.emv_certify(None, None) == -1
+    assert lib.emv_certify_table(None, 2, None, None, None, None) == -1
 
 
 INPUT_RE = re.compile(r"(.*)n = (.*), m = (.*), k = (.*), cost = (.*) \[(.*)ms\] {peak:(.*)GiB} "

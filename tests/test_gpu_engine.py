@@ -7,6 +7,7 @@ import pytest
 import emv
 import oracle
 from golden_data import META, load_instance
+from bigcheck import validate_direct_group_tree
 from stputil import check_listed_tree, Instance
 
 pytestmark = pytest.mark.gpu
@@ -115,6 +116,88 @@ def test_server_results_alone_equal_in_the_mix():
             np.testing.assert_array_equal(f, of)
             np.testing.assert_array_equal(code, ocode)
             s.release(t)
+
+
+def _group_queries(n, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        g = int(rng.integers(2, 7))
+        out.append([rng.choice(n, size=int(rng.integers(1, 6)), replace=False) for _ in range(g)])
+    return out
+
+
+@pytest.mark.parametrize("wmax", [100, 3000])
+def test_server_group_queries_match_reduced_oracle(wmax):
+    """Group queries on the resident graph against the oracle on the reduced
+    instance (virtual terminals joined to their members by weight M, the
+    reduction of README.txt:4-5): reduced cells at graph vertices are
+    |X| M + the served cells, records agree outside the singleton rows, the
+    cost is the reduced cost less g M; alone equals in the mix with terminal
+    queries (tolerance 0).  wmax 3000 takes 32-bit cells, 100 16-bit ones."""
+    R, C = 30, 40
+    n = R * C
+    e = emv.grid_graph(R, C, wmax, 47)
+    gqs = _group_queries(n, 6, 3)
+    tqs = [emv.distinct(n, int(k), 4100 + i, 2) for i, k in enumerate([3, 5, 7, 4])]
+    alone = []
+    with emv.Server(n, e, arena_bytes=512 << 20, max_inflight=16) as s:
+        M = s.group_weight()
+        assert M == 1 + int(np.asarray(e)[:, 2].sum())
+        for g in gqs:
+            t = s.submit_groups(g, emv.KEEP_TABLE | emv.LIST)
+            while s.poll(t).state != 2:
+                s.step()
+            info = s.poll(t)
+            assert info.status == 0 and info.groups == len(g)
+            nn, red, kk, M2 = emv.group_reduce(n, e, g)
+            assert M2 == M
+            ocost, of, ocode = oracle.emv(nn, red, kk)
+            assert info.reduced_cost == ocost and info.cost == ocost - len(g) * M
+            f, code = s.read_rows(t, 0, 1 << len(g), n)
+            for X in range(1, 1 << len(g)):
+                np.testing.assert_array_equal(of[X, :n], f[X] + bin(X).count("1") * M)
+                if X & (X - 1):
+                    np.testing.assert_array_equal(ocode[X, :n], code[X])
+            cert = s.certify(t)
+            assert cert.ok, cert
+            ok, why = validate_direct_group_tree(n, e, g, s.pairs(t), info.cost, info.root)
+            assert ok, why
+            alone.append((info.cost, f, code))
+            s.release(t)
+    with emv.Server(n, e, arena_bytes=512 << 20, max_inflight=16) as s:
+        tickets = []
+        for i in range(max(len(gqs), len(tqs))):
+            if i < len(gqs):
+                tickets.append(("g", i, s.submit_groups(gqs[i], emv.KEEP_TABLE)))
+            if i < len(tqs):
+                tickets.append(("t", i, s.submit(tqs[i], emv.KEEP_TABLE)))
+            s.step()  # later requests join while earlier ones are mid-way
+        while any(s.poll(t).state != 2 for _, _, t in tickets):
+            s.step()
+        for kind, i, t in tickets:
+            info = s.poll(t)
+            assert info.status == 0
+            if kind == "g":
+                cost, f0, c0 = alone[i]
+                f, code = s.read_rows(t, 0, 1 << len(gqs[i]), n)
+                assert info.cost == cost
+                np.testing.assert_array_equal(f, f0)
+                np.testing.assert_array_equal(code, c0)
+            else:
+                assert info.cost == oracle.emv(n, e, tqs[i])[0]
+            s.release(t)
+
+
+def test_server_rejects_bad_group_queries():
+    R, C = 10, 10
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 1)
+    with emv.Server(n, e, arena_bytes=64 << 20, max_inflight=4) as s:
+        for bad in ([[1, 2]], [[1], []], [[1], [n]], [[0]] * 31):
+            with pytest.raises(emv.EmvError) as ei:
+                s.submit_groups(bad)
+            assert ei.value.status == -1
 
 
 def test_server_admission_by_footprint():
