@@ -142,6 +142,19 @@ int emv_release_table(emv_ctx *ctx);
 int emv_sssp(emv_ctx *ctx, int64_t s, int64_t *d_out, int64_t *pred_out,
              double *ms);
 
+/* emv_sssp without any allocation.  No counterpart in the reference; it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Same results as
+ * emv_sssp, but d_dist and d_pred are DEVICE pointers the caller owns (n
+ * int64 each; either may be NULL), and the work memory is a fixed array of
+ * 32 MiB in the library, one per GPU, held until the process ends: graphs of
+ * more than 2,097,136 vertices are refused with EMV_ERR_CAPACITY.  The call
+ * makes no hipMalloc/hipFree; it uses the context's stream and pinned word
+ * and returns after the stream has drained.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two host threads, two
+ * contexts) race and must be serialised by the caller. */
+int emv_sssp_noalloc(emv_ctx *ctx, int64_t s, int64_t *d_dist, int64_t *d_pred);
+
 void emv_result_free(emv_result *res);
 
 /* Certificate of a table: counts of the cells that break the EMV fixed point

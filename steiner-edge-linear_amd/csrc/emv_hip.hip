@@ -106,7 +106,95 @@ struct emv_ctx {
     int64_t xs_cap = 0;
     int32_t *d_src = nullptr;                // shortest-path sources
     uint32_t *h_pin = nullptr;               // pinned readback of a work word
+    unsigned char *d_scratch = nullptr;      // this device's copy of `scratch` (emv_sssp_noalloc)
 };
+
+// ------------------------------------------------ emv_sssp_noalloc scratch
+// Device memory of emv_sssp_noalloc, which calls no allocator: a file-scope
+// array that every GPU the library runs on holds until the process ends.
+// Layout for a graph of n vertices: labels uint64[n] at byte 0, the two
+// round-parity flag arrays uint32[n] at 8n and 12n, four round words uint32
+// at 16n.  Host code reaches it only through the address hipGetSymbolAddress
+// gives for the current device (emv_ctx::d_scratch); every call on a device
+// shares that device's array.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int64_t kScratchMaxN = (int64_t)((kScratchBytes - 256) / 16);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Labels INF except 0 at s; both flag arrays and the round words cleared
+// (a fill kernel, so that no memset node is needed under capture).
+__global__ void k_ss_init(uint64_t *__restrict__ lab, uint32_t *__restrict__ flg, int64_t n,
+                          int64_t s, uint32_t *__restrict__ words) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        lab[i] = (i == s) ? 0ull : ~0ull;
+        flg[i] = 0u;
+        flg[n + i] = 0u;
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 4) words[threadIdx.x] = 0u;
+}
+
+// One pull round of the single-source labels: a flagged vertex (every vertex
+// when `dense`) takes the least neighbour label + w; a vertex whose label
+// dropped flags its neighbours for the next round and raises
+// words[(r + 1) & 3].  Round r returns at once when round r - 1 changed
+// nothing, and clears words[(r + 2) & 3] for the round after next.  Labels
+// only drop, so the rounds end at the shortest-path distances in any order.
+__global__ __launch_bounds__(256) void k_ss_round(uint64_t *__restrict__ lab,
+                                                  uint32_t *__restrict__ Fcur,
+                                                  uint32_t *__restrict__ Fnext,
+                                                  const int32_t *__restrict__ rp,
+                                                  const Edge *__restrict__ adj, int64_t n, int r,
+                                                  int dense, uint32_t *__restrict__ words) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) words[(r + 2) & 3] = 0u;
+    if (!dense && words[r & 3] == 0u) return;
+    bool raised = false;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (!dense) {
+            if (Fcur[v] == 0u) continue;
+            Fcur[v] = 0u;
+        }
+        const uint64_t cur = lab[v];
+        uint64_t best = cur;
+        for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
+            const uint64_t lu = lab[adj[e].v];
+            if (lu != ~0ull && lu + (uint64_t)adj[e].w < best) best = lu + (uint64_t)adj[e].w;
+        }
+        if (best < cur) {
+            lab[v] = best;
+            raised = true;
+            for (int32_t e = rp[v]; e < rp[v + 1]; e++) Fnext[adj[e].v] = 1u;
+        }
+    }
+    if (raised) words[(r + 1) & 3] = 1u;
+}
+
+// Distances (INT64_MAX if unreached) and predecessors (the smallest
+// neighbour u with d[u] + w(u,v) == d[v]; -1 at the source and when
+// unreached), the rule of emv_sssp.
+__global__ void k_ss_out(const uint64_t *__restrict__ lab, const int32_t *__restrict__ rp,
+                         const Edge *__restrict__ adj, int64_t n, int64_t *__restrict__ d,
+                         int64_t *__restrict__ pred) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const uint64_t dv = lab[v];
+        int64_t p = -1;
+        if (dv != 0ull && dv != ~0ull) {
+            for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
+                const int64_t u = adj[e].v;
+                const uint64_t lu = lab[u];
+                if (lu != ~0ull && lu + (uint64_t)adj[e].w == dv && (p < 0 || u < p)) p = u;
+            }
+        }
+        if (d) d[v] = (dv == ~0ull) ? INT64_MAX : (int64_t)dv;
+        if (pred) pred[v] = p;
+    }
+}
+
+}  // namespace
 
 namespace {
 
@@ -1008,6 +1096,15 @@ extern "C" int emv_open(emv_ctx **out, int device) {
             rc = EMV_ERR_HIP;
             break;
         }
+        // this device's copy of the scratch array (emv_sssp_noalloc): the
+        // array's name is a host-side stand-in, its device address comes
+        // from the runtime, per device, with the device current
+        void *sp = nullptr;
+        if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) != hipSuccess || !sp) {
+            rc = EMV_ERR_HIP;
+            break;
+        }
+        c->d_scratch = static_cast<unsigned char *>(sp);
         if (hipMalloc((void **)&c->d_any, sizeof(uint32_t) * 4) != hipSuccess ||
             hipMalloc((void **)&c->d_cnt, sizeof(uint32_t) * 4) != hipSuccess ||
             hipMalloc((void **)&c->d_rowmax, sizeof(unsigned long long) * 64) != hipSuccess ||
@@ -1641,6 +1738,60 @@ extern "C" int emv_certify_table(emv_ctx *c, int64_t k, const int64_t *kk, const
     (void)hipFree(d_f);
     if (d_b) (void)hipFree(d_b);
     return rc;
+}
+
+extern "C" int emv_sssp_noalloc(emv_ctx *c, int64_t s, int64_t *d_dist, int64_t *d_pred) {
+    if (!c) return EMV_ERR_ARG;
+    if (c->n <= 0 || !c->d_rp) {
+        set_err(c, "no graph loaded");
+        return EMV_ERR_STATE;
+    }
+    if (s < 0 || s >= c->n) {
+        set_err(c, "source %lld out of range", (long long)s);
+        return EMV_ERR_ARG;
+    }
+    if (c->n > kScratchMaxN) {
+        set_err(c, "n = %lld exceeds the %lld vertices the %zu-byte scratch holds (use emv_sssp)",
+                (long long)c->n, (long long)kScratchMaxN, kScratchBytes);
+        return EMV_ERR_CAPACITY;
+    }
+    if (!c->d_scratch) {
+        set_err(c, "scratch address unknown");
+        return EMV_ERR_STATE;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = c->n;
+    uint64_t *lab = reinterpret_cast<uint64_t *>(c->d_scratch);
+    uint32_t *flg = reinterpret_cast<uint32_t *>(c->d_scratch + 8 * n);
+    uint32_t *words = reinterpret_cast<uint32_t *>(c->d_scratch + 16 * n);
+    const int grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>((n + 255) / 256, (int64_t)c->num_cu * 8));
+    hipLaunchKernelGGL(k_ss_init, dim3(grid), dim3(256), 0, c->stream, lab, flg, n, s, words);
+    hipLaunchKernelGGL(k_ss_round, dim3(grid), dim3(256), 0, c->stream, lab, flg, flg + n, c->d_rp,
+                       c->d_adj, n, 0, 1, words);
+    HIPCHK(c, hipGetLastError());
+    int r = 1, group = 2;
+    for (;;) {
+        for (int g = 0; g < group; g++, r++)
+            hipLaunchKernelGGL(k_ss_round, dim3(grid), dim3(256), 0, c->stream, lab,
+                               flg + (r & 1) * n, flg + ((r + 1) & 1) * n, c->d_rp, c->d_adj, n,
+                               r, 0, words);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(c->h_pin, words + (r & 3), sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (*c->h_pin == 0) break;
+        group = std::min(group * 2, 64);
+        if ((int64_t)r > 2 * (n + 8)) {
+            set_err(c, "shortest paths did not converge");
+            return EMV_ERR_RANGE;
+        }
+    }
+    hipLaunchKernelGGL(k_ss_out, dim3(grid), dim3(256), 0, c->stream, lab, c->d_rp, c->d_adj, n,
+                       d_dist, d_pred);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return EMV_OK;
 }
 
 extern "C" void emv_result_free(emv_result *res) {

@@ -103,6 +103,7 @@ def lib():
         L.emv_solve_footprint.argtypes = [vp, ctypes.c_int64, ctypes.c_uint32]
         L.emv_solve_footprint.restype = ctypes.c_int64
         L.emv_certify.argtypes = [vp, ctypes.POINTER(Certificate)]
+        L.emv_sssp_noalloc.argtypes = [vp, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.emv_certify_table.argtypes = [vp, ctypes.c_int64, _i64p, _i64p,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Certificate)]
         _LIB = L
@@ -209,6 +210,11 @@ class Context:
         ms = ctypes.c_double(0)
         self._check(lib().emv_sssp(self._h, s, _p(d), _p(p), ctypes.byref(ms)))
         return d, p, ms.value
+
+    def sssp_noalloc(self, s, d_dist_ptr, d_pred_ptr):
+        """emv_sssp into caller-owned device memory (raw device addresses,
+        e.g. torch tensors' data_ptr()), with no allocation in the library."""
+        self._check(lib().emv_sssp_noalloc(self._h, s, d_dist_ptr, d_pred_ptr))
 
     def footprint(self, k, flags=0):
         return lib().emv_solve_footprint(self._h, k, flags)
