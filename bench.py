@@ -8,33 +8,39 @@ of 2^14 x 2^18 cells (16-bit: the distance bound is below 0xFFFF).  The
 graph is resident in HBM before timing.
 
 Prints one JSON line.  `value` = DP cells (2^k * n) per second of wall time
-over the timed steps.  `roofline` covers the dominant kernel (the kernel
-function with the largest share of device time: the relaxation round kernel
-here; the merge runs as split and whole-row kernels per layer, and is
-VALU/cache-bound), with achieved = algorithmic bytes / kernel time from HIP
-events on the library's stream, and traffic = HBM bytes per launch measured
-by two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled against a
-plain copy of known size.  `cpu_baseline` times the reference
-reader-el (oracle/_ref, OpenMP) on a bounded sample of the same family on
-this host's cores.
+over the timed steps.  `roofline` covers the dominant kernel (the
+relaxation round kernel), with achieved = algorithmic bytes / kernel time
+from HIP events on the library's stream, and traffic = HBM bytes per launch
+measured by two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled
+against a plain copy of known size.  The merge is reported beside it
+against its measured HBM traffic and its VALU floor (it re-reads lower rows
+from cache, so its per-X streaming model is no HBM figure).  `table32`
+repeats the solve with an int32 table.  `cpu_baseline` times the reference
+reader-el (oracle/_ref, OpenMP on every physical core up to 127) on a
+bounded sample of the same family, with the CPU model and a row on the
+n = 1,000 4-regular k = 8 instance.
 
-Two more legs go into the same line, each compared at identical costs with
-the plain one-instance-at-a-time GPU path (Context.solve):
+More legs go into the same line, each compared at identical costs with the
+plain one-instance-at-a-time GPU path (Context.solve):
   "batched": 256 uneven random regular instances (n = round(2^U[10,16]),
              d ~ U{3..8}, k ~ U{6..12}, seeds 3000+i) solved by one Batch
-             (shared launches, autotuned shape, hipGraph replay) vs one
-             plain solve per instance with the graph already loaded;
+             (shared launches, autotuned shape, hipGraph replay; the eager
+             first solve is reported too) vs one plain solve per instance
+             with the graph already loaded;
   "served":  512 x 512 grid (w ~ U[1,100], seed 4), terminal-set queries
-             (k ~ U{4..12}, seed 40) on a continuous-batching Server: a
-             burst (all queries at t=0) and Poisson streams at rates swept
-             as multiples of the plain path's serial capacity, with p50/p99
-             latency from arrival to completion; the plain serial figures
-             replay the same arrival times through a FIFO single server
-             whose service times are the measured plain solve times.
-A third leg, "group", solves the group Steiner workload through the
-reduction (1024 x 1024 grid, w ~ U[1,1000], 13 groups of 32 vertices,
-seed 5; virtual terminals joined by edges of weight M = 1 + sum of w) and
-reports its time and OPT_group = OPT(reduced) - 13 M.
+             (k ~ U{4..12}, seed 40) and group queries on a
+             continuous-batching Server: a burst (all queries at t=0) and
+             Poisson streams at rates swept as multiples of the plain path's
+             serial capacity (up to saturation), with p50/p99 latency from
+             arrival to completion; the plain serial figures replay the same
+             arrival times through a FIFO single server whose service times
+             are the measured plain solve times;
+  "group":   the group Steiner workload (1024 x 1024 grid, w ~ U[1,1000],
+             13 groups of 32 vertices, seed 5) solved through the reduction
+             (virtual terminals joined by edges of weight M = 1 + sum of w)
+             with its listed tree validated as a group Steiner tree, and the
+             same query on the Server's resident grid (the DP on the grid
+             itself, no int64 table).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--no-traffic]
                     [--legs plain,batch,serve,group] [--queries 24]
@@ -43,6 +49,7 @@ import argparse
 import json
 import os
 import re
+import signal
 import subprocess
 import sys
 import tempfile
@@ -53,6 +60,8 @@ PKG = os.path.join(REPO, "steiner-edge-linear_amd")
 sys.path.insert(0, PKG)
 
 HBM_PEAK_GBS = 8000.0
+# dense integer VALU: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-ops per second
+VALU_LANE_OPS = 256 * 64 * 2.4e9
 METRIC = "DP cells/s k=14 n=2^18 (% HBM roofline); served solves/s + p99 vs plain GPU"
 
 WORKLOAD = dict(workload="EMV plain single instance: random connected 8-regular graph "
@@ -64,57 +73,130 @@ def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline():
-    """Reference reader-el (OpenMP, its own Makefile recipe minus
-    -march=native) on an 8-regular n=65536 k=10 instance of the same family
-    (about 10-30 s of CPU work); DP cells/s from its [kernel: ms] field."""
+def _lscpu():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    info = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            info[k.strip()] = v.strip()
+    return info
+
+
+def physical_cores(info):
+    try:
+        return int(info["Core(s) per socket"]) * int(info["Socket(s)"])
+    except (KeyError, ValueError):
+        return os.cpu_count() or 1
+
+
+def _run_reference(stp, threads, timeout=900):
+    ref = os.path.join(REPO, "oracle", "_ref", "reader-el")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
+    out = subprocess.run([ref, "-in", stp, "-el"], capture_output=True, text=True, env=env,
+                         cwd=os.path.dirname(stp), timeout=timeout)
+    line = [x for x in out.stdout.splitlines() if x.startswith("erickson")][0]
+    kms = float(re.search(r"\[kernel: ([0-9.]+) ms", line).group(1))
+    done = float(re.search(r"done\. \[([0-9.]+) ms\]", line).group(1))
+    cost = int(re.search(r"\[cost: (-?[0-9]+)\]", line).group(1))
+    return kms, done, cost
+
+
+def cpu_baseline(config2=False):
+    """Reference reader-el (OpenMP on min(physical cores, 127) threads,
+    reader-el.c:62,2213; its Makefile recipe minus -march=native) on an
+    8-regular n=65536 k=10 instance of the same family (the bounded sample,
+    about 10-30 s of CPU work), on the n=1000 4-regular k=8 instance next to
+    the GPU solve of the same file, and with config2 on the full n=2^18 k=14
+    instance; DP cells/s from its [kernel: ms] field."""
+    import emv
+
+    info = _lscpu()
+    threads = min(physical_cores(info), 127)
+    model = info.get("Model name", "unknown")
     ref = os.path.join(REPO, "oracle", "_ref", "reader-el")
     gen = os.path.join(PKG, "emv-gen")
     n, d, k = 65536, 8, 10
-    threads = min(16, os.cpu_count() or 1, 127)
     sample = f"8-regular n={n} m={n * d // 2} w~U[1,1000] k={k} seed 2, full EMV kernel"
+    out = dict(unit="DP cells/s", cores=threads, cpu_model=model,
+               threads_per_core=info.get("Thread(s) per core"), sample=sample)
     with tempfile.TemporaryDirectory() as td:
         stp = os.path.join(td, "sample.stp")
         with open(stp, "w") as fh:
             subprocess.run([gen, "regular", str(n), str(d), "1000", str(k), "2"], stdout=fh,
                            check=True)
         if os.path.exists(ref):
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
-            out = subprocess.run([ref, "-in", stp, "-el"], capture_output=True, text=True,
-                                 env=env, cwd=td, timeout=600)
-            line = [x for x in out.stdout.splitlines() if x.startswith("erickson")][0]
-            ms = float(re.search(r"\[kernel: ([0-9.]+) ms", line).group(1))
-            kind = "reference"
+            kms, done, _ = _run_reference(stp, threads)
+            out.update(kind="reference", kernel_ms=kms, done_ms=done)
         else:
             sys.path.insert(0, os.path.join(REPO, "oracle"))
-            import emv
             import oracle
 
             e = emv.regular_graph(n, d, 1000, 2)
             kk = emv.terminals(n, k, 2)
             t0 = time.perf_counter()
             oracle.emv(n, e, kk, threads)
-            ms = (time.perf_counter() - t0) * 1e3
-            kind = "port"
-    cells = (1 << k) * n
-    return dict(value=cells / (ms / 1e3), unit="DP cells/s", cores=threads, kind=kind,
-                sample=sample, kernel_ms=ms)
+            kms = (time.perf_counter() - t0) * 1e3
+            out.update(kind="port", kernel_ms=kms)
+        out["value"] = (1 << k) * n / (kms / 1e3)
+        # the n=1000 4-regular k=8 instance (the reference's own STP file and
+        # the GPU solve of the same graph)
+        cfg1 = os.path.join(REPO, "tests", "golden", "regular_n1000_d4_k8_s1.stp")
+        if os.path.exists(ref) and os.path.exists(cfg1):
+            kms1, done1, cost1 = _run_reference(cfg1, threads)
+            sys.path.insert(0, os.path.join(REPO, "tests"))
+            from stputil import parse_stp
+
+            inst = parse_stp(cfg1)
+            with emv.Context(0) as ctx:
+                ctx.load_edges(inst.n, inst.edges)
+                ctx.solve(inst.terminals)  # warm
+                sols = [ctx.solve(inst.terminals) for _ in range(5)]
+            gms = min(s.kernel_ms for s in sols)
+            if sols[-1].cost != cost1:
+                raise RuntimeError(f"n=1000 instance: GPU cost {sols[-1].cost} != reference {cost1}")
+            cells = (1 << inst.k) * inst.n
+            out["n1000_k8"] = dict(workload="4-regular n=1000 m=2000 w~U[1,100] k=8 seed 1 "
+                                   "(tests/golden)", cost=cost1, cpu_kernel_ms=kms1,
+                                   cpu_done_ms=done1, cpu_cells_per_s=cells / (kms1 / 1e3),
+                                   gpu_kernel_ms=gms, gpu_cells_per_s=cells / (gms / 1e3))
+        if config2 and os.path.exists(ref):
+            stp2 = os.path.join(td, "cfg2.stp")
+            with open(stp2, "w") as fh:
+                subprocess.run([gen, "regular", "262144", "8", "1000", "14", "2"], stdout=fh,
+                               check=True)
+            log("cpu baseline: reference on n=2^18 k=14 (minutes)")
+            kms2, done2, cost2 = _run_reference(stp2, threads, timeout=3000)
+            out["n262144_k14"] = dict(cost=cost2, kernel_ms=kms2, done_ms=done2,
+                                      cells_per_s=(1 << 14) * 262144 / (kms2 / 1e3))
+    return out
 
 
-def _pmc_avg(db, prefix):
+def _pmc_rows(db):
+    import sqlite3
+
+    cur = sqlite3.connect(db).cursor()
+    return cur.execute("select name, counter_value, duration from pmc_events").fetchall()
+
+
+def _short(name):
+    return name.replace("void ", "", 1).lstrip().split("(")[0]
+
+
+def _pmc_avg(rows, prefix):
     """Average counter value per dispatch (KiB) of the kernel instance whose
     name starts with `prefix` and that takes the most time, its name and its
     number of dispatches."""
     import collections
-    import sqlite3
 
-    cur = sqlite3.connect(db).cursor()
-    rows = cur.execute("select name, counter_value, duration from pmc_events").fetchall()
     by = collections.defaultdict(lambda: [0.0, 0.0, 0])
     for name, v, dur in rows:
-        short = name.replace("void ", "", 1).lstrip()
+        short = _short(name)
         if short.startswith(prefix):
-            t = by[short.split("(")[0]]
+            t = by[short]
             t[0] += v
             t[1] += dur
             t[2] += 1
@@ -124,12 +206,38 @@ def _pmc_avg(db, prefix):
     return tot / cnt, name, cnt
 
 
+def _pmc_sum(rows, prefixes):
+    """Counter total (KiB) over every dispatch whose name starts with one of
+    the prefixes."""
+    return sum(v for name, v, _ in rows if _short(name).startswith(prefixes))
+
+
+def _run_group(cmd, cwd, env, timeout):
+    """Runs cmd in its own process group and reaps the whole group after it
+    returns, so that no helper it started outlives it."""
+    p = subprocess.Popen(cmd, cwd=cwd, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True, start_new_session=True)
+    try:
+        out, err = p.communicate(timeout=timeout)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        out, err = p.communicate()
+    try:
+        os.killpg(p.pid, signal.SIGKILL)
+    except ProcessLookupError:
+        pass
+    return p.returncode, out, err
+
+
+MERGE_KERNELS = ("emv::k_merge", "emv::k_pack")
+
+
 def measure_traffic(args, kernel_prefix):
-    """HBM bytes per launch of the dominant kernel from two rocprofv3 counter
-    passes (FETCH_SIZE, then WRITE_SIZE) over one solve of the same workload,
-    each scaled by what the counter reads for a plain copy of known size
-    (emv-copyprobe, 1 GiB read + 1 GiB written per launch).  None when
-    rocprofv3 is missing or a pass fails."""
+    """HBM bytes of one solve from two rocprofv3 counter passes (FETCH_SIZE,
+    then WRITE_SIZE), each scaled by what the counter reads for a plain copy
+    of known size (emv-copyprobe, 1 GiB read + 1 GiB written per launch): per
+    launch of the dominant kernel, and per solve summed over the merge
+    kernels.  None when rocprofv3 is missing or a pass fails."""
     import shutil
 
     prof = shutil.which("rocprofv3")
@@ -137,38 +245,40 @@ def measure_traffic(args, kernel_prefix):
     if not prof or not os.path.exists(probe):
         return None
     copy_bytes = 1 << 30
-    out = {}
+    rows = {}
     with tempfile.TemporaryDirectory(dir="/tmp") as td:
         env = dict(os.environ, TMPDIR="/tmp")
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             runs = {
                 "copy": [probe, str(copy_bytes), "3"],
                 "solve": [sys.executable, os.path.abspath(__file__), "--legs", "plain", "--steps",
-                          "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--n", str(args.n),
-                          "--k", str(args.k)],
+                          "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--no-table32",
+                          "--n", str(args.n), "--k", str(args.k)],
             }
             for tag, cmd in runs.items():
                 name = f"{tag}_{ctr}"
                 log(f"traffic: rocprofv3 --pmc {ctr} ({tag})")
-                r = subprocess.run([prof, "--pmc", ctr, "-d", td, "-o", name, "--"] + cmd, cwd=td,
-                                   env=env, capture_output=True, text=True, timeout=600)
+                rc, _, _ = _run_group([prof, "--pmc", ctr, "-d", td, "-o", name, "--"] + cmd,
+                                      td, env, 600)
                 db = os.path.join(td, f"{name}_results.db")
-                if r.returncode != 0 or not os.path.exists(db):
-                    log(f"traffic: {name} failed (rc={r.returncode})")
+                if rc != 0 or not os.path.exists(db):
+                    log(f"traffic: {name} failed (rc={rc})")
                     return None
-                prefix = "k_copy16" if tag == "copy" else kernel_prefix
-                out[name] = _pmc_avg(db, prefix)
-    fetch_copy = out["copy_FETCH_SIZE"][0]
-    write_copy = out["copy_WRITE_SIZE"][0]
-    fetch, kname, nd = out["solve_FETCH_SIZE"]
-    write = out["solve_WRITE_SIZE"][0]
+                rows[name] = _pmc_rows(db)
+    fetch_copy = _pmc_avg(rows["copy_FETCH_SIZE"], "k_copy16")[0]
+    write_copy = _pmc_avg(rows["copy_WRITE_SIZE"], "k_copy16")[0]
+    fetch, kname, nd = _pmc_avg(rows["solve_FETCH_SIZE"], kernel_prefix)
+    write = _pmc_avg(rows["solve_WRITE_SIZE"], kernel_prefix)[0]
     if not (fetch_copy and write_copy and fetch is not None and write is not None):
         return None
     fs = copy_bytes / (fetch_copy * 1024.0)
     ws = copy_bytes / (write_copy * 1024.0)
+    mf = _pmc_sum(rows["solve_FETCH_SIZE"], MERGE_KERNELS) * 1024.0 * fs
+    mw = _pmc_sum(rows["solve_WRITE_SIZE"], MERGE_KERNELS) * 1024.0 * ws
     return dict(kernel=kname, bytes=fetch * 1024.0 * fs + write * 1024.0 * ws,
                 fetch_bytes=fetch * 1024.0 * fs, write_bytes=write * 1024.0 * ws,
-                fetch_scale=fs, write_scale=ws, dispatches=nd)
+                fetch_scale=fs, write_scale=ws, dispatches=nd,
+                merge_solve_bytes=mf + mw, merge_fetch_bytes=mf, merge_write_bytes=mw)
 
 
 def plain_leg(args):
@@ -197,6 +307,22 @@ def plain_leg(args):
             f"relax={sol.relax_ms:.1f} records={sol.unpack_ms:.1f} single={sol.singleton_ms:.1f} "
             f"rounds={sol.relax_rounds}")
     wall = time.perf_counter() - t0
+    t32 = None
+    if not args.no_table32:
+        # the same solve on an int32 table (the 16-bit cells are exact because
+        # the distance bound is below 0xFFFF; this is the int32 figure)
+        ctx.solve(kk, emv.TABLE32)
+        s32 = []
+        t1 = time.perf_counter()
+        for _ in range(2):
+            s32.append(ctx.solve(kk, emv.TABLE32))
+        w32 = (time.perf_counter() - t1) / len(s32)
+        if any(s.cost != sols[-1].cost for s in s32):
+            raise RuntimeError("int32 table changed the cost")
+        t32 = dict(table="int32", ms_per_step=w32 * 1e3, cells_per_s=s32[-1].dp_cells / w32,
+                   merge_ms=s32[-1].merge_ms, relax_ms=s32[-1].relax_ms,
+                   records_ms=s32[-1].unpack_ms, label_bits=s32[-1].label_bits, cost=s32[-1].cost)
+        log(f"table32: {w32 * 1e3:.1f} ms per solve")
     ctx.close()
     for s in sols:
         if cost is not None and s.cost != cost:
@@ -210,44 +336,45 @@ def plain_leg(args):
     # its algorithmic bytes are the per-subset model without the k singleton rows
     k = sols[-1].k
     relax_bytes = sols[-1].relax_alg_bytes * ((1 << k) - 1 - k) / ((1 << k) - 1)
-    kern = {
-        "merge": dict(bytes=sols[-1].merge_alg_bytes, ms=merge_ms,
-                      launches=sols[-1].merge_launches),
-        "relax": dict(bytes=relax_bytes, ms=relax_ms, launches=sols[-1].relax_rounds),
-    }
-    for v in kern.values():
-        v["achieved_GBs"] = v["bytes"] / (v["ms"] * 1e6) if v["ms"] > 0 else 0.0
-        v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
-        v["cells_per_s"] = cells / (v["ms"] / 1e3) if v["ms"] > 0 else 0.0
-        v["avg_launch_us"] = v["ms"] * 1e3 / max(v["launches"], 1)
-    # The merge reads its lower rows from L2/L1 many times per X (cross-X
-    # reuse), so the per-X streaming model puts its algorithmic bytes above
-    # the HBM peak: it is VALU/cache-bound, not HBM-bound.
+    relax = dict(bytes=relax_bytes, ms=relax_ms, launches=sols[-1].relax_rounds)
+    relax["achieved_GBs"] = relax_bytes / (relax_ms * 1e6) if relax_ms > 0 else 0.0
+    relax["frac"] = relax["achieved_GBs"] / HBM_PEAK_GBS
+    relax["cells_per_s"] = cells / (relax_ms / 1e3) if relax_ms > 0 else 0.0
+    relax["avg_launch_us"] = relax_ms * 1e3 / max(relax["launches"], 1)
+    # merge: unordered pair-cells; its per-X streaming model counts lower rows
+    # re-read from cache, so it is reported as a model ratio, not a roofline
+    # fraction; the merge is bounded by its VALU work (about two lane-ops per
+    # pair-cell with two 16-bit cells per lane) and by its measured traffic
     split_ms = sum(s.merge_split_ms for s in sols) / len(sols)
     pairs = float(cells) / (1 << k) * (3 ** k - 2 ** (k + 1) + 1) / 2.0
-    kern["merge"].update(
-        split_ms=split_ms, whole_ms=merge_ms - split_ms,
-        pair_cells_per_s=pairs / (merge_ms / 1e3) if merge_ms > 0 else 0.0,
-        note="VALU/cache-bound: lower-layer rows are re-read from L2/L1 across X; the per-X "
-             "streaming model exceeds the HBM peak")
-    # dominant kernel = the kernel function with the most device time: the
-    # relaxation round kernel (k_relax_v), or the split merge (k_merge_split)
-    # or the whole-row merge (k_merge_v) when one of them outweighs it
-    dom = "relax" if relax_ms >= max(split_ms, merge_ms - split_ms) else "merge"
-    roof = dict(bound="hbm", kernel=dom, achieved=round(kern[dom]["achieved_GBs"], 1),
-                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(kern[dom]["frac"], 4), traffic=None,
-                merge_relax_frac=round((kern["merge"]["bytes"] + kern["relax"]["bytes"]) /
-                                       ((merge_ms + relax_ms) * 1e6) / HBM_PEAK_GBS, 4))
+    valu_floor_ms = pairs * 2.0 / VALU_LANE_OPS * 1e3
+    merge = dict(ms=merge_ms, launches=sols[-1].merge_launches, split_ms=split_ms,
+                 whole_ms=merge_ms - split_ms, pair_cells=pairs,
+                 pair_cells_per_s=pairs / (merge_ms / 1e3) if merge_ms > 0 else 0.0,
+                 cells_per_s=cells / (merge_ms / 1e3) if merge_ms > 0 else 0.0,
+                 model_bytes=sols[-1].merge_alg_bytes,
+                 model_ratio=sols[-1].merge_alg_bytes / (merge_ms * 1e6) / HBM_PEAK_GBS,
+                 valu_floor_ms=valu_floor_ms, valu_floor_frac=valu_floor_ms / merge_ms,
+                 note="per-X streaming model over the HBM peak: lower rows are re-read from "
+                      "L2/L1 across X, so model_ratio is no roofline fraction; measured_* is HBM")
+    roof = dict(bound="hbm", kernel="relax", achieved=round(relax["achieved_GBs"], 1),
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=round(relax["frac"], 4), traffic=None)
     if not args.no_traffic:
-        tr = measure_traffic(args, "emv::k_relax" if dom == "relax" else "emv::k_merge")
-        if tr and dom == "merge":
-            tr = None  # the merge runs as several kernels per layer; no single per-launch figure
+        tr = measure_traffic(args, "emv::k_relax")
         if tr:
-            launch_s = kern[dom]["ms"] / 1e3 / max(kern[dom]["launches"], 1)
+            launch_s = relax_ms / 1e3 / max(relax["launches"], 1)
             roof["traffic"] = tr["bytes"]
             roof["traffic_detail"] = tr
             roof["traffic_GBs"] = round(tr["bytes"] / launch_s / 1e9, 1)
             roof["traffic_frac"] = round(tr["bytes"] / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+            merge["measured_bytes"] = tr["merge_solve_bytes"]
+            merge["measured_GBs"] = tr["merge_solve_bytes"] / (merge_ms * 1e6)
+            merge["measured_frac"] = merge["measured_GBs"] / HBM_PEAK_GBS
+    roof["merge"] = {key: merge[key] for key in ("ms", "valu_floor_ms", "valu_floor_frac")
+                     if key in merge}
+    for key in ("measured_GBs", "measured_frac"):
+        if key in merge:
+            roof["merge"][key] = round(merge[key], 4)
     return dict(
         value=cells * args.steps / wall,
         ms_per_step=wall * 1e3 / args.steps,
@@ -261,7 +388,8 @@ def plain_leg(args):
         cost=cost,
         kernel_ms=sum(s.kernel_ms for s in sols) / len(sols),
         singleton_ms=sum(s.singleton_ms for s in sols) / len(sols),
-        merge_ms=merge_ms, relax_ms=relax_ms, unpack_ms=unpack_ms, kernels=kern, roofline=roof)
+        merge_ms=merge_ms, relax_ms=relax_ms, unpack_ms=unpack_ms,
+        kernels=dict(merge=merge, relax=relax), roofline=roof, table32=t32)
 
 
 def batch_leg(args):
@@ -288,9 +416,23 @@ def batch_leg(args):
     ctx.close()
     plain_costs = np.array(plain_costs)
     log(f"batch leg: plain {len(insts)} solves {t_solve:.3f}s (+{t_load:.3f}s graph loads)")
-    b = emv.Batch(insts)
     t0 = time.perf_counter()
-    params, table = autotune.tune_batch(b, insts, reps=1, log=log)
+    b = emv.Batch(insts)
+    create_s = time.perf_counter() - t0
+    # the first solve of a fresh batch: eager steps (default shape), rounds recorded
+    t0 = time.perf_counter()
+    costs, st = b.solve()
+    first_s = time.perf_counter() - t0
+    if not np.array_equal(costs, plain_costs):
+        raise RuntimeError("batched costs differ from the plain per-instance costs (first solve)")
+    log(f"batch leg: first (eager) solve {first_s:.3f}s")
+    t0 = time.perf_counter()
+    if args.batch_no_graph:
+        # eager steps only (no hipGraph capture or replay), default shape
+        shape, table = dict(autotune.DEFAULTS), []
+        b.set_params(use_graph=0, **shape)
+    else:
+        shape, table = autotune.tune_batch(b, insts, reps=1, log=log)
     tune_s = time.perf_counter() - t0
     times, used = [], []
     for i in range(args.batch_steps):
@@ -302,6 +444,13 @@ def batch_leg(args):
             raise RuntimeError("batched costs differ from the plain per-instance costs")
         log(f"batch leg: solve {i} {times[-1]:.3f}s graph={st.graph_used} "
             f"fallback={st.graph_fallback} rounds={st.relax_rounds}")
+    # the tuned shape without the replay (eager steps)
+    b.set_params(use_graph=0, **shape)
+    t0 = time.perf_counter()
+    costs, _ = b.solve()
+    eager_s = time.perf_counter() - t0
+    if not np.array_equal(costs, plain_costs):
+        raise RuntimeError("batched costs differ from the plain per-instance costs (eager)")
     b.close()
     bt = sum(times) / len(times)
     return dict(
@@ -312,14 +461,27 @@ def batch_leg(args):
         plain_instances_per_s=len(insts) / t_solve, plain_cells_per_s=cells / t_solve,
         batched_s=bt, batched_instances_per_s=len(insts) / bt, batched_cells_per_s=cells / bt,
         speedup=t_solve / bt, graph_replay=used,
-        autotune=dict(relax_blocks_per_cu=params[0], merge_rows_per_wave=params[1],
-                      seconds=round(tune_s, 2), table=table))
+        first_solve_s=first_s, first_solve_speedup=t_solve / first_s,
+        eager_tuned_s=eager_s, eager_tuned_speedup=t_solve / eager_s, create_s=create_s,
+        autotune=dict(shape=shape, seconds=round(tune_s, 2), evaluated=len(table), table=table))
+
+
+def _group_queries(n, count, seed):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    return [[rng.choice(n, size=int(rng.integers(4, 17)), replace=False)
+             for _ in range(int(rng.integers(3, 9)))] for _ in range(count)]
 
 
 def group_leg(args):
-    """Group Steiner through the reduction: 1024 x 1024 grid, 13 groups of 32."""
+    """Group Steiner through the reduction: 1024 x 1024 grid, 13 groups of 32;
+    then the same query on a Server holding the grid."""
     import emv
     import workloads
+
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from bigcheck import validate_group_tree
 
     R = C = 1024
     g, size = 13, 32
@@ -339,7 +501,26 @@ def group_leg(args):
         log(f"group leg: solve {i} {times[-1]:.3f}s kernel={sol.kernel_ms:.1f} ms "
             f"merge={sol.merge_ms:.1f} relax={sol.relax_ms:.1f} records={sol.unpack_ms:.1f} "
             f"relative_label_layers={sol.relative_label_layers} cost={sol.cost}")
+    lsol = ctx.solve(kk, emv.LIST)
     ctx.close()
+    ok, why, gcost, _ = validate_group_tree(n, e, grp, nn, red, M, lsol.pairs, lsol.cost)
+    if not ok or lsol.cost != sol.cost:
+        raise RuntimeError(f"group leg: listed tree invalid: {why}")
+    log(f"group leg: tree validated, group cost {gcost}")
+    # the same query on a Server holding the grid (the DP on the grid itself)
+    srv = emv.Server(n, e, max_inflight=4)
+    served = []
+    for i in range(1 + args.group_steps):
+        t0 = time.perf_counter()
+        t = srv.submit_groups([list(x) for x in grp])
+        while srv.poll(t).state != 2:
+            srv.step()
+        served.append(time.perf_counter() - t0)
+        info = srv.poll(t)
+        if info.status != 0 or info.reduced_cost != sol.cost:
+            raise RuntimeError(f"group leg: served cost {info.reduced_cost} != {sol.cost}")
+    srv.close()
+    log(f"group leg: served on the grid {min(served[1:]):.3f}s")
     return dict(
         workload=f"group Steiner via reduction: {R}x{C} grid, 4-neighbour, w~U[1,1000], "
                  f"{g} groups of {size} distinct vertices, seed 5; virtual terminals joined "
@@ -350,7 +531,11 @@ def group_leg(args):
         records_ms=sol.unpack_ms, relax_rounds=sol.relax_rounds, table_bytes=sol.table_bytes,
         label_bits=sol.label_bits, relative_label_layers=sol.relative_label_layers,
         compact_merge_layers=sol.compact_merge_layers,
-        relax_blocks=sol.relax_blocks, cost=sol.cost, group_cost=sol.cost - g * M)
+        relax_blocks=sol.relax_blocks, cost=sol.cost, group_cost=sol.cost - g * M,
+        validated=True, tree_pairs=int(len(lsol.pairs)),
+        served_on_grid_s=sum(served[1:]) / len(served[1:]),
+        served_note="Server.submit_groups on the resident grid: the DP runs on the grid with "
+                    "singleton rows = distance to the nearest member (int32 cells), same cost")
 
 
 def _percentile(xs, p):
@@ -362,7 +547,8 @@ def _percentile(xs, p):
 def _serve_stream(srv, qs, arr):
     """Submits query i at arrival time arr[i] (s, from now), steps the server
     while anything is in flight; returns latencies (arrival -> completion
-    seen after a step), costs and the completion time of the last query."""
+    seen after a step), reduced costs and the completion time of the last
+    query.  A query is a terminal list or ("groups", member lists)."""
     Q = len(qs)
     lat, costs = [None] * Q, [None] * Q
     pending = {}
@@ -372,7 +558,9 @@ def _serve_stream(srv, qs, arr):
     while i < Q or pending:
         now = time.perf_counter() - t0
         while i < Q and arr[i] <= now:
-            pending[srv.submit(qs[i])] = i
+            q = qs[i]
+            tk = srv.submit_groups(q[1]) if isinstance(q, tuple) else srv.submit(q)
+            pending[tk] = i
             i += 1
         if not pending:
             time.sleep(max(0.0, arr[i] - (time.perf_counter() - t0)))
@@ -386,7 +574,7 @@ def _serve_stream(srv, qs, arr):
                 if info.status != 0:
                     raise RuntimeError(f"served query {j} failed with status {info.status}")
                 lat[j] = tnow - arr[j]
-                costs[j] = info.cost
+                costs[j] = info.reduced_cost
                 last = tnow
     return lat, costs, last
 
@@ -401,27 +589,41 @@ def _fifo_latencies(arr, service):
 
 
 def serve_leg(args):
-    import numpy as np
-
     import emv
     import workloads
 
     R = C = 512
     n = R * C
     e = emv.grid_graph(R, C, 100, 4)
-    qs = workloads.grid_queries(n, args.queries, 40)
+    qs = list(workloads.grid_queries(n, args.queries, 40))
+    # group queries join the stream, spread over it
+    gq = _group_queries(n, args.group_queries, 42)
+    for j, g in enumerate(gq):
+        qs.insert((j + 1) * len(qs) // (len(gq) + 1), ("groups", g))
     ctx = emv.Context(0)
     ctx.load_edges(n, e)
     ctx.solve(qs[0])  # warm
-    service, plain_costs = [], []
+    service, plain_costs, group_loads = [], [], 0.0
     for q in qs:
-        t0 = time.perf_counter()
-        plain_costs.append(ctx.solve(q).cost)
-        service.append(time.perf_counter() - t0)
+        if isinstance(q, tuple):
+            # plain path: the reference's reduction, then a solve on the reduced graph
+            nn, red, kk, _ = emv.group_reduce(n, e, q[1])
+            t0 = time.perf_counter()
+            ctx.load_edges(nn, red)
+            t1 = time.perf_counter()
+            plain_costs.append(ctx.solve(kk).cost)
+            service.append(time.perf_counter() - t1)
+            group_loads += t1 - t0
+            ctx.load_edges(n, e)
+        else:
+            t0 = time.perf_counter()
+            plain_costs.append(ctx.solve(q).cost)
+            service.append(time.perf_counter() - t0)
     ctx.close()
     total = sum(service)
     log(f"serve leg: plain serial {len(qs)} queries {total:.2f}s "
-        f"(p50 {_percentile(service, 50) * 1e3:.0f} ms)")
+        f"(p50 {_percentile(service, 50) * 1e3:.0f} ms; group graph loads {group_loads:.2f}s "
+        "not counted)")
     srv = emv.Server(n, e, max_inflight=64)
     _serve_stream(srv, qs[:1], [0.0])  # warm
     lat, costs, last = _serve_stream(srv, qs, [0.0] * len(qs))
@@ -444,18 +646,22 @@ def serve_leg(args):
                           p99_ms=_percentile(lat, 99) * 1e3,
                           plain_solves_per_s=len(qs) / pend,
                           plain_p50_ms=_percentile(plat, 50) * 1e3,
-                          plain_p99_ms=_percentile(plat, 99) * 1e3))
-        log(f"serve leg: rate {lam:.2f}/s served p50 {sweep[-1]['p50_ms']:.0f} ms "
-            f"p99 {sweep[-1]['p99_ms']:.0f} ms; plain p50 {sweep[-1]['plain_p50_ms']:.0f} ms "
-            f"p99 {sweep[-1]['plain_p99_ms']:.0f} ms")
+                          plain_p99_ms=_percentile(plat, 99) * 1e3,
+                          throughput_vs_plain=(len(qs) / last) / (len(qs) / pend)))
+        log(f"serve leg: rate {lam:.2f}/s ({f}x) served {len(qs) / last:.1f}/s p50 "
+            f"{sweep[-1]['p50_ms']:.0f} ms p99 {sweep[-1]['p99_ms']:.0f} ms; plain "
+            f"{len(qs) / pend:.1f}/s p50 {sweep[-1]['plain_p50_ms']:.0f} ms p99 "
+            f"{sweep[-1]['plain_p99_ms']:.0f} ms")
     srv.close()
     return dict(
-        workload=f"512x512 grid, 4-neighbour, w~U[1,100], seed 4; {len(qs)} terminal-set "
-                 "queries k~U{4..12}, seed 40",
-        queries=len(qs), costs_equal=True, plain_serial_s=total,
+        workload=f"512x512 grid, 4-neighbour, w~U[1,100], seed 4; {args.queries} terminal-set "
+                 f"queries k~U{{4..12}}, seed 40, and {len(gq)} group queries (3-8 groups of "
+                 "4-16 vertices, seed 42); plain path = one solve per query (group queries "
+                 "through the reduction, graph loads not counted)",
+        queries=len(qs), group_queries=len(gq), costs_equal=True, plain_serial_s=total,
         plain_service_p50_ms=_percentile(service, 50) * 1e3,
         plain_service_p99_ms=_percentile(service, 99) * 1e3,
-        burst=burst, poisson=sweep,
+        burst=burst, poisson=sweep, target_speedup=5.0,
         latency="arrival to completion, p99 over the stream's queries")
 
 
@@ -465,6 +671,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-config2", action="store_true",
+                    help="also time the reference on the full n=2^18 k=14 instance (minutes)")
     ap.add_argument("--n", type=int, default=WORKLOAD["n"])
     ap.add_argument("--k", type=int, default=WORKLOAD["k"])
     ap.add_argument("--legs", default="plain,batch,serve,group")
@@ -472,9 +680,13 @@ def main():
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--queries", type=int, default=24)
-    ap.add_argument("--rates", default="0.5,1,2")
+    ap.add_argument("--group-queries", type=int, default=6)
+    ap.add_argument("--rates", default="0.5,1,2,4")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes behind roofline.traffic")
+    ap.add_argument("--no-table32", action="store_true")
+    ap.add_argument("--batch-no-graph", action="store_true",
+                    help="batch leg with eager steps only (no hipGraph capture), for profiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the "
@@ -502,7 +714,7 @@ def main():
         out["group"] = group_leg(args)
     if not args.no_cpu and "plain" in legs:
         log("cpu baseline (reference reader-el)...")
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(args.cpu_config2)
     print(json.dumps(out), flush=True)
 
 

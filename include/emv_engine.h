@@ -49,6 +49,10 @@ typedef struct {
     int32_t use_graph;            /* batch: replay steps 2..kmax from a hipGraph */
     int32_t graph_round_margin;   /* rounds added to each replayed step's recorded count
                                      (negative values force short replays, for tests) */
+    int32_t dense_rounds;         /* vertex-wise relaxation: rounds 1..R of a layer step take
+                                     every item and rounds 0..R-1 write no flags; 0 = default 6 */
+    int32_t split_items;          /* split merge of the upper layers: items per merge tile aimed
+                                     at (parts per row = items / rows); 0 = default 32 */
 } emv_engine_params;
 
 typedef struct {

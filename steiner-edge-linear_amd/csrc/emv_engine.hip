@@ -519,6 +519,11 @@ int inst_group(const emv_engine *e, const Inst &I) {
 
 bool group_blocks(int gsel) { return (gsel & 2) != 0; }
 
+// Split merge: items per tile aimed at (the params, else EMV_SPLIT_ITEMS / 32).
+int split_items(const emv_engine *e) {
+    return e->prm.split_items > 0 ? e->prm.split_items : e->split_items;
+}
+
 void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P) {
     P = StepPlan();
     int64_t dcur = 0, fcur = 0;
@@ -606,7 +611,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 // atomics), each part >= 4 outer steps
                 const int64_t steps = (I.stage - 1 <= 3) ? 1 : (1ll << (I.stage - 1 - 3));
                 M.S = (int32_t)std::max<int64_t>(
-                    1, std::min<int64_t>((e->split_items + nrows - 1) / nrows, steps / 4));
+                    1, std::min<int64_t>((split_items(e) + nrows - 1) / nrows, steps / 4));
                 M.mbase = P.macc_cells;
                 P.macc_cells += (int64_t)nrows * G.np;
             }
@@ -623,7 +628,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 fcur += nflags;
             }
             // a layer step starts near-dense (a singleton-only step does not)
-            P.dense_rounds = layer_dense_rounds();
+            P.dense_rounds = e->prm.dense_rounds > 0 ? e->prm.dense_rounds : layer_dense_rounds();
             if (splits)
                 split.push_back(M);
             else
@@ -803,6 +808,57 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         put(SG.off_pp, SG.pack_pre.data(), sizeof(int64_t) * SG.pack_pre.size());
         put(SG.off_bp, SG.bin_pre.data(), sizeof(int64_t) * SG.bin_pre.size());
     }
+}
+
+// Host-side bounds of a step plan against the buffers its launches will
+// touch: every staging chunk, bin and lane inside the staging, every flag
+// slice inside the flag arrays, every table inside the arena, accumulators
+// inside theirs, the descriptor blob the size its offsets assume.  Checked
+// before anything is enqueued, so a plan that disagrees with its buffers is
+// an error status, never an out-of-bounds access on the device.
+int check_plan(emv_engine *e, const StepPlan &P) {
+    auto fail = [&](const char *what, long long a, long long b) {
+        eset(e, "step plan out of bounds: %s (%lld > %lld)", what, a, b);
+        return EMV_ERR_STATE;
+    };
+    if (P.blob.size() != P.bytes) return fail("descriptor blob", (long long)P.blob.size(), (long long)P.bytes);
+    if (P.cells > e->stage_cells) return fail("staging cells", P.cells, e->stage_cells);
+    if (P.flagbytes > e->flag_bytes) return fail("flag words", P.flagbytes, e->flag_bytes);
+    if (P.macc_cells * (P.pbits / 8) > e->macc_bytes)
+        return fail("merge accumulators", P.macc_cells * (P.pbits / 8), e->macc_bytes);
+    for (const GChunk &c : P.ch) {
+        if (c.dbase < 0 || c.dbase + (int64_t)c.np * kWave > e->stage_cells)
+            return fail("chunk staging", c.dbase + (int64_t)c.np * kWave, e->stage_cells);
+        const int64_t tcells = 2 * (1ll << c.k) * (int64_t)c.np;
+        if (c.tbase < 0 || c.tbase + tcells > e->arena.cap)
+            return fail("instance table", c.tbase + tcells, e->arena.cap);
+        if (c.nx < 1 || c.nx > kWave || c.n > c.np) return fail("chunk rows", c.nx, kWave);
+    }
+    for (const GChunk &c : P.rch) {
+        const int64_t nf = c.nblk > 0 ? c.nblk : c.np;
+        if (c.fbase < 0 || c.fbase + nf > e->flag_bytes)
+            return fail("relaxation flags", c.fbase + nf, e->flag_bytes);
+        if (c.dbase < 0 || c.dbase + (int64_t)c.np * kWave > e->stage_cells)
+            return fail("relaxation staging", c.dbase + (int64_t)c.np * kWave, e->stage_cells);
+    }
+    for (const GBin &B : P.bins) {
+        if (B.dbase < 0 || B.dbase + (int64_t)B.np * kWave > e->stage_cells)
+            return fail("bin staging", B.dbase + (int64_t)B.np * kWave, e->stage_cells);
+        if (B.map0 < 0 || (size_t)B.map0 + kWave > P.lanes.size())
+            return fail("bin lanes", (long long)B.map0 + kWave, (long long)P.lanes.size());
+    }
+    for (const GLane &L : P.lanes)
+        if (L.dbase >= 0 && (L.lane < 0 || L.lane >= kWave || L.dbase >= e->stage_cells))
+            return fail("lane source", L.dbase, e->stage_cells);
+    for (const GMerge &M : P.mg) {
+        if (M.S > 0 && M.mbase + (int64_t)M.nrows * M.np > P.macc_cells)
+            return fail("split rows", M.mbase + (int64_t)M.nrows * M.np, P.macc_cells);
+        if (M.x0 < 0 || (size_t)M.x0 + M.nrows > P.xs.size())
+            return fail("merge rows", (long long)M.x0 + M.nrows, (long long)P.xs.size());
+    }
+    for (int64_t c : P.member_cells)
+        if (c < 0 || c >= e->stage_cells) return fail("member seed", c, e->stage_cells);
+    return EMV_OK;
 }
 
 int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
@@ -1292,6 +1348,7 @@ int step_eager(emv_engine *e, int *rounds_used) {
     build_plan(e, run, P);
     ECHK(ensure_stage(e, P.cells, P.flagbytes));
     ECHK(ensure_macc(e, P.macc_cells * (P.pbits / 8)));
+    ECHK(check_plan(e, P));
     char *db = nullptr;
     ECHK(desc_buffer(e, 0, P.bytes, &db));
     EHIP(e, hipMemcpyAsync(db, P.blob.data(), P.bytes, hipMemcpyHostToDevice, e->stream));
@@ -1334,6 +1391,8 @@ int engine_open(int device, emv_engine **out) {
     e->prm.merge_rows_per_wave = 0;
     e->prm.use_graph = 1;
     e->prm.graph_round_margin = 2;
+    e->prm.dense_rounds = 0;
+    e->prm.split_items = 0;
     *out = e;
     if (hipSetDevice(device) != hipSuccess) return EMV_ERR_HIP;
     hipDeviceProp_t prop;
@@ -1363,6 +1422,8 @@ extern "C" int emv_engine_default_params(emv_engine_params *p) {
     p->merge_rows_per_wave = 0;
     p->use_graph = 1;
     p->graph_round_margin = 2;
+    p->dense_rounds = 0;
+    p->split_items = 0;
     return EMV_OK;
 }
 
@@ -1372,7 +1433,9 @@ extern "C" int emv_engine_set_params(emv_engine *e, const emv_engine_params *p) 
         (p->merge_rows_per_wave != 0 && p->merge_rows_per_wave != 1 &&
          p->merge_rows_per_wave != 2 && p->merge_rows_per_wave != 4 &&
          p->merge_rows_per_wave != 8) ||
-        p->graph_round_margin < -1000 || p->graph_round_margin > 1000) {
+        p->graph_round_margin < -1000 || p->graph_round_margin > 1000 ||
+        p->dense_rounds < 0 || p->dense_rounds > 64 || p->split_items < 0 ||
+        p->split_items > 4096) {
         eset(e, "engine parameters out of range");
         return EMV_ERR_ARG;
     }
@@ -1571,7 +1634,8 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
     // step and of the engine's settings: a repeated solve reuses them (and
     // the descriptors already on the device)
     std::vector<int64_t> sig = {kmax, e->prm.relax_blocks_per_cu, e->prm.merge_rows_per_wave,
-                                e->split_items, e->no_split ? 1 : 0, e->no_pack ? 1 : 0};
+                                split_items(e), e->prm.dense_rounds, e->no_split ? 1 : 0,
+                                e->no_pack ? 1 : 0};
     for (const Inst &I : e->inst)
         sig.insert(sig.end(), {(int64_t)I.state, (int64_t)I.stage, (int64_t)I.k,
                                (int64_t)I.graph, (int64_t)I.dbits, (int64_t)I.pbits,
@@ -1596,6 +1660,7 @@ static int batch_solve_graph(emv_engine *e, uint32_t flags, bool *ok) {
         for (size_t t = 0; t < e->inst.size(); t++) e->inst[t].stage = saved_stage[t];
         ECHK(ensure_stage(e, max_cells, max_flags));
         ECHK(ensure_macc(e, max_macc));
+        for (const StepPlan &P : plans) ECHK(check_plan(e, P));
         for (size_t s = 0; s < plans.size(); s++) {
             ECHK(desc_buffer(e, 1 + s, plans[s].bytes, &dbs[s]));
             EHIP(e, hipMemcpyAsync(dbs[s], plans[s].blob.data(), plans[s].bytes,

@@ -331,7 +331,8 @@ class EmvInstance(ctypes.Structure):
 
 class EngineParams(ctypes.Structure):
     _fields_ = [("relax_blocks_per_cu", ctypes.c_int32), ("merge_rows_per_wave", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("graph_round_margin", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("graph_round_margin", ctypes.c_int32),
+                ("dense_rounds", ctypes.c_int32), ("split_items", ctypes.c_int32)]
 
 
 class BatchStats(ctypes.Structure):
@@ -413,8 +414,9 @@ class _Engine:
         self.close()
 
     def set_params(self, relax_blocks_per_cu=64, merge_rows_per_wave=0, use_graph=1,
-                   graph_round_margin=2):
-        p = EngineParams(relax_blocks_per_cu, merge_rows_per_wave, use_graph, graph_round_margin)
+                   graph_round_margin=2, dense_rounds=0, split_items=0):
+        p = EngineParams(relax_blocks_per_cu, merge_rows_per_wave, use_graph, graph_round_margin,
+                         dense_rounds, split_items)
         self._check(_bind_engine().emv_engine_set_params(self._h, ctypes.byref(p)))
 
     def read_rows(self, ticket, X0, nrows, n, codes=True):

@@ -69,7 +69,8 @@ int main(void){ printf("%zu %zu %zu\n", sizeof(emv_result), offsetof(emv_result,
     ("emv_certificate", "emv_hip.h", ["cells", "bad_record", "first_v", "ms"]),
     ("emv_request_info", "emv_engine.h", ["cost", "npairs", "reduced_cost", "groups", "root"]),
     ("emv_batch_stats", "emv_engine.h", ["relax_rounds", "graph_fallback"]),
-    ("emv_engine_params", "emv_engine.h", ["merge_rows_per_wave", "graph_round_margin"]),
+    ("emv_engine_params", "emv_engine.h", ["merge_rows_per_wave", "graph_round_margin",
+                                           "dense_rounds", "split_items"]),
 ])
 def test_struct_layouts_match_headers(struct, header, fields):
     """Every ctypes mirror agrees with its C struct in size and field offsets."""
