@@ -653,6 +653,7 @@ def serve_leg(args):
             f"{len(qs) / pend:.1f}/s p50 {sweep[-1]['plain_p50_ms']:.0f} ms p99 "
             f"{sweep[-1]['plain_p99_ms']:.0f} ms")
     srv.close()
+    small = _serve_small(args, n, e) if args.small_queries > 0 else None
     return dict(
         workload=f"512x512 grid, 4-neighbour, w~U[1,100], seed 4; {args.queries} terminal-set "
                  f"queries k~U{{4..12}}, seed 40, and {len(gq)} group queries (3-8 groups of "
@@ -661,8 +662,53 @@ def serve_leg(args):
         queries=len(qs), group_queries=len(gq), costs_equal=True, plain_serial_s=total,
         plain_service_p50_ms=_percentile(service, 50) * 1e3,
         plain_service_p99_ms=_percentile(service, 99) * 1e3,
-        burst=burst, poisson=sweep, target_speedup=5.0,
+        burst=burst, poisson=sweep, target_speedup=5.0, small_queries=small,
         latency="arrival to completion, p99 over the stream's queries")
+
+
+def _serve_small(args, n, e):
+    """The small-query regime on the same grid: terminal sets with k ~
+    U{4..8} (seed 43), where one query's launches leave most of the GPU idle.
+    Burst and a Poisson stream at 4x the plain serial capacity."""
+    import emv
+    import workloads
+
+    qs = list(workloads.grid_queries(n, args.small_queries, 43, kmin=4, kmax=8))
+    ctx = emv.Context(0)
+    ctx.load_edges(n, e)
+    ctx.solve(qs[0])  # warm
+    service, plain_costs = [], []
+    for q in qs:
+        t0 = time.perf_counter()
+        plain_costs.append(ctx.solve(q).cost)
+        service.append(time.perf_counter() - t0)
+    ctx.close()
+    total = sum(service)
+    srv = emv.Server(n, e, max_inflight=128)
+    _serve_stream(srv, qs[:1], [0.0])  # warm
+    lat, costs, last = _serve_stream(srv, qs, [0.0] * len(qs))
+    if costs != plain_costs:
+        raise RuntimeError("served costs differ from the plain costs (small queries, burst)")
+    out = dict(workload=f"{len(qs)} terminal-set queries k~U{{4..8}}, seed 43, same grid",
+               queries=len(qs), plain_serial_s=total, plain_solves_per_s=len(qs) / total,
+               burst=dict(solves_per_s=len(qs) / last, speedup=total / last,
+                          p50_ms=_percentile(lat, 50) * 1e3, p99_ms=_percentile(lat, 99) * 1e3))
+    lam = 4.0 * len(qs) / total
+    arr = list(workloads.poisson_arrivals(len(qs), lam, 49))
+    lat, costs, last = _serve_stream(srv, qs, arr)
+    if costs != plain_costs:
+        raise RuntimeError("served costs differ from the plain costs (small queries, stream)")
+    plat, pend = _fifo_latencies(arr, service)
+    out["poisson_4x"] = dict(rate_per_s=lam, solves_per_s=len(qs) / last,
+                             p50_ms=_percentile(lat, 50) * 1e3, p99_ms=_percentile(lat, 99) * 1e3,
+                             plain_solves_per_s=len(qs) / pend,
+                             plain_p50_ms=_percentile(plat, 50) * 1e3,
+                             plain_p99_ms=_percentile(plat, 99) * 1e3,
+                             throughput_vs_plain=(len(qs) / last) / (len(qs) / pend))
+    srv.close()
+    log(f"serve leg: small queries burst {out['burst']['speedup']:.2f}x plain, stream at 4x "
+        f"{out['poisson_4x']['throughput_vs_plain']:.2f}x, p99 {out['poisson_4x']['p99_ms']:.0f} ms")
+    return out
 
 
 def main():
@@ -681,6 +727,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--queries", type=int, default=24)
     ap.add_argument("--group-queries", type=int, default=6)
+    ap.add_argument("--small-queries", type=int, default=64)
     ap.add_argument("--rates", default="0.5,1,2,4")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes behind roofline.traffic")

@@ -48,6 +48,8 @@ extern "C" {
 #define EMV_TABLE32     0x20u /* int32 table even when the bound allows 16-bit cells */
 #define EMV_MERGE_1V    0x40u /* merges of 16-bit cells one vertex per lane, as wider cells */
 #define EMV_MERGE_ROWS  0x80u /* merges of 16-bit cells row by row, not in stars of rows sharing a base set */
+#define EMV_MERGE_CUBE  0x100u /* layers 3..8 of 16-bit cells merged from rows staged in LDS (cubes);
+                                  measured slower than the stars on MI355X, so not the default */
 
 typedef struct emv_ctx emv_ctx;
 

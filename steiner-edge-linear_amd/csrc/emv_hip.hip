@@ -79,6 +79,16 @@ struct emv_ctx {
     bool merge_h2 = false;  // this solve merges 16-bit cells two vertices per lane
     int dense_rounds = 0;   // vertex-wise rounds of a relaxation that take every item
     bool merge_star = false;  // ... in stars of rows sharing a base set
+    bool merge_cube = false;  // ... the small-subset layers from rows staged in LDS (cubes)
+    bool cube_attr = false;   // dynamic-LDS attribute of the cube kernel set
+    std::vector<MergeCube> h_cubes;  // cubes of the last batch (kept for the async copy)
+    std::vector<CubeRow> h_cube_rows;
+    std::vector<uint32_t> h_cube_tab;  // staged-row offsets per cube (k_merge_cube_h2 rowtab)
+    uint32_t *d_cube_tab = nullptr;
+    size_t cube_tab_cap = 0;
+    MergeCube *d_cubes = nullptr;
+    CubeRow *d_cube_rows = nullptr;
+    size_t cubes_cap = 0, cube_rows_cap = 0;
     double star_min_cells = 1073741824.0;  // from layer batches of this many pair cells on
     const uint32_t *h_xs = nullptr;         // host copy of the batch's rows (star building)
     std::vector<MergeStar> h_stars;         // stars of the last batch (kept for the async copy)
@@ -242,6 +252,17 @@ int64_t row_stride(int64_t n) {
     return np;
 }
 
+// The bits of x deposited into the set bits of mask, lowest first.
+uint32_t deposit_host(uint32_t x, uint32_t mask) {
+    uint32_t r = 0;
+    for (uint32_t bb = 1; mask != 0; bb <<= 1) {
+        const uint32_t low = mask & (~mask + 1u);
+        if (x & bb) r |= low;
+        mask ^= low;
+    }
+    return r;
+}
+
 int bits_for(uint64_t x) {  // number of bits to hold values 0..x
     int b = 0;
     while (b < 64 && (x >> b) != 0) b++;
@@ -326,6 +347,10 @@ void free_ws(emv_ctx *c) {
     c->rspan_cap = 0;
     dfree(c->d_stars);
     c->stars_cap = 0;
+    dfree(c->d_cubes);
+    dfree(c->d_cube_rows);
+    dfree(c->d_cube_tab);
+    c->cubes_cap = c->cube_rows_cap = c->cube_tab_cap = 0;
     dfree(c->d_q[0]);
     dfree(c->d_q[1]);
     c->qcap = 0;
@@ -732,9 +757,193 @@ int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
     return EMV_OK;
 }
 
+// Cubes covering the batch's rows xs[0..nx) (all of m elements): each
+// uncovered row, in order, goes with the u-element superset U (u = min(k,
+// kCubeU)) that holds the most uncovered rows of the batch, and every
+// uncovered row of U is assigned to it.  parts splits a row's pairs so that
+// a cube has about two tasks per wave.
+void build_cubes(const uint32_t *xs, int nx, int k, int m, std::vector<MergeCube> &cubes,
+                 std::vector<CubeRow> &rows) {
+    cubes.clear();
+    rows.clear();
+    const int u = std::min(k, kCubeU);
+    std::vector<int32_t> idx((size_t)1 << k, -1);  // batch index of an uncovered row
+    for (int i = 0; i < nx; i++) idx[xs[i]] = i;
+    // the m-subsets of a u-set, as local masks (Gosper's order)
+    std::vector<uint32_t> loc;
+    for (uint32_t L = (1u << m) - 1u; L < (1u << u);) {
+        loc.push_back(L);
+        const uint32_t lo = L & (~L + 1u), r = L + lo;
+        L = (((r ^ L) >> 2) / lo) | r;
+    }
+    for (int i = 0; i < nx; i++) {
+        const uint32_t X = xs[i];
+        if (idx[X] < 0) continue;
+        // supersets U = X + (u - m) of the other elements
+        std::vector<int> rest;
+        for (int b = 0; b < k; b++)
+            if (!(X >> b & 1u)) rest.push_back(b);
+        const int need = u - m;
+        uint32_t bestU = X;
+        int best = -1;
+        std::vector<int> pick(need);
+        for (int q = 0; q < need; q++) pick[q] = q;
+        for (;;) {
+            uint32_t U = X;
+            for (int q = 0; q < need; q++) U |= 1u << rest[pick[q]];
+            int cnt = 0;
+            for (uint32_t L : loc) {
+                const uint32_t G = deposit_host(L, U);
+                if (idx[G] >= 0) cnt++;
+            }
+            if (cnt > best) {
+                best = cnt;
+                bestU = U;
+            }
+            int q = need - 1;
+            while (q >= 0 && pick[q] == (int)rest.size() - need + q) q--;
+            if (q < 0) break;
+            pick[q]++;
+            for (int r2 = q + 1; r2 < need; r2++) pick[r2] = pick[r2 - 1] + 1;
+        }
+        MergeCube cb;
+        memset(&cb, 0, sizeof(cb));
+        cb.U = bestU;
+        cb.e0 = (int32_t)rows.size();
+        for (uint32_t L : loc) {
+            const uint32_t G = deposit_host(L, bestU);
+            if (idx[G] >= 0) {
+                rows.push_back(CubeRow{L, idx[G]});
+                idx[G] = -1;
+            }
+        }
+        cb.nx = (int32_t)rows.size() - cb.e0;
+        const int64_t npairs = ((int64_t)1 << (m - 1)) - 1;
+        const int64_t want = (2 * kCubeWaves + cb.nx - 1) / cb.nx;
+        cb.parts = (int32_t)std::max<int64_t>(1, std::min<int64_t>(want, npairs / 8));
+        cubes.push_back(cb);
+    }
+}
+
+// Whether the batch's layer takes the cube merge: 16-bit cells merged two
+// vertices per lane, layers 3 .. u - 1 (a cube holds several rows of the
+// layer), and a device that gives a workgroup the cube's LDS.
+bool use_cube(emv_ctx *c, int k, int m) {
+    static const int maxm = [] {  // EMV_CUBE_MAX_M: the largest layer merged in cubes
+        const char *e = getenv("EMV_CUBE_MAX_M");
+        return e ? atoi(e) : 64;
+    }();
+    const int u = std::min(k, kCubeU);
+    if (!c->merge_cube || m < 3 || m > u - 1 || m > maxm) return false;
+    (void)u;
+    if ((size_t)kCubeLds > c->lds_block) return false;
+    if (!c->cube_attr) {
+        if (hipFuncSetAttribute((const void *)k_merge_cube_h2<uint32_t>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kCubeLds) !=
+                hipSuccess ||
+            hipFuncSetAttribute((const void *)k_merge_cube_h2<uint64_t>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kCubeLds) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            c->merge_cube = false;
+            return false;
+        }
+        c->cube_attr = true;
+    }
+    return true;
+}
+
+template <typename P, typename DT>
+int launch_merge_cube(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, int k, int m) {
+    build_cubes(c->h_xs, nx, k, m, c->h_cubes, c->h_cube_rows);
+    const int ncubes = (int)c->h_cubes.size();
+    const size_t nrows = c->h_cube_rows.size();
+    if (nrows != (size_t)nx) {
+        set_err(c, "cube cover assigned %zu of %d rows", nrows, nx);
+        return EMV_ERR_STATE;
+    }
+    // staged-row table: local row t = w + 16 q of cube g at [g][w][q], the
+    // row's table offset in 128-byte units (rows are 2 np bytes, np a
+    // multiple of 64), all ones for rows the merges never read
+    constexpr int kPer = (1 << kCubeU) / kCubeWaves;
+    const int64_t units = 2 * c->np / 128;
+    if ((((int64_t)1 << k) - 1) * units >= (int64_t)0xFFFFFFFF) {
+        set_err(c, "table offsets exceed the cube table's 32 bits");
+        return EMV_ERR_RANGE;
+    }
+    c->h_cube_tab.assign((size_t)ncubes << kCubeU, 0xFFFFFFFFu);
+    for (int g = 0; g < ncubes; g++) {
+        const uint32_t U = c->h_cubes[(size_t)g].U;
+        const uint32_t nloc = 1u << __builtin_popcount(U);
+        for (uint32_t t = 1; t < nloc; t++) {
+            if (__builtin_popcount(t) >= m) continue;
+            const uint32_t w = t % kCubeWaves, q = t / kCubeWaves;
+            c->h_cube_tab[((size_t)g << kCubeU) + w * kPer + q] =
+                (uint32_t)((int64_t)deposit_host(t, U) * units);
+        }
+    }
+    if (c->cube_tab_cap < c->h_cube_tab.size()) {
+        dfree(c->d_cube_tab);
+        c->cube_tab_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_cube_tab, sizeof(uint32_t) * c->h_cube_tab.size()));
+        c->cube_tab_cap = c->h_cube_tab.size();
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_cube_tab, c->h_cube_tab.data(),
+                             sizeof(uint32_t) * c->h_cube_tab.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    if (c->cubes_cap < (size_t)ncubes) {
+        dfree(c->d_cubes);
+        c->cubes_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_cubes, sizeof(MergeCube) * (size_t)ncubes));
+        c->cubes_cap = (size_t)ncubes;
+    }
+    if (c->cube_rows_cap < nrows) {
+        dfree(c->d_cube_rows);
+        c->cube_rows_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_cube_rows, sizeof(CubeRow) * nrows));
+        c->cube_rows_cap = nrows;
+    }
+    // h_cubes / h_cube_rows are rebuilt only by the next batch's merge, after
+    // this batch's relaxation has synchronised the stream
+    HIPCHK(c, hipMemcpyAsync(c->d_cubes, c->h_cubes.data(), sizeof(MergeCube) * (size_t)ncubes,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_cube_rows, c->h_cube_rows.data(), sizeof(CubeRow) * nrows,
+                             hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = sizeof(P) * (size_t)nx * (size_t)c->np;
+    if (c->m_cap < bytes) {
+        dfree(c->d_M);
+        c->m_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_M, bytes));
+        c->m_cap = bytes;
+    }
+    const int64_t cells = (int64_t)nx * c->np;
+    const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL(k_fill<P>, dim3(fb), dim3(kBlock), 0, c->stream, (P *)c->d_M, cells, ~(P)0);
+    const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+    // one persistent workgroup per CU (its LDS holds one), the items of each
+    // XCD a contiguous range
+    const int64_t items = ntiles * ncubes;
+    const int64_t per = (items + c->nxcd - 1) / c->nxcd;
+    const int grid = (int)std::max<int64_t>(
+        c->nxcd, std::min<int64_t>((int64_t)c->num_cu, per * c->nxcd) / c->nxcd * c->nxcd);
+    hipLaunchKernelGGL((k_merge_cube_h2<P>), dim3(grid), dim3(kCubeWaves * kWave),
+                       (size_t)kCubeLds, c->stream, f, c->np,
+                       (const MergeCube *)c->d_cubes, (const CubeRow *)c->d_cube_rows,
+                       (const uint32_t *)c->d_cube_tab, ncubes, ntiles, k, (P *)c->d_M, c->nxcd,
+                       per);
+    const int64_t vtiles = c->np / kWave;
+    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
+                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
+                       vtiles);
+    HIPCHK(c, hipGetLastError());
+    return EMV_OK;
+}
+
 template <typename T, typename P, typename DT>
 int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
     if constexpr (std::is_same<T, uint16_t>::value) {
+        if (use_merge_h2<T>(c) && c->h_xs && use_cube(c, k, m))
+            return launch_merge_cube<P, DT>(c, f, nx, nchunks, k, m);
         // stars cost two more launches (fill, pack) than whole rows: only
         // layer batches of >= star_min_cells pair cells take them
         const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
@@ -1320,6 +1529,12 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
     c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
     c->merge_star = c->merge_h2 && !(flags & EMV_MERGE_ROWS) && merge_star_enabled();
+    // cubes staged in LDS: opt-in (EMV_MERGE_CUBE or EMV_CUBE=1); the stars are
+    // faster on every layer measured (n = 2^18, k = 14: layers 3-8 in 0.49 /
+    // 1.44 / 3.2 / 6.7 / 15.6 / 39 ms against 0.35 / 1.38 / 2.6 / 4.7 / 9.0 /
+    // 14.9 ms): a cube re-stages ~500 lower rows per tile from L2, while the
+    // stars' repeated row reads hit L1
+    c->merge_cube = c->merge_h2 && ((flags & EMV_MERGE_CUBE) || env_flag("EMV_CUBE"));
     {
         // EMV_STAR_MIN_CELLS: pair cells from which a layer batch merges in stars
         const char *e = getenv("EMV_STAR_MIN_CELLS");

@@ -1620,6 +1620,188 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
     }
 }
 
+// ------------------------------------------------------- cube merge (LDS)
+// Layers of small subsets (16-bit cells, two vertices per lane): a workgroup
+// takes one 128-vertex tile and one cube U of at most kCubeU elements.  It
+// stages in LDS the tile's segment of every lower row T of U (1 <= |T| < m),
+// at T's local mask (T's bits within U: an order-preserving relabelling), and
+// then merges the layer rows X of U that the host assigned to the cube: the
+// least packed key over the pairs {Y, X\Y}, Y a nonempty subset of X without
+// its top element, both rows read from LDS.  A staged row is read from L2/HBM
+// once per (tile, cube) and serves every pair of every assigned row that
+// holds it (0.34-0.72 row loads per pair at k = 14 with the greedy cover,
+// against 1.25-1.3 for the stars); the pairs run at LDS and VALU rate.
+// Keys are (sum << 16) | Y with Y local (the order of local masks is the
+// order of their global sets, so the least key names the same split); the
+// winner becomes (sum << k) | Y with Y global, folded into M[xi][v] with an
+// unsigned atomicMin (a row's pairs may be cut into parts over waves).
+// Items = (tile, cube), tile-major per XCD, so that the cubes of a tile run
+// together on one XCD and share its L2.
+constexpr int kCubeU = 9;       // elements per cube: 2^9 local rows x 256 B = 128 KB of LDS
+constexpr int kCubeLds = ((1 << kCubeU) * kWave + 2 * 128) * 4;  // + the row list (<= 126 rows)
+constexpr int kCubeWaves = 16;  // waves per workgroup
+
+// deposit_bits for a cube: the bits of t (t < 2^popc(U) <= 2^kCubeU) into U.
+__device__ __forceinline__ uint32_t deposit_cube(uint32_t t, uint32_t U) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int q = 0; q < kCubeU; q++) {
+        const uint32_t low = U & (~U + 1u);
+        if (t >> q & 1u) r |= low;
+        U ^= low;
+    }
+    return r;
+}
+
+struct MergeCube {
+    uint32_t U;     // the cube (global mask)
+    int32_t e0;     // its rows: entries e0 .. e0 + nx - 1 of the row list
+    int32_t nx;
+    int32_t parts;  // pair ranges per row
+};
+
+struct CubeRow {
+    uint32_t Xl;  // the row X as a local mask within its cube
+    int32_t xi;   // its index in the layer batch
+};
+
+// One persistent workgroup per CU walks the items of its XCD's range (tile-
+// major), one item at a time: while it merges item i from LDS, the loads of
+// item i + 1's rows and row list are in flight into registers; after a
+// barrier they go to LDS and the next item starts.  Rows of the table that
+// the items read are the lower layers only, which no launch of this layer
+// writes.
+// rowtab[cube * 2^kCubeU + 32 * w + q]: the table offset, in units of 128
+// bytes, of the row that wave w stages as local row t = w + 16 q (its set
+// deposited into the cube), or all ones for a local row the merges never read
+// (m or more elements, or t >= 2^u).  Built by the host, so that staging
+// costs each wave one scalar table read per row.
+template <typename P>
+__global__ __launch_bounds__(kCubeWaves * kWave) void k_merge_cube_h2(
+    const uint16_t *__restrict__ f, int64_t np, const MergeCube *__restrict__ cubes,
+    const CubeRow *__restrict__ rows, const uint32_t *__restrict__ rowtab, int ncubes,
+    int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
+    extern __shared__ uint32_t lds_rows[];  // [2^u][64] staged rows, then the row list
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    // this workgroup's items: the XCD's range [x * per, (x + 1) * per), every
+    // (gridDim / nxcd)-th from its slot
+    const int64_t slots = gridDim.x / nxcd;
+    const int64_t slot = blockIdx.x / nxcd;
+    const int64_t first = (int64_t)(blockIdx.x % nxcd) * per_xcd;
+    const int64_t total = ntiles * ncubes;
+    constexpr int kPer = (1 << kCubeU) / kCubeWaves;  // staged rows per wave
+    constexpr int kRowList = (1 << kCubeU) * kWave;   // word offset of the row list in LDS
+    uint32_t staged[kPer];
+    uint32_t cur_U = 0;
+    int cur_nx = 0, cur_parts = 0;
+    int64_t cur_tile = 0;
+    CubeRow cur_row = {0u, 0};
+    // loads of item `it` into registers: its rows (wave w: local rows w,
+    // w + 16, ..., those the merges read) and, in waves 0 and 1, row list
+    // entry 64 * wave + lane (< nx)
+    auto prefetch = [&](int64_t it, uint32_t &U, int &nx, int &parts, int64_t &tile,
+                        CubeRow &rl) {
+        tile = it / ncubes;
+        const int cb = (int)(it - tile * ncubes);
+        U = cubes[cb].U;
+        nx = cubes[cb].nx;
+        parts = cubes[cb].parts;
+        const int e0 = cubes[cb].e0;
+        const int64_t v0 = tile * (2 * kWave);
+        // lanes past the row read its last pair; their results are dropped
+        const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+        const char *fb = reinterpret_cast<const char *>(f + v0) + lb;
+        const uint32_t *tab = rowtab + (int64_t)cb * (1 << kCubeU) + wid * kPer;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const uint32_t off = tab[q];
+            staged[q] = 0u;
+            if (off != 0xFFFFFFFFu)
+                staged[q] = *reinterpret_cast<const uint32_t *>(fb + (int64_t)off * 128);
+        }
+        const int ei = wid * kWave + lane;  // row list entries: waves 0 and 1
+        if (wid < 2 && ei < nx) rl = rows[e0 + ei];
+    };
+    int64_t it = first + slot;
+    const int64_t last = min(first + per_xcd, total);
+    if (it >= last) return;
+    prefetch(it, cur_U, cur_nx, cur_parts, cur_tile, cur_row);
+    uint32_t hmask = 0xFFFF0000u;
+    asm volatile("" : "+v"(hmask));
+    for (; it < last; it += slots) {
+        __syncthreads();  // the previous item's merges are done with LDS
+        {
+            // every local row is written (unread ones with a zero word)
+#pragma unroll
+            for (int q = 0; q < kPer; q++)
+                lds_rows[(uint32_t)(wid + q * kCubeWaves) * kWave + lane] = staged[q];
+            const int ei = wid * kWave + lane;
+            if (wid < 2 && ei < cur_nx) {
+                lds_rows[kRowList + 2 * ei] = cur_row.Xl;
+                lds_rows[kRowList + 2 * ei + 1] = (uint32_t)cur_row.xi;
+            }
+        }
+        __syncthreads();
+        const uint32_t U = cur_U;
+        const int nx = cur_nx, parts = cur_parts;
+        const int64_t v0 = cur_tile * (2 * kWave);
+        if (it + slots < last) prefetch(it + slots, cur_U, cur_nx, cur_parts, cur_tile, cur_row);
+        const int ntask = nx * parts;
+        for (int task = wid; task < ntask; task += kCubeWaves) {
+            const int e = task / parts;
+            const int part = task - e * parts;
+            const uint32_t Xl = __builtin_amdgcn_readfirstlane(lds_rows[kRowList + 2 * e]);
+            const int32_t xi = (int32_t)__builtin_amdgcn_readfirstlane(lds_rows[kRowList + 2 * e + 1]);
+            const uint32_t Xp = Xl ^ (1u << (31 - __clz(Xl)));
+            const uint32_t npairs = (1u << __popc(Xp)) - 1u;
+            const uint32_t i0 = npairs * (uint32_t)part / (uint32_t)parts;
+            const uint32_t i1 = npairs * (uint32_t)(part + 1) / (uint32_t)parts;
+            if (i0 >= i1) continue;
+            // the (i0 + 1)-th nonempty subset of Xp in increasing order, then
+            // the next ones by (Y - Xp) & Xp
+            uint32_t Y = deposit_bits(i0 + 1u, Xp);
+            uint32_t klo = 0xFFFFFFFFu, khi = 0xFFFFFFFFu;
+            uint32_t i = i0;
+            for (; i + 4u <= i1; i += 4u) {
+                uint32_t Ys[4];
+                Ys[0] = Y;
+#pragma unroll
+                for (int q = 1; q < 4; q++) Ys[q] = (Ys[q - 1] - Xp) & Xp;
+                Y = (Ys[3] - Xp) & Xp;
+                uint32_t a[4], c[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    a[q] = lds_rows[Ys[q] * kWave + lane];
+                    c[q] = lds_rows[(Xl ^ Ys[q]) * kWave + lane];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q += 2) {
+                    const uint32_t s0 = add_u16x2(a[q], c[q]);
+                    const uint32_t s1 = add_u16x2(a[q + 1], c[q + 1]);
+                    klo = min(klo, min((s0 << 16) | Ys[q], (s1 << 16) | Ys[q + 1]));
+                    khi = min(khi, min((s0 & hmask) | Ys[q], (s1 & hmask) | Ys[q + 1]));
+                }
+            }
+            for (; i < i1; i++) {
+                const uint32_t s0 =
+                    add_u16x2(lds_rows[Y * kWave + lane], lds_rows[(Xl ^ Y) * kWave + lane]);
+                klo = min(klo, (s0 << 16) | Y);
+                khi = min(khi, (s0 & hmask) | Y);
+                Y = (Y - Xp) & Xp;
+            }
+            const int64_t v = v0 + 2 * lane;
+            if (v < np) {
+                const P plo = ((P)(klo >> 16) << k) | (P)deposit_cube(klo & 0xFFFFu, U);
+                const P phi = ((P)(khi >> 16) << k) | (P)deposit_cube(khi & 0xFFFFu, U);
+                P *acc = &M[(int64_t)xi * np + v];
+                atomicMin(acc, plo);
+                atomicMin(acc + 1, phi);
+            }
+        }
+    }
+}
+
 // Packed accumulators M[xi][v] -> vertex-major staging (label, packed word)
 // through an LDS transpose of 64 rows x 64 vertices per workgroup; rows past
 // nx are INF.

@@ -24,6 +24,7 @@ WIDE_LABELS = 0x10
 TABLE32 = 0x20
 MERGE_1V = 0x40
 MERGE_ROWS = 0x80
+MERGE_CUBE = 0x100
 DUMP_RECORDS = 0x1  # emv_write_table flag
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",

@@ -121,6 +121,11 @@ def test_regular_n262144_k14_certified():
         assert sol32.table_bytes == 4 and sol32.label_bits == 32 and sol32.cost == sol.cost
         assert_clean(ctx.certify(), "k=14 int32 table")
         ctx.release_table()
+        # the LDS-staged cube merge (opt-in) gives the same table
+        solc = ctx.solve(kk, emv.KEEP_TABLE | emv.MERGE_CUBE)
+        assert solc.cost == sol.cost
+        assert_clean(ctx.certify(), "k=14 table, cube merge")
+        ctx.release_table()
 
 
 # --------------------------------------------------- the 256-instance batch

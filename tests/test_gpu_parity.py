@@ -63,7 +63,7 @@ def test_regular_1000_full_table(gpu_ctx):
 @pytest.mark.parametrize("flags", [emv.WIDE_LABELS, emv.FORCE_P64, emv.FORCE_I64,
                                    emv.FORCE_I64 | emv.FORCE_P64, emv.TABLE32,
                                    emv.TABLE32 | emv.WIDE_LABELS, emv.MERGE_1V,
-                                   emv.MERGE_1V | emv.WIDE_LABELS, emv.MERGE_ROWS])
+                                   emv.MERGE_1V | emv.WIDE_LABELS, emv.MERGE_ROWS, emv.MERGE_CUBE])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
 def test_wide_paths_bit_exact(gpu_ctx, name, flags):
     """Every width combination (16/32/64-bit labels, 32/64-bit merge
@@ -96,6 +96,24 @@ def test_random_regular_vs_oracle(gpu_ctx, seed):
     inst = Instance(n, e, kk)
     print(f"case regular n={n} d={d} k={k} seed={seed}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+    assert sol.cost == ocost
+
+
+@pytest.mark.parametrize("seed,k", [(0, 10), (1, 11), (2, 12)])
+def test_cube_merge_vs_oracle(gpu_ctx, seed, k):
+    """The LDS-staged cube merge (EMV_MERGE_CUBE): with k >= 10 the layers
+    3..8 are covered by several 9-element cubes, rows split into parts over
+    the waves; tables and records equal the oracle's."""
+    n = 1500 + 500 * seed
+    e = emv.regular_graph(n, 6, 1000, 700 + seed)
+    kk = emv.terminals(n, k, 700 + seed)
+    inst = Instance(n, e, kk)
+    print(f"case cube n={n} k={k}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst, emv.MERGE_CUBE)
+    assert sol.table_bytes == 2
     ocost, of, ocode = oracle.emv(n, e, kk)
     np.testing.assert_array_equal(f, of)
     np.testing.assert_array_equal(code, ocode)
