@@ -69,6 +69,12 @@ WORKLOAD = dict(workload="EMV plain single instance: random connected 8-regular 
                 n=262144, d=8, wmax=1000, k=14, seed=2)
 
 
+def _comb(n, r):
+    import math
+
+    return math.comb(n, r)
+
+
 def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
@@ -253,6 +259,7 @@ def measure_traffic(args, kernel_prefix):
                 "copy": [probe, str(copy_bytes), "3"],
                 "solve": [sys.executable, os.path.abspath(__file__), "--legs", "plain", "--steps",
                           "1", "--warmup", "0", "--no-cpu", "--no-traffic", "--no-table32",
+                          "--no-visits",
                           "--n", str(args.n), "--k", str(args.k)],
             }
             for tag, cmd in runs.items():
@@ -307,6 +314,13 @@ def plain_leg(args):
             f"relax={sol.relax_ms:.1f} records={sol.unpack_ms:.1f} single={sol.singleton_ms:.1f} "
             f"rounds={sol.relax_rounds}")
     wall = time.perf_counter() - t0
+    # one more solve counting the relaxation's item visits (not timed)
+    visits = -1
+    if not args.no_visits:
+        vsol = ctx.solve(kk, emv.COUNT_VISITS)
+        visits = vsol.relax_item_visits
+        if vsol.cost != sols[-1].cost:
+            raise RuntimeError("visit-counting solve changed the cost")
     t32 = None
     if not args.no_table32:
         # the same solve on an int32 table (the 16-bit cells are exact because
@@ -341,6 +355,16 @@ def plain_leg(args):
     relax["frac"] = relax["achieved_GBs"] / HBM_PEAK_GBS
     relax["cells_per_s"] = cells / (relax_ms / 1e3) if relax_ms > 0 else 0.0
     relax["avg_launch_us"] = relax_ms * 1e3 / max(relax["launches"], 1)
+    # item visits: a (chunk, vertex) item is 64 rows of one vertex; one pass
+    # over every item of layers 2..k is the one-pass model's work
+    one_pass = sum(-(-_comb(k, mm) // 64) for mm in range(2, k + 1)) * wl["n"]
+    relax["item_visits"] = visits
+    relax["one_pass_items"] = one_pass
+    relax["visits_per_item"] = visits / one_pass if visits > 0 else None
+    deg = wl["d"]
+    lb = sols[-1].label_bits // 8
+    relax["visit_bytes_model"] = 64 * lb * (deg + 2) + 8 * deg  # own row read+write, neighbours, adjacency
+    relax["visit_bytes_per_solve"] = visits * relax["visit_bytes_model"] if visits > 0 else None
     # merge: unordered pair-cells; its per-X streaming model counts lower rows
     # re-read from cache, so it is reported as a model ratio, not a roofline
     # fraction; the merge is bounded by its VALU work (about two lane-ops per
@@ -732,6 +756,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 counter passes behind roofline.traffic")
     ap.add_argument("--no-table32", action="store_true")
+    ap.add_argument("--no-visits", action="store_true",
+                    help="skip the extra solve that counts the relaxation's item visits")
     ap.add_argument("--batch-no-graph", action="store_true",
                     help="batch leg with eager steps only (no hipGraph capture), for profiling")
     args = ap.parse_args()

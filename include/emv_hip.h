@@ -48,6 +48,7 @@ extern "C" {
 #define EMV_TABLE32     0x20u /* int32 table even when the bound allows 16-bit cells */
 #define EMV_MERGE_1V    0x40u /* merges of 16-bit cells one vertex per lane, as wider cells */
 #define EMV_MERGE_ROWS  0x80u /* merges of 16-bit cells row by row, not in stars of rows sharing a base set */
+#define EMV_COUNT_VISITS 0x200u /* count the (chunk, vertex) items the vertex-wise layer rounds visit */
 #define EMV_MERGE_CUBE  0x100u /* layers 3..8 of 16-bit cells merged from rows staged in LDS (cubes);
                                   measured slower than the stars on MI355X, so not the default */
 
@@ -89,6 +90,10 @@ typedef struct {
                                       values span less than 2^32 - 1 - max w */
     int64_t compact_merge_layers;  /* int64 tables: layer batches merged from the int32 copy of
                                       the rows (cell - row base) instead of the int64 cells */
+    int64_t relax_item_visits;     /* with EMV_COUNT_VISITS: (chunk, vertex) items that the
+                                      vertex-wise relaxation rounds of layers 2..k processed (each
+                                      visit reads 64 rows of its own and its neighbours' labels);
+                                      -1 otherwise */
 } emv_result;
 
 /* Context on HIP device `device`. */

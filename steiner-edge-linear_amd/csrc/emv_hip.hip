@@ -116,6 +116,8 @@ struct emv_ctx {
     int64_t xs_cap = 0;
     int32_t *d_src = nullptr;                // shortest-path sources
     uint32_t *h_pin = nullptr;               // pinned readback of a work word
+    unsigned long long *d_visits = nullptr;  // items visited by the layer rounds (EMV_COUNT_VISITS)
+    bool count_visits = false;
     unsigned char *d_scratch = nullptr;      // this device's copy of `scratch` (emv_sssp_noalloc)
 };
 
@@ -418,7 +420,8 @@ void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
                        (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
                        (const uint32_t *)c->d_q[r & 1], c->d_q[(r + 1) & 1], c->qcap, c->d_any,
-                       c->d_cnt, r, c->relax_nxcd, c->dense_rounds);
+                       c->d_cnt, r, c->relax_nxcd, c->dense_rounds,
+                       c->count_visits ? c->d_visits : nullptr);
 }
 
 template <typename DT>
@@ -1318,6 +1321,7 @@ extern "C" int emv_open(emv_ctx **out, int device) {
             hipMalloc((void **)&c->d_cnt, sizeof(uint32_t) * 4) != hipSuccess ||
             hipMalloc((void **)&c->d_rowmax, sizeof(unsigned long long) * 64) != hipSuccess ||
             hipMalloc((void **)&c->d_misc, sizeof(int64_t) * 8) != hipSuccess ||
+            hipMalloc((void **)&c->d_visits, sizeof(unsigned long long)) != hipSuccess ||
             hipHostMalloc((void **)&c->h_pin, sizeof(uint32_t) * 16, hipHostMallocDefault) !=
                 hipSuccess) {
             rc = EMV_ERR_NOMEM;
@@ -1347,6 +1351,7 @@ extern "C" void emv_close(emv_ctx *c) {
     dfree(c->d_cnt);
     dfree(c->d_rowmax);
     dfree(c->d_misc);
+    dfree(c->d_visits);
     dfree(c->d_xs);
     dfree(c->d_src);
     if (c->h_pin) (void)hipHostFree(c->h_pin);
@@ -1464,6 +1469,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     HIPCHK(c, hipSetDevice(c->device));
     auto t_start = std::chrono::steady_clock::now();
     invalidate_table(c);
+    c->count_visits = false;
 
     SolveState st;
     st.k = (int)k;
@@ -1599,6 +1605,10 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     c->tk = k;
 
     // ---- 3. layers
+    c->count_visits = (flags & EMV_COUNT_VISITS) != 0;
+    if (c->count_visits)
+        hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(64), 0, c->stream, c->d_visits,
+                           (int64_t)1, 0ull);
     const size_t ev_layers0 = ev.size();
     c->batch_split.clear();
     if (st.tbytes == 2)
@@ -1608,6 +1618,14 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     else
         CHK(run_layers_t<int64_t>(c, st, (int64_t *)c->d_f, c->d_b, res, ev));
     HIPCHK(c, hipEventRecord(e_end, c->stream));
+    res->relax_item_visits = -1;
+    if (c->count_visits) {
+        unsigned long long nv = 0;
+        HIPCHK(c, hipMemcpyAsync(&nv, c->d_visits, sizeof(nv), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        res->relax_item_visits = (int64_t)nv;
+        c->count_visits = false;
+    }
 
     // ---- 4. answer
     const int64_t C = (k == 1) ? 0 : (((int64_t)1 << (k - 1)) - 1);

@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
     const Edge *__restrict__ adj, const Edge *__restrict__ ell, int64_t nitems, int dense0,
     uint32_t *__restrict__ Fcur, uint32_t *__restrict__ Fnext, const uint32_t *__restrict__ Qcur,
     uint32_t *__restrict__ Qnext, uint32_t qcap, uint32_t *__restrict__ any,
-    uint32_t *__restrict__ cnt, int r, int nxcd, int dense_rounds) {
+    uint32_t *__restrict__ cnt, int r, int nxcd, int dense_rounds,
+    unsigned long long *__restrict__ visits) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         any[(r + 2) & 3] = 0;
         cnt[(r + 2) & 3] = 0;
@@ -211,6 +212,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
         }
         uint64_t act = __ballot(fl);
         if (act == 0) continue;
+        // optional count of the items visited (one atomic per 64 items)
+        if (visits && lane == 0) atomicAdd(visits, (unsigned long long)__popcll(act));
         const bool list_open = *(volatile uint32_t *)cnext < qcap;
         if constexpr (DMAX > 0) {
             Edge e[DMAX];

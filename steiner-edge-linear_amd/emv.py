@@ -25,6 +25,7 @@ TABLE32 = 0x20
 MERGE_1V = 0x40
 MERGE_ROWS = 0x80
 MERGE_CUBE = 0x100
+COUNT_VISITS = 0x200
 DUMP_RECORDS = 0x1  # emv_write_table flag
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",
@@ -48,7 +49,7 @@ class EmvResult(ctypes.Structure):
         ("relax_blocks", ctypes.c_int64), ("unpack_ms", ctypes.c_double),
         ("singleton_rounds", ctypes.c_int64), ("block_dist_bits", ctypes.c_int64),
         ("merge_split_ms", ctypes.c_double), ("relative_label_layers", ctypes.c_int64),
-        ("compact_merge_layers", ctypes.c_int64),
+        ("compact_merge_layers", ctypes.c_int64), ("relax_item_visits", ctypes.c_int64),
     ]
 
 
@@ -142,6 +143,7 @@ class Solution:
         self.merge_split_ms = r.merge_split_ms
         self.relative_label_layers = r.relative_label_layers
         self.compact_merge_layers = r.compact_merge_layers
+        self.relax_item_visits = r.relax_item_visits
         self.pairs = pairs
 
 
