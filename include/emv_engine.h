@@ -123,6 +123,12 @@ int64_t emv_server_group_weight(const emv_engine *e);
  * request by one DP stage in shared launches. */
 int emv_server_step(emv_engine *e, int64_t *running, int64_t *queued);
 int emv_server_poll(emv_engine *e, int64_t ticket, emv_request_info *info);
+/* Cancels a queued or running request: it leaves the queue or the running
+ * set, its table region and slot are freed for the next admission, and it
+ * ends with state 2 and status EMV_ERR_CANCELLED.  EMV_ERR_STATE for a
+ * finished request or an engine made by emv_batch_create.  (The reference
+ * solves one instance per process and has no request to cancel.) */
+int emv_server_cancel(emv_engine *e, int64_t ticket);
 int64_t emv_server_free_bytes(const emv_engine *e);
 int64_t emv_request_footprint(const emv_engine *e, int64_t k);
 
@@ -138,6 +144,14 @@ int emv_engine_release(emv_engine *e, int64_t ticket);
  * to the nearest member and edge records (2^k + the smallest neighbour on a
  * shortest path), {j} at the members. */
 int emv_engine_certify(emv_engine *e, int64_t ticket, emv_certificate *out);
+
+/* Test hook: the step allocation `after` allocations from now (0 = the next
+ * one) fails as if the device were out of memory, without calling hipMalloc;
+ * -1 disarms it.  Step allocations (staging, flags, merge accumulators,
+ * descriptors) all happen before a step launches anything, so the failing
+ * emv_server_step / emv_batch_solve returns EMV_ERR_NOMEM with every request
+ * still at its stage; a later call carries on (a batch solve starts over). */
+int emv_engine_fail_alloc(emv_engine *e, int64_t after);
 
 #ifdef __cplusplus
 }

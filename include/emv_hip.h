@@ -38,6 +38,7 @@ extern "C" {
 #define EMV_ERR_NODEV         -7   /* no HIP device */
 #define EMV_ERR_CAPACITY      -8   /* request exceeds a configured capacity */
 #define EMV_ERR_ADMISSION     -9   /* serving: request can never fit in device memory */
+#define EMV_ERR_CANCELLED     -10  /* serving: request cancelled before it finished */
 
 /* emv_solve flags */
 #define EMV_LIST        0x1u  /* run the device traceback; fill res->pairs */

@@ -213,6 +213,8 @@ struct emv_engine {
     int64_t steps = 0, relax_rounds = 0;
     double relax_ms_acc = 0, merge_ms_acc = 0;
     std::vector<int> rounds_per_step;  // eager: rounds used by step s (batch)
+    bool server = false;         // created by emv_server_create (requests may be cancelled)
+    int64_t fail_alloc = -1;     // test hook: step allocations left before one fails (-1: off)
 };
 
 namespace {
@@ -241,6 +243,21 @@ void eset(emv_engine *e, const char *fmt, ...) {
         int _r = (expr);             \
         if (_r != EMV_OK) return _r; \
     } while (0)
+
+// Device allocation of a step's scratch (staging, flags, merge accumulators,
+// descriptors).  With the test hook armed (emv_engine_fail_alloc) the
+// allocation it names fails as if the device were out of memory, without
+// calling hipMalloc.  Every such allocation happens before the step launches
+// anything, so a failed step leaves each request at its stage.
+hipError_t step_malloc(emv_engine *e, void **p, size_t bytes) {
+    if (e->fail_alloc == 0) {
+        e->fail_alloc = -1;
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    if (e->fail_alloc > 0) e->fail_alloc--;
+    return hipMalloc(p, bytes);
+}
 
 // The m-subsets of k bits in increasing numeric order (reader-el.c:2040-2046).
 void subsets_of_size(int k, int m, std::vector<uint32_t> &out) {
@@ -438,7 +455,7 @@ int ensure_macc(emv_engine *e, int64_t bytes) {
     if (bytes <= e->macc_bytes) return EMV_OK;
     dfree(e->d_macc);
     e->macc_bytes = 0;
-    EHIP(e, hipMalloc(&e->d_macc, (size_t)bytes));
+    EHIP(e, step_malloc(e, &e->d_macc, (size_t)bytes));
     e->macc_bytes = bytes;
     return EMV_OK;
 }
@@ -449,8 +466,8 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
         dfree(e->d_code);
         e->stage_cells = 0;
         const int64_t c = cells + cells / 4;
-        EHIP(e, hipMalloc((void **)&e->d_D, sizeof(uint32_t) * c));
-        EHIP(e, hipMalloc((void **)&e->d_code, sizeof(uint64_t) * c));
+        EHIP(e, step_malloc(e, (void **)&e->d_D, sizeof(uint32_t) * c));
+        EHIP(e, step_malloc(e, (void **)&e->d_code, sizeof(uint64_t) * c));
         e->stage_cells = c;
     }
     if (flags > e->flag_bytes) {
@@ -458,8 +475,8 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
         dfree(e->d_flag[1]);
         e->flag_bytes = 0;
         const int64_t c = flags + flags / 4;
-        EHIP(e, hipMalloc((void **)&e->d_flag[0], sizeof(uint32_t) * c));
-        EHIP(e, hipMalloc((void **)&e->d_flag[1], sizeof(uint32_t) * c));
+        EHIP(e, step_malloc(e, (void **)&e->d_flag[0], sizeof(uint32_t) * c));
+        EHIP(e, step_malloc(e, (void **)&e->d_flag[1], sizeof(uint32_t) * c));
         e->flag_bytes = c;
         clear_flags(e);
         EHIP(e, hipGetLastError());
@@ -870,7 +887,7 @@ int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
         if (e->desc_bufs[idx]) (void)hipFree(e->desc_bufs[idx]);
         e->desc_bufs[idx] = nullptr;
         e->desc_caps[idx] = 0;
-        EHIP(e, hipMalloc(&e->desc_bufs[idx], bytes * 2));
+        EHIP(e, step_malloc(e, &e->desc_bufs[idx], bytes * 2));
         e->desc_caps[idx] = (int64_t)bytes * 2;
     }
     *out = (char *)e->desc_bufs[idx];
@@ -1872,6 +1889,7 @@ extern "C" int emv_server_create(int device, int64_t n, int64_t m, const int64_t
     int rc = engine_open(device, &e);
     *out = e;
     if (rc != EMV_OK) return rc;
+    e->server = true;
     std::vector<std::vector<int64_t>> E(1);
     E[0].assign(edges, edges + 3 * m);
     std::vector<int64_t> ns(1, n);
@@ -1951,6 +1969,44 @@ extern "C" int emv_server_poll(emv_engine *e, int64_t ticket, emv_request_info *
     info->reduced_cost = I.reduced_cost;
     info->groups = I.groups.empty() ? 0 : I.k;
     info->root = I.root;
+    return EMV_OK;
+}
+
+extern "C" int emv_server_cancel(emv_engine *e, int64_t ticket) {
+    if (!e || ticket < 0 || ticket >= (int64_t)e->inst.size()) return EMV_ERR_ARG;
+    if (!e->server) {
+        eset(e, "cancel: requests of a batch cannot be cancelled");
+        return EMV_ERR_STATE;
+    }
+    Inst &I = e->inst[ticket];
+    if (I.state == ST_DONE) {
+        eset(e, "cancel: request %lld has finished", (long long)ticket);
+        return EMV_ERR_STATE;
+    }
+    if (I.state == ST_QUEUED) {
+        auto it = std::find(e->queue.begin(), e->queue.end(), ticket);
+        if (it != e->queue.end()) e->queue.erase(it);
+    } else {
+        // Every launch of this request is on the engine's stream, so the
+        // next request admitted to its table region or slot is ordered
+        // after them.
+        if (I.abase >= 0) e->arena.release(I.abase, I.acells);
+        if (I.slot >= 0) e->free_slots.push_back(I.slot);
+        I.abase = -1;
+        I.slot = -1;
+    }
+    I.state = ST_DONE;
+    I.status = EMV_ERR_CANCELLED;
+    I.cost = INT64_MAX;
+    I.reduced_cost = INT64_MAX;
+    I.pairs.clear();
+    I.t_done = now_ms();
+    return EMV_OK;
+}
+
+extern "C" int emv_engine_fail_alloc(emv_engine *e, int64_t after) {
+    if (!e || after < -1) return EMV_ERR_ARG;
+    e->fail_alloc = after;
     return EMV_OK;
 }
 

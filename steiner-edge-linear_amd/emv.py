@@ -30,7 +30,7 @@ DUMP_RECORDS = 0x1  # emv_write_table flag
 
 ERRORS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE",
           -5: "ERR_DISCONNECTED", -6: "ERR_RANGE", -7: "ERR_NODEV", -8: "ERR_CAPACITY",
-          -9: "ERR_ADMISSION"}
+          -9: "ERR_ADMISSION", -10: "ERR_CANCELLED"}
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
 
@@ -391,6 +391,8 @@ def _bind_engine():
                                            ctypes.POINTER(ctypes.c_int64)]
     L.emv_server_group_weight.argtypes = [vp]
     L.emv_server_group_weight.restype = ctypes.c_int64
+    L.emv_server_cancel.argtypes = [vp, ctypes.c_int64]
+    L.emv_engine_fail_alloc.argtypes = [vp, ctypes.c_int64]
     _ENG_BOUND = True
     return L
 
@@ -445,6 +447,11 @@ class _Engine:
         c = Certificate()
         self._check(_bind_engine().emv_engine_certify(self._h, ticket, ctypes.byref(c)))
         return c
+
+    def fail_alloc(self, after=0):
+        """Test hook: the step allocation `after` allocations from now fails
+        as if the device were out of memory (-1 disarms it)."""
+        self._check(_bind_engine().emv_engine_fail_alloc(self._h, after))
 
 
 class Batch(_Engine):
@@ -512,6 +519,10 @@ class Server(_Engine):
     def group_weight(self):
         """M = 1 + sum of the resident graph's weights (the reduction's edge weight)."""
         return _bind_engine().emv_server_group_weight(self._h)
+
+    def cancel(self, ticket):
+        """Cancels a queued or running request (it ends with ERR_CANCELLED)."""
+        self._check(_bind_engine().emv_server_cancel(self._h, ticket))
 
     def step(self):
         r, q = ctypes.c_int64(0), ctypes.c_int64(0)

@@ -104,6 +104,9 @@ def test_status_codes_without_device():
     assert lib.emv_solve_footprint(None, 3, 0) == -1
     assert lib.emv_certify(None, None) == -1
     assert lib.emv_certify_table(None, 2, None, None, None, None) == -1
+    eng = emv._bind_engine()
+    assert eng.emv_server_cancel(None, 0) == -1
+    assert eng.emv_engine_fail_alloc(None, 0) == -1
 
 
 INPUT_RE = re.compile(r"(.*)n = (.*), m = (.*), k = (.*), cost = (.*) \[(.*)ms\] {peak:(.*)GiB} "

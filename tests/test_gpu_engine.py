@@ -223,6 +223,81 @@ def test_server_admission_by_footprint():
         assert s.poll(t2).queue_ms > 0.0
 
 
+def test_server_cancel_queued_and_running():
+    """A cancelled request ends with ERR_CANCELLED and frees its table and
+    slot; the others' results are unchanged, and a request admitted into
+    the freed region afterwards is solved exactly."""
+    R, C = 30, 30
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 44)
+    qs = [emv.distinct(n, k, 50 + k, 2) for k in (6, 7, 8, 5)]
+    with emv.Server(n, e, arena_bytes=0, max_inflight=2) as s:
+        full = s.free_bytes()
+        t = [s.submit(q) for q in qs[:3]]
+        s.step()
+        s.step()
+        assert s.poll(t[0]).state == 1 and s.poll(t[1]).state == 1 and s.poll(t[2]).state == 0
+        s.cancel(t[2])  # queued
+        s.cancel(t[1])  # running, mid-way through its layers
+        for x in (t[1], t[2]):
+            info = s.poll(x)
+            assert info.state == 2 and info.status == -10
+        with pytest.raises(emv.EmvError) as ei:
+            s.cancel(t[1])  # already finished
+        assert ei.value.status == -4
+        t3 = s.submit(qs[3], emv.KEEP_TABLE)  # takes the freed slot and region
+        while s.poll(t[0]).state != 2 or s.poll(t3).state != 2:
+            s.step()
+        for x, q in ((t[0], qs[0]), (t3, qs[3])):
+            info = s.poll(x)
+            assert info.status == 0 and info.cost == oracle.emv(n, e, q)[0]
+        f, code = s.read_rows(t3, 0, 1 << len(qs[3]), n)
+        _, of, ocode = oracle.emv(n, e, qs[3])
+        np.testing.assert_array_equal(f, of)
+        np.testing.assert_array_equal(code, ocode)
+        s.release(t3)
+        assert s.free_bytes() == full
+        s.step()  # nothing left: an empty step
+    with emv.Batch([(n, e, qs[0])]) as b:
+        with pytest.raises(emv.EmvError) as ei:
+            b._check(emv._bind_engine().emv_server_cancel(b._h, 0))
+        assert ei.value.status == -4
+
+
+def test_server_step_survives_injected_allocation_failure():
+    """The step's scratch allocation fails (forced from the host, hipMalloc
+    is not called): the step returns ERR_NOMEM with every request at its
+    stage, and the next step carries on to the exact results."""
+    R, C = 30, 30
+    n = R * C
+    e = emv.grid_graph(R, C, 100, 45)
+    q1, q2 = emv.distinct(n, 5, 61, 2), emv.distinct(n, 9, 62, 2)
+    with emv.Server(n, e, arena_bytes=0, max_inflight=4) as s:
+        t1 = s.submit(q1)
+        s.fail_alloc(0)  # the first step allocates its staging
+        with pytest.raises(emv.EmvError) as ei:
+            s.step()
+        assert ei.value.status == -3
+        info = s.poll(t1)
+        assert info.state == 1 and info.stage == 1
+        s.step()
+        assert s.poll(t1).stage == 2
+        # a larger request grows the staging: fail each of its allocations in turn
+        t2 = s.submit(q2)
+        for after in range(3):
+            s.fail_alloc(after)
+            try:
+                s.step()
+            except emv.EmvError as err:
+                assert err.status == -3
+        s.fail_alloc(-1)
+        while s.poll(t1).state != 2 or s.poll(t2).state != 2:
+            s.step()
+        for t, q in ((t1, q1), (t2, q2)):
+            info = s.poll(t)
+            assert info.status == 0 and info.cost == oracle.emv(n, e, q)[0]
+
+
 @pytest.mark.parametrize("wide", [emv.WIDE_LABELS, emv.TABLE32, emv.WIDE_LABELS | emv.TABLE32,
                                   emv.MERGE_1V])
 def test_batch_label_widths_agree(wide):

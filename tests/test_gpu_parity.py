@@ -479,13 +479,14 @@ def test_star_merges_bit_exact(gpu_ctx, monkeypatch, name):
     np.testing.assert_array_equal(code, c2)
 
 
-@pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16)])
+@pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16), (230, 3, 13)])
 @pytest.mark.parametrize("flags", [0, emv.MERGE_ROWS, emv.MERGE_1V])
 def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags):
-    """k = 15 and 16 on 16-bit cells: the two-vertex merges' keys hold Y in
+    """k = 13 .. 16 on 16-bit cells: the two-vertex merges' keys hold Y in
     16 bits and their low rows in 32-bit lane offsets at the largest k they
     take; cells, records and cost equal the oracle's, in stars (forced for
-    these small layers), row by row and one vertex per lane."""
+    these small layers), row by row and one vertex per lane.  n = 100, 48
+    and 230 end in partly filled 128-vertex tiles."""
     monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
     e = emv.regular_graph(n, d, 1000, 7)
     kk = emv.terminals(n, k, 7)
