@@ -89,7 +89,7 @@ struct emv_ctx {
     MergeCube *d_cubes = nullptr;
     CubeRow *d_cube_rows = nullptr;
     size_t cubes_cap = 0, cube_rows_cap = 0;
-    double star_min_cells = 1073741824.0;  // from layer batches of this many pair cells on
+    double star_min_cells = 34359738368.0;  // from layer batches of this many pair cells on
     const uint32_t *h_xs = nullptr;         // host copy of the batch's rows (star building)
     std::vector<MergeStar> h_stars;         // stars of the last batch (kept for the async copy)
     MergeStar *d_stars = nullptr;
@@ -947,8 +947,10 @@ int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) 
     if constexpr (std::is_same<T, uint16_t>::value) {
         if (use_merge_h2<T>(c) && c->h_xs && use_cube(c, k, m))
             return launch_merge_cube<P, DT>(c, f, nx, nchunks, k, m);
-        // stars cost two more launches (fill, pack) than whole rows: only
-        // layer batches of >= star_min_cells pair cells take them
+        // stars cost two more launches (fill, pack) and an accumulator pass
+        // per row than whole rows: only layer batches of >= star_min_cells
+        // pair cells take them (default 2^35; at n = 2^18, k = 14 the layers
+        // from 7 elements on: layers 4-6 merged whole were 4-5 ms faster)
         const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
         if (use_merge_h2<T>(c) && c->merge_star && c->h_xs && pair_cells >= c->star_min_cells)
             return launch_merge_star<P, DT>(c, f, nx, nchunks, k, m);
@@ -1544,7 +1546,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     {
         // EMV_STAR_MIN_CELLS: pair cells from which a layer batch merges in stars
         const char *e = getenv("EMV_STAR_MIN_CELLS");
-        c->star_min_cells = e ? atof(e) : 1073741824.0;
+        c->star_min_cells = e ? atof(e) : 34359738368.0;
     }
     st.rel32 = !(flags & EMV_FORCE_P64) && !env_flag("EMV_NO_REL32");
     if (flags & EMV_FORCE_P64)
