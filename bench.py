@@ -99,8 +99,8 @@ def physical_cores(info):
         return os.cpu_count() or 1
 
 
-def _run_reference(stp, threads, timeout=900):
-    ref = os.path.join(REPO, "oracle", "_ref", "reader-el")
+def _run_reference(stp, threads, timeout=900, binary="reader-el"):
+    ref = os.path.join(REPO, "oracle", "_ref", binary)
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread")
     out = subprocess.run([ref, "-in", stp, "-el"], capture_output=True, text=True, env=env,
                          cwd=os.path.dirname(stp), timeout=timeout)
@@ -135,8 +135,16 @@ def cpu_baseline(config2=False):
             subprocess.run([gen, "regular", str(n), str(d), "1000", str(k), "2"], stdout=fh,
                            check=True)
         if os.path.exists(ref):
-            kms, done, _ = _run_reference(stp, threads)
+            kms, done, cost0 = _run_reference(stp, threads)
             out.update(kind="reference", kernel_ms=kms, done_ms=done)
+            # the same reference build at -O3 (the published recipe has no -O
+            # flag), a separately labelled row on the same sample
+            if os.path.exists(ref + "-O3"):
+                kms3, done3, cost3 = _run_reference(stp, threads, binary="reader-el-O3")
+                if cost3 != cost0:
+                    raise RuntimeError(f"-O3 reference cost {cost3} != {cost0}")
+                out["O3"] = dict(kernel_ms=kms3, done_ms=done3,
+                                 value=(1 << k) * n / (kms3 / 1e3))
         else:
             sys.path.insert(0, os.path.join(REPO, "oracle"))
             import oracle
