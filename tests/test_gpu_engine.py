@@ -52,6 +52,27 @@ def test_batch_matches_oracle_eager_and_graph():
             np.testing.assert_array_equal(code, ocode)
 
 
+def test_batch_more_than_16_terminals():
+    """A batch mixing k = 17 and 18 instances (16-bit cells, generic merges:
+    the two-vertex merges hold Y in 16 bits) with small ones: every table,
+    record and cost equals the oracle's, eagerly and from the replay."""
+    insts = []
+    for i, (n, k) in enumerate([(24, 17), (26, 18), (200, 6), (64, 9)]):
+        e = emv.regular_graph(n, 3, 1000, 70 + i)
+        insts.append((n, e, emv.terminals(n, k, 70 + i)))
+    want = [oracle.emv(n, e, kk) for n, e, kk in insts]
+    with emv.Batch(insts) as b:
+        for rep in range(2):
+            costs, st = b.solve(emv.KEEP_TABLE)
+            print(f"batch solve {rep}: graph={st.graph_used}", flush=True)
+            for i, (n, e, kk) in enumerate(insts):
+                ocost, of, ocode = want[i]
+                assert costs[i] == ocost, i
+                f, code = b.read_rows(i, 0, 1 << len(kk), n)
+                np.testing.assert_array_equal(f, of)
+                np.testing.assert_array_equal(code, ocode)
+
+
 def test_batch_of_testset_instances():
     names = ["small-instances/B/b01", "small-instances/C/c01", "small-instances/P6E/p619",
              "small-instances/I080/i080-011", "small-instances/D/d06", "small-instances/SP/se03"]

@@ -499,6 +499,43 @@ def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags):
     np.testing.assert_array_equal(code, ocode)
 
 
+@pytest.mark.parametrize("k", [17, 19])
+def test_more_than_16_terminals_vs_oracle(gpu_ctx, k):
+    """k = 17 and 19: 16-bit cells, but the two-vertex merges' keys hold Y in
+    16 bits, so these layers take the generic merges with 64-bit packed words
+    ((sum << k) | Y no longer fits 32 bits).  Cells, records and cost equal
+    the oracle's."""
+    n = 24
+    e = emv.regular_graph(n, 3, 1000, 11)
+    kk = emv.terminals(n, k, 11)
+    print(f"case n={n} k={k}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+    check_listed_tree(Instance(n, e, kk), sol.pairs, sol.cost)
+
+
+def test_k22_certified(gpu_ctx):
+    """k = 22 (2^22 rows; the oracle would take minutes): the fixed-point
+    certificate over all 2^22 x n cells and a validated listed tree."""
+    from bigcheck import validate_tree
+
+    n, k = 24, 22
+    e = emv.regular_graph(n, 3, 1000, 12)
+    kk = emv.terminals(n, k, 12)
+    print(f"case n={n} k={k}", flush=True)
+    gpu_ctx.load_edges(n, e)
+    sol = gpu_ctx.solve(kk, emv.LIST | emv.KEEP_TABLE)
+    cert = gpu_ctx.certify()
+    print(f"certificate {cert}", flush=True)
+    assert cert.ok and cert.cells == (1 << k) * n, cert
+    ok, why = validate_tree(n, e, kk, sol.pairs, sol.cost)
+    assert ok, why
+    gpu_ctx.release_table()
+
+
 @pytest.mark.parametrize("blocks", ["1", "0"])
 def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks):
     """Group-reduced grid with weights up to 2^20: the bound needs 64-bit
