@@ -1555,6 +1555,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
             uint32_t t[8];
 #pragma unroll
             for (int j = 0; j < 8; j++) t[j] = ld_pair(pt, va[j]);
+            // top(S) lies in H, so whether the key's set flips depends on Th
+            // alone: the set of pair j is B_i ^ yl[j] with B_i per row and step
+            const bool flip = (Th & topS) != 0u;
 #pragma unroll
             for (int i = 0; i < kStarMax; i++) {
                 if (i < r) {
@@ -1563,11 +1566,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
                     uint32_t pv[8];
 #pragma unroll
                     for (int j = 0; j < 8; j++) pv[j] = ld_pair(pp, vc[j]);
+                    const uint32_t B = (flip && a[i] < topS) ? ((S | a[i]) ^ Th) : Th;
 #pragma unroll
                     for (int j = 0; j < 8; j++) {
                         const uint32_t sm = add_u16x2(t[j], pv[j]);
-                        const uint32_t Y = yset(i, Th | yl[j]);
-                        klo[i] = min(klo[i], (sm << 16) | Y);
+                        // wave-uniform set; the low half's key as a shift-add
+                        // (the shifted sum has zero low bits)
+                        const uint32_t Y = __builtin_amdgcn_readfirstlane(B ^ yl[j]);
+                        klo[i] = min(klo[i], (sm << 16) + Y);
                         khi[i] = min(khi[i], (sm & hmask) | Y);
                     }
                 }
