@@ -80,6 +80,8 @@ struct emv_ctx {
     int dense_rounds = 0;   // vertex-wise rounds of a relaxation that take every item
     bool merge_star = false;  // ... in stars of rows sharing a base set
     bool merge_cube = false;  // ... the small-subset layers from rows staged in LDS (cubes)
+    bool chunk128 = true;     // 16-bit labels staged in chunks of 128 rows (packed pairs) ...
+    int cw = 64;              // ... the current layer batch's chunk width in rows
     bool cube_attr = false;   // dynamic-LDS attribute of the cube kernel set
     std::vector<MergeCube> h_cubes;  // cubes of the last batch (kept for the async copy)
     std::vector<CubeRow> h_cube_rows;
@@ -410,13 +412,13 @@ int ensure_xs(emv_ctx *c, int64_t count) {
 // the "round has work" word, until a round flags nothing.  Rounds launched
 // after convergence return at once.
 
-template <typename DT, int DMAX>
+template <typename DT, int DMAX, bool PK = false>
 void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
     // no more workgroups than 64-item blocks of waves (small instances)
     const int64_t need = (nitems + (int64_t)kBlock - 1) / kBlock;
     int grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cu * c->relax_wg, need));
     if (c->relax_nxcd > 1) grid = (grid + c->relax_nxcd - 1) / c->relax_nxcd * c->relax_nxcd;
-    hipLaunchKernelGGL((k_relax_v<DT, DMAX>), dim3(grid), dim3(kBlock), 0, c->stream,
+    hipLaunchKernelGGL((k_relax_v<DT, DMAX, PK>), dim3(grid), dim3(kBlock), 0, c->stream,
                        (DT *)c->d_D, c->np, (int)c->n, c->d_rp, c->d_adj, c->d_ell,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1],
                        (const uint32_t *)c->d_q[r & 1], c->d_q[(r + 1) & 1], c->qcap, c->d_any,
@@ -426,6 +428,18 @@ void launch_relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
 
 template <typename DT>
 void relax_round(emv_ctx *c, int64_t nitems, int dense, int r) {
+    if constexpr (std::is_same<DT, uint16_t>::value) {
+        if (c->cw == 2 * kWave) {
+            // chunks of 128 16-bit rows: packed pairs in 32-bit words
+            switch (c->ell_deg) {
+            case 4: launch_relax_round<uint32_t, 4, true>(c, nitems, dense, r); break;
+            case 8: launch_relax_round<uint32_t, 8, true>(c, nitems, dense, r); break;
+            case 16: launch_relax_round<uint32_t, 16, true>(c, nitems, dense, r); break;
+            default: launch_relax_round<uint32_t, 0, true>(c, nitems, dense, r); break;
+            }
+            return;
+        }
+    }
     switch (c->ell_deg) {
     case 4: launch_relax_round<DT, 4>(c, nitems, dense, r); break;
     case 8: launch_relax_round<DT, 8>(c, nitems, dense, r); break;
@@ -593,16 +607,16 @@ bool use_merge_h2(const emv_ctx *c) {
     return std::is_same<T, uint16_t>::value && c->merge_h2;
 }
 
-template <typename T, typename P, typename DT, int XG>
-int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
-    const int nsub = (int)(nchunks * (kWave / XG));
+template <typename T, typename P, typename DT, int XG, int CW>
+int launch_merge_xg_cw(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
+    const int nsub = (int)(nchunks * (CW / XG));
     if constexpr (std::is_same<T, uint16_t>::value) {
         if (use_merge_h2<T>(c)) {
             const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
             int64_t per;
             const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
-            hipLaunchKernelGGL((k_merge_h2<P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream,
-                               f, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D,
+            hipLaunchKernelGGL((k_merge_h2<P, DT, XG, CW>), dim3(grid), dim3(kBlock), 0,
+                               c->stream, f, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D,
                                (P *)c->d_code, c->nxcd, per);
             HIPCHK(c, hipGetLastError());
             return EMV_OK;
@@ -611,11 +625,41 @@ int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     const int64_t ntiles = c->np / kWave;
     int64_t per;
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
-    hipLaunchKernelGGL((k_merge_v<T, P, DT, XG>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+    hipLaunchKernelGGL((k_merge_v<T, P, DT, XG, CW>), dim3(grid), dim3(kBlock), 0, c->stream, f,
                        c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
                        per, merge_fbase<T>(c));
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
+}
+
+// Chunks of 128 rows exist only for 16-bit labels with 32-bit packed words.
+template <typename P, typename DT>
+constexpr bool wide_chunks_ok() {
+    return std::is_same<DT, uint16_t>::value && std::is_same<P, uint32_t>::value;
+}
+
+template <typename T, typename P, typename DT, int XG>
+int launch_merge_xg(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
+    if constexpr (wide_chunks_ok<P, DT>())
+        if (c->cw == 2 * kWave) return launch_merge_xg_cw<T, P, DT, XG, 2 * kWave>(c, f, nx, nchunks, k);
+    return launch_merge_xg_cw<T, P, DT, XG, kWave>(c, f, nx, nchunks, k);
+}
+
+// Accumulators M -> staging for the batch's nchunks chunks.
+template <typename P, typename DT>
+void launch_pack(emv_ctx *c, int nx, int64_t nchunks, int k) {
+    const int64_t vtiles = c->np / kWave;
+    if constexpr (wide_chunks_ok<P, DT>()) {
+        if (c->cw == 2 * kWave) {
+            hipLaunchKernelGGL((k_pack_v<P, DT, 2 * kWave>), dim3((unsigned)(2 * nchunks * vtiles)),
+                               dim3(kBlock), 0, c->stream, (const P *)c->d_M, c->np, nx, k,
+                               (DT *)c->d_D, (P *)c->d_code, vtiles);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
+                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
+                       vtiles);
 }
 
 // Split merge for layers whose rows have >= kSplitPairs pairs: fill the
@@ -656,10 +700,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
                            c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
                            merge_fbase<T>(c));
     }
-    const int64_t vtiles = c->np / kWave;
-    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
-                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
-                       vtiles);
+    launch_pack<P, DT>(c, nx, nchunks, k);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
 }
@@ -672,6 +713,18 @@ bool merge_star_enabled() {
         return !(e && atoi(e) == 0);
     }();
     return on;
+}
+
+// Waves a star launch gives each tile (its stars times their parts);
+// EMV_STAR_WPT overrides 512.  At n = 2^18, k = 14 the merges take 114.6 /
+// 114.5 ms at 256 / 512 and 119.5 / 121.1 ms at 128 / 1024 (one part per
+// star: 129.4 ms).
+int64_t star_waves_per_tile() {
+    static const int64_t v = [] {
+        const char *e = getenv("EMV_STAR_WPT");
+        return e ? std::max<int64_t>(1, atoll(e)) : (int64_t)512;
+    }();
+    return v;
 }
 
 // Stars covering the batch's rows xs[0..nx) (all of m elements): each
@@ -744,18 +797,18 @@ int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
     hipLaunchKernelGGL(k_fill<P>, dim3(fb), dim3(kBlock), 0, c->stream, (P *)c->d_M, cells, ~(P)0);
     // outer steps of a star: subsets of S's high part (S has m - 1 elements)
     const int64_t steps = (m - 1 <= 3) ? 1 : (1ll << (m - 1 - 3));
+    // parts per star: about star_waves_per_tile() waves per tile, each taking
+    // at least four outer steps
+    const int64_t wpt = star_waves_per_tile();
     const int64_t parts =
-        std::max<int64_t>(1, std::min<int64_t>((512 + nstars - 1) / nstars, steps / 4));
+        std::max<int64_t>(1, std::min<int64_t>((wpt + nstars - 1) / nstars, steps / 4));
     const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
     int64_t per;
     const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
     hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
                        (const MergeStar *)c->d_stars, nstars, (int)parts, ntiles, k,
                        (P *)c->d_M, c->nxcd, per);
-    const int64_t vtiles = c->np / kWave;
-    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
-                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
-                       vtiles);
+    launch_pack<P, DT>(c, nx, nchunks, k);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
 }
@@ -934,10 +987,7 @@ int launch_merge_cube(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
                        (const MergeCube *)c->d_cubes, (const CubeRow *)c->d_cube_rows,
                        (const uint32_t *)c->d_cube_tab, ncubes, ntiles, k, (P *)c->d_M, c->nxcd,
                        per);
-    const int64_t vtiles = c->np / kWave;
-    hipLaunchKernelGGL((k_pack_v<P, DT>), dim3((unsigned)(nchunks * vtiles)), dim3(kBlock), 0,
-                       c->stream, (const P *)c->d_M, c->np, nx, k, (DT *)c->d_D, (P *)c->d_code,
-                       vtiles);
+    launch_pack<P, DT>(c, nx, nchunks, k);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
 }
@@ -1018,6 +1068,14 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
     const int64_t np = c->np;
     const int64_t n = c->n;
     const int64_t rows_per_batch = std::min<int64_t>(c->ws_chunks * kWave, c->xs_cap);
+    // 16-bit labels relaxed vertex-wise stage in chunks of 128 rows (one
+    // 256-byte gather per neighbour); everything else in chunks of 64.  A
+    // batch of rows_per_batch rows then needs at most as many bytes of staging
+    // as 64-row chunks of 8-byte cells reserve (2 + 4 bytes per cell).
+    int cw = kWave;
+    if constexpr (wide_chunks_ok<P, DT>())
+        if (c->chunk128 && !use_blocks<DT>(c)) cw = 2 * kWave;
+    c->cw = cw;
     std::vector<uint32_t> xs;
     for (int m = 2; m <= k; m++) {
         xs.clear();
@@ -1030,7 +1088,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
         }
         for (size_t off = 0; off < xs.size(); off += (size_t)rows_per_batch) {
             const int nx = (int)std::min<int64_t>(rows_per_batch, (int64_t)(xs.size() - off));
-            const int64_t nchunks = (nx + kWave - 1) / kWave;
+            const int64_t nchunks = (nx + cw - 1) / cw;
             HIPCHK(c, hipMemcpyAsync(c->d_xs, xs.data() + off, sizeof(uint32_t) * nx,
                                      hipMemcpyHostToDevice, c->stream));
             hipEvent_t e0, e1, e2, e3;
@@ -1053,7 +1111,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             HIPCHK(c, hipEventRecord(e1, c->stream));
             const int64_t vtiles = (n + kWave - 1) / kWave;
             int64_t per;
-            const int grid = grid_for(nchunks * vtiles, c->nxcd, &per);
+            const int grid = grid_for(nchunks * (cw / kWave) * vtiles, c->nxcd, &per);
             bool rel = false, copy_fit = false;
             if constexpr (sizeof(DT) == 8) {
                 if (st.rel32) CHK(relative_labels_fit<P>(c, nx, nchunks, k, &rel, &copy_fit));
@@ -1075,17 +1133,31 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
             } else {
                 CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
                 HIPCHK(c, hipEventRecord(e2, c->stream));
-                hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0,
-                                   c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
-                                   c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
-                                   vtiles, (const unsigned long long *)nullptr,
-                                   (int32_t *)nullptr, (int64_t *)nullptr);
+                bool wide = false;
+                if constexpr (wide_chunks_ok<P, DT>()) {
+                    if (cw == 2 * kWave) {
+                        hipLaunchKernelGGL((k_unpack_v<T, DT, P, 2 * kWave>), dim3(grid),
+                                           dim3(kBlock), 0, c->stream, (const DT *)c->d_D,
+                                           (const P *)c->d_code, np, c->d_xs, nx, (int)n, k,
+                                           c->d_rp, c->d_adj, f, b, c->nxcd, per, vtiles,
+                                           (const unsigned long long *)nullptr,
+                                           (int32_t *)nullptr, (int64_t *)nullptr);
+                        wide = true;
+                    }
+                }
+                if (!wide)
+                    hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0,
+                                       c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
+                                       c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd,
+                                       per, vtiles, (const unsigned long long *)nullptr,
+                                       (int32_t *)nullptr, (int64_t *)nullptr);
             }
             c->dense_rounds = 0;
             HIPCHK(c, hipGetLastError());
             HIPCHK(c, hipEventRecord(e3, c->stream));
         }
     }
+    c->cw = kWave;
     return EMV_OK;
 }
 
@@ -1543,6 +1615,12 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // 14.9 ms): a cube re-stages ~500 lower rows per tile from L2, while the
     // stars' repeated row reads hit L1
     c->merge_cube = c->merge_h2 && ((flags & EMV_MERGE_CUBE) || env_flag("EMV_CUBE"));
+    {
+        // EMV_CHUNK128=0: 16-bit labels in chunks of 64 rows
+        const char *w = getenv("EMV_CHUNK128");
+        c->chunk128 = !(w && atoi(w) == 0);
+        c->cw = kWave;
+    }
     {
         // EMV_STAR_MIN_CELLS: pair cells from which a layer batch merges in stars
         const char *e = getenv("EMV_STAR_MIN_CELLS");

@@ -108,6 +108,49 @@ __device__ __forceinline__ DT relax_min(DT du, int u, int w, DT INF, DT best) {
     return best;
 }
 
+// Packed pairs (PK): with 16-bit labels in chunks of 128 rows, one 32-bit
+// word holds the labels of rows 2j and 2j + 1 of a (chunk, vertex) item, and
+// a lane takes both: a neighbour's labels for the whole chunk are one 256-byte
+// load (random 256-byte reads run at 1.4x the bytes per second of 128-byte
+// ones), and one packed saturating add and one packed min relax two rows.
+// Every finite label + w stays below 0xFFFF (the 16-bit label bound), and
+// INF (0xFFFF) + w saturates to INF, so the halves never interact.
+typedef unsigned short emv_u16x2 __attribute__((ext_vector_type(2)));
+
+template <typename DT, bool PK>
+__device__ __forceinline__ DT relax_lbl(DT du, int u, int w, DT INF, DT best) {
+    if constexpr (PK) {
+        const emv_u16x2 ww = {(unsigned short)w, (unsigned short)w};
+        const emv_u16x2 nd = __builtin_elementwise_add_sat(__builtin_bit_cast(emv_u16x2, du), ww);
+        return __builtin_bit_cast(DT, __builtin_elementwise_min(__builtin_bit_cast(emv_u16x2, best), nd));
+    } else {
+        return relax_min(du, u, w, INF, best);
+    }
+}
+
+// Whether some label of the item dropped (labels only decrease).
+template <typename DT, bool PK>
+__device__ __forceinline__ bool lbl_dropped(DT best, DT cur) {
+    if constexpr (PK) return best != cur;
+    else return best < cur;
+}
+
+// Whether a row that dropped now improves the neighbour whose labels are d
+// (label + w below the neighbour's label).  A dropped label is finite.
+template <typename DT, bool PK>
+__device__ __forceinline__ bool lbl_improves(DT best, DT cur, int w, DT d) {
+    if constexpr (PK) {
+        const emv_u16x2 b = __builtin_bit_cast(emv_u16x2, best);
+        const emv_u16x2 c = __builtin_bit_cast(emv_u16x2, cur);
+        const emv_u16x2 dd = __builtin_bit_cast(emv_u16x2, d);
+        const emv_u16x2 ww = {(unsigned short)w, (unsigned short)w};
+        const auto m = (b < c) & (__builtin_elementwise_add_sat(b, ww) < dd);
+        return (m.x | m.y) != 0;
+    } else {
+        return best < cur && (DT)(best + (DT)w) < d;
+    }
+}
+
 // Wave-aggregated append of `entry` to a list of capacity cap (one atomic per
 // wave).  Every lane of the wave must call it.  Entries past cap are dropped;
 // the counter still counts them, which tells the next round to scan flags.
@@ -160,7 +203,7 @@ __device__ __forceinline__ void flag_neighbours(int64_t c, int64_t np, int n, in
 // stores they replace.  Round R writes flags again for round R + 1; the
 // rounds without flags leave F_cur / F_next clean, as a dense round reads
 // no flags.
-template <typename DT, int DMAX>
+template <typename DT, int DMAX, bool PK = false>
 __global__ __launch_bounds__(kBlock) void k_relax_v(
     DT *__restrict__ D, int64_t np, int n, const int32_t *__restrict__ rp,
     const Edge *__restrict__ adj, const Edge *__restrict__ ell, int64_t nitems, int dense0,
@@ -255,10 +298,11 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                 DT besta = cura, bestb = curb;
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    besta = relax_min(da[q], ua[q], wa[q], INF, besta);
-                    bestb = relax_min(db[q], ub[q], wb[q], INF, bestb);
+                    besta = relax_lbl<DT, PK>(da[q], ua[q], wa[q], INF, besta);
+                    bestb = relax_lbl<DT, PK>(db[q], ub[q], wb[q], INF, bestb);
                 }
-                const bool chga = besta < cura, chgb = two && bestb < curb;
+                const bool chga = lbl_dropped<DT, PK>(besta, cura);
+                const bool chgb = two && lbl_dropped<DT, PK>(bestb, curb);
                 if (chga) Da[ia] = besta;
                 if (chgb) Db[ib] = bestb;
                 const bool anya = __ballot(chga) != 0, anyb = __ballot(chgb) != 0;
@@ -274,8 +318,8 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                     uint32_t fa = 0, fb = 0;
 #pragma unroll
                     for (int q = 0; q < DMAX; q++) {
-                        const bool ia = chga && (DT)(besta + (DT)wa[q]) < da[q];
-                        const bool ib = chgb && (DT)(bestb + (DT)wb[q]) < db[q];
+                        const bool ia = chga && lbl_improves<DT, PK>(besta, cura, wa[q], da[q]);
+                        const bool ib = chgb && lbl_improves<DT, PK>(bestb, curb, wb[q], db[q]);
                         if (__ballot(ia) != 0) fa |= 1u << q;
                         if (__ballot(ib) != 0) fb |= 1u << q;
                     }
@@ -327,10 +371,10 @@ __global__ __launch_bounds__(kBlock) void k_relax_v(
                         }
 #pragma unroll
                         for (int q = 0; q < 4; q++)
-                            if (t0 + q < cnt_e) best = relax_min(du[q], uu[q], ww[q], INF, best);
+                            if (t0 + q < cnt_e) best = relax_lbl<DT, PK>(du[q], uu[q], ww[q], INF, best);
                     }
                 }
-                const bool dchg = best < cur;
+                const bool dchg = lbl_dropped<DT, PK>(best, cur);
                 if (dchg) Dc[idx] = best;
                 if (__ballot(dchg) != 0) {
                     if (!noflags) {
@@ -1134,7 +1178,7 @@ __device__ __forceinline__ P merge_any(const T *__restrict__ fv, const int64_t *
 // stay in that XCD's L2.  Results go straight to the vertex-major staging:
 // each lane writes its XG labels (and packed words) as one contiguous run.
 // Rows past nx (the tail of the last chunk) are written as INF.
-template <typename T, typename P, typename DT, int XG>
+template <typename T, typename P, typename DT, int XG, int CW = kWave>
 __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int64_t np,
                                                     const uint32_t *__restrict__ xs, int nx,
                                                     int nsub, int64_t ntiles, int k,
@@ -1164,10 +1208,11 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
             dv[jx] = (DT)(pv[jx] >> k);
         }
     }
-    const int chunk = sub / (kWave / XG);
-    const int loff = (sub % (kWave / XG)) * XG;
-    DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
-    P *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
+    // the XG rows of the item lie in one chunk of CW rows
+    const int chunk = sub / (CW / XG);
+    const int loff = (sub % (CW / XG)) * XG;
+    DT *dd = D + ((int64_t)chunk * np + v) * CW + loff;
+    P *cd = CODE + ((int64_t)chunk * np + v) * CW + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         dd[jx] = dv[jx];
@@ -1217,8 +1262,6 @@ __global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f,
 // half is keyed as (sum << 16) | Y (one shift-or, or one and-or for the high
 // half), which orders like (sum << k) | Y for k <= 16: the smallest sum, then
 // the smallest Y.  Half the load instructions of merge_outer per cell pair.
-typedef unsigned short emv_u16x2 __attribute__((ext_vector_type(2)));
-
 __device__ __forceinline__ uint32_t add_u16x2(uint32_t a, uint32_t c) {
     const emv_u16x2 s = __builtin_bit_cast(emv_u16x2, a) + __builtin_bit_cast(emv_u16x2, c);
     return __builtin_bit_cast(uint32_t, s);
@@ -1387,7 +1430,7 @@ __device__ __forceinline__ P key_packed(uint32_t key, int k) {
 // lane) x XG rows.  np is a multiple of 64, so in the last tile of an np
 // that is not a multiple of 128 the upper half of the lanes has no vertices:
 // they read the last pair and store nothing.
-template <typename P, typename DT, int XG>
+template <typename P, typename DT, int XG, int CW = kWave>
 __global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict__ f, int64_t np,
                                                      const uint32_t *__restrict__ xs, int nx,
                                                      int nsub, int64_t ntiles, int k,
@@ -1417,17 +1460,17 @@ __global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict_
         }
     }
     if (!live) return;
-    const int chunk = sub / (kWave / XG);
-    const int loff = (sub % (kWave / XG)) * XG;
-    DT *dd = D + ((int64_t)chunk * np + v) * kWave + loff;
-    P *cd = CODE + ((int64_t)chunk * np + v) * kWave + loff;
+    const int chunk = sub / (CW / XG);
+    const int loff = (sub % (CW / XG)) * XG;
+    DT *dd = D + ((int64_t)chunk * np + v) * CW + loff;
+    P *cd = CODE + ((int64_t)chunk * np + v) * CW + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
         const P plo = key_packed<P>(klo[jx], k), phi = key_packed<P>(khi[jx], k);
         dd[jx] = klo[jx] == 0xFFFFFFFFu ? INF : (DT)(klo[jx] >> 16);
         cd[jx] = plo;
-        dd[kWave + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
-        cd[kWave + jx] = phi;
+        dd[CW + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
+        cd[CW + jx] = phi;
     }
 }
 
@@ -1813,14 +1856,17 @@ __global__ __launch_bounds__(kCubeWaves * kWave) void k_merge_cube_h2(
 
 // Packed accumulators M[xi][v] -> vertex-major staging (label, packed word)
 // through an LDS transpose of 64 rows x 64 vertices per workgroup; rows past
-// nx are INF.
-template <typename P, typename DT>
+// nx are INF.  Workgroup = (group of 64 rows, vertex tile); a chunk of CW
+// rows holds CW / 64 groups side by side.
+template <typename P, typename DT, int CW = kWave>
 __global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int64_t np, int nx,
                                                    int k, DT *__restrict__ D,
                                                    P *__restrict__ CODE, int64_t vtiles) {
     __shared__ P sm[kWave][kWave + 1];
-    const int64_t chunk = blockIdx.x / vtiles;
-    const int64_t vt = blockIdx.x - chunk * vtiles;
+    const int64_t grp64 = blockIdx.x / vtiles;
+    const int64_t vt = blockIdx.x - grp64 * vtiles;
+    const int64_t chunk = grp64 / (CW / kWave);
+    const int64_t roff = (grp64 % (CW / kWave)) * kWave;  // the group's first row in its chunk
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int grp = t / kWave;
@@ -1828,7 +1874,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int6
 #pragma unroll
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int xl = grp + r * (kBlock / kWave);
-        const int64_t xi = chunk * kWave + xl;
+        const int64_t xi = grp64 * kWave + xl;
         sm[xl][lane] = (xi < nx) ? M[xi * np + v0 + lane] : ~(P)0;
     }
     __syncthreads();
@@ -1836,8 +1882,8 @@ __global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int6
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int vv = grp + r * (kBlock / kWave);
         const P a = sm[lane][vv];
-        const int64_t idx = (chunk * np + v0 + vv) * kWave + lane;
-        const bool real = (chunk * kWave + lane) < nx;
+        const int64_t idx = (chunk * np + v0 + vv) * CW + roff + lane;
+        const bool real = (grp64 * kWave + lane) < nx;
         D[idx] = real ? (DT)(a >> k) : ~(DT)0;
         CODE[idx] = real ? a : ~(P)0;
     }
@@ -1849,7 +1895,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int6
 // (packed word pk = (sum << k) | Y), else 2^k + the smallest neighbour u with
 // label(u) + w == label(v).  Neighbour entries are read as wave-uniform
 // (scalar) loads, eight gathers in flight.
-template <typename DT, typename P>
+template <typename DT, typename P, int CW = kWave>
 __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, P pk, int k,
                                                bool real, int v, const int32_t *__restrict__ rp,
                                                const Edge *__restrict__ adj) {
@@ -1868,7 +1914,7 @@ __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, 
             const int e = min(eb + q, e1 - 1);
             uu[q] = adj[e].v;
             ww[q] = adj[e].w;
-            du[q] = Dc[(int64_t)uu[q] * kWave + lane];
+            du[q] = Dc[(int64_t)uu[q] * CW + lane];
         }
 #pragma unroll
         for (int q = 0; q < 8; q++)
@@ -1883,9 +1929,10 @@ __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, 
 // ----------------------------------------------------------------- unpack
 // Vertex-major staging -> table rows f[X], b[X] through an LDS transpose of
 // 64 vertices x 64 rows per workgroup; each wave decides the records of 16
-// vertices (cell_record) on the way into LDS.
+// vertices (cell_record) on the way into LDS.  Item = (group of 64 rows,
+// vertex tile); a chunk of CW rows holds CW / 64 groups side by side.
 // rspan (relative labels, else null): row xi's labels are f - rspan[2 xi].
-template <typename T, typename DT, typename P>
+template <typename T, typename DT, typename P, int CW = kWave>
 __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      const P *__restrict__ CODE, int64_t np,
                                                      const uint32_t *__restrict__ xs, int nx,
@@ -1900,17 +1947,19 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
     __shared__ int32_t sc[kWave][kWave + 1];
     if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
     const int64_t item = xcd_item(nxcd, per_xcd);
-    const int64_t chunk = item / vtiles;
-    const int64_t vt = item - chunk * vtiles;
-    if (chunk * kWave >= nx) return;
+    const int64_t g64 = item / vtiles;
+    const int64_t vt = item - g64 * vtiles;
+    if (g64 * kWave >= nx) return;
+    const int64_t chunk = g64 / (CW / kWave);
+    const int64_t roff = (g64 % (CW / kWave)) * kWave;  // the group's first row in its chunk
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
     const int64_t v0 = vt * kWave;
-    const DT *Dc = D + chunk * np * kWave;
-    const bool real = chunk * kWave + lane < nx;
+    const DT *Dc = D + chunk * np * CW + roff;
+    const bool real = g64 * kWave + lane < nx;
     // relative labels: compare with the merged value less the row's base
-    const P pbase = (rspan && real) ? ((P)rspan[2 * (chunk * kWave + lane)] << k) : (P)0;
+    const P pbase = (rspan && real) ? ((P)rspan[2 * (g64 * kWave + lane)] << k) : (P)0;
     // the labels and packed words of kPre vertices are loaded together, then
     // their records decided one vertex at a time
     constexpr int kPre = 4;
@@ -1921,7 +1970,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
 #pragma unroll
         for (int u = 0; u < kPre; u++) {
             const int64_t idx =
-                ((chunk * np) + v0 + grp + (r0 + u) * (kBlock / kWave)) * kWave + lane;
+                ((chunk * np) + v0 + grp + (r0 + u) * (kBlock / kWave)) * CW + roff + lane;
             Ls[u] = D[idx];
             Ps[u] = CODE[idx];
         }
@@ -1931,7 +1980,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
             const int64_t v = v0 + vv;
             int32_t code = 0;
             if (v < n)
-                code = cell_record<DT, P>(Dc, Ls[u], Ps[u] - pbase, k, real, (int)v, rp, adj);
+                code = cell_record<DT, P, CW>(Dc, Ls[u], Ps[u] - pbase, k, real, (int)v, rp, adj);
             sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }
@@ -1940,7 +1989,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
 #pragma unroll
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int xl = grp + r * (kBlock / kWave);
-        const int64_t xi = chunk * kWave + xl;
+        const int64_t xi = g64 * kWave + xl;
         const int64_t v = v0 + lane;
         if (xi < nx && v < n) {
             const int64_t X = xs[xi];
