@@ -1135,7 +1135,18 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 HIPCHK(c, hipEventRecord(e2, c->stream));
                 bool wide = false;
                 if constexpr (wide_chunks_ok<P, DT>()) {
-                    if (cw == 2 * kWave) {
+                    // (32-bit packed words imply a table of at most 4-byte cells)
+                    if (cw == 2 * kWave && sizeof(T) <= 4) {
+                        // one workgroup per (chunk, vertex tile)
+                        int64_t wper;
+                        const int wgrid = grid_for(nchunks * vtiles, c->nxcd, &wper);
+                        hipLaunchKernelGGL((k_unpack_w<T, P>), dim3(wgrid),
+                                           dim3(kWideWaves * kWave), 0, c->stream,
+                                           (const uint32_t *)c->d_D, (const P *)c->d_code, np,
+                                           c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b,
+                                           c->nxcd, wper, vtiles);
+                        wide = true;
+                    } else if (cw == 2 * kWave) {
                         hipLaunchKernelGGL((k_unpack_v<T, DT, P, 2 * kWave>), dim3(grid),
                                            dim3(kBlock), 0, c->stream, (const DT *)c->d_D,
                                            (const P *)c->d_code, np, c->d_xs, nx, (int)n, k,

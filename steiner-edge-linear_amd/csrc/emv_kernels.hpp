@@ -2006,6 +2006,119 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
     }
 }
 
+// ------------------------------------------- unpack, chunks of 128 16-bit rows
+// k_unpack_v for 16-bit labels staged in chunks of 128 rows: a wave takes all
+// 128 rows of a vertex, two per lane (the 32-bit word of rows 2l, 2l + 1), so
+// the neighbour labels an edge record looks for are one 256-byte read per
+// neighbour.  Item = (chunk, 64-vertex tile), one workgroup of kWideWaves
+// waves, each deciding the records of 64 / kWideWaves vertices; then the
+// workgroup writes the chunk's rows of the tile's vertices through LDS.
+constexpr int kWideWaves = 8;
+
+// Records of rows 2l (lo) and 2l + 1 (hi) of a converged word L at vertex v,
+// the merged packed words p0, p1: the rule of cell_record, for two rows.
+template <typename P>
+__device__ __forceinline__ void cell_record2(const uint32_t *__restrict__ Dw, uint32_t L, P p0,
+                                             P p1, int k, bool real0, bool real1, int v,
+                                             const int32_t *__restrict__ rp,
+                                             const Edge *__restrict__ adj, int32_t &code0,
+                                             int32_t &code1) {
+    const int lane = __lane_id();
+    const uint32_t L0 = L & 0xFFFFu, L1 = L >> 16;
+    const bool edge0 = real0 && (P)L0 < (p0 >> k);
+    const bool edge1 = real1 && (P)L1 < (p1 >> k);
+    code0 = (int32_t)(p0 & (((P)1 << k) - 1));
+    code1 = (int32_t)(p1 & (((P)1 << k) - 1));
+    if (__ballot(edge0 || edge1) == 0) return;
+    uint32_t bu0 = 0xFFFFFFFFu, bu1 = 0xFFFFFFFFu;
+    const int e0 = rp[v], e1 = rp[v + 1];
+    for (int eb = e0; eb < e1; eb += 8) {
+        int uu[8], ww[8];
+        uint32_t du[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = min(eb + q, e1 - 1);
+            uu[q] = adj[e].v;
+            ww[q] = adj[e].w;
+            du[q] = Dw[(int64_t)uu[q] * kWave + lane];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (eb + q >= e1) continue;
+            const uint32_t d0 = du[q] & 0xFFFFu, d1 = du[q] >> 16;
+            // INF stays out: a finite label + w is below 0xFFFF
+            if (d0 != 0xFFFFu && d0 + (uint32_t)ww[q] == L0 && (uint32_t)uu[q] < bu0) bu0 = (uint32_t)uu[q];
+            if (d1 != 0xFFFFu && d1 + (uint32_t)ww[q] == L1 && (uint32_t)uu[q] < bu1) bu1 = (uint32_t)uu[q];
+        }
+    }
+    if (edge0) code0 = (int32_t)((1u << k) + bu0);
+    if (edge1) code1 = (int32_t)((1u << k) + bu1);
+}
+
+template <typename T, typename P>
+__global__ __launch_bounds__(kWideWaves * kWave) void k_unpack_w(
+    const uint32_t *__restrict__ D, const P *__restrict__ CODE, int64_t np,
+    const uint32_t *__restrict__ xs, int nx, int n, int k, const int32_t *__restrict__ rp,
+    const Edge *__restrict__ adj, T *__restrict__ f, int32_t *__restrict__ b, int nxcd,
+    int64_t per_xcd, int64_t vtiles) {
+    constexpr int CW = 2 * kWave;
+    // [vertex][row]; odd row strides (in 32-bit words) keep the column reads
+    // of the write-out free of bank conflicts
+    __shared__ uint32_t sd[kWave][kWave + 1];  // the 128 labels of a vertex as 64 words
+    __shared__ int32_t sc[kWave][CW + 1];
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd);
+    const int64_t chunk = item / vtiles;
+    const int64_t vt = item - chunk * vtiles;
+    if (chunk * CW >= nx) return;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(t / kWave);
+    const int64_t v0 = vt * kWave;
+    const uint32_t *Dw = D + chunk * np * kWave;
+    const bool real0 = chunk * CW + 2 * lane < nx;
+    const bool real1 = chunk * CW + 2 * lane + 1 < nx;
+    constexpr int kPer = kWave / kWideWaves;  // vertices per wave
+    constexpr int kPre = 4;                   // loaded together
+#pragma unroll 1
+    for (int r0 = 0; r0 < kPer; r0 += kPre) {
+        uint32_t Ls[kPre];
+        P P0[kPre], P1[kPre];
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int64_t vv = v0 + wv + (r0 + u) * kWideWaves;
+            Ls[u] = Dw[vv * kWave + lane];
+            const P *cp = CODE + (chunk * np + vv) * CW + 2 * lane;
+            P0[u] = cp[0];
+            P1[u] = cp[1];
+        }
+#pragma unroll
+        for (int u = 0; u < kPre; u++) {
+            const int vl = wv + (r0 + u) * kWideWaves;
+            const int64_t v = v0 + vl;
+            int32_t c0 = 0, c1 = 0;
+            if (v < n)
+                cell_record2<P>(Dw, Ls[u], P0[u], P1[u], k, real0, real1, (int)v, rp, adj, c0, c1);
+            sd[vl][lane] = Ls[u];
+            sc[vl][2 * lane] = c0;
+            sc[vl][2 * lane + 1] = c1;
+        }
+    }
+    __syncthreads();
+    // write-out: row xl of the chunk, lane = vertex
+    const int64_t v = v0 + lane;
+#pragma unroll 1
+    for (int xl = wv; xl < CW; xl += kWideWaves) {
+        const int64_t xi = chunk * CW + xl;
+        if (xi < nx && v < n) {
+            const int64_t X = xs[xi];
+            const uint32_t w = sd[lane][xl >> 1];
+            f[X * np + v] = (T)((xl & 1) ? (w >> 16) : (w & 0xFFFFu));
+            b[X * np + v] = sc[lane][xl];
+        }
+    }
+}
+
 // Compact copy of the singleton rows 2^t (base 0): int32 distances.
 template <typename CT>
 __global__ void k_copy_singletons(const int64_t *__restrict__ f, int64_t np, int n, int k,

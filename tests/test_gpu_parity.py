@@ -360,22 +360,23 @@ def test_csr_and_ell_paths_agree(gpu_ctx, monkeypatch):
 
 
 @pytest.mark.parametrize("chunk128", ["1", "0"])
-@pytest.mark.parametrize("ell", [True, False])
+@pytest.mark.parametrize("ell,flags", [(True, 0), (False, 0), (True, emv.TABLE32)])
 @pytest.mark.parametrize("n,d,k", [(2999, 8, 10), (1234, 5, 8), (778, 3, 3), (4100, 7, 9)])
-def test_chunk_widths_vs_oracle(gpu_ctx, monkeypatch, n, d, k, ell, chunk128):
+def test_chunk_widths_vs_oracle(gpu_ctx, monkeypatch, n, d, k, ell, flags, chunk128):
     """16-bit labels relax vertex-wise in chunks of 128 rows (two labels per
-    32-bit word, packed saturating add and min) or, with EMV_CHUNK128=0, in
-    chunks of 64; both adjacency paths; layers whose last chunk is partly
-    filled (k = 8: 70 rows, k = 10: 252) and k = 3.  Cells, records and cost
-    equal the oracle's."""
+    32-bit word, packed saturating add and min; records two rows per lane)
+    or, with EMV_CHUNK128=0, in chunks of 64; both adjacency paths; 16-bit
+    and int32 tables; layers whose last chunk is partly filled (k = 8: 70
+    rows, k = 10: 252) and k = 3.  Cells, records and cost equal the
+    oracle's."""
     monkeypatch.setenv("EMV_CHUNK128", chunk128)
     if not ell:
         monkeypatch.setenv("EMV_NO_ELL", "1")
     e = emv.regular_graph(n, d, 1000, 900 + k)
     kk = emv.terminals(n, k, 900 + k)
     inst = Instance(n, e, kk)
-    print(f"case chunks n={n} d={d} k={k} ell={ell} chunk128={chunk128}", flush=True)
-    sol, f, code = solve_full(gpu_ctx, inst)
+    print(f"case chunks n={n} d={d} k={k} ell={ell} flags={flags} chunk128={chunk128}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst, flags)
     assert sol.label_bits == 16
     ocost, of, ocode = oracle.emv(n, e, kk)
     assert sol.cost == ocost
