@@ -80,6 +80,7 @@ struct emv_ctx {
     int dense_rounds = 0;   // vertex-wise rounds of a relaxation that take every item
     bool merge_star = false;  // ... in stars of rows sharing a base set
     bool merge_cube = false;  // ... the small-subset layers from rows staged in LDS (cubes)
+    bool merge_star32 = true; // 32-bit cells (int32 table, compact copy) merge in stars too
     bool chunk128 = true;     // 16-bit labels staged in chunks of 128 rows (packed pairs) ...
     int cw = 64;              // ... the current layer batch's chunk width in rows
     bool cube_attr = false;   // dynamic-LDS attribute of the cube kernel set
@@ -771,8 +772,8 @@ void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out)
     }
 }
 
-template <typename P, typename DT>
-int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, int k, int m) {
+template <typename T, typename P, typename DT>
+int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
     build_stars(c->h_xs, nx, k, c->h_stars);
     const int nstars = (int)c->h_stars.size();
     if (c->stars_cap < (size_t)nstars) {
@@ -802,12 +803,27 @@ int launch_merge_star(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
     const int64_t wpt = star_waves_per_tile();
     const int64_t parts =
         std::max<int64_t>(1, std::min<int64_t>((wpt + nstars - 1) / nstars, steps / 4));
-    const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
     int64_t per;
-    const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
-    hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
-                       (const MergeStar *)c->d_stars, nstars, (int)parts, ntiles, k,
-                       (P *)c->d_M, c->nxcd, per);
+    if constexpr (std::is_same<T, uint16_t>::value) {
+        const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+        const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
+        hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
+                           (const MergeStar *)c->d_stars, nstars, (int)parts, ntiles, k,
+                           (P *)c->d_M, c->nxcd, per);
+    } else {
+        // 32-bit cells (int32 table or compact copy): 64-vertex tiles
+        const int64_t ntiles = c->np / kWave;
+        const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
+        const int64_t *fb = merge_fbase<T>(c);
+        if (fb)
+            hipLaunchKernelGGL((k_merge_star_w<P, true>), dim3(grid), dim3(kBlock), 0, c->stream,
+                               f, fb, c->np, (const MergeStar *)c->d_stars, nstars, (int)parts,
+                               ntiles, k, (P *)c->d_M, c->nxcd, per);
+        else
+            hipLaunchKernelGGL((k_merge_star_w<P, false>), dim3(grid), dim3(kBlock), 0, c->stream,
+                               f, fb, c->np, (const MergeStar *)c->d_stars, nstars, (int)parts,
+                               ntiles, k, (P *)c->d_M, c->nxcd, per);
+    }
     launch_pack<P, DT>(c, nx, nchunks, k);
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
@@ -1003,7 +1019,13 @@ int launch_merge(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) 
         // from 7 elements on: layers 4-6 merged whole were 4-5 ms faster)
         const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
         if (use_merge_h2<T>(c) && c->merge_star && c->h_xs && pair_cells >= c->star_min_cells)
-            return launch_merge_star<P, DT>(c, f, nx, nchunks, k, m);
+            return launch_merge_star<T, P, DT>(c, f, nx, nchunks, k, m);
+    }
+    if constexpr (std::is_same<T, int32_t>::value) {
+        // int32 table or the compact copy of an int64 table
+        const double pair_cells = (double)nx * (double)((1u << (m - 1)) - 1u) * (double)c->n;
+        if (c->merge_star32 && c->h_xs && pair_cells >= c->star_min_cells)
+            return launch_merge_star<T, P, DT>(c, f, nx, nchunks, k, m);
     }
     if (((1u << (m - 1)) - 1u) >= split_pairs())
         return launch_merge_split<T, P, DT>(c, f, nx, nchunks, k, m);
@@ -1627,6 +1649,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     // stars' repeated row reads hit L1
     c->merge_cube = c->merge_h2 && ((flags & EMV_MERGE_CUBE) || env_flag("EMV_CUBE"));
     {
+        // EMV_STAR32=0: 32-bit cells merge row by row
+        const char *s32 = getenv("EMV_STAR32");
+        c->merge_star32 = !(flags & EMV_MERGE_ROWS) && !(s32 && atoi(s32) == 0);
         // EMV_CHUNK128=0: 16-bit labels in chunks of 64 rows
         const char *w = getenv("EMV_CHUNK128");
         c->chunk128 = !(w && atoi(w) == 0);

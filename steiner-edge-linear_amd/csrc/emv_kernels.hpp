@@ -1672,6 +1672,149 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
     }
 }
 
+// --------------------------------------------- merge by stars (32-bit cells)
+// The stars of k_merge_star_h2 over an int32 table or over the compact int32
+// copy of an int64 table (REL: a row holds f - fbase[row], and a pair's sum
+// adds both rows' bases back), one vertex per lane on 64-vertex tiles (256 B
+// per row).  The lane offsets of the low rows would not fit 32 bits with
+// 4-byte cells, so every row is read from a scalar 64-bit base (the row's
+// tile segment) at the lane's fixed offset.  The loop body is instantiated per
+// star size R, so that the loads of all R + 1 rows of a step are in flight
+// together.
+template <typename P, bool REL, int R>
+__device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
+                                            const int64_t *__restrict__ fbase, int64_t np,
+                                            uint32_t S, const uint32_t *a, uint32_t o0,
+                                            uint32_t o1, int k, P *acc) {
+    const int lane = __lane_id();
+    const uint32_t topS = 1u << (31 - __clz(S));
+    uint32_t L = 0, H = S;
+    for (int q = 0; q < 3 && H; q++) {
+        const uint32_t lo = H & (~H + 1u);
+        L |= lo;
+        H ^= lo;
+    }
+    const uint32_t nH = 1u << __popc(H);
+    // (sum of the two cells, with their rows' bases) << k | Y
+    auto key = [&](int32_t x, int32_t y, int64_t b, uint32_t Y) -> P {
+        P sum = (P)(uint32_t)x + (P)(uint32_t)y;
+        if constexpr (REL) sum += (P)b;
+        return (sum << k) | (P)Y;
+    };
+    if (__popc(S) >= 4) {
+        uint32_t yl[8];
+        low_subsets(L, yl);
+        uint32_t Th = deposit_bits(nH - 1u - o0, H);
+        const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Th == 0 last, apart
+        for (uint32_t o = o0; o < oend; o++) {
+            const int32_t *tb = fv0 + (int64_t)Th * np;
+            int32_t t[8], pv[R][8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) t[j] = tb[(int64_t)yl[j] * np + lane];
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const int32_t *pb = fv0 + (int64_t)((H ^ Th) | a[i]) * np;
+#pragma unroll
+                for (int j = 0; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * np + lane];
+            }
+            const bool flip = (Th & topS) != 0u;
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t B = (flip && a[i] < topS) ? ((S | a[i]) ^ Th) : Th;
+                const uint32_t Pi = (H ^ Th) | a[i];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    int64_t b = 0;
+                    if constexpr (REL) b = fbase[Th | yl[j]] + fbase[Pi | (L - yl[j])];
+                    const P s2 = key(t[j], pv[i][j], b, B ^ yl[j]);
+                    acc[i] = s2 < acc[i] ? s2 : acc[i];
+                }
+            }
+            Th = (Th - 1u) & H;
+        }
+        if (o1 == nH) {  // Th == 0: the seven nonempty subsets of L (topS lies in H: Y = T)
+            int32_t t[8], pv[R][8];
+#pragma unroll
+            for (int j = 1; j < 8; j++) t[j] = fv0[(int64_t)yl[j] * np + lane];
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const int32_t *pb = fv0 + (int64_t)(H | a[i]) * np;
+#pragma unroll
+                for (int j = 1; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * np + lane];
+            }
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+#pragma unroll
+                for (int j = 1; j < 8; j++) {
+                    int64_t b = 0;
+                    if constexpr (REL) b = fbase[yl[j]] + fbase[(H | a[i]) | (L - yl[j])];
+                    const P s2 = key(t[j], pv[i][j], b, yl[j]);
+                    acc[i] = s2 < acc[i] ? s2 : acc[i];
+                }
+            }
+        }
+    } else {
+        // |S| <= 3: one outer step over the nonempty subsets of S
+        for (uint32_t T = S; T != 0; T = (T - 1u) & S) {
+            const int32_t t = fv0[(int64_t)T * np + lane];
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                const uint32_t Pr = (S ^ T) | a[i];
+                int64_t b = 0;
+                if constexpr (REL) b = fbase[T] + fbase[Pr];
+                const uint32_t Y = (a[i] < topS && (T & topS)) ? ((S | a[i]) ^ T) : T;
+                const P s2 = key(t, fv0[(int64_t)Pr * np + lane], b, Y);
+                acc[i] = s2 < acc[i] ? s2 : acc[i];
+            }
+        }
+    }
+}
+
+// Item = (64-vertex tile, star, part), tile-major per XCD; parts fold into M
+// with unsigned atomicMin.  np is a multiple of 64, so every lane's vertex
+// lies inside the rows (padding columns feed only dropped results).
+template <typename P, bool REL>
+__global__ __launch_bounds__(kBlock) void k_merge_star_w(
+    const int32_t *__restrict__ f, const int64_t *__restrict__ fbase, int64_t np,
+    const MergeStar *__restrict__ stars, int nstars, int parts, int64_t ntiles, int k,
+    P *__restrict__ M, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)nstars * parts;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int g = rem / parts;
+    const int part = rem - g * parts;
+    const uint32_t S = stars[g].S;
+    const int r = (int)stars[g].r;
+    uint32_t a[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) a[i] = stars[g].a[i];
+    const uint32_t steps = star_steps(S);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / parts);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * kWave;
+    const int32_t *fv0 = f + v0;
+    P acc[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) acc[i] = ~(P)0;
+    static_assert(kStarMax == 4, "one instantiation per star size");
+    switch (r) {
+    case 1: starw_pairs<P, REL, 1>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
+    case 2: starw_pairs<P, REL, 2>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
+    case 3: starw_pairs<P, REL, 3>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
+    default: starw_pairs<P, REL, 4>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
+    }
+    const int64_t v = v0 + lane;
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++)
+        if (i < r) atomicMin(&M[(int64_t)stars[g].xi[i] * np + v], acc[i]);
+}
+
 // ------------------------------------------------------- cube merge (LDS)
 // Layers of small subsets (16-bit cells, two vertices per lane): a workgroup
 // takes one 128-vertex tile and one cube U of at most kCubeU elements.  It

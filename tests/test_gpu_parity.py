@@ -504,6 +504,45 @@ def test_star_merges_bit_exact(gpu_ctx, monkeypatch, name):
     np.testing.assert_array_equal(code, c2)
 
 
+@pytest.mark.parametrize("flags", [emv.TABLE32, emv.TABLE32 | emv.FORCE_P64,
+                                   emv.TABLE32 | emv.WIDE_LABELS])
+@pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
+def test_star_merges_32bit_cells(gpu_ctx, monkeypatch, name, flags):
+    """Stars of rows sharing a base set on an int32 table (forced for small
+    layers), with 32- and 64-bit packed words: the reference's table and
+    record kinds, as row by row.  (The compact int32 copy of an int64 table
+    takes the same stars with row bases: the group-instance test.)"""
+    monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    inst = load_instance(name)
+    print(f"case stars32 {name} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, inst, flags)
+    assert sol.table_bytes == 4
+    assert digest(f) == META[name]["f_sha256"]
+    assert digest(kinds(code, inst.k)) == META[name]["kind_sha256"]
+    s2, f2, c2 = solve_full(gpu_ctx, inst, flags | emv.MERGE_ROWS)
+    np.testing.assert_array_equal(f, f2)
+    np.testing.assert_array_equal(code, c2)
+
+
+@pytest.mark.parametrize("flags", [emv.TABLE32, emv.TABLE32 | emv.FORCE_P64])
+@pytest.mark.parametrize("n,d,k", [(3001, 6, 11), (2000, 8, 9), (230, 3, 13)])
+def test_star_merges_32bit_cells_vs_oracle(gpu_ctx, monkeypatch, n, d, k, flags):
+    """32-bit-cell stars on random regular graphs (k = 9 .. 13: stars of up
+    to four rows, parts over the outer steps): cells, records and cost equal
+    the oracle's."""
+    if n * d % 2:
+        n += 1
+    monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    e = emv.regular_graph(n, d, 1000, 31 + k)
+    kk = emv.terminals(n, k, 31 + k)
+    print(f"case stars32 n={n} d={d} k={k} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
 @pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16), (230, 3, 13)])
 @pytest.mark.parametrize("flags", [0, emv.MERGE_ROWS, emv.MERGE_1V])
 def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags):
@@ -561,18 +600,22 @@ def test_k22_certified(gpu_ctx):
     gpu_ctx.release_table()
 
 
+@pytest.mark.parametrize("stars", ["0", "1"])
 @pytest.mark.parametrize("blocks", ["1", "0"])
-def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks):
+def test_relative_32bit_labels_on_group_instance(gpu_ctx, monkeypatch, blocks, stars):
     """Group-reduced grid with weights up to 2^20: the bound needs 64-bit
     labels (M is about 9e8), but every row's merged values span less than
     2^32, so the layers relax on 32-bit labels relative to each row's
-    minimum, and merge from the int32 copy of the rows (cell - row base).
-    Cells, records and cost equal the oracle's, with blocks and with
-    vertex-wise rounds, and with 64-bit labels forced (FORCE_P64)."""
+    minimum, and merge from the int32 copy of the rows (cell - row base),
+    row by row or in stars (forced for these small layers).  Cells, records
+    and cost equal the oracle's, with blocks and with vertex-wise rounds,
+    and with 64-bit labels forced (FORCE_P64)."""
     import workloads
 
     n, e, grp, nn, red, kk, M = workloads.group_instance(30, 30, 5, 6, 1 << 20, 35)
     monkeypatch.setenv("EMV_BLOCKS", blocks)
+    if stars == "1":
+        monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
     inst = Instance(nn, red, kk)
     print(f"case relative labels group grid nn={nn} k={len(kk)} M={M} blocks={blocks}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst)
