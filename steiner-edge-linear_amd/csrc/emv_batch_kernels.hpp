@@ -28,11 +28,19 @@
 
 namespace emv {
 
-// One staging chunk: up to 64 rows of one instance.
+// One staging chunk: up to 64 rows of one instance.  Its packed words sit at
+// CODE[dbase + v * 64 + lane]; its labels at ((DT *)(D + 4 * lbase))[v *
+// lstride + lane].  lbase == dbase and lstride == 64 but for paired chunks
+// (batched solves, 16-bit labels, vertex-wise rounds): chunks 2p and 2p + 1
+// of an instance's layer keep their labels side by side, 128 rows per vertex
+// in the first chunk's label region (lstride 128, the second at + 64 rows),
+// so that the pair relaxes as one 128-row item whose neighbour reads are
+// 256-byte blocks.
 struct GChunk {
     int64_t dbase;  // cell offset of the chunk in the staging (np * 64 cells)
     int64_t fbase;  // offset of the chunk's activity flags (np words)
     int64_t tbase;  // element offset of the instance's table
+    int64_t lbase;  // cell offset of the chunk's labels
     int32_t n, np;  // instance vertices, table row stride
     int32_t voff;   // first vertex of the instance's graph in the shared adjacency
     int32_t k;      // instance terminals; codes of edge records start at 2^k
@@ -44,6 +52,7 @@ struct GChunk {
     int32_t b0;     // block relaxation: first block of the graph in the shared block arrays
     int32_t nblk;   //   and its blocks (0: vertex-wise relaxation)
     int32_t tbits;  // table cells: 16 or 32 bits
+    int32_t lstride;  // labels per vertex: 64, or 128 for paired chunks
 };
 
 // One instance at a layer: its chunks are consecutive, its merge items are
@@ -62,7 +71,19 @@ struct GMerge {
     int64_t mbase;   // split merge: accumulator of row xi at Macc[mbase + xi * np + v]
     int32_t h2;      // 16-bit cells merged two vertices per lane (128-vertex merge tiles)
     int32_t mtiles;  // merge tiles: np / 64, or ceil(np / 128) when h2
+    int32_t lpair;   // chunks paired (an even count): labels as GChunk.lbase / lstride
+    int32_t pad_;
 };
+
+// Label base (cells) and labels per vertex of chunk q of a GMerge.
+__device__ __forceinline__ int64_t merge_lbase(const GMerge &M, int64_t q) {
+    const int64_t own = M.dbase0 + q * M.np * kWave;
+    return M.lpair ? own - (q & 1) * M.np * kWave + (q & 1) * (kWave / 2) : own;
+}
+
+__device__ __forceinline__ int merge_lstride(const GMerge &M) {
+    return M.lpair ? 2 * kWave : kWave;
+}
 
 template <typename DT>
 __device__ __forceinline__ DT *chunk_labels(void *D, int64_t dbase) {
@@ -190,14 +211,15 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
         const int chunk = sub / (kWave / XG);
         const int loff = (sub % (kWave / XG)) * XG;
         const int64_t dbase = M.dbase0 + (int64_t)chunk * M.np * kWave;
-        DT *dd = chunk_labels<DT>(D, dbase) + v * kWave + loff;
+        const int ls = merge_lstride(M);
+        DT *dd = chunk_labels<DT>(D, merge_lbase(M, chunk)) + v * ls + loff;
         P *cd = CODE + dbase + v * kWave + loff;
         const DT INF = (DT)~(DT)0;
 #pragma unroll
         for (int jx = 0; jx < XG; jx++) {
             dd[jx] = klo[jx] == 0xFFFFFFFFu ? INF : (DT)(klo[jx] >> 16);
             cd[jx] = key_packed<P>(klo[jx], k);
-            dd[kWave + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
+            dd[ls + jx] = khi[jx] == 0xFFFFFFFFu ? INF : (DT)(khi[jx] >> 16);
             cd[kWave + jx] = key_packed<P>(khi[jx], k);
         }
         return;
@@ -220,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_gmerge(const int32_t *__restrict__ f
     const int chunk = sub / (kWave / XG);
     const int loff = (sub % (kWave / XG)) * XG;
     const int64_t dbase = M.dbase0 + (int64_t)chunk * M.np * kWave;
-    DT *dd = chunk_labels<DT>(D, dbase) + v * kWave + loff;
+    DT *dd = chunk_labels<DT>(D, merge_lbase(M, chunk)) + v * merge_lstride(M) + loff;
     P *cd = CODE + dbase + v * kWave + loff;
 #pragma unroll
     for (int jx = 0; jx < XG; jx++) {
@@ -301,14 +323,15 @@ __global__ __launch_bounds__(kBlock) void k_gpack(const P *__restrict__ Macc,
     }
     __syncthreads();
     const int64_t dbase = M.dbase0 + chunk * M.np * kWave;
-    DT *Dd = chunk_labels<DT>(D, dbase);
+    DT *Dd = chunk_labels<DT>(D, merge_lbase(M, chunk));
+    const int ls = merge_lstride(M);
     const bool real = chunk * kWave + lane < M.nrows;
 #pragma unroll
     for (int r = 0; r < kWave / (kBlock / kWave); r++) {
         const int vv = grp + r * (kBlock / kWave);
         const P a = sm[lane][vv];
         const int64_t idx = (v0 + vv) * kWave + lane;
-        Dd[idx] = real ? (DT)(a >> M.k) : (DT)~(DT)0;
+        Dd[(v0 + vv) * ls + lane] = real ? (DT)(a >> M.k) : (DT)~(DT)0;
         CODE[dbase + idx] = real ? a : ~(P)0;
     }
 }
@@ -433,7 +456,7 @@ __device__ __forceinline__ void gflag(const GChunk &c, int64_t base_item, int nb
 // chunk's instance; blk_chunk[b] names the chunk of item 64 * b, so a wave
 // finds its items' chunks with one load.  Adjacency is the shared padded fixed-degree copy
 // (DMAX > 0, neighbour ids local to the instance's graph) or the shared CSR.
-template <typename DT, int DMAX>
+template <typename DT, int DMAX, bool PK = false>
 __global__ __launch_bounds__(kBlock) void k_grelax(
     void *__restrict__ D, const GChunk *__restrict__ ch,
     const int64_t *__restrict__ item_pre, const int32_t *__restrict__ blk_chunk, int nch, const int32_t *__restrict__ rp,
@@ -521,8 +544,8 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const int vb = __builtin_amdgcn_readlane(mv, t2);
                 const GChunk c1 = ch[ga];
                 const GChunk c2 = ch[gb];
-                DT *Da = chunk_labels<DT>(D, c1.dbase);
-                DT *Db = chunk_labels<DT>(D, c2.dbase);
+                DT *Da = chunk_labels<DT>(D, c1.lbase);
+                DT *Db = chunk_labels<DT>(D, c2.lbase);
                 const int64_t ia = (int64_t)va * kWave + lane, ib = (int64_t)vb * kWave + lane;
                 const DT cura = Da[ia];
                 const DT curb = two ? Db[ib] : INF;
@@ -543,10 +566,11 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 DT besta = cura, bestb = curb;
 #pragma unroll
                 for (int q = 0; q < DMAX; q++) {
-                    besta = relax_min(da[q], ua[q], wa[q], INF, besta);
-                    bestb = relax_min(db[q], ub[q], wb[q], INF, bestb);
+                    besta = relax_lbl<DT, PK>(da[q], ua[q], wa[q], INF, besta);
+                    bestb = relax_lbl<DT, PK>(db[q], ub[q], wb[q], INF, bestb);
                 }
-                const bool chga = besta < cura, chgb = two && bestb < curb;
+                const bool chga = lbl_dropped<DT, PK>(besta, cura);
+                const bool chgb = two && lbl_dropped<DT, PK>(bestb, curb);
                 if (chga) Da[ia] = besta;
                 if (chgb) Db[ib] = bestb;
                 if (noflags) {
@@ -557,8 +581,8 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     int na = -1, nb = -1;
 #pragma unroll
                     for (int q = 0; q < DMAX; q++) {
-                        const bool ia2 = chga && (DT)(besta + (DT)wa[q]) < da[q];
-                        const bool ib2 = chgb && (DT)(bestb + (DT)wb[q]) < db[q];
+                        const bool ia2 = chga && lbl_improves<DT, PK>(besta, cura, wa[q], da[q]);
+                        const bool ib2 = chgb && lbl_improves<DT, PK>(bestb, curb, wb[q], db[q]);
                         const bool fa = __ballot(ia2) != 0, fb = __ballot(ib2) != 0;
                         if (lane == q) {
                             na = fa ? ua[q] : -1;
@@ -578,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                 const int g = __builtin_amdgcn_readlane(mg, t);
                 const int v = __builtin_amdgcn_readlane(mv, t);
                 const GChunk c = ch[g];
-                DT *Dc = chunk_labels<DT>(D, c.dbase);
+                DT *Dc = chunk_labels<DT>(D, c.lbase);
                 const int64_t idx = (int64_t)v * kWave + lane;
                 const DT cur = Dc[idx];
                 DT best = cur;
@@ -591,10 +615,10 @@ __global__ __launch_bounds__(kBlock) void k_grelax(
                     for (int tt = 0; tt < cnt_e; tt++) {
                         const int u = __builtin_amdgcn_readlane(my.v, tt);
                         const int w = __builtin_amdgcn_readlane(my.w, tt);
-                        best = relax_min(Dc[(int64_t)u * kWave + lane], u, w, INF, best);
+                        best = relax_lbl<DT, PK>(Dc[(int64_t)u * kWave + lane], u, w, INF, best);
                     }
                 }
-                const bool dchg = best < cur;
+                const bool dchg = lbl_dropped<DT, PK>(best, cur);
                 if (dchg) Dc[idx] = best;
                 if (__ballot(dchg) != 0) {
                     if (!noflags) {
@@ -665,7 +689,7 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<D
             const int g = find_seg(item_pre, nch, item);
             const GChunk c = ch[g];
             const int b = c.b0 + (int)(item - item_pre[g]);
-            blk_visit<DT, DW>(chunk_labels<DT>(D, c.dbase), boff[b], boff[b + 1] - boff[b],
+            blk_visit<DT, DW>(chunk_labels<DT>(D, c.lbase), boff[b], boff[b + 1] - boff[b],
                               bvert, brp, bext, badj, vblk + c.voff, Dm ? Dm + bdoff[b] : nullptr,
                               Fnext + c.fbase, dense, L, s_flag, s_dirty, s_chg, nullptr, raised);
             __syncthreads();  // LDS is reused by the next item
@@ -700,7 +724,8 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int grp = __builtin_amdgcn_readfirstlane(t / kWave);
-    const DT *Dc = chunk_labels<DT>(const_cast<void *>(D), c.dbase);
+    const DT *Dc = chunk_labels<DT>(const_cast<void *>(D), c.lbase);
+    const int64_t ls = c.lstride;
     const bool real = lane < c.nx;
     // the labels and packed words of kPre vertices are loaded together, then
     // their records decided one vertex at a time
@@ -711,9 +736,9 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
         P Ps[kPre];
 #pragma unroll
         for (int u = 0; u < kPre; u++) {
-            const int64_t idx = (v0 + grp + (r0 + u) * (kBlock / kWave)) * kWave + lane;
-            Ls[u] = Dc[idx];
-            Ps[u] = c.kind == 0 ? CODE[c.dbase + idx] : (P)0;
+            const int64_t vu = v0 + grp + (r0 + u) * (kBlock / kWave);
+            Ls[u] = Dc[vu * ls + lane];
+            Ps[u] = c.kind == 0 ? CODE[c.dbase + vu * kWave + lane] : (P)0;
         }
 #pragma unroll
         for (int u = 0; u < kPre; u++) {
@@ -721,14 +746,14 @@ __global__ __launch_bounds__(kBlock) void k_gunpack(
             const int64_t v = v0 + vv;
             int32_t code = 0;
             if (c.kind == 0 && v < c.n)
-                code = cell_record<DT, P>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v, rp, adj);
+                code = cell_record<DT, P>(Dc, Ls[u], Ps[u], c.k, real, c.voff + (int)v, rp, adj, ls);
             else if (c.kind == 2 && v < c.n)
                 // group rows: {j} at the members (label 0: weights are >= 1),
                 // elsewhere the edge from the smallest neighbour on a
                 // shortest path (the merge word's sum is out of reach)
                 code = cell_record<DT, uint64_t>(
                     Dc, Ls[u], Ls[u] == (DT)0 ? (uint64_t)(1ull << lane) : ~0ull, c.k, real,
-                    c.voff + (int)v, rp, adj);
+                    c.voff + (int)v, rp, adj, ls);
             sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }

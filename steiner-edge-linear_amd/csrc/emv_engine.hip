@@ -189,6 +189,7 @@ struct emv_engine {
     int64_t macc_bytes = 0;
     bool no_split = false;   // EMV_NO_SPLIT: every layer through k_gmerge
     bool no_pack = false;    // EMV_NO_PACK: every staging chunk relaxes in place
+    bool pair_chunks = false;  // batched solves: 16-bit vertex-wise chunks relax in pairs
     int split_items = 32;    // split merge: items per tile aimed at (EMV_SPLIT_ITEMS)
     uint32_t *d_any = nullptr;
     uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
@@ -492,6 +493,7 @@ int ensure_stage(emv_engine *e, int64_t cells, int64_t flags) {
 // dbase counts cells of 4 bytes in the staging; a 16-bit chunk uses the
 // first half of its region.
 struct StepGroup {
+    bool paired = false;   // chunks relax in pairs of 128 rows (packed 16-bit labels)
     int ch0 = 0, nch = 0;  // chunk slice [ch0, ch0 + nch)
     int mg0 = 0, nmg = 0;  // merge slice (GMerge.chunk0 is relative to ch0)
     int ms0 = 0, nms = 0;  // split-merge slice, right after the merge slice
@@ -547,10 +549,14 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
     std::vector<GMerge> split;
+    std::vector<GChunk> pairs[4];  // per group: the relaxation items of paired chunks
     for (int gsel = 0; gsel < 4; gsel++) {
         StepGroup &SG = P.grp[gsel];
         SG.ch0 = (int)P.ch.size();
         SG.mg0 = (int)P.mg.size();
+        // 16-bit labels relaxed vertex-wise: chunks in pairs when the engine
+        // pairs (batched solves: no lane packing across instances)
+        SG.paired = e->pair_chunks && gsel == 1;
         split.clear();
         for (int64_t t : tickets) {
             Inst &I = e->inst[t];
@@ -570,8 +576,10 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             c.tbits = I.tbits;
             // activity flags: one per vertex, or one per block
             const int64_t nflags = G.nblk > 0 ? G.nblk : G.np;
+            c.lstride = kWave;
             if (I.stage == 1) {
                 c.dbase = dcur;
+                c.lbase = dcur;
                 c.fbase = fcur;
                 c.x0 = 0;
                 c.nx = I.k;
@@ -608,7 +616,11 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             GMerge M;
             memset(&M, 0, sizeof(M));
             M.chunk0 = (int)P.ch.size() - SG.ch0;
-            M.nchunks = (nrows + kWave - 1) / kWave;
+            const int nreal = (nrows + kWave - 1) / kWave;
+            // paired: an even number of chunks; a last odd chunk's partner
+            // holds INF rows only (merged and packed, never unpacked)
+            M.nchunks = SG.paired ? (nreal + 1) / 2 * 2 : nreal;
+            M.lpair = SG.paired ? 1 : 0;
             M.ntiles = (int32_t)(G.np / kWave);
             M.nsub = 0;  // set once the step's rows per wave is known
             M.nrows = nrows;
@@ -638,9 +650,24 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 c.dbase = dcur;
                 c.fbase = fcur;
                 c.x0 = x0 + q * kWave;
-                c.nx = std::min(kWave, nrows - q * kWave);
+                c.nx = std::max(0, std::min(kWave, nrows - q * kWave));
                 c.kind = 0;
-                P.ch.push_back(c);
+                if (SG.paired) {
+                    // labels of the pair in the first chunk's region, 128 per vertex
+                    const int64_t pbase = dcur - (q & 1) * G.np * kWave;
+                    c.lbase = pbase + (q & 1) * (kWave / 2);
+                    c.lstride = 2 * kWave;
+                    if ((q & 1) == 0) {
+                        GChunk pc = c;
+                        pc.lbase = pbase;
+                        pc.nx = std::min(2 * kWave, nrows - q * kWave);
+                        pairs[gsel].push_back(pc);
+                    }
+                } else {
+                    c.lbase = dcur;
+                    c.lstride = kWave;
+                }
+                if (q < nreal) P.ch.push_back(c);
                 dcur += G.np * kWave;
                 fcur += nflags;
             }
@@ -683,6 +710,13 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             fcur += c.nblk > 0 ? c.nblk : c.np;
             P.rch.push_back(c);
         };
+        if (SG.paired) {
+            // one item per (pair, vertex); pairs relax in place
+            for (const GChunk &pc : pairs[gsel]) add_relax(pc);
+            SG.nrch = (int)P.rch.size() - SG.rch0;
+            SG.nbin = 0;
+            continue;
+        }
         std::map<int32_t, std::vector<int>> partial;  // graph (voff) -> chunks
         for (int i = 0; i < SG.nch; i++) {
             const GChunk &c = P.ch[SG.ch0 + i];
@@ -714,9 +748,11 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 B.dbase = dcur;
                 B.np = c.np;
                 B.map0 = (int32_t)P.lanes.size();
+                c.lbase = dcur;
+                c.lstride = kWave;
                 for (int ci : bn.second)
                     for (int j = 0; j < P.ch[ci].nx; j++)
-                        P.lanes.push_back(GLane{P.ch[ci].dbase, j, 0});
+                        P.lanes.push_back(GLane{P.ch[ci].lbase, j, 0});
                 while ((int)P.lanes.size() < B.map0 + kWave) P.lanes.push_back(GLane{-1, 0, 0});
                 P.bins.push_back(B);
                 c.dbase = dcur;
@@ -843,9 +879,19 @@ int check_plan(emv_engine *e, const StepPlan &P) {
     if (P.flagbytes > e->flag_bytes) return fail("flag words", P.flagbytes, e->flag_bytes);
     if (P.macc_cells * (P.pbits / 8) > e->macc_bytes)
         return fail("merge accumulators", P.macc_cells * (P.pbits / 8), e->macc_bytes);
+    // end (cells) of a chunk's labels: lstride 64 labels of up to 4 bytes,
+    // np * 64 cells; a paired chunk's 64 16-bit rows at 128 per vertex end
+    // 32 cells (64 rows) before that, a pair's relaxation item (all 128 rows
+    // from the pair's base) at np * 64
+    auto label_end = [](const GChunk &c, bool item) {
+        return c.lbase + (int64_t)c.np * kWave - ((c.lstride == 2 * kWave && !item) ? kWave / 2 : 0);
+    };
     for (const GChunk &c : P.ch) {
         if (c.dbase < 0 || c.dbase + (int64_t)c.np * kWave > e->stage_cells)
             return fail("chunk staging", c.dbase + (int64_t)c.np * kWave, e->stage_cells);
+        if (c.lbase < 0 || label_end(c, false) > e->stage_cells ||
+            (c.lstride != kWave && c.lstride != 2 * kWave))
+            return fail("chunk labels", label_end(c, false), e->stage_cells);
         const int64_t tcells = 2 * (1ll << c.k) * (int64_t)c.np;
         if (c.tbase < 0 || c.tbase + tcells > e->arena.cap)
             return fail("instance table", c.tbase + tcells, e->arena.cap);
@@ -857,6 +903,8 @@ int check_plan(emv_engine *e, const StepPlan &P) {
             return fail("relaxation flags", c.fbase + nf, e->flag_bytes);
         if (c.dbase < 0 || c.dbase + (int64_t)c.np * kWave > e->stage_cells)
             return fail("relaxation staging", c.dbase + (int64_t)c.np * kWave, e->stage_cells);
+        if (c.lbase < 0 || label_end(c, true) > e->stage_cells)
+            return fail("relaxation labels", label_end(c, true), e->stage_cells);
     }
     for (const GBin &B : P.bins) {
         if (B.dbase < 0 || B.dbase + (int64_t)B.np * kWave > e->stage_cells)
@@ -894,7 +942,7 @@ int desc_buffer(emv_engine *e, size_t idx, size_t bytes, char **out) {
     return EMV_OK;
 }
 
-template <typename DT, int DMAX>
+template <typename DT, int DMAX, bool PK = false>
 void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
                    int r) {
     // no more workgroups than 64-item blocks of waves
@@ -905,7 +953,7 @@ void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
     int groups = 0;
     for (const StepGroup &G : P.grp) groups += G.nrch > 0 ? 1 : 0;
     const uint32_t qcap = groups > 1 ? 0u : e->qcap;
-    hipLaunchKernelGGL((k_grelax<DT, DMAX>), dim3(grid), dim3(kBlock), 0, e->stream,
+    hipLaunchKernelGGL((k_grelax<DT, DMAX, PK>), dim3(grid), dim3(kBlock), 0, e->stream,
                        (void *)e->d_D, (const GChunk *)(db + P.off_rch) + SG.rch0,
                        (const int64_t *)(db + SG.off_rp), (const int32_t *)(db + SG.off_rb),
                        SG.nrch, e->d_rp, e->d_adj, e->d_ell, dense, e->d_flag[r & 1],
@@ -916,6 +964,18 @@ void launch_grelax(emv_engine *e, char *db, const StepPlan &P, const StepGroup &
 template <typename DT>
 void grelax_group(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG, int dense,
                   int r) {
+    if constexpr (std::is_same<DT, uint16_t>::value) {
+        if (SG.paired) {
+            // pairs of 16-bit chunks: two labels per 32-bit word
+            switch (e->ell_deg) {
+            case 4: launch_grelax<uint32_t, 4, true>(e, db, P, SG, dense, r); break;
+            case 8: launch_grelax<uint32_t, 8, true>(e, db, P, SG, dense, r); break;
+            case 16: launch_grelax<uint32_t, 16, true>(e, db, P, SG, dense, r); break;
+            default: launch_grelax<uint32_t, 0, true>(e, db, P, SG, dense, r); break;
+            }
+            return;
+        }
+    }
     switch (e->ell_deg) {
     case 4: launch_grelax<DT, 4>(e, db, P, SG, dense, r); break;
     case 8: launch_grelax<DT, 8>(e, db, P, SG, dense, r); break;
@@ -1563,6 +1623,12 @@ extern "C" int emv_batch_create(int device, int64_t ninst, const emv_instance *i
     if (rc != EMV_OK) {
         emv_engine_destroy(e);
         return rc;
+    }
+    // every instance has its own graph, so no lane packing: 16-bit
+    // vertex-wise chunks relax in pairs (EMV_PAIR=0: one by one)
+    {
+        const char *pe = getenv("EMV_PAIR");
+        e->pair_chunks = !(pe && atoi(pe) == 0);
     }
     std::vector<std::vector<int64_t>> edges((size_t)ninst);
     std::vector<int64_t> ns((size_t)ninst);

@@ -2038,10 +2038,12 @@ __global__ __launch_bounds__(kBlock) void k_pack_v(const P *__restrict__ M, int6
 // (packed word pk = (sum << k) | Y), else 2^k + the smallest neighbour u with
 // label(u) + w == label(v).  Neighbour entries are read as wave-uniform
 // (scalar) loads, eight gathers in flight.
-template <typename DT, typename P, int CW = kWave>
+// Dc[u * stride + lane] holds the label of the lane's row at vertex u.
+template <typename DT, typename P>
 __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, P pk, int k,
                                                bool real, int v, const int32_t *__restrict__ rp,
-                                               const Edge *__restrict__ adj) {
+                                               const Edge *__restrict__ adj,
+                                               int64_t stride = kWave) {
     const int lane = __lane_id();
     const bool edge = real && (P)L < (pk >> k);
     int32_t code = (int32_t)(pk & (((P)1 << k) - 1));
@@ -2057,7 +2059,7 @@ __device__ __forceinline__ int32_t cell_record(const DT *__restrict__ Dc, DT L, 
             const int e = min(eb + q, e1 - 1);
             uu[q] = adj[e].v;
             ww[q] = adj[e].w;
-            du[q] = Dc[(int64_t)uu[q] * CW + lane];
+            du[q] = Dc[(int64_t)uu[q] * stride + lane];
         }
 #pragma unroll
         for (int q = 0; q < 8; q++)
@@ -2123,7 +2125,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
             const int64_t v = v0 + vv;
             int32_t code = 0;
             if (v < n)
-                code = cell_record<DT, P, CW>(Dc, Ls[u], Ps[u] - pbase, k, real, (int)v, rp, adj);
+                code = cell_record<DT, P>(Dc, Ls[u], Ps[u] - pbase, k, real, (int)v, rp, adj, CW);
             sd[vv][lane] = Ls[u];
             sc[vv][lane] = code;
         }

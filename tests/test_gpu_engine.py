@@ -393,6 +393,32 @@ def test_batch_grids_and_regular_graphs_in_one_step(monkeypatch, closure):
         _check_batch_vs_oracle(insts, c2, b)
 
 
+@pytest.mark.parametrize("pair,no_ell", [("1", False), ("1", True), ("0", False)])
+def test_batch_paired_chunks(monkeypatch, pair, no_ell):
+    """Batched solves relax 16-bit vertex-wise chunks in pairs (two chunks'
+    labels side by side, 128 rows per vertex, packed saturating add and
+    min), a last odd chunk with an INF partner, or one by one (EMV_PAIR=0);
+    both adjacency paths.  Layers of 1 .. 4 chunks (k = 9 .. 11), a 32-bit
+    label instance in the same step; tables and records equal the oracle's,
+    eagerly and from the replay."""
+    monkeypatch.setenv("EMV_PAIR", pair)
+    if no_ell:
+        monkeypatch.setenv("EMV_NO_ELL", "1")
+    insts = []
+    for i, (n, d, k, w) in enumerate([(640, 4, 9, 1000), (512, 6, 10, 1000), (300, 3, 11, 100),
+                                      (410, 5, 7, 1000), (350, 4, 8, 200000)]):
+        if (n * d) % 2:
+            n += 1
+        e = emv.regular_graph(n, d, w, 970 + i)
+        insts.append((n, e, emv.terminals(n, k, 970 + i)))
+    with emv.Batch(insts) as b:
+        c1, _ = b.solve(emv.KEEP_TABLE)
+        _check_batch_vs_oracle(insts, c1, b)
+        c2, s2 = b.solve(emv.KEEP_TABLE)
+        assert s2.graph_used == 1 or s2.graph_fallback == 1
+        _check_batch_vs_oracle(insts, c2, b)
+
+
 @pytest.mark.parametrize("no_split", [False, True])
 def test_batch_split_merge_upper_layers(monkeypatch, no_split):
     """Instances with k >= 9 reach layers whose rows have >= 128 pairs: the
