@@ -72,6 +72,8 @@ struct GMerge {
     int32_t h2;      // 16-bit cells merged two vertices per lane (128-vertex merge tiles)
     int32_t mtiles;  // merge tiles: np / 64, or ceil(np / 128) when h2
     int32_t lpair;   // chunks paired (an even count): labels as GChunk.lbase / lstride
+    int32_t star0;   // star merge (nstars > 0): its stars in the step's star list
+    int32_t nstars;
     int32_t pad_;
 };
 
@@ -294,6 +296,60 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_split(
     const int64_t v = tile * kWave + lane;
     const P acc = merge_cells<P>(f, M, v, X, o0, o1);
     atomicMin(&Macc[M.mbase + (int64_t)xi * M.np + v], acc);
+}
+
+// Star merges of the 16-bit instances whose layer rows have many pairs: the
+// stars of k_merge_star_h2 per instance (its stars are stars[M.star0 ..
+// M.star0 + M.nstars), row indices local to the instance's layer), item =
+// (tile, star, part) numbered tile-major per instance, M.S parts per star,
+// each folded into the instance's accumulators with an unsigned atomicMin
+// (k_gpack moves them to the staging, as for the split merges).
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_gmerge_star(
+    const int32_t *__restrict__ f, const GMerge *__restrict__ mg,
+    const int64_t *__restrict__ item_pre, int nmg, const MergeStar *__restrict__ stars,
+    P *__restrict__ Macc, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    __shared__ int64_t sh_pre[kSegCache];
+    const int64_t *pre = cache_pre(item_pre, nmg, sh_pre);
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) +
+                         __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (item >= pre[nmg]) return;
+    const int s = __builtin_amdgcn_readfirstlane(find_seg(pre, nmg, item));
+    const GMerge M = mg[s];
+    const int64_t local = wave_uniform(item - pre[s]);
+    const int64_t per_tile = (int64_t)M.nstars * M.S;
+    const int64_t tile = local / per_tile;
+    const int rem = (int)(local - tile * per_tile);
+    const int g = rem / M.S;
+    const int part = rem - g * M.S;
+    const MergeStar *st = stars + M.star0 + g;
+    const uint32_t S = st->S;
+    const int r = (int)st->r;
+    uint32_t a[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) a[i] = st->a[i];
+    const uint32_t steps = star_steps(S);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / M.S);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / M.S);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    // lanes past np read the last pair; their results are dropped
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, ((int64_t)M.np - v0) / 2 - 1);
+    const char *fb = reinterpret_cast<const char *>(f + M.tbase) + 2 * v0;
+    uint32_t klo[kStarMax], khi[kStarMax];
+    star_h2_pairs(fb, lb, (int64_t)M.np, S, r, a, o0, o1, klo, khi);
+    if (v >= M.np) return;
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) {
+        if (i < r) {
+            P *acc = &Macc[M.mbase + (int64_t)st->xi[i] * M.np + v];
+            atomicMin(acc, key_packed<P>(klo[i], M.k));
+            atomicMin(acc + 1, key_packed<P>(khi[i], M.k));
+        }
+    }
 }
 
 // Accumulators of the split instances -> vertex-major staging (label, packed

@@ -190,6 +190,7 @@ struct emv_engine {
     bool no_split = false;   // EMV_NO_SPLIT: every layer through k_gmerge
     bool no_pack = false;    // EMV_NO_PACK: every staging chunk relaxes in place
     bool pair_chunks = false;  // batched solves: 16-bit vertex-wise chunks relax in pairs
+    bool star_merge = true;    // split 16-bit layers merge in stars (EMV_MERGE_STAR=0: by rows)
     int split_items = 32;    // split merge: items per tile aimed at (EMV_SPLIT_ITEMS)
     uint32_t *d_any = nullptr;
     uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
@@ -496,7 +497,10 @@ struct StepGroup {
     bool paired = false;   // chunks relax in pairs of 128 rows (packed 16-bit labels)
     int ch0 = 0, nch = 0;  // chunk slice [ch0, ch0 + nch)
     int mg0 = 0, nmg = 0;  // merge slice (GMerge.chunk0 is relative to ch0)
-    int ms0 = 0, nms = 0;  // split-merge slice, right after the merge slice
+    int ms0 = 0, nms = 0;  // split-merge slice, right after the merge slice: rows cut into
+    int nss = 0;           // parts, then the last nss merging in stars
+    std::vector<int64_t> star_pre;
+    size_t off_stp = 0;
     int rch0 = 0, nrch = 0;  // relaxation chunks (staging chunks in place, or packed bins)
     int bin0 = 0, nbin = 0;  // packed bins
     std::vector<int64_t> relax_pre, merge_pre, unpack_pre, split_pre, pack_pre, bin_pre;
@@ -512,6 +516,7 @@ struct StepPlan {
     std::vector<GBin> bins;
     std::vector<GLane> lanes;
     std::vector<GMerge> mg;
+    std::vector<MergeStar> stars;  // star merges' stars (rows local to their instance's layer)
     StepGroup grp[4];  // label width (u32, u16) x relaxation (vertex-wise, block-local)
     std::vector<int64_t> seed_pre;
     std::vector<uint32_t> xs;
@@ -526,6 +531,7 @@ struct StepPlan {
     // device layout (byte offsets into the descriptor buffer)
     size_t off_ch = 0, off_mg = 0, off_sp = 0, off_xs = 0, off_seedch = 0, bytes = 0;
     size_t off_rch = 0, off_bins = 0, off_lanes = 0, off_pl = 0, off_plp = 0, off_mc = 0;
+    size_t off_stars = 0;
     std::vector<char> blob;
     std::vector<GChunk> seed_ch;
 };
@@ -548,7 +554,8 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     int64_t dcur = 0, fcur = 0;
     int maxrows = 0, maxstage = 0;
     std::vector<uint32_t> rows;
-    std::vector<GMerge> split;
+    std::vector<GMerge> split, starsplit;
+    std::vector<MergeStar> tmp_stars;
     std::vector<GChunk> pairs[4];  // per group: the relaxation items of paired chunks
     for (int gsel = 0; gsel < 4; gsel++) {
         StepGroup &SG = P.grp[gsel];
@@ -558,6 +565,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         // pairs (batched solves: no lane packing across instances)
         SG.paired = e->pair_chunks && gsel == 1;
         split.clear();
+        starsplit.clear();
         for (int64_t t : tickets) {
             Inst &I = e->inst[t];
             if (inst_group(e, I) != gsel) continue;
@@ -639,8 +647,17 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 // holds many instances' tiles, so few parts pay: fewer
                 // atomics), each part >= 4 outer steps
                 const int64_t steps = (I.stage - 1 <= 3) ? 1 : (1ll << (I.stage - 1 - 3));
+                int64_t units = nrows;
+                if (M.h2 && e->star_merge) {
+                    // 16-bit cells: stars of rows sharing a base set
+                    build_stars(rows.data(), nrows, I.k, tmp_stars);
+                    M.star0 = (int32_t)P.stars.size();
+                    M.nstars = (int32_t)tmp_stars.size();
+                    P.stars.insert(P.stars.end(), tmp_stars.begin(), tmp_stars.end());
+                    units = M.nstars;
+                }
                 M.S = (int32_t)std::max<int64_t>(
-                    1, std::min<int64_t>((split_items(e) + nrows - 1) / nrows, steps / 4));
+                    1, std::min<int64_t>((split_items(e) + units - 1) / units, steps / 4));
                 M.mbase = P.macc_cells;
                 P.macc_cells += (int64_t)nrows * G.np;
             }
@@ -673,7 +690,9 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             }
             // a layer step starts near-dense (a singleton-only step does not)
             P.dense_rounds = e->prm.dense_rounds > 0 ? e->prm.dense_rounds : layer_dense_rounds();
-            if (splits)
+            if (splits && M.nstars > 0)
+                starsplit.push_back(M);
+            else if (splits)
                 split.push_back(M);
             else
                 P.mg.push_back(M);
@@ -681,8 +700,10 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         SG.nch = (int)P.ch.size() - SG.ch0;
         SG.nmg = (int)P.mg.size() - SG.mg0;
         SG.ms0 = (int)P.mg.size();
-        SG.nms = (int)split.size();
+        SG.nms = (int)(split.size() + starsplit.size());
+        SG.nss = (int)starsplit.size();
         P.mg.insert(P.mg.end(), split.begin(), split.end());
+        P.mg.insert(P.mg.end(), starsplit.begin(), starsplit.end());
     }
     // rows per merge wave: many rows while rows have few pairs (a wave's
     // fixed costs are spread over more work), one row per wave on the layers
@@ -786,11 +807,15 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             SG.merge_pre.push_back(SG.merge_pre.back() + (int64_t)M.mtiles * M.nsub);
         }
         SG.split_pre.assign(1, 0);
+        SG.star_pre.assign(1, 0);
         SG.pack_pre.assign(1, 0);
         for (int i = 0; i < SG.nms; i++) {
             GMerge &M = P.mg[SG.ms0 + i];
             M.nsub = M.nchunks * kWave;
-            SG.split_pre.push_back(SG.split_pre.back() + (int64_t)M.mtiles * M.nrows * M.S);
+            if (i < SG.nms - SG.nss)
+                SG.split_pre.push_back(SG.split_pre.back() + (int64_t)M.mtiles * M.nrows * M.S);
+            else
+                SG.star_pre.push_back(SG.star_pre.back() + (int64_t)M.mtiles * M.nstars * M.S);
             SG.pack_pre.push_back(SG.pack_pre.back() + (int64_t)M.ntiles * M.nchunks);
         }
         const int64_t nitems = group_blocks(gsel) ? 0 : SG.relax_pre.back();
@@ -828,6 +853,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     P.off_pl = place(sizeof(GPlace) * P.places.size());
     P.off_plp = place(sizeof(int64_t) * P.place_pre.size());
     P.off_mc = place(sizeof(int64_t) * P.member_cells.size());
+    P.off_stars = place(sizeof(MergeStar) * P.stars.size());
     for (StepGroup &SG : P.grp) {
         SG.off_rp = place(sizeof(int64_t) * SG.relax_pre.size());
         SG.off_mp = place(sizeof(int64_t) * SG.merge_pre.size());
@@ -836,6 +862,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         SG.off_sp = place(sizeof(int64_t) * SG.split_pre.size());
         SG.off_pp = place(sizeof(int64_t) * SG.pack_pre.size());
         SG.off_bp = place(sizeof(int64_t) * SG.bin_pre.size());
+        SG.off_stp = place(sizeof(int64_t) * SG.star_pre.size());
     }
     P.blob.assign(P.bytes, 0);
     auto put = [&](size_t off, const void *src, size_t n) {
@@ -852,6 +879,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
     put(P.off_pl, P.places.data(), sizeof(GPlace) * P.places.size());
     put(P.off_plp, P.place_pre.data(), sizeof(int64_t) * P.place_pre.size());
     put(P.off_mc, P.member_cells.data(), sizeof(int64_t) * P.member_cells.size());
+    put(P.off_stars, P.stars.data(), sizeof(MergeStar) * P.stars.size());
     for (StepGroup &SG : P.grp) {
         put(SG.off_rp, SG.relax_pre.data(), sizeof(int64_t) * SG.relax_pre.size());
         put(SG.off_mp, SG.merge_pre.data(), sizeof(int64_t) * SG.merge_pre.size());
@@ -860,6 +888,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
         put(SG.off_sp, SG.split_pre.data(), sizeof(int64_t) * SG.split_pre.size());
         put(SG.off_pp, SG.pack_pre.data(), sizeof(int64_t) * SG.pack_pre.size());
         put(SG.off_bp, SG.bin_pre.data(), sizeof(int64_t) * SG.bin_pre.size());
+        put(SG.off_stp, SG.star_pre.data(), sizeof(int64_t) * SG.star_pre.size());
     }
 }
 
@@ -918,6 +947,17 @@ int check_plan(emv_engine *e, const StepPlan &P) {
     for (const GMerge &M : P.mg) {
         if (M.S > 0 && M.mbase + (int64_t)M.nrows * M.np > P.macc_cells)
             return fail("split rows", M.mbase + (int64_t)M.nrows * M.np, P.macc_cells);
+        if (M.nstars > 0) {
+            if (M.star0 < 0 || (size_t)M.star0 + M.nstars > P.stars.size() || M.S < 1 || !M.h2)
+                return fail("merge stars", (long long)M.star0 + M.nstars, (long long)P.stars.size());
+            for (int i = 0; i < M.nstars; i++) {
+                const MergeStar &st = P.stars[(size_t)M.star0 + i];
+                if (st.r < 1 || st.r > (uint32_t)kStarMax) return fail("star rows", st.r, kStarMax);
+                for (uint32_t j = 0; j < st.r; j++)
+                    if (st.xi[j] < 0 || st.xi[j] >= M.nrows)
+                        return fail("star row index", st.xi[j], M.nrows);
+            }
+        }
         if (M.x0 < 0 || (size_t)M.x0 + M.nrows > P.xs.size())
             return fail("merge rows", (long long)M.x0 + M.nrows, (long long)P.xs.size());
     }
@@ -1047,14 +1087,24 @@ void launch_gmerge_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup
 template <typename P_, typename DT>
 void launch_gsplit_p(emv_engine *e, char *db, const StepPlan &P, const StepGroup &SG) {
     const int64_t items = SG.split_pre.back();
-    if (items == 0) return;
+    if (items == 0 && SG.star_pre.back() == 0) return;
     const int64_t blocks = (items + (kBlock / kWave) - 1) / (kBlock / kWave);
     const int nxcd = (e->num_cu % 32 == 0 && e->num_cu >= 64) ? e->num_cu / 32 : 1;
     const int64_t per = (blocks + nxcd - 1) / nxcd;
     const GMerge *mg = (const GMerge *)(db + P.off_mg) + SG.ms0;
-    hipLaunchKernelGGL((k_gmerge_split<P_>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
-                       e->stream, e->d_arena, mg, (const int64_t *)(db + SG.off_sp), SG.nms,
-                       (const uint32_t *)(db + P.off_xs), (P_ *)e->d_macc, nxcd, per);
+    const int nrow = SG.nms - SG.nss;  // rows cut into parts, then the star merges
+    if (items > 0)
+        hipLaunchKernelGGL((k_gmerge_split<P_>), dim3((unsigned)(per * nxcd)), dim3(kBlock), 0,
+                           e->stream, e->d_arena, mg, (const int64_t *)(db + SG.off_sp), nrow,
+                           (const uint32_t *)(db + P.off_xs), (P_ *)e->d_macc, nxcd, per);
+    if (SG.star_pre.back() > 0) {
+        const int64_t sblocks = (SG.star_pre.back() + (kBlock / kWave) - 1) / (kBlock / kWave);
+        const int64_t sper = (sblocks + nxcd - 1) / nxcd;
+        hipLaunchKernelGGL((k_gmerge_star<P_>), dim3((unsigned)(sper * nxcd)), dim3(kBlock), 0,
+                           e->stream, e->d_arena, mg + nrow, (const int64_t *)(db + SG.off_stp),
+                           SG.nss, (const MergeStar *)(db + P.off_stars), (P_ *)e->d_macc, nxcd,
+                           sper);
+    }
     hipLaunchKernelGGL((k_gpack<P_, DT>), dim3((unsigned)SG.pack_pre.back()), dim3(kBlock), 0,
                        e->stream, (const P_ *)e->d_macc, mg, (const int64_t *)(db + SG.off_pp),
                        SG.nms, (void *)e->d_D, (P_ *)e->d_code);
@@ -1482,6 +1532,7 @@ int engine_open(int device, emv_engine **out) {
     if (const char *qc = getenv("EMV_QCAP")) e->qcap = (uint32_t)std::max(0, atoi(qc));
     e->no_split = getenv("EMV_NO_SPLIT") != nullptr;
     e->no_pack = getenv("EMV_NO_PACK") != nullptr;
+    if (const char *ms = getenv("EMV_MERGE_STAR")) e->star_merge = atoi(ms) != 0;
     if (const char *si = getenv("EMV_SPLIT_ITEMS")) e->split_items = std::max(1, atoi(si));
     EHIP(e, hipMalloc((void **)&e->d_q[0], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipMalloc((void **)&e->d_q[1], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));

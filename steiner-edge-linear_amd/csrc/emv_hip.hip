@@ -728,50 +728,6 @@ int64_t star_waves_per_tile() {
     return v;
 }
 
-// Stars covering the batch's rows xs[0..nx) (all of m elements): each
-// uncovered row, in order, takes the base set S = X without the element that
-// leaves the most uncovered rows S | a in the batch, and up to kStarMax of
-// them (X first, then by increasing a) form its star.
-void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out) {
-    out.clear();
-    std::vector<int32_t> idx((size_t)1 << k, -1);
-    for (int i = 0; i < nx; i++) idx[xs[i]] = i;
-    std::vector<char> covered((size_t)nx, 0);
-    auto free_row = [&](uint32_t Y) { return idx[Y] >= 0 && !covered[(size_t)idx[Y]]; };
-    for (int i = 0; i < nx; i++) {
-        if (covered[(size_t)i]) continue;
-        const uint32_t X = xs[i];
-        uint32_t bestS = 0;
-        int best = -1;
-        for (uint32_t rest = X; rest; rest &= rest - 1) {
-            const uint32_t S = X ^ (rest & (~rest + 1u));
-            int cnt = 0;
-            for (int b = 0; b < k; b++)
-                if (!(S >> b & 1u) && free_row(S | (1u << b))) cnt++;
-            if (cnt > best) {
-                best = cnt;
-                bestS = S;
-            }
-        }
-        MergeStar st;
-        memset(&st, 0, sizeof(st));
-        st.S = bestS;
-        st.a[0] = X ^ bestS;
-        st.xi[0] = i;
-        covered[(size_t)i] = 1;
-        st.r = 1;
-        for (int b = 0; b < k && st.r < (uint32_t)kStarMax; b++) {
-            const uint32_t Y = bestS | (1u << b);
-            if ((bestS >> b & 1u) || !free_row(Y)) continue;
-            st.a[st.r] = 1u << b;
-            st.xi[st.r] = idx[Y];
-            covered[(size_t)idx[Y]] = 1;
-            st.r++;
-        }
-        out.push_back(st);
-    }
-}
-
 template <typename T, typename P, typename DT>
 int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
     build_stars(c->h_xs, nx, k, c->h_stars);

@@ -34,7 +34,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <type_traits>
+#include <vector>
 
 namespace emv {
 
@@ -1525,6 +1527,50 @@ struct MergeStar {
     int32_t xi[kStarMax];       // the row's index in the layer batch
 };
 
+// Stars covering the batch's rows xs[0..nx) (all of m elements): each
+// uncovered row, in order, takes the base set S = X without the element that
+// leaves the most uncovered rows S | a in the batch, and up to kStarMax of
+// them (X first, then by increasing a) form its star.
+inline void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out) {
+    out.clear();
+    std::vector<int32_t> idx((size_t)1 << k, -1);
+    for (int i = 0; i < nx; i++) idx[xs[i]] = i;
+    std::vector<char> covered((size_t)nx, 0);
+    auto free_row = [&](uint32_t Y) { return idx[Y] >= 0 && !covered[(size_t)idx[Y]]; };
+    for (int i = 0; i < nx; i++) {
+        if (covered[(size_t)i]) continue;
+        const uint32_t X = xs[i];
+        uint32_t bestS = 0;
+        int best = -1;
+        for (uint32_t rest = X; rest; rest &= rest - 1) {
+            const uint32_t S = X ^ (rest & (~rest + 1u));
+            int cnt = 0;
+            for (int b = 0; b < k; b++)
+                if (!(S >> b & 1u) && free_row(S | (1u << b))) cnt++;
+            if (cnt > best) {
+                best = cnt;
+                bestS = S;
+            }
+        }
+        MergeStar st;
+        memset(&st, 0, sizeof(st));
+        st.S = bestS;
+        st.a[0] = X ^ bestS;
+        st.xi[0] = i;
+        covered[(size_t)i] = 1;
+        st.r = 1;
+        for (int b = 0; b < k && st.r < (uint32_t)kStarMax; b++) {
+            const uint32_t Y = bestS | (1u << b);
+            if ((bestS >> b & 1u) || !free_row(Y)) continue;
+            st.a[st.r] = 1u << b;
+            st.xi[st.r] = idx[Y];
+            covered[(size_t)idx[Y]] = 1;
+            st.r++;
+        }
+        out.push_back(st);
+    }
+}
+
 // Outer steps of a star: subsets Th of S's high part H (S without its three
 // lowest bits L), as merge_steps.
 __device__ __forceinline__ uint32_t star_steps(uint32_t S) {
@@ -1532,36 +1578,15 @@ __device__ __forceinline__ uint32_t star_steps(uint32_t S) {
     return r <= 3 ? 1u : (1u << (r - 3));
 }
 
-// Item = (tile, star, part), tile-major per XCD; parts fold into M with
-// unsigned atomicMin like k_merge_split_h2.  Two vertices per lane.
-template <typename P>
-__global__ __launch_bounds__(kBlock) void k_merge_star_h2(
-    const uint16_t *__restrict__ f, int64_t np, const MergeStar *__restrict__ stars, int nstars,
-    int parts, int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
-    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
-    const int64_t per_tile = (int64_t)nstars * parts;
-    const int64_t tile = item / per_tile;
-    if (tile >= ntiles) return;
-    const int rem = (int)(item - tile * per_tile);
-    const int g = rem / parts;
-    const int part = rem - g * parts;
-    const uint32_t S = stars[g].S;
-    const int r = (int)stars[g].r;
-    uint32_t a[kStarMax];
-#pragma unroll
-    for (int i = 0; i < kStarMax; i++) a[i] = stars[g].a[i];
-    const uint32_t steps = star_steps(S);
-    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / parts);
-    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
-    if (o0 >= o1) return;
-    const int64_t v0 = tile * (2 * kWave);
-    const int64_t v = v0 + 2 * lane;
-    const bool live = v < np;
-    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
-    const char *fb = reinterpret_cast<const char *>(f + v0);
+// The pairs of a star's rows over the outer steps [o0, o1), for the vertex
+// pair at byte offset lb (4-aligned, < 256) from the wave-uniform byte base fb
+// of a 128-vertex tile; klo / khi fold the (sum << 16) | Y keys of the low /
+// high vertex of each row (all ones where a row has no pair yet).
+__device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint32_t lb,
+                                              int64_t np, uint32_t S, int r,
+                                              const uint32_t (&a)[kStarMax], uint32_t o0,
+                                              uint32_t o1, uint32_t (&klo)[kStarMax],
+                                              uint32_t (&khi)[kStarMax]) {
     uint32_t hmask = 0xFFFF0000u;
     asm volatile("" : "+v"(hmask));
     const int64_t rb = 2 * np;
@@ -1573,7 +1598,6 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
         H ^= lo;
     }
     const uint32_t nH = 1u << __popc(H);
-    uint32_t klo[kStarMax], khi[kStarMax];
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) klo[i] = khi[i] = 0xFFFFFFFFu;
     // the key's set for row i: T, or X_i ^ T when top(X_i) = top(S) lies in T
@@ -1661,6 +1685,40 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
             }
         }
     }
+}
+
+// Item = (tile, star, part), tile-major per XCD; parts fold into M with
+// unsigned atomicMin like k_merge_split_h2.  Two vertices per lane.
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_star_h2(
+    const uint16_t *__restrict__ f, int64_t np, const MergeStar *__restrict__ stars, int nstars,
+    int parts, int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)nstars * parts;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int g = rem / parts;
+    const int part = rem - g * parts;
+    const uint32_t S = stars[g].S;
+    const int r = (int)stars[g].r;
+    uint32_t a[kStarMax];
+#pragma unroll
+    for (int i = 0; i < kStarMax; i++) a[i] = stars[g].a[i];
+    const uint32_t steps = star_steps(S);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / parts);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    const bool live = v < np;
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+    const char *fb = reinterpret_cast<const char *>(f + v0);
+    uint32_t klo[kStarMax], khi[kStarMax];
+    star_h2_pairs(fb, lb, np, S, r, a, o0, o1, klo, khi);
     if (!live) return;
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) {

@@ -419,17 +419,21 @@ def test_batch_paired_chunks(monkeypatch, pair, no_ell):
         _check_batch_vs_oracle(insts, c2, b)
 
 
-@pytest.mark.parametrize("no_split", [False, True])
-def test_batch_split_merge_upper_layers(monkeypatch, no_split):
+@pytest.mark.parametrize("no_split,stars", [(False, "1"), (False, "0"), (True, "1")])
+def test_batch_split_merge_upper_layers(monkeypatch, no_split, stars):
     """Instances with k >= 9 reach layers whose rows have >= 128 pairs: the
-    engine cuts those rows into parts folded with atomicMin (16/32-bit labels,
-    32- and 64-bit packed words: the last instance's bound needs 64) or, with
-    EMV_NO_SPLIT, merges them whole.  Tables and records equal the oracle's."""
+    engine merges the 16-bit tables' rows in stars (rows sharing a base set,
+    128-vertex tiles, a last tile half past the vertices), cuts the other
+    rows into parts (EMV_MERGE_STAR=0 for all), both folded with atomicMin
+    (16/32-bit labels, 32- and 64-bit packed words: the last instance's bound
+    needs 64), or, with EMV_NO_SPLIT, merges them whole.  Tables and records
+    equal the oracle's, eagerly and from the replay."""
+    monkeypatch.setenv("EMV_MERGE_STAR", stars)
     if no_split:
         monkeypatch.setenv("EMV_NO_SPLIT", "1")
     insts = []
     for i, (n, d, k, w) in enumerate([(300, 4, 9, 1000), (260, 3, 10, 1000), (200, 5, 11, 100),
-                                      (240, 4, 10, 800000)]):
+                                      (240, 4, 10, 800000), (384, 3, 12, 1000)]):
         e = emv.regular_graph(n, d, w, 950 + i)
         insts.append((n, e, emv.terminals(n, k, 950 + i)))
     with emv.Batch(insts) as b:
