@@ -1582,11 +1582,17 @@ __device__ __forceinline__ uint32_t star_steps(uint32_t S) {
 // pair at byte offset lb (4-aligned, < 256) from the wave-uniform byte base fb
 // of a 128-vertex tile; klo / khi fold the (sum << 16) | Y keys of the low /
 // high vertex of each row (all ones where a row has no pair yet).
-__device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint32_t lb,
-                                              int64_t np, uint32_t S, int r,
-                                              const uint32_t (&a)[kStarMax], uint32_t o0,
-                                              uint32_t o1, uint32_t (&klo)[kStarMax],
-                                              uint32_t (&khi)[kStarMax]) {
+// R = the star's rows, a compile-time count: the loads of all R rows of an
+// outer step then issue before the first wait (a runtime count puts a branch
+// and a wait between the rows, and a wave keeps 8 loads in flight instead
+// of 8 (R + 1)).
+template <int R>
+__device__ __forceinline__ void star_h2_pairs_r(const char *__restrict__ fb, uint32_t lb,
+                                                int64_t np, uint32_t S,
+                                                const uint32_t (&a)[kStarMax], uint32_t o0,
+                                                uint32_t o1, uint32_t (&klo)[kStarMax],
+                                                uint32_t (&khi)[kStarMax]) {
+    constexpr int r = R;
     uint32_t hmask = 0xFFFF0000u;
     asm volatile("" : "+v"(hmask));
     const int64_t rb = 2 * np;
@@ -1684,6 +1690,20 @@ __device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint3
                 }
             }
         }
+    }
+}
+
+__device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint32_t lb,
+                                              int64_t np, uint32_t S, int r,
+                                              const uint32_t (&a)[kStarMax], uint32_t o0,
+                                              uint32_t o1, uint32_t (&klo)[kStarMax],
+                                              uint32_t (&khi)[kStarMax]) {
+    static_assert(kStarMax == 4, "one case per star size");
+    switch (r) {
+    case 1: star_h2_pairs_r<1>(fb, lb, np, S, a, o0, o1, klo, khi); break;
+    case 2: star_h2_pairs_r<2>(fb, lb, np, S, a, o0, o1, klo, khi); break;
+    case 3: star_h2_pairs_r<3>(fb, lb, np, S, a, o0, o1, klo, khi); break;
+    default: star_h2_pairs_r<4>(fb, lb, np, S, a, o0, o1, klo, khi); break;
     }
 }
 
