@@ -363,15 +363,18 @@ def plain_leg(args):
     relax["frac"] = relax["achieved_GBs"] / HBM_PEAK_GBS
     relax["cells_per_s"] = cells / (relax_ms / 1e3) if relax_ms > 0 else 0.0
     relax["avg_launch_us"] = relax_ms * 1e3 / max(relax["launches"], 1)
-    # item visits: a (chunk, vertex) item is 64 rows of one vertex; one pass
-    # over every item of layers 2..k is the one-pass model's work
-    one_pass = sum(-(-_comb(k, mm) // 64) for mm in range(2, k + 1)) * wl["n"]
+    # item visits: a (chunk, vertex) item is the chunk's rows (128 or 64) of
+    # one vertex; one pass over every item of layers 2..k is the one-pass
+    # model's work
+    cw = sols[-1].relax_chunk_rows or 64
+    one_pass = sum(-(-_comb(k, mm) // cw) for mm in range(2, k + 1)) * wl["n"]
+    relax["chunk_rows"] = cw
     relax["item_visits"] = visits
     relax["one_pass_items"] = one_pass
     relax["visits_per_item"] = visits / one_pass if visits > 0 else None
     deg = wl["d"]
     lb = sols[-1].label_bits // 8
-    relax["visit_bytes_model"] = 64 * lb * (deg + 2) + 8 * deg  # own row read+write, neighbours, adjacency
+    relax["visit_bytes_model"] = cw * lb * (deg + 2) + 8 * deg  # own rows read+write, neighbours, adjacency
     relax["visit_bytes_per_solve"] = visits * relax["visit_bytes_model"] if visits > 0 else None
     # merge: unordered pair-cells; its per-X streaming model counts lower rows
     # re-read from cache, so it is reported as a model ratio, not a roofline

@@ -93,8 +93,10 @@ typedef struct {
                                       the rows (cell - row base) instead of the int64 cells */
     int64_t relax_item_visits;     /* with EMV_COUNT_VISITS: (chunk, vertex) items that the
                                       vertex-wise relaxation rounds of layers 2..k processed (each
-                                      visit reads 64 rows of its own and its neighbours' labels);
-                                      -1 otherwise */
+                                      visit reads relax_chunk_rows rows of its own and its
+                                      neighbours' labels); -1 otherwise */
+    int64_t relax_chunk_rows;      /* rows of a relaxation item (chunk, vertex) in layers 2..k:
+                                      128 when 16-bit labels ride two per 32-bit word, else 64 */
 } emv_result;
 
 /* Context on HIP device `device`. */

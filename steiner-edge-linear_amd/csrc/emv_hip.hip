@@ -1054,6 +1054,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
     if constexpr (wide_chunks_ok<P, DT>())
         if (c->chunk128 && !use_blocks<DT>(c)) cw = 2 * kWave;
     c->cw = cw;
+    res->relax_chunk_rows = cw;
     std::vector<uint32_t> xs;
     for (int m = 2; m <= k; m++) {
         xs.clear();

@@ -50,6 +50,7 @@ class EmvResult(ctypes.Structure):
         ("singleton_rounds", ctypes.c_int64), ("block_dist_bits", ctypes.c_int64),
         ("merge_split_ms", ctypes.c_double), ("relative_label_layers", ctypes.c_int64),
         ("compact_merge_layers", ctypes.c_int64), ("relax_item_visits", ctypes.c_int64),
+        ("relax_chunk_rows", ctypes.c_int64),
     ]
 
 
@@ -144,6 +145,7 @@ class Solution:
         self.relative_label_layers = r.relative_label_layers
         self.compact_merge_layers = r.compact_merge_layers
         self.relax_item_visits = r.relax_item_visits
+        self.relax_chunk_rows = r.relax_chunk_rows
         self.pairs = pairs
 
 
