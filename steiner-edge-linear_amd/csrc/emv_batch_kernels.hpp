@@ -169,7 +169,7 @@ __device__ __forceinline__ void merge_cells_h2(const int32_t *__restrict__ f, co
                                                int64_t v0, uint32_t X, uint32_t o0, uint32_t o1,
                                                uint32_t &klo, uint32_t &khi) {
     const uint32_t lb = 4u * (uint32_t)min((int64_t)__lane_id(), ((int64_t)M.np - v0) / 2 - 1);
-    merge_outer_h2(reinterpret_cast<const char *>(f + M.tbase) + 2 * v0, lb, (int64_t)M.np, X,
+    merge_outer_h2(reinterpret_cast<const char *>(f + M.tbase) + 2 * v0, lb, 2 * (int64_t)M.np, X,
                    o0, o1, klo, khi);
 }
 
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_gmerge_star(
     const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, ((int64_t)M.np - v0) / 2 - 1);
     const char *fb = reinterpret_cast<const char *>(f + M.tbase) + 2 * v0;
     uint32_t klo[kStarMax], khi[kStarMax];
-    star_h2_pairs(fb, lb, (int64_t)M.np, S, r, a, o0, o1, klo, khi);
+    star_h2_pairs(fb, lb, 2 * (int64_t)M.np, S, r, a, o0, o1, klo, khi);
     if (v >= M.np) return;
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) {

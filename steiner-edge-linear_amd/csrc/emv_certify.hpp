@@ -63,7 +63,8 @@ __device__ __forceinline__ int64_t cert_add(int64_t a, int64_t b) {
 // row {t} (shortcut records; unused with edge_singletons).
 // Rows X = row0 .. row0 + nrows - 1; one workgroup per (row, 256 vertices).
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_certify(
-    const void *__restrict__ f, int tb, const int32_t *__restrict__ b, int64_t np, int n, int k,
+    const void *__restrict__ f, FLayout fl, int tb, const int32_t *__restrict__ b, int64_t np,
+    int n, int k,
     const int32_t *__restrict__ rp, const Edge *__restrict__ adj, const uint32_t *__restrict__ mark,
     const int32_t *__restrict__ kk, int edge_singletons, int64_t row0, int64_t nrows,
     unsigned long long *__restrict__ out) {
@@ -74,8 +75,8 @@ __device__ __forceinline__ int64_t cert_add(int64_t a, int64_t b) {
         const int64_t X = row0 + item / vblocks;
         const int64_t v = (item % vblocks) * 256 + threadIdx.x;
         if (v >= n) continue;
-        const int64_t cell = X * np + v;
-        const int64_t fv = cert_cell(f, tb, cell);
+        const int64_t cell = X * np + v;  // records: rows of np; cells: layout fl
+        const int64_t fv = cert_cell(f, tb, fl.cell(X, v));
         const int32_t code = b[cell];
         bool bad_value = false, bad_record = false, not_finite = false;
         if (X == 0) {
@@ -88,7 +89,7 @@ __device__ __forceinline__ int64_t cert_add(int64_t a, int64_t b) {
             int64_t bu = -1;
             for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
                 const int64_t u = adj[e].v;
-                const int64_t c = cert_add(cert_cell(f, tb, X * np + u), (int64_t)adj[e].w);
+                const int64_t c = cert_add(cert_cell(f, tb, fl.cell(X, u)), (int64_t)adj[e].w);
                 if (c < best_e || (c == best_e && c != kCertInf && u < bu)) {
                     best_e = c;
                     bu = u;
@@ -117,8 +118,8 @@ __device__ __forceinline__ int64_t cert_add(int64_t a, int64_t b) {
                 int64_t Y = 0;
                 do {
                     Y = (Y - Xl) & Xl;
-                    const int64_t s = cert_add(cert_cell(f, tb, Y * np + v),
-                                               cert_cell(f, tb, (X ^ Y) * np + v));
+                    const int64_t s = cert_add(cert_cell(f, tb, fl.cell(Y, v)),
+                                               cert_cell(f, tb, fl.cell(X ^ Y, v)));
                     if (s < best_s) {
                         best_s = s;
                         bY = Y;
@@ -142,7 +143,7 @@ __device__ __forceinline__ int64_t cert_add(int64_t a, int64_t b) {
 // fills out[0..3] as above (out[3] = ~0 when no cell fails).  srcs[t] lists
 // the sources of singleton row {t}; kk[t] its terminal.  Returns the HIP
 // error of the first failing call.
-static inline hipError_t certify_table(hipStream_t stream, const void *f, int tb,
+static inline hipError_t certify_table(hipStream_t stream, const void *f, FLayout fl, int tb,
                                        const int32_t *b, int64_t np, int n, int k,
                                        const int32_t *rp, const Edge *adj,
                                        const std::vector<std::vector<int32_t>> &srcs,
@@ -175,7 +176,7 @@ static inline hipError_t certify_table(hipStream_t stream, const void *f, int tb
         const int64_t nr = std::min<int64_t>(rows, S2 - X0);
         const int64_t items = nr * vblocks;
         hipLaunchKernelGGL(k_certify, dim3((unsigned)std::min<int64_t>(items, 1 << 20)), dim3(256),
-                           0, stream, f, tb, b, np, n, k, rp, adj, d_mark, d_kk, edge_singletons,
+                           0, stream, f, fl, tb, b, np, n, k, rp, adj, d_mark, d_kk, edge_singletons,
                            X0, nr, d_out);
         err = hipGetLastError();
     }

@@ -191,6 +191,7 @@ struct emv_engine {
     bool no_pack = false;    // EMV_NO_PACK: every staging chunk relaxes in place
     bool pair_chunks = false;  // batched solves: 16-bit vertex-wise chunks relax in pairs
     bool star_merge = true;    // split 16-bit layers merge in stars (EMV_MERGE_STAR=0: by rows)
+    uint32_t star_pairs = 63;  // and 16-bit rows of this many pairs or more (EMV_STAR_PAIRS)
     int split_items = 32;    // split merge: items per tile aimed at (EMV_SPLIT_ITEMS)
     uint32_t *d_any = nullptr;
     uint32_t *d_cnt = nullptr;   // 4 rotating worklist counters
@@ -614,8 +615,12 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             rows.clear();
             subsets_of_size(I.k, I.stage, rows);
             const int nrows = (int)rows.size();
-            // rows of >= kSplitPairs pairs take the split merge
-            const bool splits = ((1u << (I.stage - 1)) - 1u) >= kSplitPairs && !e->no_split;
+            // rows of >= kSplitPairs pairs take the split merge, and so do 16-bit
+            // rows of >= star_pairs pairs, merged in stars
+            const uint32_t row_pairs = (1u << (I.stage - 1)) - 1u;
+            const bool h2 = !(I.flags & EMV_MERGE_1V) && merge_h2_ok(I.tbits, I.k, G.np);
+            const bool stars = h2 && e->star_merge && row_pairs >= e->star_pairs;
+            const bool splits = (row_pairs >= kSplitPairs || stars) && !e->no_split;
             if (!splits) {
                 maxrows = std::max(maxrows, nrows);
                 maxstage = std::max(maxstage, I.stage);
@@ -640,7 +645,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
             M.S = 0;
             M.tbits = I.tbits;
             M.mbase = 0;
-            M.h2 = (!(I.flags & EMV_MERGE_1V) && merge_h2_ok(I.tbits, I.k, G.np)) ? 1 : 0;
+            M.h2 = h2 ? 1 : 0;
             M.mtiles = M.h2 ? (int32_t)((G.np + 2 * kWave - 1) / (2 * kWave)) : M.ntiles;
             if (splits) {
                 // parts so that a tile has about split_items items (a step
@@ -648,7 +653,7 @@ void build_plan(emv_engine *e, const std::vector<int64_t> &tickets, StepPlan &P)
                 // atomics), each part >= 4 outer steps
                 const int64_t steps = (I.stage - 1 <= 3) ? 1 : (1ll << (I.stage - 1 - 3));
                 int64_t units = nrows;
-                if (M.h2 && e->star_merge) {
+                if (stars) {
                     // 16-bit cells: stars of rows sharing a base set
                     build_stars(rows.data(), nrows, I.k, tmp_stars);
                     M.star0 = (int32_t)P.stars.size();
@@ -1533,6 +1538,7 @@ int engine_open(int device, emv_engine **out) {
     e->no_split = getenv("EMV_NO_SPLIT") != nullptr;
     e->no_pack = getenv("EMV_NO_PACK") != nullptr;
     if (const char *ms = getenv("EMV_MERGE_STAR")) e->star_merge = atoi(ms) != 0;
+    if (const char *sp = getenv("EMV_STAR_PAIRS")) e->star_pairs = (uint32_t)std::max(1, atoi(sp));
     if (const char *si = getenv("EMV_SPLIT_ITEMS")) e->split_items = std::max(1, atoi(si));
     EHIP(e, hipMalloc((void **)&e->d_q[0], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
     EHIP(e, hipMalloc((void **)&e->d_q[1], sizeof(uint32_t) * std::max<uint32_t>(e->qcap, 1)));
@@ -2148,7 +2154,7 @@ extern "C" int emv_engine_certify(emv_engine *e, int64_t ticket, emv_certificate
         for (int j = 0; j < I.k; j++) srcs.push_back(std::vector<int32_t>(1, I.kk[(size_t)j]));
     const auto t0 = std::chrono::steady_clock::now();
     unsigned long long cnt[4];
-    EHIP(e, certify_table(e->stream, e->d_arena + I.abase, I.tbits / 8,
+    EHIP(e, certify_table(e->stream, e->d_arena + I.abase, rows_layout(G.np), I.tbits / 8,
                           e->d_arena + I.abase + S2 * G.np, G.np, (int)G.n, I.k,
                           e->d_rp + G.voff, e->d_adj, srcs, I.kk, I.groups.empty() ? 0 : 1, cnt));
     out->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -69,6 +69,7 @@ struct emv_ctx {
     int32_t *d_b = nullptr;
     size_t f_cap = 0, b_cap = 0;
     int64_t np = 0;
+    FLayout flay = rows_layout(0);  // cell layout of the table (tiles for 16-bit cells)
     int tbytes = 0;  // nonzero while a kept table is readable
     int64_t kept_cost = 0;  // answer cell of the last solve
     std::vector<int32_t> kept_kk;  // terminals of the last solve
@@ -217,6 +218,12 @@ namespace {
 bool env_flag(const char *name) {
     const char *e = getenv(name);
     return e && atoi(e) != 0;
+}
+
+// A default-on switch turned off: set to "0".
+bool env_off(const char *name) {
+    const char *e = getenv(name);
+    return e && atoi(e) == 0;
 }
 
 void set_err(emv_ctx *c, const char *fmt, ...) {
@@ -617,7 +624,7 @@ int launch_merge_xg_cw(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
             int64_t per;
             const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
             hipLaunchKernelGGL((k_merge_h2<P, DT, XG, CW>), dim3(grid), dim3(kBlock), 0,
-                               c->stream, f, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D,
+                               c->stream, f, c->flay, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D,
                                (P *)c->d_code, c->nxcd, per);
             HIPCHK(c, hipGetLastError());
             return EMV_OK;
@@ -627,7 +634,7 @@ int launch_merge_xg_cw(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     int64_t per;
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
     hipLaunchKernelGGL((k_merge_v<T, P, DT, XG, CW>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                       c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
+                       c->flay, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
                        per, merge_fbase<T>(c));
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
@@ -688,7 +695,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
             int64_t per;
             const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
             hipLaunchKernelGGL((k_merge_split_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                               c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd,
+                               c->flay, c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd,
                                per);
             h2 = true;
         }
@@ -698,7 +705,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
         int64_t per;
         const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
         hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                           c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
+                           c->flay, c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
                            merge_fbase<T>(c));
     }
     launch_pack<P, DT>(c, nx, nchunks, k);
@@ -763,7 +770,8 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
     if constexpr (std::is_same<T, uint16_t>::value) {
         const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
         const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
-        hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->np,
+        hipLaunchKernelGGL((k_merge_star_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
+                           c->flay, c->np,
                            (const MergeStar *)c->d_stars, nstars, (int)parts, ntiles, k,
                            (P *)c->d_M, c->nxcd, per);
     } else {
@@ -894,7 +902,7 @@ int launch_merge_cube(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
     // row's table offset in 128-byte units (rows are 2 np bytes, np a
     // multiple of 64), all ones for rows the merges never read
     constexpr int kPer = (1 << kCubeU) / kCubeWaves;
-    const int64_t units = 2 * c->np / 128;
+    const int64_t units = 2 * c->flay.rcells / 128;
     if ((((int64_t)1 << k) - 1) * units >= (int64_t)0xFFFFFFFF) {
         set_err(c, "table offsets exceed the cube table's 32 bits");
         return EMV_ERR_RANGE;
@@ -955,7 +963,7 @@ int launch_merge_cube(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, in
     const int grid = (int)std::max<int64_t>(
         c->nxcd, std::min<int64_t>((int64_t)c->num_cu, per * c->nxcd) / c->nxcd * c->nxcd);
     hipLaunchKernelGGL((k_merge_cube_h2<P>), dim3(grid), dim3(kCubeWaves * kWave),
-                       (size_t)kCubeLds, c->stream, f, c->np,
+                       (size_t)kCubeLds, c->stream, f, c->flay, c->np,
                        (const MergeCube *)c->d_cubes, (const CubeRow *)c->d_cube_rows,
                        (const uint32_t *)c->d_cube_tab, ncubes, ntiles, k, (P *)c->d_M, c->nxcd,
                        per);
@@ -1105,8 +1113,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 HIPCHK(c, hipEventRecord(e2, c->stream));
                 hipLaunchKernelGGL((k_unpack_v<T, uint32_t, P>), dim3(grid), dim3(kBlock), 0,
                                    c->stream, (const uint32_t *)c->d_D, (const P *)c->d_code, np,
-                                   c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd, per,
-                                   vtiles, (const unsigned long long *)c->d_rspan,
+                                   c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, c->flay, b, c->nxcd,
+                                   per, vtiles, (const unsigned long long *)c->d_rspan,
                                    c->fcopy_on ? c->d_fcopy : nullptr, c->d_fbase);
                 res->relative_label_layers++;
             } else {
@@ -1122,14 +1130,14 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                         hipLaunchKernelGGL((k_unpack_w<T, P>), dim3(wgrid),
                                            dim3(kWideWaves * kWave), 0, c->stream,
                                            (const uint32_t *)c->d_D, (const P *)c->d_code, np,
-                                           c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b,
-                                           c->nxcd, wper, vtiles);
+                                           c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, c->flay,
+                                           b, c->nxcd, wper, vtiles);
                         wide = true;
                     } else if (cw == 2 * kWave) {
                         hipLaunchKernelGGL((k_unpack_v<T, DT, P, 2 * kWave>), dim3(grid),
                                            dim3(kBlock), 0, c->stream, (const DT *)c->d_D,
                                            (const P *)c->d_code, np, c->d_xs, nx, (int)n, k,
-                                           c->d_rp, c->d_adj, f, b, c->nxcd, per, vtiles,
+                                           c->d_rp, c->d_adj, f, c->flay, b, c->nxcd, per, vtiles,
                                            (const unsigned long long *)nullptr,
                                            (int32_t *)nullptr, (int64_t *)nullptr);
                         wide = true;
@@ -1138,8 +1146,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                 if (!wide)
                     hipLaunchKernelGGL((k_unpack_v<T, DT, P>), dim3(grid), dim3(kBlock), 0,
                                        c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
-                                       c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, b, c->nxcd,
-                                       per, vtiles, (const unsigned long long *)nullptr,
+                                       c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, c->flay, b,
+                                       c->nxcd, per, vtiles, (const unsigned long long *)nullptr,
                                        (int32_t *)nullptr, (int64_t *)nullptr);
             }
             c->dense_rounds = 0;
@@ -1313,12 +1321,12 @@ void unpack_singletons(emv_ctx *c, int64_t k, T *f, int32_t *b) {
     const dim3 grid((unsigned)((c->n + kBlock - 1) / kBlock), (unsigned)k);
     if (c->sssp_bits == 32)
         hipLaunchKernelGGL((k_unpack_singletons_v<T, uint32_t>), grid, dim3(kBlock), 0, c->stream,
-                           (const uint32_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, b,
-                           c->d_rowmax);
+                           (const uint32_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, c->flay,
+                           b, c->d_rowmax);
     else
         hipLaunchKernelGGL((k_unpack_singletons_v<T, uint64_t>), grid, dim3(kBlock), 0, c->stream,
-                           (const uint64_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, b,
-                           c->d_rowmax);
+                           (const uint64_t *)c->d_D, c->np, c->d_src, (int)k, (int)c->n, f, c->flay,
+                           b, c->d_rowmax);
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -1597,7 +1605,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
     if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
-    c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->np);
+    // 16-bit tables in 128-vertex tiles of 2^k rows (EMV_TILE_TABLE=0: rows)
+    c->flay = (st.tbytes == 2 && !env_off("EMV_TILE_TABLE")) ? tiles_layout(k) : rows_layout(c->np);
+    c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->flay.rcells);
     c->merge_star = c->merge_h2 && !(flags & EMV_MERGE_ROWS) && merge_star_enabled();
     // cubes staged in LDS: opt-in (EMV_MERGE_CUBE or EMV_CUBE=1); the stars are
     // faster on every layer measured (n = 2^18, k = 14: layers 3-8 in 0.49 /
@@ -1628,17 +1638,19 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         st.dbits = (top < 0xFFFFFFFFull) ? 32 : 64;
 
     // ---- table allocation + fill
-    const int64_t cells = st.S2 * np;
+    const int64_t cells = st.S2 * np;  // records (and table cells in rows)
+    // tiles: ceil(np / 128) blocks of 2^k rows of 128 cells
+    const int64_t vcells = c->flay.ts < 62 ? (np + 127) / 128 * 128 : np;
     HIPCHK(c, hipEventRecord(e_zero0, c->stream));
-    CHK(ensure_table(c, (size_t)cells * st.tbytes, (size_t)cells * 4));
+    CHK(ensure_table(c, (size_t)(st.S2 * vcells) * st.tbytes, (size_t)cells * 4));
     {
         // Every cell of rows 1 .. 2^k-1 at v < n is written by the solve, and
         // the padding columns v >= n only feed lanes whose results are
         // dropped, so only row 0 (never written; INF and record -1 as the
         // reference's initialisation, reader-el.c:2244-2259) is filled.
         if (st.tbytes == 2)
-            hipLaunchKernelGGL(k_fill<uint16_t>, dim3(4), dim3(kBlock), 0, c->stream,
-                               (uint16_t *)c->d_f, np, (uint16_t)0xFFFF);
+            hipLaunchKernelGGL(k_fill_row<uint16_t>, dim3(4), dim3(kBlock), 0, c->stream,
+                               (uint16_t *)c->d_f, c->flay, (int64_t)0, vcells, (uint16_t)0xFFFF);
         else if (st.tbytes == 4)
             hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (int32_t *)c->d_f, np, INT32_MAX);
@@ -1708,8 +1720,8 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         cost = INT64_MAX;  // row 0 is never written (reader-el.c:2137 with k == 1)
     } else if (st.tbytes == 2) {
         uint16_t v16;
-        HIPCHK(c, hipMemcpyAsync(&v16, (uint16_t *)c->d_f + C * np + q, 2, hipMemcpyDeviceToHost,
-                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(&v16, (uint16_t *)c->d_f + c->flay.cell(C, q), 2,
+                                 hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         cost = v16;
     } else if (st.tbytes == 4) {
@@ -1884,8 +1896,17 @@ extern "C" int emv_read_rows(emv_ctx *c, int64_t X0, int64_t nrows, int64_t *f_o
     for (int64_t r = 0; r < nrows; r++) {
         const int64_t X = X0 + r;
         if (f_out) {
-            HIPCHK(c, hipMemcpy(buf.data(), (char *)c->d_f + (size_t)(X * np) * c->tbytes,
-                                (size_t)n * c->tbytes, hipMemcpyDeviceToHost));
+            if (c->flay.ts < 62) {
+                // tiles: 128 cells of row X per tile, one tile block apart
+                const size_t w = (size_t)128 * c->tbytes;
+                HIPCHK(c, hipMemcpy2D(buf.data(), w,
+                                      (char *)c->d_f + (size_t)c->flay.cell(X, 0) * c->tbytes,
+                                      (size_t)c->flay.tcells * c->tbytes, w,
+                                      (size_t)((n + 127) / 128), hipMemcpyDeviceToHost));
+            } else {
+                HIPCHK(c, hipMemcpy(buf.data(), (char *)c->d_f + (size_t)(X * np) * c->tbytes,
+                                    (size_t)n * c->tbytes, hipMemcpyDeviceToHost));
+            }
             for (int64_t v = 0; v < n; v++) {
                 int64_t val;
                 if (c->tbytes == 2) {
@@ -1963,7 +1984,7 @@ extern "C" int emv_sssp(emv_ctx *c, int64_t s, int64_t *d_out, int64_t *pred_out
 namespace {
 
 // Runs the certificate over a table on the device and fills *out.
-int run_certificate(emv_ctx *c, const void *f, int tb, const int32_t *b, int64_t np, int k,
+int run_certificate(emv_ctx *c, const void *f, FLayout fl, int tb, const int32_t *b, int64_t np, int k,
                     const std::vector<int32_t> &kk, emv_certificate *out) {
     std::vector<std::vector<int32_t>> srcs((size_t)k);
     for (int t = 0; t < k; t++) srcs[(size_t)t].push_back(kk[(size_t)t]);
@@ -1975,7 +1996,7 @@ int run_certificate(emv_ctx *c, const void *f, int tb, const int32_t *b, int64_t
     evl.v.push_back(e);
     HIPCHK(c, hipEventRecord(a, c->stream));
     unsigned long long cnt[4];
-    HIPCHK(c, certify_table(c->stream, f, tb, b, np, (int)c->n, k, c->d_rp, c->d_adj, srcs, kk, 0,
+    HIPCHK(c, certify_table(c->stream, f, fl, tb, b, np, (int)c->n, k, c->d_rp, c->d_adj, srcs, kk, 0,
                             cnt));
     HIPCHK(c, hipEventRecord(e, c->stream));
     HIPCHK(c, hipEventSynchronize(e));
@@ -2002,7 +2023,8 @@ extern "C" int emv_certify(emv_ctx *c, emv_certificate *out) {
         return EMV_ERR_STATE;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    return run_certificate(c, c->d_f, c->tbytes, c->d_b, c->np, (int)c->tk, c->kept_kk, out);
+    return run_certificate(c, c->d_f, c->flay, c->tbytes, c->d_b, c->np, (int)c->tk, c->kept_kk,
+                           out);
 }
 
 extern "C" int emv_certify_table(emv_ctx *c, int64_t k, const int64_t *kk, const int64_t *f,
@@ -2039,7 +2061,7 @@ extern "C" int emv_certify_table(emv_ctx *c, int64_t k, const int64_t *kk, const
         set_err(c, "table upload failed: %s", hipGetErrorString(err));
         rc = (err == hipErrorOutOfMemory) ? EMV_ERR_NOMEM : EMV_ERR_HIP;
     } else {
-        rc = run_certificate(c, d_f, 8, d_b, c->n, (int)k, kk32, out);
+        rc = run_certificate(c, d_f, rows_layout(c->n), 8, d_b, c->n, (int)k, kk32, out);
     }
     (void)hipFree(d_f);
     if (d_b) (void)hipFree(d_b);

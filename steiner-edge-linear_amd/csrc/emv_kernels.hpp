@@ -56,12 +56,40 @@ __device__ __forceinline__ int64_t xcd_item(int nxcd, int64_t per) {
     return (b % nxcd) * per + (b / nxcd);
 }
 
+// Cell layout of a DP table: cell (X, v) at (v >> ts) * tcells + X * rcells +
+// (v & (2^ts - 1)).  Rows (ts = 62, tcells = 0, rcells = np): row X is
+// contiguous, the reference's f[X*n + v] with a padded stride.  Tiles
+// (16-bit tables, ts = 7): the 2^k rows of a 128-vertex tile are one
+// contiguous block of 2^k * 256 bytes (tcells = 2^k * 128, rcells = 128), so
+// the merges, which read one tile's blocks of many rows, stay within a few
+// pages instead of touching one page per row.
+struct FLayout {
+    int64_t tcells;
+    int64_t rcells;
+    int32_t ts;
+    int32_t pad_;
+    __host__ __device__ int64_t cell(int64_t X, int64_t v) const {
+        return (v >> ts) * tcells + X * rcells + (v & ((1ll << ts) - 1));
+    }
+};
+
+inline FLayout rows_layout(int64_t np) { return FLayout{0, np, 62, 0}; }
+inline FLayout tiles_layout(int k) { return FLayout{(int64_t)128 << k, 128, 7, 0}; }
+
 // ---------------------------------------------------------------- fills
 template <typename T>
 __global__ void k_fill(T *__restrict__ p, int64_t count, T val) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < count; i += stride) p[i] = val;
+}
+
+// Row X of a table in layout fl, cells v < nv, set to val.
+template <typename T>
+__global__ void k_fill_row(T *__restrict__ f, FLayout fl, int64_t X, int64_t nv, T val) {
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; v < nv; v += stride) f[fl.cell(X, v)] = val;
 }
 
 // ------------------------------------------------------------- seeding
@@ -1181,7 +1209,7 @@ __device__ __forceinline__ P merge_any(const T *__restrict__ fv, const int64_t *
 // each lane writes its XG labels (and packed words) as one contiguous run.
 // Rows past nx (the tail of the last chunk) are written as INF.
 template <typename T, typename P, typename DT, int XG, int CW = kWave>
-__global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int64_t np,
+__global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, FLayout fl, int64_t np,
                                                     const uint32_t *__restrict__ xs, int nx,
                                                     int nsub, int64_t ntiles, int k,
                                                     DT *__restrict__ D, P *__restrict__ CODE,
@@ -1195,7 +1223,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
     const int sub = (int)(item - tile * nsub);
     if (tile >= ntiles) return;
     const int64_t v = tile * kWave + lane;
-    const T *fv = f + v;
+    const T *fv = f + fl.cell(0, v);
     const DT INF = (DT)~(DT)0;
     DT dv[XG];
     P pv[XG];
@@ -1206,7 +1234,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
         pv[jx] = ~(P)0;
         if (xi < nx) {
             const uint32_t X = xs[xi];
-            pv[jx] = merge_any<T, P>(fv, fbase, np, X, k, 0u, merge_steps(X));
+            pv[jx] = merge_any<T, P>(fv, fbase, fl.rcells, X, k, 0u, merge_steps(X));
             dv[jx] = (DT)(pv[jx] >> k);
         }
     }
@@ -1231,7 +1259,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_v(const T *__restrict__ f, int
 // does not depend on the order, so the split leaves every result unchanged.
 // M starts at all ones; k_pack_v moves it to the staging.
 template <typename T, typename P>
-__global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f, int64_t np,
+__global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f, FLayout fl, int64_t np,
                                                         const uint32_t *__restrict__ xs, int nx,
                                                         int S, int64_t ntiles, int k,
                                                         P *__restrict__ M, int nxcd,
@@ -1253,7 +1281,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_split(const T *__restrict__ f,
     const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / S);
     if (o0 >= o1) return;
     const int64_t v = tile * kWave + lane;
-    const P acc = merge_any<T, P>(f + v, fbase, np, X, k, o0, o1);
+    const P acc = merge_any<T, P>(f + fl.cell(0, v), fbase, fl.rcells, X, k, o0, o1);
     atomicMin(&M[(int64_t)xi * np + v], acc);
 }
 
@@ -1283,16 +1311,17 @@ __device__ __forceinline__ uint32_t ld_pair(__amdgpu_buffer_rsrc_t r, uint32_t v
 
 // As merge_outer over the outer steps [o0, o1), for the vertex pair at byte
 // offset lb (4-aligned, < 256) from the wave-uniform byte base fb; klo/khi
-// fold the keys of the low / high vertex.  Row offsets of the three low bits
-// L stay in per-lane 32-bit offsets, which needs 2 * L * np + 256 < 2^32 for
-// the largest L of the solve (the host checks it).
+// fold the keys of the low / high vertex; rows are rb bytes apart.  Row
+// offsets of the three low bits L stay in per-lane 32-bit offsets, which
+// needs L * rb + 256 < 2^32 for the largest L of the solve (the host checks
+// it).
 __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint32_t lb,
-                                               int64_t np, uint32_t X, uint32_t o0, uint32_t o1,
+                                               int64_t rb, uint32_t X, uint32_t o0, uint32_t o1,
                                                uint32_t &klo, uint32_t &khi) {
     // kept in a VGPR so that (sum & hmask) | Y is one v_and_or_b32 with Y in an SGPR
     uint32_t hmask = 0xFFFF0000u;
     asm volatile("" : "+v"(hmask));
-    const int64_t rb = 2 * np;  // bytes per row
+    // rb: bytes from a row's cell to the next row's (2 np rows, 256 tiles)
     const uint32_t Xp = X ^ (1u << (31 - __clz(X)));
     uint32_t L = 0, H = Xp;
     for (int i = 0; i < 3 && H; i++) {
@@ -1402,14 +1431,14 @@ __device__ __forceinline__ void merge_outer_h2(const char *__restrict__ fb, uint
 // bits, and lane offsets of the low rows within 32 bits (the low rows are
 // the three lowest bits of X without its top bit, at most bits k-4..k-2).
 // EMV_MERGE_H2=0 keeps one vertex per lane.
-inline bool merge_h2_ok(int tbits, int k, int64_t np) {
+inline bool merge_h2_ok(int tbits, int k, int64_t rcells) {
     static const bool on = [] {
         const char *e = getenv("EMV_MERGE_H2");
         return !(e && atoi(e) == 0);
     }();
     if (!on || tbits != 16 || k > 16 || k < 2) return false;
     const uint64_t lmax = k >= 4 ? 7ull << (k - 4) : (1ull << (k - 1)) - 1ull;
-    return 2ull * lmax * (uint64_t)np + 256ull < (1ull << 32);
+    return 2ull * lmax * (uint64_t)rcells + 256ull < (1ull << 32);
 }
 
 // Host: vertex-wise rounds 1 .. R of a layer's relaxation that take every
@@ -1433,7 +1462,7 @@ __device__ __forceinline__ P key_packed(uint32_t key, int k) {
 // that is not a multiple of 128 the upper half of the lanes has no vertices:
 // they read the last pair and store nothing.
 template <typename P, typename DT, int XG, int CW = kWave>
-__global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict__ f, int64_t np,
+__global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict__ f, FLayout fl, int64_t np,
                                                      const uint32_t *__restrict__ xs, int nx,
                                                      int nsub, int64_t ntiles, int k,
                                                      DT *__restrict__ D, P *__restrict__ CODE,
@@ -1448,7 +1477,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict_
     const int64_t v0 = tile * (2 * kWave);
     const int64_t v = v0 + 2 * lane;
     const bool live = v < np;
-    const char *fb = reinterpret_cast<const char *>(f + v0);
+    const char *fb = reinterpret_cast<const char *>(f + fl.cell(0, v0));
     const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
     const DT INF = (DT)~(DT)0;
     uint32_t klo[XG], khi[XG];
@@ -1458,7 +1487,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict_
         klo[jx] = khi[jx] = 0xFFFFFFFFu;
         if (xi < nx) {
             const uint32_t X = xs[xi];
-            merge_outer_h2(fb, lb, np, X, 0u, merge_steps(X), klo[jx], khi[jx]);
+            merge_outer_h2(fb, lb, 2 * fl.rcells, X, 0u, merge_steps(X), klo[jx], khi[jx]);
         }
     }
     if (!live) return;
@@ -1479,7 +1508,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_h2(const uint16_t *__restrict_
 // k_merge_split for 16-bit cells, two vertices per lane (128-vertex tiles).
 template <typename P>
 __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__restrict__ f,
-                                                           int64_t np,
+                                                           FLayout fl, int64_t np,
                                                            const uint32_t *__restrict__ xs,
                                                            int nx, int S, int64_t ntiles, int k,
                                                            P *__restrict__ M, int nxcd,
@@ -1504,7 +1533,8 @@ __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__res
     const bool live = v < np;
     const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
     uint32_t klo = 0xFFFFFFFFu, khi = 0xFFFFFFFFu;
-    merge_outer_h2(reinterpret_cast<const char *>(f + v0), lb, np, X, o0, o1, klo, khi);
+    merge_outer_h2(reinterpret_cast<const char *>(f + fl.cell(0, v0)), lb, 2 * fl.rcells, X, o0,
+                   o1, klo, khi);
     if (!live) return;
     atomicMin(&M[(int64_t)xi * np + v], key_packed<P>(klo, k));
     atomicMin(&M[(int64_t)xi * np + v + 1], key_packed<P>(khi, k));
@@ -1588,14 +1618,14 @@ __device__ __forceinline__ uint32_t star_steps(uint32_t S) {
 // of 8 (R + 1)).
 template <int R>
 __device__ __forceinline__ void star_h2_pairs_r(const char *__restrict__ fb, uint32_t lb,
-                                                int64_t np, uint32_t S,
+                                                int64_t rb, uint32_t S,
                                                 const uint32_t (&a)[kStarMax], uint32_t o0,
                                                 uint32_t o1, uint32_t (&klo)[kStarMax],
                                                 uint32_t (&khi)[kStarMax]) {
     constexpr int r = R;
     uint32_t hmask = 0xFFFF0000u;
     asm volatile("" : "+v"(hmask));
-    const int64_t rb = 2 * np;
+    // rb: bytes from a row's cell to the next row's
     const uint32_t topS = 1u << (31 - __clz(S));
     uint32_t L = 0, H = S;
     for (int q = 0; q < 3 && H; q++) {
@@ -1694,16 +1724,16 @@ __device__ __forceinline__ void star_h2_pairs_r(const char *__restrict__ fb, uin
 }
 
 __device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint32_t lb,
-                                              int64_t np, uint32_t S, int r,
+                                              int64_t rb, uint32_t S, int r,
                                               const uint32_t (&a)[kStarMax], uint32_t o0,
                                               uint32_t o1, uint32_t (&klo)[kStarMax],
                                               uint32_t (&khi)[kStarMax]) {
     static_assert(kStarMax == 4, "one case per star size");
     switch (r) {
-    case 1: star_h2_pairs_r<1>(fb, lb, np, S, a, o0, o1, klo, khi); break;
-    case 2: star_h2_pairs_r<2>(fb, lb, np, S, a, o0, o1, klo, khi); break;
-    case 3: star_h2_pairs_r<3>(fb, lb, np, S, a, o0, o1, klo, khi); break;
-    default: star_h2_pairs_r<4>(fb, lb, np, S, a, o0, o1, klo, khi); break;
+    case 1: star_h2_pairs_r<1>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 2: star_h2_pairs_r<2>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 3: star_h2_pairs_r<3>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    default: star_h2_pairs_r<4>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
     }
 }
 
@@ -1711,7 +1741,8 @@ __device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint3
 // unsigned atomicMin like k_merge_split_h2.  Two vertices per lane.
 template <typename P>
 __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
-    const uint16_t *__restrict__ f, int64_t np, const MergeStar *__restrict__ stars, int nstars,
+    const uint16_t *__restrict__ f, FLayout fl, int64_t np, const MergeStar *__restrict__ stars,
+    int nstars,
     int parts, int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1736,9 +1767,9 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
     const int64_t v = v0 + 2 * lane;
     const bool live = v < np;
     const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
-    const char *fb = reinterpret_cast<const char *>(f + v0);
+    const char *fb = reinterpret_cast<const char *>(f + fl.cell(0, v0));
     uint32_t klo[kStarMax], khi[kStarMax];
-    star_h2_pairs(fb, lb, np, S, r, a, o0, o1, klo, khi);
+    star_h2_pairs(fb, lb, 2 * fl.rcells, S, r, a, o0, o1, klo, khi);
     if (!live) return;
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) {
@@ -1951,7 +1982,7 @@ struct CubeRow {
 // costs each wave one scalar table read per row.
 template <typename P>
 __global__ __launch_bounds__(kCubeWaves * kWave) void k_merge_cube_h2(
-    const uint16_t *__restrict__ f, int64_t np, const MergeCube *__restrict__ cubes,
+    const uint16_t *__restrict__ f, FLayout fl, int64_t np, const MergeCube *__restrict__ cubes,
     const CubeRow *__restrict__ rows, const uint32_t *__restrict__ rowtab, int ncubes,
     int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
     extern __shared__ uint32_t lds_rows[];  // [2^u][64] staged rows, then the row list
@@ -1984,7 +2015,7 @@ __global__ __launch_bounds__(kCubeWaves * kWave) void k_merge_cube_h2(
         const int64_t v0 = tile * (2 * kWave);
         // lanes past the row read its last pair; their results are dropped
         const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
-        const char *fb = reinterpret_cast<const char *>(f + v0) + lb;
+        const char *fb = reinterpret_cast<const char *>(f + fl.cell(0, v0)) + lb;
         const uint32_t *tab = rowtab + (int64_t)cb * (1 << kCubeU) + wid * kPer;
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
@@ -2161,7 +2192,8 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      const uint32_t *__restrict__ xs, int nx,
                                                      int n, int k, const int32_t *__restrict__ rp,
                                                      const Edge *__restrict__ adj,
-                                                     T *__restrict__ f, int32_t *__restrict__ b,
+                                                     T *__restrict__ f, FLayout fl,
+                                                     int32_t *__restrict__ b,
                                                      int nxcd, int64_t per_xcd, int64_t vtiles,
                                                      const unsigned long long *__restrict__ rspan,
                                                      int32_t *__restrict__ fcopy,
@@ -2217,7 +2249,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
         if (xi < nx && v < n) {
             const int64_t X = xs[xi];
             const unsigned long long base = rspan ? rspan[2 * xi] : 0ull;
-            f[X * np + v] = (T)((unsigned long long)sd[lane][xl] + base);
+            f[fl.cell(X, v)] = (T)((unsigned long long)sd[lane][xl] + base);
             b[X * np + v] = sc[lane][xl];
             // compact copy (relative labels that fit int32): the relative
             // label itself, with the row's base
@@ -2282,8 +2314,8 @@ template <typename T, typename P>
 __global__ __launch_bounds__(kWideWaves * kWave) void k_unpack_w(
     const uint32_t *__restrict__ D, const P *__restrict__ CODE, int64_t np,
     const uint32_t *__restrict__ xs, int nx, int n, int k, const int32_t *__restrict__ rp,
-    const Edge *__restrict__ adj, T *__restrict__ f, int32_t *__restrict__ b, int nxcd,
-    int64_t per_xcd, int64_t vtiles) {
+    const Edge *__restrict__ adj, T *__restrict__ f, FLayout fl, int32_t *__restrict__ b,
+    int nxcd, int64_t per_xcd, int64_t vtiles) {
     constexpr int CW = 2 * kWave;
     // [vertex][row]; odd row strides (in 32-bit words) keep the column reads
     // of the write-out free of bank conflicts
@@ -2336,7 +2368,7 @@ __global__ __launch_bounds__(kWideWaves * kWave) void k_unpack_w(
         if (xi < nx && v < n) {
             const int64_t X = xs[xi];
             const uint32_t w = sd[lane][xl >> 1];
-            f[X * np + v] = (T)((xl & 1) ? (w >> 16) : (w & 0xFFFFu));
+            f[fl.cell(X, v)] = (T)((xl & 1) ? (w >> 16) : (w & 0xFFFFu));
             b[X * np + v] = sc[lane][xl];
         }
     }
@@ -2418,7 +2450,7 @@ __global__ void k_rel_labels(const P *__restrict__ CODE, int64_t np, int n, int 
 template <typename T, typename DT>
 __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
     const DT *__restrict__ D, int64_t np, const int32_t *__restrict__ kk, int k, int n,
-    T *__restrict__ f, int32_t *__restrict__ b, unsigned long long *__restrict__ rowmax) {
+    T *__restrict__ f, FLayout fl, int32_t *__restrict__ b, unsigned long long *__restrict__ rowmax) {
     const int t = blockIdx.y;
     const int v = blockIdx.x * kBlock + threadIdx.x;
     unsigned long long d = 0;
@@ -2427,7 +2459,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_singletons_v(
         d = (x == (DT)~(DT)0) ? ~0ull : (unsigned long long)x;
         if (f) {
             const int64_t X = 1ll << t;
-            f[X * np + v] = (T)d;
+            f[fl.cell(X, v)] = (T)d;
             b[X * np + v] = (v == kk[t]) ? (int32_t)X : (int32_t)(((int64_t)1 << k) + kk[t]);
         }
     }

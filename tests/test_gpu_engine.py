@@ -419,16 +419,20 @@ def test_batch_paired_chunks(monkeypatch, pair, no_ell):
         _check_batch_vs_oracle(insts, c2, b)
 
 
-@pytest.mark.parametrize("no_split,stars", [(False, "1"), (False, "0"), (True, "1")])
-def test_batch_split_merge_upper_layers(monkeypatch, no_split, stars):
+@pytest.mark.parametrize("no_split,stars,star_pairs", [(False, "1", "63"), (False, "1", "3"),
+                                                        (False, "0", "63"), (True, "1", "63")])
+def test_batch_split_merge_upper_layers(monkeypatch, no_split, stars, star_pairs):
     """Instances with k >= 9 reach layers whose rows have >= 128 pairs: the
     engine merges the 16-bit tables' rows in stars (rows sharing a base set,
-    128-vertex tiles, a last tile half past the vertices), cuts the other
-    rows into parts (EMV_MERGE_STAR=0 for all), both folded with atomicMin
-    (16/32-bit labels, 32- and 64-bit packed words: the last instance's bound
-    needs 64), or, with EMV_NO_SPLIT, merges them whole.  Tables and records
-    equal the oracle's, eagerly and from the replay."""
+    128-vertex tiles, a last tile half past the vertices; also the 16-bit
+    rows of >= EMV_STAR_PAIRS pairs, 63 by default, 3 here for every layer
+    from 3 terminals up), cuts the other rows into parts (EMV_MERGE_STAR=0
+    for all), both folded with atomicMin (16/32-bit labels, 32- and 64-bit
+    packed words: the last instance's bound needs 64), or, with
+    EMV_NO_SPLIT, merges them whole.  Tables and records equal the oracle's,
+    eagerly and from the replay."""
     monkeypatch.setenv("EMV_MERGE_STAR", stars)
+    monkeypatch.setenv("EMV_STAR_PAIRS", star_pairs)
     if no_split:
         monkeypatch.setenv("EMV_NO_SPLIT", "1")
     insts = []
