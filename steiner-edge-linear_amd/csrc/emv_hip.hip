@@ -69,7 +69,8 @@ struct emv_ctx {
     int32_t *d_b = nullptr;
     size_t f_cap = 0, b_cap = 0;
     int64_t np = 0;
-    FLayout flay = rows_layout(0);  // cell layout of the table (tiles for 16-bit cells)
+    FLayout flay = rows_layout(0);  // cell layout of the table (tiles for 16/32-bit cells)
+    FLayout fclay = rows_layout(0); // cell layout of the int32 copy of an int64 table
     int tbytes = 0;  // nonzero while a kept table is readable
     int64_t kept_cost = 0;  // answer cell of the last solve
     std::vector<int32_t> kept_kk;  // terminals of the last solve
@@ -610,6 +611,13 @@ const int64_t *merge_fbase(const emv_ctx *c) {
     return (std::is_same<T, int32_t>::value && c->fcopy_on) ? c->d_fbase : nullptr;
 }
 
+// Cell layout of the cells a merge of T reads: the int32 copy's while an
+// int64 table merges from it, else the table's.
+template <typename T>
+FLayout merge_layout(const emv_ctx *c) {
+    return (std::is_same<T, int32_t>::value && c->fcopy_on) ? c->fclay : c->flay;
+}
+
 template <typename T>
 bool use_merge_h2(const emv_ctx *c) {
     return std::is_same<T, uint16_t>::value && c->merge_h2;
@@ -634,7 +642,7 @@ int launch_merge_xg_cw(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k) {
     int64_t per;
     const int grid = grid_for((ntiles * nsub + 3) / 4, c->nxcd, &per);
     hipLaunchKernelGGL((k_merge_v<T, P, DT, XG, CW>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                       c->flay, c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
+                       merge_layout<T>(c), c->np, c->d_xs, nx, nsub, ntiles, k, (DT *)c->d_D, (P *)c->d_code, c->nxcd,
                        per, merge_fbase<T>(c));
     HIPCHK(c, hipGetLastError());
     return EMV_OK;
@@ -705,7 +713,7 @@ int launch_merge_split(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, i
         int64_t per;
         const int grid = grid_for((ntiles * nx * S + 3) / 4, c->nxcd, &per);
         hipLaunchKernelGGL((k_merge_split<T, P>), dim3(grid), dim3(kBlock), 0, c->stream, f,
-                           c->flay, c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
+                           merge_layout<T>(c), c->np, c->d_xs, nx, (int)S, ntiles, k, (P *)c->d_M, c->nxcd, per,
                            merge_fbase<T>(c));
     }
     launch_pack<P, DT>(c, nx, nchunks, k);
@@ -781,11 +789,13 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
         const int64_t *fb = merge_fbase<T>(c);
         if (fb)
             hipLaunchKernelGGL((k_merge_star_w<P, true>), dim3(grid), dim3(kBlock), 0, c->stream,
-                               f, fb, c->np, (const MergeStar *)c->d_stars, nstars, (int)parts,
+                               f, merge_layout<T>(c), fb, c->np, (const MergeStar *)c->d_stars,
+                               nstars, (int)parts,
                                ntiles, k, (P *)c->d_M, c->nxcd, per);
         else
             hipLaunchKernelGGL((k_merge_star_w<P, false>), dim3(grid), dim3(kBlock), 0, c->stream,
-                               f, fb, c->np, (const MergeStar *)c->d_stars, nstars, (int)parts,
+                               f, merge_layout<T>(c), fb, c->np, (const MergeStar *)c->d_stars,
+                               nstars, (int)parts,
                                ntiles, k, (P *)c->d_M, c->nxcd, per);
     }
     launch_pack<P, DT>(c, nx, nchunks, k);
@@ -1115,7 +1125,7 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                                    c->stream, (const uint32_t *)c->d_D, (const P *)c->d_code, np,
                                    c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, c->flay, b, c->nxcd,
                                    per, vtiles, (const unsigned long long *)c->d_rspan,
-                                   c->fcopy_on ? c->d_fcopy : nullptr, c->d_fbase);
+                                   c->fcopy_on ? c->d_fcopy : nullptr, c->fclay, c->d_fbase);
                 res->relative_label_layers++;
             } else {
                 CHK(relax_chunks<DT>(c, nchunks, &res->relax_rounds));
@@ -1139,7 +1149,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                                            (const P *)c->d_code, np, c->d_xs, nx, (int)n, k,
                                            c->d_rp, c->d_adj, f, c->flay, b, c->nxcd, per, vtiles,
                                            (const unsigned long long *)nullptr,
-                                           (int32_t *)nullptr, (int64_t *)nullptr);
+                                           (int32_t *)nullptr, c->fclay,
+                                           (int64_t *)nullptr);
                         wide = true;
                     }
                 }
@@ -1148,7 +1159,8 @@ int run_layers(emv_ctx *c, const SolveState &st, T *f, int32_t *b, emv_result *r
                                        c->stream, (const DT *)c->d_D, (const P *)c->d_code, np,
                                        c->d_xs, nx, (int)n, k, c->d_rp, c->d_adj, f, c->flay, b,
                                        c->nxcd, per, vtiles, (const unsigned long long *)nullptr,
-                                       (int32_t *)nullptr, (int64_t *)nullptr);
+                                       (int32_t *)nullptr, c->fclay,
+                                           (int64_t *)nullptr);
             }
             c->dense_rounds = 0;
             HIPCHK(c, hipGetLastError());
@@ -1605,8 +1617,14 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
     st.tbytes = (B < (unsigned long long)INT32_MAX && !(flags & EMV_FORCE_I64)) ? 4 : 8;
     if (st.tbytes == 4 && B < 0xFFFFull && !(flags & EMV_TABLE32)) st.tbytes = 2;
     st.pbits = (st.tbytes <= 4 && bits_for(B) + k <= 32 && !(flags & EMV_FORCE_P64)) ? 32 : 64;
-    // 16-bit tables in 128-vertex tiles of 2^k rows (EMV_TILE_TABLE=0: rows)
-    c->flay = (st.tbytes == 2 && !env_off("EMV_TILE_TABLE")) ? tiles_layout(k) : rows_layout(c->np);
+    // 16-bit tables in 128-vertex tiles of 2^k rows, int32 tables and the
+    // int32 copy of an int64 table in 64-vertex tiles (256 bytes a row either
+    // way); int64 tables in rows (EMV_TILE_TABLE=0: every table in rows)
+    const bool tiles = !env_off("EMV_TILE_TABLE");
+    c->flay = (tiles && st.tbytes == 2)   ? tiles_layout(k, 7)
+              : (tiles && st.tbytes == 4) ? tiles_layout(k, 6)
+                                          : rows_layout(c->np);
+    c->fclay = tiles ? tiles_layout(k, 6) : rows_layout(c->np);
     c->merge_h2 = st.tbytes == 2 && !(flags & EMV_MERGE_1V) && merge_h2_ok(16, k, c->flay.rcells);
     c->merge_star = c->merge_h2 && !(flags & EMV_MERGE_ROWS) && merge_star_enabled();
     // cubes staged in LDS: opt-in (EMV_MERGE_CUBE or EMV_CUBE=1); the stars are
@@ -1639,8 +1657,9 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
 
     // ---- table allocation + fill
     const int64_t cells = st.S2 * np;  // records (and table cells in rows)
-    // tiles: ceil(np / 128) blocks of 2^k rows of 128 cells
-    const int64_t vcells = c->flay.ts < 62 ? (np + 127) / 128 * 128 : np;
+    // tiles: ceil(np / 2^ts) blocks of 2^k rows of 2^ts cells
+    const int64_t tw = c->flay.ts < 62 ? (int64_t)1 << c->flay.ts : 1;
+    const int64_t vcells = (np + tw - 1) / tw * tw;
     HIPCHK(c, hipEventRecord(e_zero0, c->stream));
     CHK(ensure_table(c, (size_t)(st.S2 * vcells) * st.tbytes, (size_t)cells * 4));
     {
@@ -1652,8 +1671,8 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
             hipLaunchKernelGGL(k_fill_row<uint16_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (uint16_t *)c->d_f, c->flay, (int64_t)0, vcells, (uint16_t)0xFFFF);
         else if (st.tbytes == 4)
-            hipLaunchKernelGGL(k_fill<int32_t>, dim3(4), dim3(kBlock), 0, c->stream,
-                               (int32_t *)c->d_f, np, INT32_MAX);
+            hipLaunchKernelGGL(k_fill_row<int32_t>, dim3(4), dim3(kBlock), 0, c->stream,
+                               (int32_t *)c->d_f, c->flay, (int64_t)0, vcells, INT32_MAX);
         else
             hipLaunchKernelGGL(k_fill<int64_t>, dim3(4), dim3(kBlock), 0, c->stream,
                                (int64_t *)c->d_f, np, INT64_MAX);
@@ -1680,7 +1699,7 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
                                dim3(kBlock), 0, c->stream, c->d_fbase, st.S2, (int64_t)0);
             hipLaunchKernelGGL(k_copy_singletons<int32_t>, dim3((unsigned)((n + kBlock - 1) / kBlock), (unsigned)k),
                                dim3(kBlock), 0, c->stream, (const int64_t *)c->d_f, np, (int)n, (int)k,
-                               c->d_fcopy);
+                               c->d_fcopy, c->fclay);
             HIPCHK(c, hipGetLastError());
             c->fcopy_on = true;
         }
@@ -1726,8 +1745,8 @@ extern "C" int emv_solve(emv_ctx *c, int64_t k, const int64_t *kk, uint32_t flag
         cost = v16;
     } else if (st.tbytes == 4) {
         int32_t v32;
-        HIPCHK(c, hipMemcpyAsync(&v32, (int32_t *)c->d_f + C * np + q, 4, hipMemcpyDeviceToHost,
-                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(&v32, (int32_t *)c->d_f + c->flay.cell(C, q), 4,
+                                 hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         cost = v32;
     } else {
@@ -1897,12 +1916,13 @@ extern "C" int emv_read_rows(emv_ctx *c, int64_t X0, int64_t nrows, int64_t *f_o
         const int64_t X = X0 + r;
         if (f_out) {
             if (c->flay.ts < 62) {
-                // tiles: 128 cells of row X per tile, one tile block apart
-                const size_t w = (size_t)128 * c->tbytes;
+                // tiles: 2^ts cells of row X per tile, one tile block apart
+                const int64_t tw = (int64_t)1 << c->flay.ts;
+                const size_t w = (size_t)tw * c->tbytes;
                 HIPCHK(c, hipMemcpy2D(buf.data(), w,
                                       (char *)c->d_f + (size_t)c->flay.cell(X, 0) * c->tbytes,
                                       (size_t)c->flay.tcells * c->tbytes, w,
-                                      (size_t)((n + 127) / 128), hipMemcpyDeviceToHost));
+                                      (size_t)((n + tw - 1) / tw), hipMemcpyDeviceToHost));
             } else {
                 HIPCHK(c, hipMemcpy(buf.data(), (char *)c->d_f + (size_t)(X * np) * c->tbytes,
                                     (size_t)n * c->tbytes, hipMemcpyDeviceToHost));

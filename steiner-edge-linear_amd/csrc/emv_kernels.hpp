@@ -74,7 +74,10 @@ struct FLayout {
 };
 
 inline FLayout rows_layout(int64_t np) { return FLayout{0, np, 62, 0}; }
-inline FLayout tiles_layout(int k) { return FLayout{(int64_t)128 << k, 128, 7, 0}; }
+// tiles of 2^ts vertices (7 for 16-bit cells, 6 for 32-bit: 256 bytes a row)
+inline FLayout tiles_layout(int k, int ts = 7) {
+    return FLayout{(int64_t)1 << (k + ts), (int64_t)1 << ts, ts, 0};
+}
 
 // ---------------------------------------------------------------- fills
 template <typename T>
@@ -1792,7 +1795,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
 // together.
 template <typename P, bool REL, int R>
 __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
-                                            const int64_t *__restrict__ fbase, int64_t np,
+                                            const int64_t *__restrict__ fbase, int64_t rs,
                                             uint32_t S, const uint32_t *a, uint32_t o0,
                                             uint32_t o1, int k, P *acc) {
     const int lane = __lane_id();
@@ -1816,15 +1819,15 @@ __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
         uint32_t Th = deposit_bits(nH - 1u - o0, H);
         const uint32_t oend = o1 < nH - 1u ? o1 : nH - 1u;  // Th == 0 last, apart
         for (uint32_t o = o0; o < oend; o++) {
-            const int32_t *tb = fv0 + (int64_t)Th * np;
+            const int32_t *tb = fv0 + (int64_t)Th * rs;
             int32_t t[8], pv[R][8];
 #pragma unroll
-            for (int j = 0; j < 8; j++) t[j] = tb[(int64_t)yl[j] * np + lane];
+            for (int j = 0; j < 8; j++) t[j] = tb[(int64_t)yl[j] * rs + lane];
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int32_t *pb = fv0 + (int64_t)((H ^ Th) | a[i]) * np;
+                const int32_t *pb = fv0 + (int64_t)((H ^ Th) | a[i]) * rs;
 #pragma unroll
-                for (int j = 0; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * np + lane];
+                for (int j = 0; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * rs + lane];
             }
             const bool flip = (Th & topS) != 0u;
 #pragma unroll
@@ -1844,12 +1847,12 @@ __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
         if (o1 == nH) {  // Th == 0: the seven nonempty subsets of L (topS lies in H: Y = T)
             int32_t t[8], pv[R][8];
 #pragma unroll
-            for (int j = 1; j < 8; j++) t[j] = fv0[(int64_t)yl[j] * np + lane];
+            for (int j = 1; j < 8; j++) t[j] = fv0[(int64_t)yl[j] * rs + lane];
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int32_t *pb = fv0 + (int64_t)(H | a[i]) * np;
+                const int32_t *pb = fv0 + (int64_t)(H | a[i]) * rs;
 #pragma unroll
-                for (int j = 1; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * np + lane];
+                for (int j = 1; j < 8; j++) pv[i][j] = pb[(int64_t)(L - yl[j]) * rs + lane];
             }
 #pragma unroll
             for (int i = 0; i < R; i++) {
@@ -1865,14 +1868,14 @@ __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
     } else {
         // |S| <= 3: one outer step over the nonempty subsets of S
         for (uint32_t T = S; T != 0; T = (T - 1u) & S) {
-            const int32_t t = fv0[(int64_t)T * np + lane];
+            const int32_t t = fv0[(int64_t)T * rs + lane];
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const uint32_t Pr = (S ^ T) | a[i];
                 int64_t b = 0;
                 if constexpr (REL) b = fbase[T] + fbase[Pr];
                 const uint32_t Y = (a[i] < topS && (T & topS)) ? ((S | a[i]) ^ T) : T;
-                const P s2 = key(t, fv0[(int64_t)Pr * np + lane], b, Y);
+                const P s2 = key(t, fv0[(int64_t)Pr * rs + lane], b, Y);
                 acc[i] = s2 < acc[i] ? s2 : acc[i];
             }
         }
@@ -1884,7 +1887,7 @@ __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
 // lies inside the rows (padding columns feed only dropped results).
 template <typename P, bool REL>
 __global__ __launch_bounds__(kBlock) void k_merge_star_w(
-    const int32_t *__restrict__ f, const int64_t *__restrict__ fbase, int64_t np,
+    const int32_t *__restrict__ f, FLayout fl, const int64_t *__restrict__ fbase, int64_t np,
     const MergeStar *__restrict__ stars, int nstars, int parts, int64_t ntiles, int k,
     P *__restrict__ M, int nxcd, int64_t per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -1907,16 +1910,16 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_w(
     const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
     if (o0 >= o1) return;
     const int64_t v0 = tile * kWave;
-    const int32_t *fv0 = f + v0;
+    const int32_t *fv0 = f + fl.cell(0, v0);
     P acc[kStarMax];
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) acc[i] = ~(P)0;
     static_assert(kStarMax == 4, "one instantiation per star size");
     switch (r) {
-    case 1: starw_pairs<P, REL, 1>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
-    case 2: starw_pairs<P, REL, 2>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
-    case 3: starw_pairs<P, REL, 3>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
-    default: starw_pairs<P, REL, 4>(fv0, fbase, np, S, a, o0, o1, k, acc); break;
+    case 1: starw_pairs<P, REL, 1>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 2: starw_pairs<P, REL, 2>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 3: starw_pairs<P, REL, 3>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    default: starw_pairs<P, REL, 4>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     }
     const int64_t v = v0 + lane;
 #pragma unroll
@@ -2196,7 +2199,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
                                                      int32_t *__restrict__ b,
                                                      int nxcd, int64_t per_xcd, int64_t vtiles,
                                                      const unsigned long long *__restrict__ rspan,
-                                                     int32_t *__restrict__ fcopy,
+                                                     int32_t *__restrict__ fcopy, FLayout fcl,
                                                      int64_t *__restrict__ fbase) {
     __shared__ DT sd[kWave][kWave + 1];
     __shared__ int32_t sc[kWave][kWave + 1];
@@ -2254,7 +2257,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_v(const DT *__restrict__ D,
             // compact copy (relative labels that fit int32): the relative
             // label itself, with the row's base
             if (fcopy) {
-                fcopy[X * np + v] = (int32_t)sd[lane][xl];
+                fcopy[fcl.cell(X, v)] = (int32_t)sd[lane][xl];
                 if (v == 0) fbase[X] = (int64_t)base;
             }
         }
@@ -2377,12 +2380,12 @@ __global__ __launch_bounds__(kWideWaves * kWave) void k_unpack_w(
 // Compact copy of the singleton rows 2^t (base 0): int32 distances.
 template <typename CT>
 __global__ void k_copy_singletons(const int64_t *__restrict__ f, int64_t np, int n, int k,
-                                  CT *__restrict__ fcopy) {
+                                  CT *__restrict__ fcopy, FLayout fcl) {
     const int t = blockIdx.y;
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < k && v < n) {
         const int64_t X = 1ll << t;
-        fcopy[X * np + v] = (CT)f[X * np + v];
+        fcopy[fcl.cell(X, v)] = (CT)f[X * np + v];
     }
 }
 
