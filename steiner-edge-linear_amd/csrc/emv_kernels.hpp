@@ -1551,7 +1551,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__res
 // pairs instead of 2 r.  The packed key takes the side without X's top bit
 // (the smaller set), as merge_outer does: T, unless top(X) is in T (a_i below
 // top(S) and top(S) in T), then X ^ T.
-constexpr int kStarMax = 4;  // rows per star
+#ifndef EMV_STAR_MAX
+#define EMV_STAR_MAX 8
+#endif
+constexpr int kStarMax = EMV_STAR_MAX;  // rows per star (4 .. 8)
+static_assert(kStarMax >= 4 && kStarMax <= 8, "stars of 4 to 8 rows");
+// R clamped to kStarMax, so every case of a dispatch over star sizes compiles
+template <int R>
+constexpr int star_r() { return R <= kStarMax ? R : kStarMax; }
 
 struct MergeStar {
     uint32_t S;                 // the shared set, m - 1 elements
@@ -1731,12 +1738,15 @@ __device__ __forceinline__ void star_h2_pairs(const char *__restrict__ fb, uint3
                                               const uint32_t (&a)[kStarMax], uint32_t o0,
                                               uint32_t o1, uint32_t (&klo)[kStarMax],
                                               uint32_t (&khi)[kStarMax]) {
-    static_assert(kStarMax == 4, "one case per star size");
     switch (r) {
     case 1: star_h2_pairs_r<1>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
     case 2: star_h2_pairs_r<2>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
     case 3: star_h2_pairs_r<3>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
-    default: star_h2_pairs_r<4>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 4: star_h2_pairs_r<4>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 5: star_h2_pairs_r<star_r<5>()>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 6: star_h2_pairs_r<star_r<6>()>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    case 7: star_h2_pairs_r<star_r<7>()>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
+    default: star_h2_pairs_r<star_r<8>()>(fb, lb, rb, S, a, o0, o1, klo, khi); break;
     }
 }
 
@@ -1914,12 +1924,15 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_w(
     P acc[kStarMax];
 #pragma unroll
     for (int i = 0; i < kStarMax; i++) acc[i] = ~(P)0;
-    static_assert(kStarMax == 4, "one instantiation per star size");
     switch (r) {
     case 1: starw_pairs<P, REL, 1>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     case 2: starw_pairs<P, REL, 2>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     case 3: starw_pairs<P, REL, 3>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
-    default: starw_pairs<P, REL, 4>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 4: starw_pairs<P, REL, 4>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 5: starw_pairs<P, REL, star_r<5>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 6: starw_pairs<P, REL, star_r<6>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 7: starw_pairs<P, REL, star_r<7>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    default: starw_pairs<P, REL, star_r<8>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     }
     const int64_t v = v0 + lane;
 #pragma unroll
