@@ -99,6 +99,9 @@ struct emv_ctx {
     std::vector<MergeStar> h_stars;         // stars of the last batch (kept for the async copy)
     MergeStar *d_stars = nullptr;
     size_t stars_cap = 0;
+    std::vector<MergeGrid> h_grids;         // grids of the last batch (16-bit cells)
+    MergeGrid *d_grids = nullptr;
+    size_t grids_cap = 0;
     void *d_rspan = nullptr;  // per staged row: min and max merged value
     size_t rspan_cap = 0;
     // int64 tables: compact int32 copy of the rows (cell - row base) that the
@@ -360,6 +363,8 @@ void free_ws(emv_ctx *c) {
     c->rspan_cap = 0;
     dfree(c->d_stars);
     c->stars_cap = 0;
+    dfree(c->d_grids);
+    c->grids_cap = 0;
     dfree(c->d_cubes);
     dfree(c->d_cube_rows);
     dfree(c->d_cube_tab);
@@ -743,8 +748,61 @@ int64_t star_waves_per_tile() {
     return v;
 }
 
+// Grids instead of stars for 16-bit cells (EMV_MERGE_GRID=0: stars; read
+// per layer batch).
+bool merge_grid_enabled() { return !env_off("EMV_MERGE_GRID"); }
+
+// Layer batch of m >= 5 elements on 16-bit cells merged by grids
+// (build_grids) into the accumulators, then packed into the staging.
+template <typename P, typename DT>
+int launch_merge_grid(emv_ctx *c, const uint16_t *f, int nx, int64_t nchunks, int k, int m) {
+    build_grids(c->h_xs, nx, k, c->h_grids);
+    const int ngrids = (int)c->h_grids.size();
+    if (c->grids_cap < (size_t)ngrids) {
+        dfree(c->d_grids);
+        c->grids_cap = 0;
+        HIPCHK(c, hipMalloc((void **)&c->d_grids, sizeof(MergeGrid) * (size_t)ngrids));
+        c->grids_cap = (size_t)ngrids;
+    }
+    // h_grids is rebuilt only by the next batch's merge, after this batch's
+    // relaxation has synchronised the stream
+    HIPCHK(c, hipMemcpyAsync(c->d_grids, c->h_grids.data(), sizeof(MergeGrid) * (size_t)ngrids,
+                             hipMemcpyHostToDevice, c->stream));
+    const size_t bytes = sizeof(P) * (size_t)nx * (size_t)c->np;
+    if (c->m_cap < bytes) {
+        dfree(c->d_M);
+        c->m_cap = 0;
+        HIPCHK(c, hipMalloc(&c->d_M, bytes));
+        c->m_cap = bytes;
+    }
+    const int64_t cells = (int64_t)nx * c->np;
+    const int fb = (int)std::min<int64_t>((cells + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL(k_fill<P>, dim3(fb), dim3(kBlock), 0, c->stream, (P *)c->d_M, cells, ~(P)0);
+    // outer steps of a grid: subsets of S (m - 2 elements) without its two
+    // lowest; parts as for the stars, each at least four outer steps
+    const int64_t steps = 1ll << (m - 4);
+    // waves per tile: EMV_STAR_WPT, else 256 (n = 2^18, k = 14: merges 85.7 /
+    // 88.4 / 97.0 / 97.9 ms at 256 / 512 / 1,024 / 128; one part per grid
+    // 104.7 ms)
+    const int64_t wpt = getenv("EMV_STAR_WPT") ? star_waves_per_tile() : 256;
+    const int64_t parts =
+        std::max<int64_t>(1, std::min<int64_t>((wpt + ngrids - 1) / ngrids, steps / 4));
+    const int64_t ntiles = (c->np + 2 * kWave - 1) / (2 * kWave);
+    int64_t per;
+    const int grid = grid_for((ntiles * ngrids * parts + 3) / 4, c->nxcd, &per);
+    hipLaunchKernelGGL((k_merge_grid_h2<P>), dim3(grid), dim3(kBlock), 0, c->stream, f, c->flay,
+                       c->np, (const MergeGrid *)c->d_grids, ngrids, (int)parts, ntiles, k,
+                       (P *)c->d_M, c->nxcd, per);
+    launch_pack<P, DT>(c, nx, nchunks, k);
+    HIPCHK(c, hipGetLastError());
+    return EMV_OK;
+}
+
 template <typename T, typename P, typename DT>
 int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, int m) {
+    if constexpr (std::is_same<T, uint16_t>::value)
+        if (m >= 5 && merge_grid_enabled())
+            return launch_merge_grid<P, DT>(c, f, nx, nchunks, k, m);
     build_stars(c->h_xs, nx, k, c->h_stars);
     const int nstars = (int)c->h_stars.size();
     if (c->stars_cap < (size_t)nstars) {

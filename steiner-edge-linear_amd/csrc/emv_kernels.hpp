@@ -1794,6 +1794,252 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_h2(
     }
 }
 
+// ------------------------------------------ merge by grids (16-bit cells)
+// A grid is the rows X_ij = S | a_i | b_j (i < p, j < q, p, q <= 3) of a layer
+// batch: S has m - 2 elements, the a_i and b_j are distinct elements outside
+// it.  Every pair {T, X_ij \ T} has one side T without b_j, so T runs over
+// the nonempty U of S and over U | a_i (every U of S, the empty one too),
+// with partners (S \ U) | a_i | b_j and (S \ U) | b_j.  For one U the grid
+// loads f[U], the p rows f[U | a_i], the q rows f[(S \ U) | b_j] and the p q
+// rows f[(S \ U) | a_i | b_j]: 1 + p + q + p q loads for 2 p q pairs (16 for
+// 18 at 3 x 3), against the (1 + r) for r of a star of r rows.
+struct MergeGrid {
+    uint32_t S;
+    uint32_t p, q;
+    uint32_t a[3], b[3];
+    int32_t xi[9];  // X_ij's index in the layer batch at 3 i + j
+};
+
+// Greedy grid cover of the batch's rows xs[0..nx) (all of m >= 4 elements):
+// each uncovered row X, in order, tries its element pairs (a < b) as (a_0,
+// b_0) with S = X \ {a, b}, grows A = {a_i} and B = {b_j} by the lowest
+// elements outside S whose rows with every member of the other side are
+// uncovered rows of the batch (A first, alternating, up to 3 each), and keeps
+// the largest grid (the first 3 x 3 ends the search).
+inline void build_grids(const uint32_t *xs, int nx, int k, std::vector<MergeGrid> &out) {
+    out.clear();
+    std::vector<int32_t> idx((size_t)1 << k, -1);
+    for (int i = 0; i < nx; i++) idx[xs[i]] = i;
+    std::vector<char> covered((size_t)nx, 0);
+    auto free_row = [&](uint32_t Y) { return idx[Y] >= 0 && !covered[(size_t)idx[Y]]; };
+    for (int i = 0; i < nx; i++) {
+        if (covered[(size_t)i]) continue;
+        const uint32_t X = xs[i];
+        MergeGrid best;
+        memset(&best, 0, sizeof(best));
+        int best_sz = 0;
+        for (uint32_t ra = X; ra && best_sz < 9; ra &= ra - 1) {
+            const uint32_t a = ra & (~ra + 1u);
+            for (uint32_t rb = ra & (ra - 1); rb && best_sz < 9; rb &= rb - 1) {
+                const uint32_t b = rb & (~rb + 1u);
+                const uint32_t S = X ^ a ^ b;
+                uint32_t A[3] = {a, 0, 0}, B[3] = {b, 0, 0};
+                int p = 1, q = 1;
+                uint32_t used = a | b;
+                bool grown = true;
+                while (grown) {
+                    grown = false;
+                    if (p < 3)
+                        for (int c = 0; c < k; c++) {
+                            const uint32_t cb = 1u << c;
+                            if ((S | used) & cb) continue;
+                            bool ok = true;
+                            for (int j = 0; j < q && ok; j++) ok = free_row(S | cb | B[j]);
+                            if (ok) {
+                                A[p++] = cb;
+                                used |= cb;
+                                grown = true;
+                                break;
+                            }
+                        }
+                    if (q < 3)
+                        for (int c = 0; c < k; c++) {
+                            const uint32_t cb = 1u << c;
+                            if ((S | used) & cb) continue;
+                            bool ok = true;
+                            for (int j = 0; j < p && ok; j++) ok = free_row(S | A[j] | cb);
+                            if (ok) {
+                                B[q++] = cb;
+                                used |= cb;
+                                grown = true;
+                                break;
+                            }
+                        }
+                }
+                if (p * q > best_sz) {
+                    best_sz = p * q;
+                    best.S = S;
+                    best.p = (uint32_t)p;
+                    best.q = (uint32_t)q;
+                    for (int t = 0; t < 3; t++) {
+                        best.a[t] = A[t];
+                        best.b[t] = B[t];
+                    }
+                }
+            }
+        }
+        for (int t = 0; t < 9; t++) best.xi[t] = -1;
+        for (uint32_t ii = 0; ii < best.p; ii++)
+            for (uint32_t jj = 0; jj < best.q; jj++) {
+                const int32_t r = idx[best.S | best.a[ii] | best.b[jj]];
+                best.xi[3 * ii + jj] = r;
+                covered[(size_t)r] = 1;
+            }
+        out.push_back(best);
+    }
+}
+
+// The pairs of a p x q grid over the outer steps [o0, o1) of U's high part
+// (S without its two lowest elements L; U = Uh | ul over the four subsets ul
+// of L), for the vertex pair at byte offset lb from the wave-uniform base fb
+// of a 128-vertex tile, rows rb bytes apart; klo / khi[3 i + j] fold the
+// (sum << 16) | Y keys of row X_ij's low / high vertex.  Y, the side without
+// X_ij's top element: T, or X_ij ^ T when the top lies in T (a_i in T = U |
+// a_i, or S's top in U), so Y = T ^ m with m = X_ij or 0 per step.
+template <int P_, int Q_>
+__device__ __forceinline__ void grid_h2_pairs(const char *__restrict__ fb, uint32_t lb, int64_t rb,
+                                              uint32_t S, const uint32_t (&a)[3],
+                                              const uint32_t (&b)[3], uint32_t o0, uint32_t o1,
+                                              uint32_t (&klo)[9], uint32_t (&khi)[9]) {
+    uint32_t hmask = 0xFFFF0000u;
+    asm volatile("" : "+v"(hmask));
+    const uint32_t l0 = S & (~S + 1u);
+    const uint32_t l1 = (S ^ l0) & (~(S ^ l0) + 1u);
+    const uint32_t L = l0 | l1;
+    const uint32_t H = S ^ L;
+    const uint32_t topS = 1u << (31 - __clz(S));  // in H: S has at least three elements
+    const uint32_t nH = 1u << __popc(H);
+    const uint32_t ul[4] = {0u, l0, l1, L};
+    uint32_t va[4], vc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        va[u] = lb + (uint32_t)((int64_t)ul[u] * rb);
+        vc[u] = lb + (uint32_t)((int64_t)(L ^ ul[u]) * rb);
+    }
+#pragma unroll
+    for (int t = 0; t < 9; t++) klo[t] = khi[t] = 0xFFFFFFFFu;
+    uint32_t Uh = deposit_bits(nH - 1u - o0, H);
+    for (uint32_t o = o0; o < o1; o++) {
+        const uint32_t Hc = H ^ Uh;  // (S \ U) = Hc | (L ^ ul)
+        uint32_t t[4], ta[P_][4], pb[Q_][4], pab[P_][Q_][4];
+        {
+            const __amdgpu_buffer_rsrc_t r = row_rsrc(fb + (int64_t)Uh * rb);
+#pragma unroll
+            for (int u = 0; u < 4; u++) t[u] = ld_pair(r, va[u]);
+        }
+#pragma unroll
+        for (int i = 0; i < P_; i++) {
+            const __amdgpu_buffer_rsrc_t r = row_rsrc(fb + (int64_t)(Uh | a[i]) * rb);
+#pragma unroll
+            for (int u = 0; u < 4; u++) ta[i][u] = ld_pair(r, va[u]);
+        }
+#pragma unroll
+        for (int j = 0; j < Q_; j++) {
+            const __amdgpu_buffer_rsrc_t r = row_rsrc(fb + (int64_t)(Hc | b[j]) * rb);
+#pragma unroll
+            for (int u = 0; u < 4; u++) pb[j][u] = ld_pair(r, vc[u]);
+        }
+#pragma unroll
+        for (int i = 0; i < P_; i++)
+#pragma unroll
+            for (int j = 0; j < Q_; j++) {
+                const __amdgpu_buffer_rsrc_t r = row_rsrc(fb + (int64_t)(Hc | a[i] | b[j]) * rb);
+#pragma unroll
+                for (int u = 0; u < 4; u++) pab[i][j][u] = ld_pair(r, vc[u]);
+            }
+        const bool stop = (Uh & topS) != 0u;  // S's top in U
+        const bool last = Uh == 0u;           // U = ul: U = 0 has no first pair
+#pragma unroll
+        for (int i = 0; i < P_; i++)
+#pragma unroll
+            for (int j = 0; j < Q_; j++) {
+                const uint32_t X = S | a[i] | b[j];
+                const uint32_t top = 1u << (31 - __clz(X));
+                // first pair {U, X \ U}: flips when S's top is X's top and in U;
+                // second {U | a_i, X \ (U | a_i)}: when a_i is X's top, or S's
+                // top is and lies in U
+                const uint32_t m1 = (top == topS && stop) ? X : 0u;
+                const uint32_t m2 = (top == a[i] || (top == topS && stop)) ? X : 0u;
+                const uint32_t B1 = Uh ^ m1, B2 = (Uh | a[i]) ^ m2;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint32_t s2 = add_u16x2(ta[i][u], pb[j][u]);
+                    const uint32_t Y2 = __builtin_amdgcn_readfirstlane(B2 ^ ul[u]);
+                    klo[3 * i + j] = min(klo[3 * i + j], (s2 << 16) + Y2);
+                    khi[3 * i + j] = min(khi[3 * i + j], (s2 & hmask) | Y2);
+                    if (u > 0 || !last) {
+                        const uint32_t s1 = add_u16x2(t[u], pab[i][j][u]);
+                        const uint32_t Y1 = __builtin_amdgcn_readfirstlane(B1 ^ ul[u]);
+                        klo[3 * i + j] = min(klo[3 * i + j], (s1 << 16) + Y1);
+                        khi[3 * i + j] = min(khi[3 * i + j], (s1 & hmask) | Y1);
+                    }
+                }
+            }
+        Uh = (Uh - 1u) & H;
+    }
+}
+
+// Outer steps of a grid: subsets of S without its two lowest elements.
+__device__ __forceinline__ uint32_t grid_steps(uint32_t S) { return 1u << (__popc(S) - 2); }
+
+// Item = (tile, grid, part), tile-major per XCD; parts fold into M with
+// unsigned atomicMin as the stars do.  Two vertices per lane.
+template <typename P>
+__global__ __launch_bounds__(kBlock) void k_merge_grid_h2(
+    const uint16_t *__restrict__ f, FLayout fl, int64_t np, const MergeGrid *__restrict__ grids,
+    int ngrids, int parts, int64_t ntiles, int k, P *__restrict__ M, int nxcd, int64_t per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if ((int64_t)(blockIdx.x / nxcd) >= per_xcd) return;
+    const int64_t item = xcd_item(nxcd, per_xcd) * (kBlock / kWave) + wid;
+    const int64_t per_tile = (int64_t)ngrids * parts;
+    const int64_t tile = item / per_tile;
+    if (tile >= ntiles) return;
+    const int rem = (int)(item - tile * per_tile);
+    const int g = rem / parts;
+    const int part = rem - g * parts;
+    const MergeGrid *gr = grids + g;
+    const uint32_t S = gr->S;
+    const int pq = (int)(gr->p * 4 + gr->q);
+    uint32_t a[3], b[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        a[i] = gr->a[i];
+        b[i] = gr->b[i];
+    }
+    const uint32_t steps = grid_steps(S);
+    const uint32_t o0 = (uint32_t)((uint64_t)steps * part / parts);
+    const uint32_t o1 = (uint32_t)((uint64_t)steps * (part + 1) / parts);
+    if (o0 >= o1) return;
+    const int64_t v0 = tile * (2 * kWave);
+    const int64_t v = v0 + 2 * lane;
+    const uint32_t lb = 4u * (uint32_t)min((int64_t)lane, (np - v0) / 2 - 1);
+    const char *fb = reinterpret_cast<const char *>(f + fl.cell(0, v0));
+    const int64_t rb = 2 * fl.rcells;
+    uint32_t klo[9], khi[9];
+    switch (pq) {
+    case 1 * 4 + 1: grid_h2_pairs<1, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 1 * 4 + 2: grid_h2_pairs<1, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 1 * 4 + 3: grid_h2_pairs<1, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 1: grid_h2_pairs<2, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 2: grid_h2_pairs<2, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 3: grid_h2_pairs<2, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 3 * 4 + 1: grid_h2_pairs<3, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 3 * 4 + 2: grid_h2_pairs<3, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    default: grid_h2_pairs<3, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    }
+    if (v >= np) return;
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const int32_t xi = gr->xi[t];
+        if (xi >= 0) {
+            P *acc = &M[(int64_t)xi * np + v];
+            atomicMin(acc, key_packed<P>(klo[t], k));
+            atomicMin(acc + 1, key_packed<P>(khi[t], k));
+        }
+    }
+}
+
 // --------------------------------------------- merge by stars (32-bit cells)
 // The stars of k_merge_star_h2 over an int32 table or over the compact int32
 // copy of an int64 table (REL: a row holds f - fbase[row], and a pair's sum

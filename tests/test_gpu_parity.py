@@ -487,12 +487,15 @@ def test_large_weights_64bit_labels(gpu_ctx, shape):
 
 
 
+@pytest.mark.parametrize("grid", ["1", "0"])
 @pytest.mark.parametrize("name", ["small-instances/B/b01", "small-instances/D/d17", CFG1])
-def test_star_merges_bit_exact(gpu_ctx, monkeypatch, name):
-    """Every layer merged in stars of rows sharing a base set (forced for
-    small layers, which otherwise merge row by row): the reference's table
-    and record kinds, as row by row."""
+def test_star_merges_bit_exact(gpu_ctx, monkeypatch, name, grid):
+    """Every layer merged in grids of rows S | a_i | b_j (layers of >= 5
+    terminals) or in stars of rows sharing a base set (EMV_MERGE_GRID=0, and
+    the smaller layers), forced for small layers, which otherwise merge row
+    by row: the reference's table and record kinds, as row by row."""
     monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    monkeypatch.setenv("EMV_MERGE_GRID", grid)
     inst = load_instance(name)
     print(f"case stars {name}", flush=True)
     sol, f, code = solve_full(gpu_ctx, inst)
@@ -543,15 +546,42 @@ def test_star_merges_32bit_cells_vs_oracle(gpu_ctx, monkeypatch, n, d, k, flags)
     np.testing.assert_array_equal(code, ocode)
 
 
+@pytest.mark.parametrize("tiles", ["1", "0"])
+@pytest.mark.parametrize("n,d,k", [(3001, 6, 11), (2000, 8, 9), (777, 4, 12), (130, 3, 10)])
+def test_grid_merges_vs_oracle(gpu_ctx, monkeypatch, n, d, k, tiles):
+    """16-bit cells merged in grids of rows S | a_i | b_j on every layer of
+    >= 5 terminals (forced; 1 x 1 up to 3 x 3 grids, parts over the outer
+    steps), on the tiled table and (EMV_TILE_TABLE=0) on rows; n = 3001, 777
+    and 130 end in partly filled tiles.  Cells, records and cost equal the
+    oracle's."""
+    if n * d % 2:
+        n += 1
+    monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    monkeypatch.setenv("EMV_MERGE_GRID", "1")
+    monkeypatch.setenv("EMV_TILE_TABLE", tiles)
+    e = emv.regular_graph(n, d, 1000, 53 + k)
+    kk = emv.terminals(n, k, 53 + k)
+    print(f"case grids n={n} d={d} k={k} tiles={tiles}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk))
+    assert sol.table_bytes == 2
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
 @pytest.mark.parametrize("n,d,k", [(100, 4, 15), (48, 3, 16), (230, 3, 13)])
-@pytest.mark.parametrize("flags", [0, emv.MERGE_ROWS, emv.MERGE_1V])
-def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags):
+@pytest.mark.parametrize("flags,grid", [(0, "1"), (0, "0"), (emv.MERGE_ROWS, "1"),
+                                        (emv.MERGE_1V, "1")])
+def test_many_terminals_16bit_cells(gpu_ctx, monkeypatch, n, d, k, flags, grid):
     """k = 13 .. 16 on 16-bit cells: the two-vertex merges' keys hold Y in
     16 bits and their low rows in 32-bit lane offsets at the largest k they
     take; cells, records and cost equal the oracle's, in stars (forced for
-    these small layers), row by row and one vertex per lane.  n = 100, 48
-    and 230 end in partly filled 128-vertex tiles."""
+    these small layers; grids of rows S | a_i | b_j, or stars with
+    EMV_MERGE_GRID=0), row by row and one vertex per lane.  n = 100, 48 and
+    230 end in partly filled 128-vertex tiles."""
     monkeypatch.setenv("EMV_STAR_MIN_CELLS", "0")
+    monkeypatch.setenv("EMV_MERGE_GRID", grid)
     e = emv.regular_graph(n, d, 1000, 7)
     kk = emv.terminals(n, k, 7)
     print(f"case n={n} d={d} k={k} flags={flags}", flush=True)
