@@ -803,7 +803,11 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
     if constexpr (std::is_same<T, uint16_t>::value)
         if (m >= 5 && merge_grid_enabled())
             return launch_merge_grid<P, DT>(c, f, nx, nchunks, k, m);
-    build_stars(c->h_xs, nx, k, c->h_stars);
+    // stars of up to kStarMax rows, but of at most four over the int32 copy
+    // of an int64 table (row bases per pair: more rows cost it occupancy and
+    // scalar registers; config 5's merges take 421 ms at four, 627 at eight)
+    const bool rel = !std::is_same<T, uint16_t>::value && merge_fbase<T>(c) != nullptr;
+    build_stars(c->h_xs, nx, k, c->h_stars, rel ? 4 : kStarMax);
     const int nstars = (int)c->h_stars.size();
     if (c->stars_cap < (size_t)nstars) {
         dfree(c->d_stars);
@@ -846,12 +850,13 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
         const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
         const int64_t *fb = merge_fbase<T>(c);
         if (fb)
-            hipLaunchKernelGGL((k_merge_star_w<P, true>), dim3(grid), dim3(kBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_merge_star_w<P, true, 4>), dim3(grid), dim3(kBlock), 0, c->stream,
                                f, merge_layout<T>(c), fb, c->np, (const MergeStar *)c->d_stars,
                                nstars, (int)parts,
                                ntiles, k, (P *)c->d_M, c->nxcd, per);
         else
-            hipLaunchKernelGGL((k_merge_star_w<P, false>), dim3(grid), dim3(kBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_merge_star_w<P, false, kStarMax>), dim3(grid), dim3(kBlock), 0,
+                               c->stream,
                                f, merge_layout<T>(c), fb, c->np, (const MergeStar *)c->d_stars,
                                nstars, (int)parts,
                                ntiles, k, (P *)c->d_M, c->nxcd, per);

@@ -1556,9 +1556,12 @@ __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__res
 #endif
 constexpr int kStarMax = EMV_STAR_MAX;  // rows per star (4 .. 8)
 static_assert(kStarMax >= 4 && kStarMax <= 8, "stars of 4 to 8 rows");
-// R clamped to kStarMax, so every case of a dispatch over star sizes compiles
+// R clamped to kStarMax (to RMAX), so every case of a dispatch over star
+// sizes compiles; stars of at most RMAX rows never reach the clamped cases
 template <int R>
 constexpr int star_r() { return R <= kStarMax ? R : kStarMax; }
+template <int RMAX, int R>
+constexpr int star_rw() { return R <= RMAX ? R : RMAX; }
 
 struct MergeStar {
     uint32_t S;                 // the shared set, m - 1 elements
@@ -1571,7 +1574,8 @@ struct MergeStar {
 // uncovered row, in order, takes the base set S = X without the element that
 // leaves the most uncovered rows S | a in the batch, and up to kStarMax of
 // them (X first, then by increasing a) form its star.
-inline void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out) {
+inline void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar> &out,
+                        int rmax = kStarMax) {
     out.clear();
     std::vector<int32_t> idx((size_t)1 << k, -1);
     for (int i = 0; i < nx; i++) idx[xs[i]] = i;
@@ -1599,7 +1603,7 @@ inline void build_stars(const uint32_t *xs, int nx, int k, std::vector<MergeStar
         st.xi[0] = i;
         covered[(size_t)i] = 1;
         st.r = 1;
-        for (int b = 0; b < k && st.r < (uint32_t)kStarMax; b++) {
+        for (int b = 0; b < k && st.r < (uint32_t)rmax; b++) {
             const uint32_t Y = bestS | (1u << b);
             if ((bestS >> b & 1u) || !free_row(Y)) continue;
             st.a[st.r] = 1u << b;
@@ -2141,7 +2145,7 @@ __device__ __forceinline__ void starw_pairs(const int32_t *__restrict__ fv0,
 // Item = (64-vertex tile, star, part), tile-major per XCD; parts fold into M
 // with unsigned atomicMin.  np is a multiple of 64, so every lane's vertex
 // lies inside the rows (padding columns feed only dropped results).
-template <typename P, bool REL>
+template <typename P, bool REL, int RMAX>
 __global__ __launch_bounds__(kBlock) void k_merge_star_w(
     const int32_t *__restrict__ f, FLayout fl, const int64_t *__restrict__ fbase, int64_t np,
     const MergeStar *__restrict__ stars, int nstars, int parts, int64_t ntiles, int k,
@@ -2175,10 +2179,10 @@ __global__ __launch_bounds__(kBlock) void k_merge_star_w(
     case 2: starw_pairs<P, REL, 2>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     case 3: starw_pairs<P, REL, 3>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     case 4: starw_pairs<P, REL, 4>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
-    case 5: starw_pairs<P, REL, star_r<5>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
-    case 6: starw_pairs<P, REL, star_r<6>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
-    case 7: starw_pairs<P, REL, star_r<7>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
-    default: starw_pairs<P, REL, star_r<8>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 5: starw_pairs<P, REL, star_rw<RMAX, 5>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 6: starw_pairs<P, REL, star_rw<RMAX, 6>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    case 7: starw_pairs<P, REL, star_rw<RMAX, 7>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
+    default: starw_pairs<P, REL, star_rw<RMAX, 8>()>(fv0, fbase, fl.rcells, S, a, o0, o1, k, acc); break;
     }
     const int64_t v = v0 + lane;
 #pragma unroll
