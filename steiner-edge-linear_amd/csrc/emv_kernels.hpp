@@ -1983,6 +1983,24 @@ __device__ __forceinline__ void grid_h2_pairs(const char *__restrict__ fb, uint3
     }
 }
 
+// grid_h2_pairs for a grid of p x q rows, pq = 4 p + q.
+__device__ __forceinline__ void grid_h2_dispatch(int pq, const char *__restrict__ fb, uint32_t lb,
+                                                 int64_t rb, uint32_t S, const uint32_t (&a)[3],
+                                                 const uint32_t (&b)[3], uint32_t o0, uint32_t o1,
+                                                 uint32_t (&klo)[9], uint32_t (&khi)[9]) {
+    switch (pq) {
+    case 1 * 4 + 1: grid_h2_pairs<1, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 1 * 4 + 2: grid_h2_pairs<1, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 1 * 4 + 3: grid_h2_pairs<1, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 1: grid_h2_pairs<2, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 2: grid_h2_pairs<2, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 2 * 4 + 3: grid_h2_pairs<2, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 3 * 4 + 1: grid_h2_pairs<3, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    case 3 * 4 + 2: grid_h2_pairs<3, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    default: grid_h2_pairs<3, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
+    }
+}
+
 // Outer steps of a grid: subsets of S without its two lowest elements.
 __device__ __forceinline__ uint32_t grid_steps(uint32_t S) { return 1u << (__popc(S) - 2); }
 
@@ -2021,17 +2039,7 @@ __global__ __launch_bounds__(kBlock) void k_merge_grid_h2(
     const char *fb = reinterpret_cast<const char *>(f + fl.cell(0, v0));
     const int64_t rb = 2 * fl.rcells;
     uint32_t klo[9], khi[9];
-    switch (pq) {
-    case 1 * 4 + 1: grid_h2_pairs<1, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 1 * 4 + 2: grid_h2_pairs<1, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 1 * 4 + 3: grid_h2_pairs<1, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 2 * 4 + 1: grid_h2_pairs<2, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 2 * 4 + 2: grid_h2_pairs<2, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 2 * 4 + 3: grid_h2_pairs<2, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 3 * 4 + 1: grid_h2_pairs<3, 1>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    case 3 * 4 + 2: grid_h2_pairs<3, 2>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    default: grid_h2_pairs<3, 3>(fb, lb, rb, S, a, b, o0, o1, klo, khi); break;
-    }
+    grid_h2_dispatch(pq, fb, lb, rb, S, a, b, o0, o1, klo, khi);
     if (v >= np) return;
 #pragma unroll
     for (int t = 0; t < 9; t++) {
