@@ -425,6 +425,30 @@ def test_block_relax_grid_vs_oracle(gpu_ctx, monkeypatch, flags, closure):
     np.testing.assert_array_equal(code, ocode)
 
 
+@pytest.mark.parametrize("shape,flags", [("grid", 0), ("grid", emv.WIDE_LABELS), ("group", 0)])
+def test_block_relax_many_chunks_vs_oracle(gpu_ctx, shape, flags):
+    """Block-local relaxation over layers of many 64-row chunks (k = 12:
+    layers 4 .. 8 hold 8 to 15 chunks, so one round's items span chunks and
+    blocks): a grid, and a group-reduced grid whose virtual terminals join
+    blocks far apart. Every cell and record equal the oracle's."""
+    import workloads
+
+    if shape == "grid":
+        R = C = 48
+        n = R * C
+        e = emv.grid_graph(R, C, 100, 43)
+        kk = emv.terminals(n, 12, 43)
+    else:
+        _, _, _, n, e, kk, _ = workloads.group_instance(36, 36, 12, 5, 100, 44)
+    print(f"case block relax {shape} n={n} k={len(kk)} flags={flags}", flush=True)
+    sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
+    assert sol.relax_blocks > 0
+    ocost, of, ocode = oracle.emv(n, e, kk)
+    assert sol.cost == ocost
+    np.testing.assert_array_equal(f, of)
+    np.testing.assert_array_equal(code, ocode)
+
+
 @pytest.mark.parametrize("n,d,seed", [(1000, 4, 1), (300, 20, 2), (1200, 16, 3)])
 def test_block_relax_forced_vs_oracle(gpu_ctx, monkeypatch, n, d, seed):
     """EMV_BLOCKS=1 forces blocks on graphs without locality: most neighbours
