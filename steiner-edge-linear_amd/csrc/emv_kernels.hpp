@@ -730,24 +730,42 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         DT x[kBQ], x0[kBQ];
 #pragma unroll
         for (int t = 0; t < kBQ; t++) x0[t] = x[t] = t < nv ? L[(t * kBW + wid) * kWave + lane] : INF;
-        // two lowered slots per step (a lone last one is taken twice: min
-        // is idempotent); entries past the band (t >= nv) are padding and
-        // their results are dropped below
-        for (int w2 = 0; w2 < kBW; w2++) {
-            uint64_t m = uniform64(s_chg[w2]);
-            while (m) {
-                const int b1 = pick_bit(m, true) * kBW + w2;
-                m &= m - 1;
-                const int b2 = m ? pick_bit(m, true) * kBW + w2 : b1;
-                m &= m - 1;
-                const DT l1 = L[b1 * kWave + lane];
-                const DT l2 = L[b2 * kWave + lane];
-                DW d1[kBQ], d2[kBQ];
-                band_row<DW>(Db + (int64_t)b1 * kBRow + wid * kBQ, d1);
-                band_row<DW>(Db + (int64_t)b2 * kBRow + wid * kBQ, d2);
+        // lowered slots NB at a time, taken across the waves' masks by a
+        // scalar cursor (w2, m) over s_chg (a short last step repeats its
+        // first slot: min is idempotent), so that the row loads of NB slots
+        // are in flight together; entries past the band (t >= nv) are
+        // padding and their results are dropped below
+        constexpr int NB = sizeof(DW) == 2 ? 4 : 2;
+        int w2 = 0;
+        uint64_t m = uniform64(s_chg[0]);
+        auto next_low = [&]() -> int {
+            while (m == 0 && w2 < kBW - 1) m = uniform64(s_chg[++w2]);
+            if (m == 0) return -1;
+            const int b = pick_bit(m, true) * kBW + w2;
+            m &= m - 1;
+            return b;
+        };
+        for (int b0 = next_low(); b0 >= 0; b0 = next_low()) {
+            int bb[NB];
+            bb[0] = b0;
 #pragma unroll
-                for (int t = 0; t < kBQ; t++)
-                    x[t] = closure_min<DT, DW>(closure_min<DT, DW>(x[t], l1, d1[t]), l2, d2[t]);
+            for (int t = 1; t < NB; t++) {
+                const int bt = next_low();
+                bb[t] = bt >= 0 ? bt : b0;
+            }
+            DT lv[NB];
+            DW dv[NB][kBQ];
+#pragma unroll
+            for (int t = 0; t < NB; t++) {
+                lv[t] = L[bb[t] * kWave + lane];
+                band_row<DW>(Db + (int64_t)bb[t] * kBRow + wid * kBQ, dv[t]);
+            }
+#pragma unroll
+            for (int q = 0; q < kBQ; q++) {
+                DT v = x[q];
+#pragma unroll
+                for (int t = 0; t < NB; t++) v = closure_min<DT, DW>(v, lv[t], dv[t][q]);
+                x[q] = v;
             }
         }
 #pragma unroll
