@@ -644,17 +644,18 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
     const bool any_ovf = __ballot(ovf != 0) != 0;
     if (tid < 3) s_flag[tid] = 0;
     const uint64_t own_mask = nv >= 64 ? ~0ull : ((1ull << nv) - 1ull);
-    // 1a. the block's labels, eight slots per step
-    for (int j0 = 0; j0 < nv; j0 += 8) {
-        DT own[8];
+    // 1a. the block's labels, all of the wave's band (nv <= kBQ slots) in
+    //     one step, so that its loads are in flight together
+    if (nv > 0) {
+        DT own[kBQ];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-            const int j = min(j0 + t, nv - 1);
+        for (int t = 0; t < kBQ; t++) {
+            const int j = min(t, nv - 1);
             own[t] = Dc[(int64_t)__builtin_amdgcn_readlane(vj, j) * kWave + lane];
         }
 #pragma unroll
-        for (int t = 0; t < 8; t++)
-            if (j0 + t < nv) L[((j0 + t) * kBW + wid) * kWave + lane] = own[t];
+        for (int t = 0; t < kBQ; t++)
+            if (t < nv) L[(t * kBW + wid) * kWave + lane] = own[t];
     }
     // 1b. fold in the outside neighbours of the border slots, four per
     //     step; a slot that drops puts its block neighbours on the
