@@ -789,38 +789,86 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
         //      anywhere in the block (a barrier per iteration; labels only
         //      drop, so at most cnt iterations).  No worklists: a visit of
         //      a round that starts every slot dirty is cheaper this way.
+        //      Odd iterations take the wave's slots in reverse order, so
+        //      that a drop runs against the slot order (breadth-first from
+        //      the block's seed) as far in one iteration as along it
         for (int it = 0;; it++) {
             if (tid == 0) s_flag[(it + 1) % 3] = 0;
             bool any_low = false;
+            // (16-bit labels with 32-bit distances keep one order: there the
+            // two loops spill registers to scratch)
+            if constexpr (sizeof(DT) == 2 && sizeof(DW) == 4) {
 #pragma unroll
-            for (int t = 0; t < kBQ; t++) {
-                if (t >= nv) break;
-                const int sl0 = t * kBW + wid;
-                const DT x0 = L[sl0 * kWave + lane];
-                DT nb[kBD];
+                for (int t = 0; t < kBQ; t++) {
+                    if (t >= nv) break;
+                    const int sl0 = t * kBW + wid;
+                    const DT x0 = L[sl0 * kWave + lane];
+                    DT nb[kBD];
 #pragma unroll
-                for (int q = 0; q < kBD; q++) {
-                    const int sl = __builtin_amdgcn_readlane(ic[q], t);
-                    nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
-                }
-                DT best = x0;
+                    for (int q = 0; q < kBD; q++) {
+                        const int sl = __builtin_amdgcn_readlane(ic[q], t);
+                        nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                    }
+                    DT best = x0;
 #pragma unroll
-                for (int q = 0; q < kBD; q++)
-                    best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
-                                     __builtin_amdgcn_readlane(iw[q], t), INF, best);
-                if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
-                    const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
-                    const int eb = __builtin_amdgcn_readlane(em, t);
-                    for (int e = ea; e < eb; e++) {
-                        const Edge ed = badj[e];
-                        best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                    for (int q = 0; q < kBD; q++)
+                        best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
+                                         __builtin_amdgcn_readlane(iw[q], t), INF, best);
+                    if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
+                        const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
+                        const int eb = __builtin_amdgcn_readlane(em, t);
+                        for (int e = ea; e < eb; e++) {
+                            const Edge ed = badj[e];
+                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                        }
+                    }
+                    const bool lower = best < x0;
+                    if (lower) L[sl0 * kWave + lane] = best;
+                    if (__ballot(lower) != 0) {
+                        chg_mask |= 1ull << t;
+                        any_low = true;
                     }
                 }
-                const bool lower = best < x0;
-                if (lower) L[sl0 * kWave + lane] = best;
-                if (__ballot(lower) != 0) {
-                    chg_mask |= 1ull << t;
-                    any_low = true;
+            } else {
+                auto relax_slot = [&](int t) {
+                    const int sl0 = t * kBW + wid;
+                    const DT x0 = L[sl0 * kWave + lane];
+                    DT nb[kBD];
+#pragma unroll
+                    for (int q = 0; q < kBD; q++) {
+                        const int sl = __builtin_amdgcn_readlane(ic[q], t);
+                        nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                    }
+                    DT best = x0;
+#pragma unroll
+                    for (int q = 0; q < kBD; q++)
+                        best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
+                                         __builtin_amdgcn_readlane(iw[q], t), INF, best);
+                    if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
+                        const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
+                        const int eb = __builtin_amdgcn_readlane(em, t);
+                        for (int e = ea; e < eb; e++) {
+                            const Edge ed = badj[e];
+                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                        }
+                    }
+                    const bool lower = best < x0;
+                    if (lower) L[sl0 * kWave + lane] = best;
+                    if (__ballot(lower) != 0) {
+                        chg_mask |= 1ull << t;
+                        any_low = true;
+                    }
+                };
+                if (it & 1) {
+#pragma unroll
+                    for (int t = kBQ - 1; t >= 0; t--)
+                        if (t < nv) relax_slot(t);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < kBQ; t++) {
+                        if (t >= nv) break;
+                        relax_slot(t);
+                    }
                 }
             }
             if (any_low && lane == 0) s_flag[it % 3] = 1;
