@@ -494,6 +494,14 @@ bool use_blocks_any(emv_ctx *c, int dbits) {
     return use_blocks<uint64_t>(c);
 }
 
+// Full sweeps of a block's first visit (EMV_BLOCK_FULL_SWEEPS overrides the
+// default; large values keep full sweeps to the end).
+int blk_full_sweeps() {
+    const char *e = getenv("EMV_BLOCK_FULL_SWEEPS");
+    const int v = e ? atoi(e) : 0;
+    return v > 0 ? v : kBlkFullSweeps;
+}
+
 template <typename DT, typename DW>
 void blk_round_w(emv_ctx *c, int64_t nitems, int dense, int r) {
     const size_t bytes = blk_lds_bytes<DT>();
@@ -503,7 +511,7 @@ void blk_round_w(emv_ctx *c, int64_t nitems, int dense, int r) {
                        c->stream, (DT *)c->d_D, c->np, c->nblk, c->d_boff, c->d_bvert, c->d_brp,
                        c->d_bext, c->d_badj, c->d_vblk, (const DW *)c->d_bdist, c->d_bdoff,
                        nitems, dense, c->d_flag[r & 1], c->d_flag[(r + 1) & 1], c->d_any, r,
-                       c->d_stats);
+                       c->d_stats, blk_full_sweeps());
 }
 
 template <typename DT>

@@ -456,6 +456,9 @@ constexpr int kBD = 4;    // adjacency entries per vertex and kind held in regis
 constexpr int kBW = 16;   // waves per block workgroup (one workgroup fills a CU's LDS)
 constexpr int kBT = kBW * kWave;
 constexpr int kBPiece = 256;  // flags read per collection step
+// Full sweeps of a block's first visit before its worklists take over
+// (EMV_BLOCK_FULL_SWEEPS for the single-instance path).
+constexpr int kBlkFullSweeps = 12;
 constexpr int kBQ = (kBV + kBW - 1) / kBW;  // slots per wave of a block, at most
 constexpr int kBRow = kBW * kBQ;             // entries per row of an in-block distance matrix
 
@@ -597,7 +600,8 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
                                           const DW *__restrict__ Db,
                                           uint32_t *__restrict__ Fc, int dense, DT *L,
                                           uint32_t *s_flag, uint8_t *s_dirty, uint64_t *s_chg,
-                                          unsigned long long *__restrict__ stats, bool &raised) {
+                                          unsigned long long *__restrict__ stats, bool &raised,
+                                          int fsw = kBlkFullSweeps) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(tid / kWave);
@@ -783,173 +787,174 @@ __device__ __forceinline__ void blk_visit(DT *__restrict__ Dc, int s0, int cnt,
             if (low == 0) atomicAdd(&stats[6], 1ull);
             atomicAdd(&stats[7], low);
         }
-    } else if (dense) {
-        // 2''. the first visit: every wave relaxes all its slots over their
-        //      block neighbours, in place, until an iteration lowers nothing
-        //      anywhere in the block (a barrier per iteration; labels only
-        //      drop, so at most cnt iterations).  No worklists: a visit of
-        //      a round that starts every slot dirty is cheaper this way.
-        //      Odd iterations take the wave's slots in reverse order, so
-        //      that a drop runs against the slot order (breadth-first from
-        //      the block's seed) as far in one iteration as along it
-        for (int it = 0;; it++) {
-            if (tid == 0) s_flag[(it + 1) % 3] = 0;
-            bool any_low = false;
-            // (16-bit labels with 32-bit distances keep one order: there the
-            // two loops spill registers to scratch)
-            if constexpr (sizeof(DT) == 2 && sizeof(DW) == 4) {
-#pragma unroll
-                for (int t = 0; t < kBQ; t++) {
-                    if (t >= nv) break;
-                    const int sl0 = t * kBW + wid;
-                    const DT x0 = L[sl0 * kWave + lane];
-                    DT nb[kBD];
-#pragma unroll
-                    for (int q = 0; q < kBD; q++) {
-                        const int sl = __builtin_amdgcn_readlane(ic[q], t);
-                        nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
-                    }
-                    DT best = x0;
-#pragma unroll
-                    for (int q = 0; q < kBD; q++)
-                        best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
-                                         __builtin_amdgcn_readlane(iw[q], t), INF, best);
-                    if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
-                        const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
-                        const int eb = __builtin_amdgcn_readlane(em, t);
-                        for (int e = ea; e < eb; e++) {
-                            const Edge ed = badj[e];
-                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
-                        }
-                    }
-                    const bool lower = best < x0;
-                    if (lower) L[sl0 * kWave + lane] = best;
-                    if (__ballot(lower) != 0) {
-                        chg_mask |= 1ull << t;
-                        any_low = true;
-                    }
-                }
-            } else {
-                auto relax_slot = [&](int t) {
-                    const int sl0 = t * kBW + wid;
-                    const DT x0 = L[sl0 * kWave + lane];
-                    DT nb[kBD];
-#pragma unroll
-                    for (int q = 0; q < kBD; q++) {
-                        const int sl = __builtin_amdgcn_readlane(ic[q], t);
-                        nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
-                    }
-                    DT best = x0;
-#pragma unroll
-                    for (int q = 0; q < kBD; q++)
-                        best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
-                                         __builtin_amdgcn_readlane(iw[q], t), INF, best);
-                    if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
-                        const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
-                        const int eb = __builtin_amdgcn_readlane(em, t);
-                        for (int e = ea; e < eb; e++) {
-                            const Edge ed = badj[e];
-                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
-                        }
-                    }
-                    const bool lower = best < x0;
-                    if (lower) L[sl0 * kWave + lane] = best;
-                    if (__ballot(lower) != 0) {
-                        chg_mask |= 1ull << t;
-                        any_low = true;
-                    }
-                };
-                if (it & 1) {
-#pragma unroll
-                    for (int t = kBQ - 1; t >= 0; t--)
-                        if (t < nv) relax_slot(t);
-                } else {
-#pragma unroll
-                    for (int t = 0; t < kBQ; t++) {
-                        if (t >= nv) break;
-                        relax_slot(t);
-                    }
-                }
-            }
-            if (any_low && lane == 0) s_flag[it % 3] = 1;
-            __syncthreads();
-            if (s_flag[it % 3] == 0) {
-                if (stats && tid == 0) {
-                    atomicAdd(&stats[0], 1ull);
-                    atomicAdd(&stats[1], (unsigned long long)(it + 1));
-                }
-                break;
-            }
-        }
     } else {
-        // 2. sweeps.  Each wave drains a worklist of its own slots (all of
-        //    them on a dense round, else the block neighbours of the slots
-        //    the outside lowered): a slot whose label drops puts its block
-        //    neighbours on the list -- its own wave's at once, another
-        //    wave's through the LDS marks, which that wave takes up after
-        //    the barrier.  The visit ends when a sweep leaves no mark for
-        //    another wave.
+        // 2. sweeps.  A block's first visit of a relaxation (dense) starts
+        //    with full sweeps: every wave relaxes all its slots over their
+        //    block neighbours, in place, a barrier per sweep; odd sweeps take
+        //    a wave's slots in reverse order, so that a drop runs against the
+        //    slot order (breadth-first from the block's seed) as fast as along
+        //    it (16-bit labels with 32-bit distances keep one order and full
+        //    sweeps to the end: there the second loop spilled registers to
+        //    scratch).  After fsw full sweeps, the last of which marks the
+        //    block neighbours of the slots it lowered, and on every later
+        //    visit of a round without in-block distances, each wave drains a
+        //    worklist of its own slots: a slot whose label drops puts its
+        //    block neighbours on the list -- its own wave's at once, another
+        //    wave's through the LDS marks, which that wave takes up after the
+        //    barrier.  The visit ends when a full sweep lowers nothing or a
+        //    worklist sweep leaves no mark for another wave.
+        constexpr bool one_order = sizeof(DT) == 2 && sizeof(DW) == 4;
+        const int K = dense ? (one_order ? (1 << 30) : fsw) : 0;
         unsigned long long processed = 0;
         for (int it = 0;; it++) {
             uint8_t *dcur = s_dirty + (it & 1) * kBV;
             uint8_t *dnext = s_dirty + ((it + 1) & 1) * kBV;
             if (tid == 0) s_flag[(it + 1) % 3] = 0;
-            {
-                const bool dl = lane < nv && dcur[lane * kBW + wid] != 0;
-                if (dl) dcur[lane * kBW + wid] = 0;
-                wmask = uniform64(wmask | __ballot(dl));
-            }
-            bool cross = false;
-            while (wmask) {
-                int jj[4];
-                int nt = 0;
+            bool cross = false, flag;
+            if (it < K) {
+                const bool last = it == K - 1;
+                if (last) wmask = 0;
+                bool any_low = false;
+                if constexpr (one_order) {
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    if (wmask) {
-                        const int j = pick_bit(wmask, true);
-                        wmask &= wmask - 1;
-                        jj[t] = j;
-                        nt++;
+                    for (int t = 0; t < kBQ; t++) {
+                        if (t >= nv) break;
+                        const int sl0 = t * kBW + wid;
+                        const DT x0 = L[sl0 * kWave + lane];
+                        DT nb[kBD];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int sl = __builtin_amdgcn_readlane(ic[q], t);
+                            nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                        }
+                        DT best = x0;
+#pragma unroll
+                        for (int q = 0; q < kBD; q++)
+                            best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
+                                             __builtin_amdgcn_readlane(iw[q], t), INF, best);
+                        if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
+                            const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
+                            const int eb = __builtin_amdgcn_readlane(em, t);
+                            for (int e = ea; e < eb; e++) {
+                                const Edge ed = badj[e];
+                                best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                            }
+                        }
+                        const bool lower = best < x0;
+                        if (lower) L[sl0 * kWave + lane] = best;
+                        if (__ballot(lower) != 0) {
+                            chg_mask |= 1ull << t;
+                            any_low = true;
+                        }
+                    }
+                } else {
+                    auto relax_slot = [&](int t) {
+                        const int sl0 = t * kBW + wid;
+                        const DT x0 = L[sl0 * kWave + lane];
+                        DT nb[kBD];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int sl = __builtin_amdgcn_readlane(ic[q], t);
+                            nb[q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                        }
+                        DT best = x0;
+#pragma unroll
+                        for (int q = 0; q < kBD; q++)
+                            best = relax_min(nb[q], __builtin_amdgcn_readlane(ic[q], t),
+                                             __builtin_amdgcn_readlane(iw[q], t), INF, best);
+                        if (any_ovf && __builtin_amdgcn_readlane(ovf, t)) {
+                            const int ea = __builtin_amdgcn_readlane(e0, t) + kBD;
+                            const int eb = __builtin_amdgcn_readlane(em, t);
+                            for (int e = ea; e < eb; e++) {
+                                const Edge ed = badj[e];
+                                best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                            }
+                        }
+                        const bool lower = best < x0;
+                        if (lower) L[sl0 * kWave + lane] = best;
+                        if (__ballot(lower) != 0) {
+                            chg_mask |= 1ull << t;
+                            any_low = true;
+                            if (last) {
+                                const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, t);
+                                mark_block_neighbours(t, ic, wid, ov ? __builtin_amdgcn_readlane(e0, t) + kBD : 0,
+                                                      ov ? __builtin_amdgcn_readlane(em, t) : 0, badj, dnext,
+                                                      wmask, cross);
+                            }
+                        }
+                    };
+                    if (it & 1) {
+#pragma unroll
+                        for (int t = kBQ - 1; t >= 0; t--)
+                            if (t < nv) relax_slot(t);
                     } else {
-                        jj[t] = jj[0];
+#pragma unroll
+                        for (int t = 0; t < kBQ; t++) {
+                            if (t >= nv) break;
+                            relax_slot(t);
+                        }
                     }
                 }
-                processed += nt;
-                DT x[4], nb[4][kBD];
+                flag = any_low;
+            } else {
+                {
+                    const bool dl = lane < nv && dcur[lane * kBW + wid] != 0;
+                    if (dl) dcur[lane * kBW + wid] = 0;
+                    wmask = uniform64(wmask | __ballot(dl));
+                }
+                bool cross = false;
+                while (wmask) {
+                    int jj[4];
+                    int nt = 0;
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    x[t] = L[(jj[t] * kBW + wid) * kWave + lane];
+                    for (int t = 0; t < 4; t++) {
+                        if (wmask) {
+                            const int j = pick_bit(wmask, true);
+                            wmask &= wmask - 1;
+                            jj[t] = j;
+                            nt++;
+                        } else {
+                            jj[t] = jj[0];
+                        }
+                    }
+                    processed += nt;
+                    DT x[4], nb[4][kBD];
 #pragma unroll
-                    for (int q = 0; q < kBD; q++) {
-                        const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
-                        nb[t][q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                    for (int t = 0; t < 4; t++) {
+                        x[t] = L[(jj[t] * kBW + wid) * kWave + lane];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++) {
+                            const int sl = __builtin_amdgcn_readlane(ic[q], jj[t]);
+                            nb[t][q] = sl >= 0 ? L[sl * kWave + lane] : INF;
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if (t >= nt) break;
+                        const int j = jj[t];
+                        DT best = x[t];
+#pragma unroll
+                        for (int q = 0; q < kBD; q++)
+                            best = relax_min(nb[t][q], __builtin_amdgcn_readlane(ic[q], j),
+                                             __builtin_amdgcn_readlane(iw[q], j), INF, best);
+                        const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
+                        const int ea = ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0;
+                        const int eb = ov ? __builtin_amdgcn_readlane(em, j) : 0;
+                        for (int e = ea; e < eb; e++) {
+                            const Edge ed = badj[e];
+                            best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
+                        }
+                        const bool lower = best < x[t];
+                        if (lower) L[(j * kBW + wid) * kWave + lane] = best;
+                        if (__ballot(lower) != 0) {
+                            chg_mask |= 1ull << j;
+                            mark_block_neighbours(j, ic, wid, ea, eb, badj, dnext, wmask, cross);
+                        }
                     }
                 }
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    if (t >= nt) break;
-                    const int j = jj[t];
-                    DT best = x[t];
-#pragma unroll
-                    for (int q = 0; q < kBD; q++)
-                        best = relax_min(nb[t][q], __builtin_amdgcn_readlane(ic[q], j),
-                                         __builtin_amdgcn_readlane(iw[q], j), INF, best);
-                    const bool ov = any_ovf && __builtin_amdgcn_readlane(ovf, j);
-                    const int ea = ov ? __builtin_amdgcn_readlane(e0, j) + kBD : 0;
-                    const int eb = ov ? __builtin_amdgcn_readlane(em, j) : 0;
-                    for (int e = ea; e < eb; e++) {
-                        const Edge ed = badj[e];
-                        best = relax_min(L[ed.v * kWave + lane], ed.v, ed.w, INF, best);
-                    }
-                    const bool lower = best < x[t];
-                    if (lower) L[(j * kBW + wid) * kWave + lane] = best;
-                    if (__ballot(lower) != 0) {
-                        chg_mask |= 1ull << j;
-                        mark_block_neighbours(j, ic, wid, ea, eb, badj, dnext, wmask, cross);
-                    }
-                }
+                flag = cross;
             }
-            if (cross && lane == 0) s_flag[it % 3] = 1;
+            if (flag && lane == 0) s_flag[it % 3] = 1;
             __syncthreads();
             if (s_flag[it % 3] == 0) {
                 if (stats && lane == 0) {
@@ -1026,7 +1031,7 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<D
     const int32_t *__restrict__ vblk, const DW *__restrict__ Dm,
     const int64_t *__restrict__ bdoff, int64_t nitems, int dense, uint32_t *__restrict__ Fcur,
     uint32_t *__restrict__ Fnext, uint32_t *__restrict__ any, int r,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, int fsw) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     DT *L = reinterpret_cast<DT *>(lds_raw);  // L[slot * 64 + row]
     uint32_t *s_misc = reinterpret_cast<uint32_t *>(lds_raw + sizeof(DT) * kBV * kWave);
@@ -1065,7 +1070,7 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(kBlkWaves<D
             blk_visit<DT, DW>(D + (int64_t)c * np * kWave, boff[b], boff[b + 1] - boff[b], bvert,
                               brp, bext, badj, vblk, Dm ? Dm + bdoff[b] : nullptr,
                               Fnext + (int64_t)c * nblk, dense, L, s_flag, s_dirty, s_chg, stats,
-                              raised);
+                              raised, fsw);
             __syncthreads();  // LDS is reused by the next item
         }
         __syncthreads();  // s_cnt is reset by the next piece
