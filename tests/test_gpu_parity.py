@@ -425,12 +425,15 @@ def test_block_relax_grid_vs_oracle(gpu_ctx, monkeypatch, flags, closure):
     np.testing.assert_array_equal(code, ocode)
 
 
+@pytest.mark.parametrize("full_sweeps", ["1", "12"])
 @pytest.mark.parametrize("shape,flags", [("grid", 0), ("grid", emv.WIDE_LABELS), ("group", 0)])
-def test_block_relax_many_chunks_vs_oracle(gpu_ctx, shape, flags):
+def test_block_relax_many_chunks_vs_oracle(gpu_ctx, monkeypatch, shape, flags, full_sweeps):
     """Block-local relaxation over layers of many 64-row chunks (k = 12:
     layers 4 .. 8 hold 8 to 15 chunks, so one round's items span chunks and
     blocks): a grid, and a group-reduced grid whose virtual terminals join
-    blocks far apart. Every cell and record equal the oracle's."""
+    blocks far apart. A block's first visit hands over from full sweeps to
+    worklists after one sweep or after twelve (EMV_BLOCK_FULL_SWEEPS). Every
+    cell and record equal the oracle's."""
     import workloads
 
     if shape == "grid":
@@ -440,7 +443,9 @@ def test_block_relax_many_chunks_vs_oracle(gpu_ctx, shape, flags):
         kk = emv.terminals(n, 12, 43)
     else:
         _, _, _, n, e, kk, _ = workloads.group_instance(36, 36, 12, 5, 100, 44)
-    print(f"case block relax {shape} n={n} k={len(kk)} flags={flags}", flush=True)
+    monkeypatch.setenv("EMV_BLOCK_FULL_SWEEPS", full_sweeps)
+    print(f"case block relax {shape} n={n} k={len(kk)} flags={flags} full_sweeps={full_sweeps}",
+          flush=True)
     sol, f, code = solve_full(gpu_ctx, Instance(n, e, kk), flags)
     assert sol.relax_blocks > 0
     ocost, of, ocode = oracle.emv(n, e, kk)
