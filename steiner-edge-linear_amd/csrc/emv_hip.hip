@@ -815,7 +815,7 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
     // of an int64 table (row bases per pair: more rows cost it occupancy and
     // scalar registers; config 5's merges take 421 ms at four, 627 at eight)
     const bool rel = !std::is_same<T, uint16_t>::value && merge_fbase<T>(c) != nullptr;
-    build_stars(c->h_xs, nx, k, c->h_stars, rel ? 4 : kStarMax);
+    build_stars(c->h_xs, nx, k, c->h_stars, rel ? kStarRel : kStarMax);
     const int nstars = (int)c->h_stars.size();
     if (c->stars_cap < (size_t)nstars) {
         dfree(c->d_stars);
@@ -858,7 +858,7 @@ int launch_merge_star(emv_ctx *c, const T *f, int nx, int64_t nchunks, int k, in
         const int grid = grid_for((ntiles * nstars * parts + 3) / 4, c->nxcd, &per);
         const int64_t *fb = merge_fbase<T>(c);
         if (fb)
-            hipLaunchKernelGGL((k_merge_star_w<P, true, 4>), dim3(grid), dim3(kBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_merge_star_w<P, true, kStarRel>), dim3(grid), dim3(kBlock), 0, c->stream,
                                f, merge_layout<T>(c), fb, c->np, (const MergeStar *)c->d_stars,
                                nstars, (int)parts,
                                ntiles, k, (P *)c->d_M, c->nxcd, per);

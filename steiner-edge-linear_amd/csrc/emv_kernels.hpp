@@ -1628,6 +1628,14 @@ __global__ __launch_bounds__(kBlock) void k_merge_split_h2(const uint16_t *__res
 #endif
 constexpr int kStarMax = EMV_STAR_MAX;  // rows per star (4 .. 8)
 static_assert(kStarMax >= 4 && kStarMax <= 8, "stars of 4 to 8 rows");
+// rows per star over the compact int32 copy with row bases (more registers a
+// pair: 64-bit keys with the bases added; a 1024 x 1024 grid at k = 13 merged
+// in 428 / 425 / 506 ms with stars of 3 / 4 / 6 rows)
+#ifndef EMV_STAR_REL
+#define EMV_STAR_REL 4
+#endif
+constexpr int kStarRel = EMV_STAR_REL;
+static_assert(kStarRel >= 1 && kStarRel <= kStarMax, "relative stars of 1 to kStarMax rows");
 // R clamped to kStarMax (to RMAX), so every case of a dispatch over star
 // sizes compiles; stars of at most RMAX rows never reach the clamped cases
 template <int R>
